@@ -1,0 +1,172 @@
+/*
+ * sdddst_amd: C ABI of the MI355X (gfx950) implicit-step hot path of the
+ * SDDDST 2-D periodic discrete dislocation dynamics simulator.
+ *
+ * The reference (pgabor/sdddst) has no FFI on this path: its boundary is the
+ * C++ class API of sdddstCore::Simulation / Field / SimulationData.  The
+ * entry points below are what those methods bind to in this build; each one
+ * names the reference code it replaces.  The C++ mirror of that API
+ * (include/sdddst/simulation.h and friends) calls only these functions.
+ *
+ * Conventions
+ *   - Every function returns an int status: SDDDST_OK (0) or a negative
+ *     SDDDST_ERR_* code; sdddst_last_error() gives the message of the last
+ *     failure on the calling thread.  Nothing throws across the ABI.
+ *   - FP64 everywhere on the path; indices are int32 (UMFPACK "di").
+ *   - Host buffers are caller-owned; device buffers are context-owned.
+ *   - One HIP stream per context; a context is not thread-safe.  Calls that
+ *     return host data synchronise the context's stream.
+ *   - Position slots follow SimulationData (simulation_data.h:69-127):
+ *     0 dislocations, 1 bigStep, 2 firstSmall, 3 secondSmall.  Only x moves
+ *     (simulation.cpp:80); y and b are shared by all slots.
+ *   - Speed buffers: 0 initSpeed, 1 initSpeed2, 2 speed, 3 speed2
+ *     (simulation_data.h:75-81).
+ */
+#ifndef SDDDST_AMD_H
+#define SDDDST_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDDDST_OK 0
+#define SDDDST_ERR_ARG (-1)      /* bad argument or size                 */
+#define SDDDST_ERR_HIP (-2)      /* HIP runtime / device failure          */
+#define SDDDST_ERR_STATE (-3)    /* call out of order (e.g. no Jacobian)  */
+#define SDDDST_ERR_SOLVER (-4)   /* factorisation / solve failure         */
+#define SDDDST_ERR_NOMEM (-5)    /* device allocation failed              */
+#define SDDDST_ERR_UNSUPPORTED (-6)
+
+#define SDDDST_SLOT_DISLOCATIONS 0
+#define SDDDST_SLOT_BIG_STEP 1
+#define SDDDST_SLOT_FIRST_SMALL 2
+#define SDDDST_SLOT_SECOND_SMALL 3
+
+#define SDDDST_V_INIT 0
+#define SDDDST_V_INIT2 1
+#define SDDDST_V_SPEED 2
+#define SDDDST_V_SPEED2 3
+
+/* Field evaluation modes of the pair-speed kernel. */
+#define SDDDST_FIELD_FAST 0     /* near image as the reference, far images from one shared exp */
+#define SDDDST_FIELD_EXACT 1    /* every image as the reference (bit-compatible per pair term) */
+#define SDDDST_FIELD_ORDERED 2  /* exact terms summed in the reference's order (validation; 2x pair work) */
+
+/* Linear solvers for the Newton correction. */
+#define SDDDST_SOLVER_DENSE_LU 0   /* device LU with partial pivoting (rocSOLVER getrf/getrs) */
+#define SDDDST_SOLVER_FGMRES 1     /* device FGMRES, FP64, relative residual tolerance        */
+#define SDDDST_SOLVER_HOST 2       /* host hand-off: CSC to the caller (UMFPACK)              */
+
+typedef struct sdddst_ctx sdddst_ctx;
+
+/* Scalars of SimulationData that the kernels need
+ * (simulation_data.h:84-107, constants.h:23-35). */
+typedef struct sdddst_params {
+    double cutoff_multiplier; /* cutOffMultiplier; rc = m/sqrt(N), m := 1e20 if m >= sqrt(2N)/12 */
+    double precision;         /* prec: tolerance floor prec^2                                     */
+    double A;                 /* point-defect interaction strength                                */
+    double KASQR;             /* point-defect range parameter                                     */
+    int iteration_count;      /* ic: fixed Newton iterations per integrate (2)                    */
+    int field_mode;           /* SDDDST_FIELD_*                                                   */
+} sdddst_params;
+
+/* Library version string and last error message (thread-local). */
+const char *sdddst_version(void);
+const char *sdddst_last_error(void);
+
+/* Context lifetime.  Replaces SimulationData's buffer allocation
+ * (simulation_data.cpp:277-298); device memory is O(nnz), not O(N^2). */
+int sdddst_ctx_create(int n, int pc, const sdddst_params *params, sdddst_ctx **out);
+int sdddst_ctx_destroy(sdddst_ctx *ctx);
+int sdddst_set_params(sdddst_ctx *ctx, const sdddst_params *params);
+
+/* Cutoff of SimulationData::updateCutOff (simulation_data.cpp:208-218). */
+int sdddst_get_cutoff(sdddst_ctx *ctx, double *cutoff, double *cutoff_sqr);
+
+/* Positions.  upload: x (and y, b when non-NULL) of a slot; y and b are
+ * shared.  download_x: x of a slot.  copy_slot: x[dst] = x[src]
+ * (bigStep = dislocations etc., simulation.cpp:482, 496, 504). */
+int sdddst_upload_state(sdddst_ctx *ctx, int slot, const double *x, const double *y, const double *b, int n);
+int sdddst_download_x(sdddst_ctx *ctx, int slot, double *x, int n);
+int sdddst_copy_slot(sdddst_ctx *ctx, int dst, int src);
+int sdddst_swap_slots(sdddst_ctx *ctx, int a, int b);
+int sdddst_upload_points(sdddst_ctx *ctx, const double *px, const double *py, int pc);
+
+/* Simulation::calculateSpeeds (simulation.cpp:86-131) with
+ * PrecisionHandler::updateTolerance (precision_handler.cpp:72-87) folded in:
+ * v[out_buf] = pair speeds of slot cfg_slot + point-defect terms + b*sigma.
+ * update_tol != 0 lowers the device tolerance as the reference does.
+ * v_out (nullable) receives a host copy. */
+int sdddst_speeds(sdddst_ctx *ctx, int cfg_slot, int out_buf, double sigma, int update_tol, double *v_out);
+
+/* PrecisionHandler state: tolerance download / reset (precision_handler.cpp:62-70). */
+int sdddst_get_tolerance(sdddst_ctx *ctx, double *tol_out, int n);
+int sdddst_reset_tolerance(sdddst_ctx *ctx);
+int sdddst_download_speed(sdddst_ctx *ctx, int buf, double *v_out, int n);
+int sdddst_upload_speed(sdddst_ctx *ctx, int buf, const double *v, int n);
+int sdddst_copy_speed(sdddst_ctx *ctx, int dst, int src);
+
+/* Simulation::calculateJacobian (simulation.cpp:236-384): the CSC of
+ * A = I + diag((1+d)/2) * M for slot cfg_slot and step h, built on the
+ * device with the reference's exact sparsity pattern.  Two-phase: with
+ * Ap == NULL the matrix is built on the device and only *nnz is returned;
+ * a second call with non-NULL pointers copies Ap[n+1], Ai[nnz], Ax[nnz],
+ * dVec[n] and the diagonal positions diag[n] (SimulationData::indexes) to
+ * the host without rebuilding (pass rebuild = 0), or rebuilds (rebuild = 1). */
+int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, int *Ap, int *Ai, double *Ax,
+                        double *dVec, int *diag, int64_t *nnz);
+
+/* Simulation::calculateG (simulation.cpp:189-192): g = x_new -
+ * (1+d)/2*h*v[csp_buf] - x_old - (1-d)/2*h*v[isp_buf]; g_out nullable. */
+int sdddst_residual(sdddst_ctx *ctx, double h, int new_slot, int old_slot, int csp_buf, int isp_buf, double *g_out);
+
+/* Device linear solve of A delta = g with the current Jacobian
+ * (replaces calculateSparseFormForJacobian + solveEQSys,
+ * simulation.cpp:386-396, when the solver is a device solver).
+ * factor: once per Jacobian; solve: delta on the device, delta_out nullable. */
+int sdddst_solver_select(sdddst_ctx *ctx, int solver, double rel_tol, int max_iter, int restart);
+int sdddst_solver_factor(sdddst_ctx *ctx);
+int sdddst_solver_solve(sdddst_ctx *ctx, double *delta_out);
+int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks);
+
+/* Newton update x[slot] -= delta (simulation.cpp:78-81); delta_host NULL
+ * uses the device delta of the last solve. */
+int sdddst_newton_update(sdddst_ctx *ctx, int slot, const double *delta_host);
+
+/* Simulation::calculateXError + PrecisionHandler::updateError /
+ * getNewStepSize / reset + the max-stepsize clamp (simulation.cpp:398-405,
+ * 516, 643-649; precision_handler.cpp:62-119), on the device:
+ * max_ratio = max_i |x_big - x_small2|^2 / tol_i, accept = max_ratio < 1,
+ * h_next = new step size (clamped to hmax when hmax > 0), tolerance reset. */
+int sdddst_error_control(sdddst_ctx *ctx, double h, double hmax, int *accept, double *h_next, double *max_ratio,
+                         int *selected_id);
+
+/* Reductions of the log (simulation.cpp:460-461, 551, 557, utility.h:92-100):
+ * sum |v|, sum v^2 of a speed buffer (sequential order), strain increment
+ * sum b*(x[new]-x[old]), order parameter sum b*v. */
+int sdddst_speed_sums(sdddst_ctx *ctx, int buf, double *sum_abs, double *sum_sq, double *sum_bv);
+int sdddst_strain_increment(sdddst_ctx *ctx, int old_slot, int new_slot, double *out);
+
+int sdddst_synchronize(sdddst_ctx *ctx);
+
+/* AnalyticField::xy / xy_diff_x (AnalyticField.cpp:145-181) over arrays, on
+ * the device, with the context's field mode (which = 0 xy, 1 xy_diff_x;
+ * which = 2 evaluates the production form of xy regardless of the mode).
+ * Used to compare per-pair values with the reference's Field virtuals. */
+int sdddst_field_eval(sdddst_ctx *ctx, int which, int64_t n, const double *dx, const double *dy, double *out);
+
+/* Device elementary functions (cos, sin, exp, expm1, cosh, sinh, sqrt,
+ * pow(x, 1/3) for fn = 0..7) over an array: the bit-compatibility check of
+ * the device math against the host C library. */
+int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double *out);
+
+/* Timing of the last kernel launches (HIP events on the context stream). */
+int sdddst_last_kernel_ms(sdddst_ctx *ctx, const char *kernel, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDDDST_AMD_H */

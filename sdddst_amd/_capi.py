@@ -1,0 +1,314 @@
+"""ctypes binding of the C ABI in include/sdddst_amd.h (libsdddst_amd.so).
+
+The library is loaded from this package directory; there is no fallback: if
+it is missing or the device is absent the calls raise.  Array arguments are
+numpy float64/int32 arrays; every call checks the returned status and raises
+SdddstError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdddst_amd.so")
+
+OK = 0
+SLOT_DISLOCATIONS, SLOT_BIG_STEP, SLOT_FIRST_SMALL, SLOT_SECOND_SMALL = 0, 1, 2, 3
+V_INIT, V_INIT2, V_SPEED, V_SPEED2 = 0, 1, 2, 3
+FIELD_FAST, FIELD_EXACT, FIELD_ORDERED = 0, 1, 2
+SOLVER_DENSE_LU, SOLVER_FGMRES, SOLVER_HOST = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class SdddstError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("cutoff_multiplier", ctypes.c_double),
+        ("precision", ctypes.c_double),
+        ("A", ctypes.c_double),
+        ("KASQR", ctypes.c_double),
+        ("iteration_count", ctypes.c_int),
+        ("field_mode", ctypes.c_int),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "sdddst_version": (ctypes.c_char_p, []),
+    "sdddst_last_error": (ctypes.c_char_p, []),
+    "sdddst_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(_vp)]),
+    "sdddst_ctx_destroy": (ctypes.c_int, [_vp]),
+    "sdddst_set_params": (ctypes.c_int, [_vp, ctypes.POINTER(Params)]),
+    "sdddst_get_cutoff": (ctypes.c_int, [_vp, _dp, _dp]),
+    "sdddst_upload_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    "sdddst_download_x": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
+    "sdddst_copy_slot": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "sdddst_swap_slots": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "sdddst_upload_points": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int]),
+    "sdddst_speeds": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
+    "sdddst_get_tolerance": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "sdddst_reset_tolerance": (ctypes.c_int, [_vp]),
+    "sdddst_download_speed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
+    "sdddst_upload_speed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
+    "sdddst_copy_speed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "sdddst_jacobian_csc": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _dp, _ip, _i64p],
+    ),
+    "sdddst_residual": (
+        ctypes.c_int,
+        [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
+    ),
+    "sdddst_solver_select": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "sdddst_solver_factor": (ctypes.c_int, [_vp]),
+    "sdddst_solver_solve": (ctypes.c_int, [_vp, _dp]),
+    "sdddst_solver_stats": (ctypes.c_int, [_vp, _ip, _dp, _ip]),
+    "sdddst_newton_update": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "sdddst_error_control": (
+        ctypes.c_int,
+        [_vp, ctypes.c_double, ctypes.c_double, _ip, _dp, _dp, _ip],
+    ),
+    "sdddst_speed_sums": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
+    "sdddst_strain_increment": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "sdddst_synchronize": (ctypes.c_int, [_vp]),
+    "sdddst_field_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]),
+    "sdddst_libm_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+    "sdddst_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libsdddst_amd.so and declare every exported entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SdddstError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__; __graft_entry__.build()'` "
+            "or `make -C sdddst_amd`"
+        )
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def _dptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _iptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context (sdddst_ctx) holding the state of a system of n
+    dislocations and pc point defects."""
+
+    def __init__(self, n: int, pc: int = 0, cutoff_multiplier: float = 1.0, precision: float = 1e-6,
+                 A: float = 1e-4 * 16.0, KASQR: float = 1.65 * 1.65 * 1e6 / 256.0, iteration_count: int = 2,
+                 field_mode: int = FIELD_FAST):
+        self.lib = load_library()
+        self.n, self.pc = int(n), int(pc)
+        self.params = Params(cutoff_multiplier, precision, A, KASQR, iteration_count, field_mode)
+        h = _vp()
+        self._check(self.lib.sdddst_ctx_create(self.n, self.pc, ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    # -- plumbing --------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != OK:
+            msg = self.lib.sdddst_last_error().decode(errors="replace")
+            raise SdddstError(f"status {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sdddst_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self._check(self.lib.sdddst_set_params(self.h, ctypes.byref(self.params)))
+
+    def cutoff(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.sdddst_get_cutoff(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    # -- state -----------------------------------------------------------
+    def upload_state(self, slot: int, x, y=None, b=None):
+        x = _f64(x)
+        y = None if y is None else _f64(y)
+        b = None if b is None else _f64(b)
+        self._check(self.lib.sdddst_upload_state(self.h, slot, _dptr(x), _dptr(y), _dptr(b), self.n))
+
+    def download_x(self, slot: int) -> np.ndarray:
+        out = np.empty(self.n)
+        self._check(self.lib.sdddst_download_x(self.h, slot, _dptr(out), self.n))
+        return out
+
+    def copy_slot(self, dst: int, src: int):
+        self._check(self.lib.sdddst_copy_slot(self.h, dst, src))
+
+    def swap_slots(self, a: int, b: int):
+        self._check(self.lib.sdddst_swap_slots(self.h, a, b))
+
+    def upload_points(self, px, py):
+        px, py = _f64(px), _f64(py)
+        self._check(self.lib.sdddst_upload_points(self.h, _dptr(px), _dptr(py), len(px)))
+
+    # -- K1 --------------------------------------------------------------
+    def speeds(self, cfg_slot: int = 0, out_buf: int = V_INIT, sigma: float = 0.0, update_tol: bool = True,
+               download: bool = True) -> Optional[np.ndarray]:
+        out = np.empty(self.n) if download else None
+        self._check(self.lib.sdddst_speeds(self.h, cfg_slot, out_buf, sigma, int(update_tol), _dptr(out)))
+        return out
+
+    def tolerance(self) -> np.ndarray:
+        out = np.empty(self.n)
+        self._check(self.lib.sdddst_get_tolerance(self.h, _dptr(out), self.n))
+        return out
+
+    def reset_tolerance(self):
+        self._check(self.lib.sdddst_reset_tolerance(self.h))
+
+    def download_speed(self, buf: int) -> np.ndarray:
+        out = np.empty(self.n)
+        self._check(self.lib.sdddst_download_speed(self.h, buf, _dptr(out), self.n))
+        return out
+
+    def upload_speed(self, buf: int, v):
+        v = _f64(v)
+        self._check(self.lib.sdddst_upload_speed(self.h, buf, _dptr(v), self.n))
+
+    def copy_speed(self, dst: int, src: int):
+        self._check(self.lib.sdddst_copy_speed(self.h, dst, src))
+
+    # -- K2 --------------------------------------------------------------
+    def jacobian_nnz(self, cfg_slot: int, h: float) -> int:
+        nnz = ctypes.c_int64()
+        self._check(self.lib.sdddst_jacobian_csc(self.h, cfg_slot, h, 1, None, None, None, None, None,
+                                                 ctypes.byref(nnz)))
+        return nnz.value
+
+    def jacobian_csc(self, cfg_slot: int, h: float, rebuild: bool = True):
+        """Build and download (Ap, Ai, Ax, dVec, diag)."""
+        nnz = self.jacobian_nnz(cfg_slot, h) if rebuild else None
+        if nnz is None:
+            tmp = ctypes.c_int64()
+            self._check(self.lib.sdddst_jacobian_csc(self.h, cfg_slot, h, 0, None, None, None, None, None,
+                                                     ctypes.byref(tmp)))
+            nnz = tmp.value
+        Ap = np.empty(self.n + 1, dtype=np.int32)
+        Ai = np.empty(nnz, dtype=np.int32)
+        Ax = np.empty(nnz)
+        dv = np.empty(self.n)
+        dg = np.empty(self.n, dtype=np.int32)
+        out_nnz = ctypes.c_int64()
+        self._check(self.lib.sdddst_jacobian_csc(self.h, cfg_slot, h, 0, _iptr(Ap), _iptr(Ai), _dptr(Ax), _dptr(dv),
+                                                 _iptr(dg), ctypes.byref(out_nnz)))
+        return Ap, Ai, Ax, dv, dg
+
+    # -- K3/K4 and solver ------------------------------------------------
+    def residual(self, h: float, new_slot: int, old_slot: int, csp: int, isp: int, download: bool = False):
+        out = np.empty(self.n) if download else None
+        self._check(self.lib.sdddst_residual(self.h, h, new_slot, old_slot, csp, isp, _dptr(out)))
+        return out
+
+    def solver_select(self, solver: int, rel_tol: float = 0.0, max_iter: int = 0, restart: int = 0):
+        self._check(self.lib.sdddst_solver_select(self.h, solver, rel_tol, max_iter, restart))
+
+    def solver_factor(self):
+        self._check(self.lib.sdddst_solver_factor(self.h))
+
+    def solver_solve(self, download: bool = False):
+        out = np.empty(self.n) if download else None
+        self._check(self.lib.sdddst_solver_solve(self.h, _dptr(out)))
+        return out
+
+    def solver_stats(self):
+        it, rr, fb = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.sdddst_solver_stats(self.h, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(fb)))
+        return it.value, rr.value, fb.value
+
+    def newton_update(self, slot: int, delta=None):
+        d = None if delta is None else _f64(delta)
+        self._check(self.lib.sdddst_newton_update(self.h, slot, _dptr(d)))
+
+    def error_control(self, h: float, hmax: float = 0.0):
+        acc, hn, mr, sel = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.sdddst_error_control(self.h, h, hmax, ctypes.byref(acc), ctypes.byref(hn),
+                                                  ctypes.byref(mr), ctypes.byref(sel)))
+        return bool(acc.value), hn.value, mr.value, sel.value
+
+    def speed_sums(self, buf: int):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.sdddst_speed_sums(self.h, buf, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def strain_increment(self, old_slot: int, new_slot: int) -> float:
+        r = ctypes.c_double()
+        self._check(self.lib.sdddst_strain_increment(self.h, old_slot, new_slot, ctypes.byref(r)))
+        return r.value
+
+    def synchronize(self):
+        self._check(self.lib.sdddst_synchronize(self.h))
+
+    # -- per-pair evaluation and timing ---------------------------------
+    def field_eval(self, which: int, dx, dy) -> np.ndarray:
+        dx, dy = _f64(dx), _f64(dy)
+        out = np.empty_like(dx)
+        self._check(self.lib.sdddst_field_eval(self.h, which, len(dx), _dptr(dx), _dptr(dy), _dptr(out)))
+        return out
+
+    def libm_eval(self, fn: int, x) -> np.ndarray:
+        x = _f64(x)
+        out = np.empty_like(x)
+        self._check(self.lib.sdddst_libm_eval(self.h, fn, len(x), _dptr(x), _dptr(out)))
+        return out
+
+    def timing(self, on: bool):
+        ms = ctypes.c_double()
+        self._check(self.lib.sdddst_last_kernel_ms(self.h, b"enable" if on else b"disable", ctypes.byref(ms)))
+
+    def last_ms(self, kernel: str) -> float:
+        ms = ctypes.c_double()
+        self._check(self.lib.sdddst_last_kernel_ms(self.h, kernel.encode(), ctypes.byref(ms)))
+        return ms.value

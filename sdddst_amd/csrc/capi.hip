@@ -1,0 +1,470 @@
+// The C ABI of include/sdddst_amd.h: argument checking, device memory and
+// stream ownership, and the thin calls into the kernel launchers.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "context.h"
+#include "solvers.h"
+
+namespace sdamd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+// simulation_data.cpp:208-218
+void compute_cutoff(Context &c)
+{
+    double multiplier = c.p.cutoff_multiplier;
+    if (c.p.cutoff_multiplier >= 1. / 12. * sqrt(2.0 * double(c.n))) multiplier = 1e20;
+    c.cutoff = 1. / sqrt((double)c.n) * multiplier;
+    c.cutoff_sqr = c.cutoff * c.cutoff;
+    c.one_per_cutoff_sqr = 1. / c.cutoff_sqr;
+}
+
+static int check_params(const sdddst_params *p)
+{
+    SD_CHECK(p, SDDDST_ERR_ARG, "params is NULL");
+    SD_CHECK(p->precision > 0, SDDDST_ERR_ARG, "precision must be > 0");
+    SD_CHECK(p->iteration_count >= 1, SDDDST_ERR_ARG, "iteration_count must be >= 1");
+    SD_CHECK(p->field_mode >= SDDDST_FIELD_FAST && p->field_mode <= SDDDST_FIELD_ORDERED, SDDDST_ERR_ARG,
+             "unknown field mode %d", p->field_mode);
+    return SDDDST_OK;
+}
+
+static void free_all(Context &c)
+{
+    solver_free(c);
+    for (int s = 0; s < 4; ++s) {
+        if (c.x[s]) (void)hipFree(c.x[s]);
+        if (c.v[s]) (void)hipFree(c.v[s]);
+    }
+    void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
+                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c.hs) (void)hipHostFree(c.hs);
+    if (c.ev0) (void)hipEventDestroy(c.ev0);
+    if (c.ev1) (void)hipEventDestroy(c.ev1);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+template <class T>
+static int dalloc(T **p, size_t count)
+{
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, sizeof(T) * count) != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc of %zu bytes failed", sizeof(T) * count);
+        return SDDDST_ERR_NOMEM;
+    }
+    return SDDDST_OK;
+}
+
+static int create(int n, int pc, const sdddst_params *params, Context &c)
+{
+    SD_CHECK(n >= 2, SDDDST_ERR_ARG, "need at least 2 dislocations (n=%d)", n);
+    SD_CHECK(pc >= 0, SDDDST_ERR_ARG, "negative point-defect count");
+    int rc = check_params(params);
+    if (rc) return rc;
+    int ndev = 0;
+    SD_HIP(hipGetDeviceCount(&ndev));
+    SD_CHECK(ndev > 0, SDDDST_ERR_HIP, "no HIP device");
+    SD_HIP(hipGetDevice(&c.device));
+    c.n = n;
+    c.pc = pc;
+    c.p = *params;
+    compute_cutoff(c);
+    c.nb = (n + 63) / 64;
+    SD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    SD_HIP(hipEventCreate(&c.ev0));
+    SD_HIP(hipEventCreate(&c.ev1));
+    for (int s = 0; s < 4; ++s) {
+        if ((rc = dalloc(&c.x[s], n))) return rc;
+        if ((rc = dalloc(&c.v[s], n))) return rc;
+    }
+    if ((rc = dalloc(&c.y, n)) || (rc = dalloc(&c.b, n)) || (rc = dalloc(&c.px, pc)) || (rc = dalloc(&c.py, pc)) ||
+        (rc = dalloc(&c.tol, n)) || (rc = dalloc(&c.minr2, n)) || (rc = dalloc(&c.partial, (size_t)c.nb * n)) ||
+        (rc = dalloc(&c.dvec, n)) || (rc = dalloc(&c.g, n)) || (rc = dalloc(&c.delta, n)) ||
+        (rc = dalloc(&c.Ap, (size_t)n + 1)) || (rc = dalloc(&c.diag, n)) || (rc = dalloc(&c.colcount, n)) ||
+        (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, 1)))
+        return rc;
+    SD_HIP(hipHostMalloc((void **)&c.hs, sizeof(DevScalars), hipHostMallocDefault));
+    SD_HIP(hipMemsetAsync(c.dvec, 0, sizeof(double) * n, c.stream));
+    SD_HIP(hipMemsetAsync(c.x[0], 0, sizeof(double) * n, c.stream));
+    rc = launch_reset_tolerance(c);
+    if (rc) return rc;
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+static int valid_slot(int s) { return s >= 0 && s < 4; }
+
+}  // namespace sdamd
+
+using namespace sdamd;
+
+#define CTX_CHECK(ctx) SD_CHECK((ctx) != nullptr, SDDDST_ERR_ARG, "context is NULL")
+
+extern "C" {
+
+const char *sdddst_version(void) { return "sdddst_amd 0.1 (gfx950, FP64)"; }
+
+const char *sdddst_last_error(void) { return g_last_error.c_str(); }
+
+int sdddst_ctx_create(int n, int pc, const sdddst_params *params, sdddst_ctx **out)
+{
+    SD_CHECK(out != nullptr, SDDDST_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    sdddst_ctx *ctx = new (std::nothrow) sdddst_ctx();
+    SD_CHECK(ctx != nullptr, SDDDST_ERR_NOMEM, "host allocation failed");
+    int rc = create(n, pc, params, ctx->c);
+    if (rc) {
+        free_all(ctx->c);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return SDDDST_OK;
+}
+
+int sdddst_ctx_destroy(sdddst_ctx *ctx)
+{
+    if (!ctx) return SDDDST_OK;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    free_all(ctx->c);
+    delete ctx;
+    return SDDDST_OK;
+}
+
+int sdddst_set_params(sdddst_ctx *ctx, const sdddst_params *params)
+{
+    CTX_CHECK(ctx);
+    int rc = check_params(params);
+    if (rc) return rc;
+    ctx->c.p = *params;
+    compute_cutoff(ctx->c);
+    ctx->c.jac_valid = false;
+    return SDDDST_OK;
+}
+
+int sdddst_get_cutoff(sdddst_ctx *ctx, double *cutoff, double *cutoff_sqr)
+{
+    CTX_CHECK(ctx);
+    if (cutoff) *cutoff = ctx->c.cutoff;
+    if (cutoff_sqr) *cutoff_sqr = ctx->c.cutoff_sqr;
+    return SDDDST_OK;
+}
+
+int sdddst_upload_state(sdddst_ctx *ctx, int slot, const double *x, const double *y, const double *b, int n)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(slot), SDDDST_ERR_ARG, "bad slot %d", slot);
+    SD_CHECK(n == c.n, SDDDST_ERR_ARG, "n=%d does not match the context (%d)", n, c.n);
+    SD_CHECK(x != nullptr, SDDDST_ERR_ARG, "x is NULL");
+    SD_HIP(hipMemcpyAsync(c.x[slot], x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    if (y) SD_HIP(hipMemcpyAsync(c.y, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    if (b) SD_HIP(hipMemcpyAsync(c.b, b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    c.jac_valid = false;
+    return SDDDST_OK;
+}
+
+int sdddst_download_x(sdddst_ctx *ctx, int slot, double *x, int n)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(slot) && x && n == c.n, SDDDST_ERR_ARG, "bad arguments");
+    SD_HIP(hipMemcpyAsync(x, c.x[slot], sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_copy_slot(sdddst_ctx *ctx, int dst, int src)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(dst) && valid_slot(src), SDDDST_ERR_ARG, "bad slot");
+    if (dst != src)
+        SD_HIP(hipMemcpyAsync(c.x[dst], c.x[src], sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_swap_slots(sdddst_ctx *ctx, int a, int b)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(a) && valid_slot(b), SDDDST_ERR_ARG, "bad slot");
+    double *t = c.x[a];
+    c.x[a] = c.x[b];
+    c.x[b] = t;
+    return SDDDST_OK;
+}
+
+int sdddst_upload_points(sdddst_ctx *ctx, const double *px, const double *py, int pc)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(pc == c.pc, SDDDST_ERR_ARG, "pc=%d does not match the context (%d)", pc, c.pc);
+    if (pc == 0) return SDDDST_OK;
+    SD_CHECK(px && py, SDDDST_ERR_ARG, "NULL point arrays");
+    SD_HIP(hipMemcpyAsync(c.px, px, sizeof(double) * pc, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipMemcpyAsync(c.py, py, sizeof(double) * pc, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_speeds(sdddst_ctx *ctx, int cfg_slot, int out_buf, double sigma, int update_tol, double *v_out)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(cfg_slot) && out_buf >= 0 && out_buf < 4, SDDDST_ERR_ARG, "bad slot or buffer");
+    int rc = launch_speeds(c, cfg_slot, out_buf, sigma, update_tol);
+    if (rc) return rc;
+    if (v_out) {
+        SD_HIP(hipMemcpyAsync(v_out, c.v[out_buf], sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
+        SD_HIP(hipStreamSynchronize(c.stream));
+    }
+    return SDDDST_OK;
+}
+
+int sdddst_get_tolerance(sdddst_ctx *ctx, double *tol_out, int n)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(tol_out && n == c.n, SDDDST_ERR_ARG, "bad arguments");
+    SD_HIP(hipMemcpyAsync(tol_out, c.tol, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_reset_tolerance(sdddst_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    return launch_reset_tolerance(ctx->c);
+}
+
+int sdddst_download_speed(sdddst_ctx *ctx, int buf, double *v_out, int n)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(buf >= 0 && buf < 4 && v_out && n == c.n, SDDDST_ERR_ARG, "bad arguments");
+    SD_HIP(hipMemcpyAsync(v_out, c.v[buf], sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_upload_speed(sdddst_ctx *ctx, int buf, const double *v, int n)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(buf >= 0 && buf < 4 && v && n == c.n, SDDDST_ERR_ARG, "bad arguments");
+    SD_HIP(hipMemcpyAsync(c.v[buf], v, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_copy_speed(sdddst_ctx *ctx, int dst, int src)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(dst >= 0 && dst < 4 && src >= 0 && src < 4, SDDDST_ERR_ARG, "bad buffer");
+    if (dst != src)
+        SD_HIP(hipMemcpyAsync(c.v[dst], c.v[src], sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, int *Ap, int *Ai, double *Ax,
+                        double *dVec, int *diag, int64_t *nnz)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(cfg_slot), SDDDST_ERR_ARG, "bad slot");
+    if (Ap == nullptr || rebuild || !c.jac_valid) {
+        int rc = launch_jacobian(c, cfg_slot, h);
+        if (rc) return rc;
+    }
+    if (nnz) *nnz = c.nnz;
+    if (Ap) {
+        SD_CHECK(Ai && Ax, SDDDST_ERR_ARG, "Ai/Ax are NULL");
+        SD_HIP(hipMemcpyAsync(Ap, c.Ap, sizeof(int) * (c.n + 1), hipMemcpyDeviceToHost, c.stream));
+        SD_HIP(hipMemcpyAsync(Ai, c.Ai, sizeof(int) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+        SD_HIP(hipMemcpyAsync(Ax, c.Ax, sizeof(double) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+    }
+    if (dVec) SD_HIP(hipMemcpyAsync(dVec, c.dvec, sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
+    if (diag) SD_HIP(hipMemcpyAsync(diag, c.diag, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_residual(sdddst_ctx *ctx, double h, int new_slot, int old_slot, int csp_buf, int isp_buf, double *g_out)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(new_slot) && valid_slot(old_slot) && csp_buf >= 0 && csp_buf < 4 && isp_buf >= 0 &&
+                 isp_buf < 4,
+             SDDDST_ERR_ARG, "bad slot or buffer");
+    int rc = launch_residual(c, h, new_slot, old_slot, csp_buf, isp_buf);
+    if (rc) return rc;
+    if (g_out) {
+        SD_HIP(hipMemcpyAsync(g_out, c.g, sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
+        SD_HIP(hipStreamSynchronize(c.stream));
+    }
+    return SDDDST_OK;
+}
+
+int sdddst_solver_select(sdddst_ctx *ctx, int solver, double rel_tol, int max_iter, int restart)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(solver == SDDDST_SOLVER_DENSE_LU || solver == SDDDST_SOLVER_FGMRES || solver == SDDDST_SOLVER_HOST,
+             SDDDST_ERR_ARG, "unknown solver %d", solver);
+    c.solver = solver;
+    if (rel_tol > 0) c.solver_tol = rel_tol;
+    if (max_iter > 0) c.solver_max_iter = max_iter;
+    if (restart > 0) c.solver_restart = restart;
+    return SDDDST_OK;
+}
+
+int sdddst_solver_factor(sdddst_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    return solver_factor(ctx->c);
+}
+
+int sdddst_solver_solve(sdddst_ctx *ctx, double *delta_out)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    int rc = solver_solve(c);
+    if (rc) return rc;
+    if (delta_out) {
+        SD_HIP(hipMemcpyAsync(delta_out, c.delta, sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
+        SD_HIP(hipStreamSynchronize(c.stream));
+    }
+    return SDDDST_OK;
+}
+
+int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks)
+{
+    CTX_CHECK(ctx);
+    if (last_iters) *last_iters = ctx->c.last_iters;
+    if (last_rel_residual) *last_rel_residual = ctx->c.last_rel_res;
+    if (fallbacks) *fallbacks = ctx->c.fallbacks;
+    return SDDDST_OK;
+}
+
+int sdddst_newton_update(sdddst_ctx *ctx, int slot, const double *delta_host)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(slot), SDDDST_ERR_ARG, "bad slot");
+    if (delta_host) SD_HIP(hipMemcpyAsync(c.delta, delta_host, sizeof(double) * c.n, hipMemcpyHostToDevice, c.stream));
+    return launch_newton_update(c, slot);
+}
+
+int sdddst_error_control(sdddst_ctx *ctx, double h, double hmax, int *accept, double *h_next, double *max_ratio,
+                         int *selected_id)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    int rc = launch_error_control(c, h, hmax);
+    if (rc) return rc;
+    SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    if (accept) *accept = c.hs->accept;
+    if (h_next) *h_next = c.hs->h_next;
+    if (max_ratio) *max_ratio = c.hs->max_ratio;
+    if (selected_id) *selected_id = c.hs->selected;
+    return SDDDST_OK;
+}
+
+int sdddst_speed_sums(sdddst_ctx *ctx, int buf, double *sum_abs, double *sum_sq, double *sum_bv)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(buf >= 0 && buf < 4, SDDDST_ERR_ARG, "bad buffer");
+    int rc = launch_speed_sums(c, buf);
+    if (rc) return rc;
+    SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    if (sum_abs) *sum_abs = c.hs->sum_abs;
+    if (sum_sq) *sum_sq = c.hs->sum_sq;
+    if (sum_bv) *sum_bv = c.hs->sum_bv;
+    return SDDDST_OK;
+}
+
+int sdddst_strain_increment(sdddst_ctx *ctx, int old_slot, int new_slot, double *out)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(valid_slot(old_slot) && valid_slot(new_slot) && out, SDDDST_ERR_ARG, "bad arguments");
+    int rc = launch_strain(c, old_slot, new_slot);
+    if (rc) return rc;
+    SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    *out = c.hs->strain;
+    return SDDDST_OK;
+}
+
+int sdddst_synchronize(sdddst_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    SD_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SDDDST_OK;
+}
+
+int sdddst_field_eval(sdddst_ctx *ctx, int which, int64_t n, const double *dx, const double *dy, double *out)
+{
+    CTX_CHECK(ctx);
+    return field_eval(ctx->c, which, n, dx, dy, out);
+}
+
+int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double *out)
+{
+    CTX_CHECK(ctx);
+    return libm_eval(ctx->c, fn, n, x, out);
+}
+
+int sdddst_last_kernel_ms(sdddst_ctx *ctx, const char *kernel, double *ms)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(kernel && ms, SDDDST_ERR_ARG, "bad arguments");
+    if (strcmp(kernel, "enable") == 0) {
+        c.timing = true;
+        *ms = 0;
+        return SDDDST_OK;
+    }
+    if (strcmp(kernel, "disable") == 0) {
+        c.timing = false;
+        *ms = 0;
+        return SDDDST_OK;
+    }
+    if (strcmp(kernel, "pair") == 0) *ms = c.last_pair_kernel_ms;
+    else if (strcmp(kernel, "speeds") == 0) *ms = c.last_speeds_ms;
+    else if (strcmp(kernel, "jacobian") == 0) *ms = c.last_jacobian_ms;
+    else {
+        set_error("unknown kernel name '%s'", kernel);
+        return SDDDST_ERR_ARG;
+    }
+    return SDDDST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// Internal state of one simulation context: device buffers, the stream and
+// the launch helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/sdddst_amd.h"
+
+namespace sdamd {
+
+void set_error(const char *fmt, ...);
+
+#define SD_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            ::sdamd::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                               __LINE__);                                                       \
+            return SDDDST_ERR_HIP;                                                              \
+        }                                                                                       \
+    } while (0)
+
+#define SD_CHECK(cond, code, ...)            \
+    do {                                     \
+        if (!(cond)) {                       \
+            ::sdamd::set_error(__VA_ARGS__); \
+            return (code);                   \
+        }                                    \
+    } while (0)
+
+struct DenseLu;
+struct Fgmres;
+
+// Device scalars of the step controller (one per context).
+struct DevScalars {
+    double max_ratio;      // PrecisionHandler::maxErrorRatioSqr
+    double h_next;         // getNewStepSize result (after the clamp)
+    int accept;            // maxErrorRatioSqr < 1
+    int selected;          // PrecisionHandler::selectedID
+    double sum_abs, sum_sq, sum_bv, strain;
+};
+
+struct Context {
+    int n = 0, pc = 0;
+    sdddst_params p{};
+    double cutoff = 0, cutoff_sqr = 0, one_per_cutoff_sqr = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // positions: x per slot, y and b shared
+    double *x[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *y = nullptr, *b = nullptr;
+    double *px = nullptr, *py = nullptr;
+    // speeds
+    double *v[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *tol = nullptr;              // tolerance per dislocation
+    unsigned long long *minr2 = nullptr; // per-call minimum r^2 bits
+    double *partial = nullptr;          // [nb][n] pair partial sums
+    int nb = 0;                         // number of 64-wide blocks
+    // Newton
+    double *dvec = nullptr, *g = nullptr, *delta = nullptr;
+    // Jacobian (CSC)
+    int *Ap = nullptr, *Ai = nullptr, *diag = nullptr, *colcount = nullptr;
+    double *Ax = nullptr;               // final values
+    double *AxRow = nullptr;            // CSR-of-A values (same index arrays): for SpMV
+    int64_t nnz = 0, cap = 0;
+    bool jac_valid = false;
+    double jac_h = 0;
+    int *d_flags = nullptr;             // device flags: [0] zero-valued upper entries found
+    void *scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    // controller scalars
+    DevScalars *ds = nullptr;
+    DevScalars *hs = nullptr;           // pinned host mirror
+    // solver
+    int solver = SDDDST_SOLVER_DENSE_LU;
+    double solver_tol = 1e-10;
+    int solver_max_iter = 200, solver_restart = 30;
+    DenseLu *lu = nullptr;
+    Fgmres *gm = nullptr;
+    int last_iters = 0, fallbacks = 0;
+    double last_rel_res = 0;
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_speeds_ms = 0, last_jacobian_ms = 0, last_pair_kernel_ms = 0;
+    bool timing = false;
+};
+
+// cutoff of SimulationData::updateCutOff (simulation_data.cpp:208-218)
+void compute_cutoff(Context &c);
+
+// kernel launchers (return SDDDST status)
+int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol);
+int launch_jacobian(Context &c, int slot, double h);
+int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
+int launch_newton_update(Context &c, int slot);
+int launch_error_control(Context &c, double h, double hmax);
+int launch_speed_sums(Context &c, int buf);
+int launch_strain(Context &c, int old_slot, int new_slot);
+int launch_reset_tolerance(Context &c);
+int field_eval(Context &c, int which, int64_t n, const double *dx, const double *dy, double *out);
+int libm_eval(Context &c, int fn, int64_t n, const double *x, double *out);
+int solver_factor(Context &c);
+int solver_solve(Context &c);
+void solver_free(Context &c);
+
+}  // namespace sdamd
+
+struct sdddst_ctx {
+    sdamd::Context c;
+};

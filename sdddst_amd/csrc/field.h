@@ -1,0 +1,173 @@
+// The analytic periodic shear-stress field of an edge dislocation in the unit
+// cell [-0.5, 0.5)^2 (reference: src/Fields/AnalyticField.cpp) and the
+// periodic wrap of a coordinate difference (include/utility.h:29-40).
+//
+// Each function keeps the reference's expression tree and evaluation order
+// so that, with the bit-compatible elementary functions of libm_compat.h and
+// contraction disabled, the device reproduces the reference's per-pair
+// values.  pow(v, 2.0) is written v * v: an optimised build of the reference
+// folds it the same way and the two agree bit for bit.
+//
+// xy_fast() is the production form of xy(): the image at the
+// minimum-image offset (the only one that can suffer cancellation) is
+// evaluated exactly as the reference does; the nine far images take their
+// cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors.  Far
+// image terms then differ from the reference by a few ulps of terms that are
+// at least e^-pi smaller than the near term.
+#pragma once
+
+#include "libm_compat.h"
+
+namespace sdfield {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr double kPiSq = kPi * kPi;       // == pow(M_PI, 2.0) as folded by the compiler
+constexpr int kImages = 4;                // ANALYTIC_FIELD_N (include/constants.h:25)
+
+// utility.h:29-40.  Each +-1 step of the reference loop is exact, so the
+// loop equals d - k for the integer k that lands in [-0.5, 0.5).
+SDLIBM_FN double normalize(double d)
+{
+    double r = d - __builtin_rint(d);
+    if (r >= 0.5) r -= 1.0;
+    if (r < -0.5) r += 1.0;
+    return r;
+}
+
+// AnalyticField.cpp:33-38 (taylor branch of f, r^2 <= 1e-6).
+SDLIBM_FN double f_taylor(double dx, double dy)
+{
+    const double dx2pi = dx * 2.0 * kPi;
+    const double dy2pi = dy * 2.0 * kPi;
+    const double x2 = dx2pi * dx2pi, x4 = sdlibm::powi(dx2pi, 4), x6 = sdlibm::powi(dx2pi, 6);
+    const double y2 = dy2pi * dy2pi, y4 = sdlibm::powi(dy2pi, 4), y6 = sdlibm::powi(dy2pi, 6);
+    const double a = x6 / 720.0 + x4 / 24.0 + x2 * 0.5;
+    const double b = -y6 / 720.0 + y4 / 24.0 - y2 * 0.5;
+    const double c = (y6 + x6) / 720. + (x4 - y4) / 24.0 + (x2 + y2) * 0.5;
+    return ((a * b + b + a) * dx) / (c * c) * 2.0 * kPiSq;
+}
+
+// AnalyticField.cpp:24-39 with cosh(2 pi dx) supplied by the caller.
+SDLIBM_FN double f_given_cosh(double dx, double cosh2pix, double cos2piy)
+{
+    return dx * (cosh2pix * cos2piy - 1.0) / ((cosh2pix - cos2piy) * (cosh2pix - cos2piy)) * 2. * kPi * kPi;
+}
+
+// AnalyticField.cpp:24-39.
+SDLIBM_FN double f(double dx, double cos2piy, double dy)
+{
+    if (dx * dx + dy * dy > 1e-6) return f_given_cosh(dx, sdlibm::cosh(dx * 2.0 * kPi), cos2piy);
+    return f_taylor(dx, dy);
+}
+
+// AnalyticField.cpp:65-90 (taylor branch of f_dx).
+SDLIBM_FN double f_dx_taylor(double dx, double dy)
+{
+    const double X = dx * 2.0 * kPi;
+    const double Y = dy * 2.0 * kPi;
+    const double X2 = X * X, X3 = sdlibm::powi(X, 3), X4 = sdlibm::powi(X, 4), X5 = sdlibm::powi(X, 5),
+                 X6 = sdlibm::powi(X, 6), X7 = sdlibm::powi(X, 7);
+    const double Y2 = Y * Y, Y4 = sdlibm::powi(Y, 4), Y6 = sdlibm::powi(Y, 6);
+    const double a = X6 / 720.0 + X4 / 24.0 + X2 * 0.5;
+    const double b = -Y6 / 720.0 + Y4 / 24.0 - Y2 * 0.5;
+    const double c = (Y6 + X6) / 720. + (X4 - Y4) / 24.0 + (X2 + Y2) * 0.5;
+    const double cs = -(X7 * Y6) / 3628800.0 - (X5 * Y6) / 86400.0 - (X3 * Y6) / 4320.0 - (Y6 * X) / 720.0 +
+                      (X7 * Y4) / 120960.0 + (X5 * Y4) / 2880.0 + (X3 * Y4) / 144.0 + (Y4 * X) / 24.0 -
+                      (X7 * Y2) / 1080.0 - (X5 * Y2) / 240.0 - (X3 * Y2) / 12.0 - (Y2 * X) * 0.5 +
+                      X7 / 5040.0 + X5 / 120.0 + X3 / 6.0 + X;
+    const double c2 = c * c, c3 = sdlibm::powi(c, 3);
+    return ((a * b + b + a) / c2 + ((cs * dx) / c2 - (a * b + b + a) / c3 * dx * sdlibm::sinh(X) * 2.0) * kPi * 2.0) *
+           2.0 * kPiSq;
+}
+
+// AnalyticField.cpp:53-93.
+SDLIBM_FN double f_dx(double dx, double cos2piy, double dy)
+{
+    const double ch = sdlibm::cosh(kPi * 2.0 * dx);
+    const double sh = sdlibm::sinh(kPi * 2.0 * dx);
+    if (dx * dx + dy * dy > 1e-6) {
+        const double d = ch - cos2piy;
+        const double d2 = d * d;
+        const double d3 = sdlibm::powi(d, 3);
+        return ((ch * cos2piy - 1.0) / d2 +
+                dx * (sh * cos2piy * 2.0 * kPi / d2 - (ch * cos2piy - 1.0) / d3 * 4.0 * kPi * sh)) *
+               kPi * kPi * 2.0;
+    }
+    return f_dx_taylor(dx, dy);
+}
+
+// AnalyticField.cpp:145-160, exact form.  Branch-free selection of the
+// dx < 0 / dx >= 0 image sets: with s = +1 for dx < 0 and -1 otherwise the
+// reference's two expressions are -s*dx*f(dx+5s) + (1+s*dx)*f(dx-4s) + f(dx+4s)
+// + g3(dx), and every sign flip is exact.
+SDLIBM_FN double xy(double dx, double dy)
+{
+    const double cy = sdlibm::cos(kPi * 2.0 * dy);
+    const bool neg = dx < 0;
+    const double s = neg ? 1.0 : -1.0;
+    const double sdx = neg ? -dx : dx;          // -s*dx
+    double g = f(dx, cy, dy);
+    for (int k = 1; k < kImages; ++k) g = g + f(dx - (double)k, cy, dy) + f(dx + (double)k, cy, dy);
+    return sdx * f(dx + 5.0 * s, cy, dy) + (1.0 + s * dx) * f(dx - 4.0 * s, cy, dy) + f(dx + 4.0 * s, cy, dy) + g;
+}
+
+// AnalyticField.cpp:162-181.
+SDLIBM_FN double xy_diff_x(double dx, double dy)
+{
+    const double cy = sdlibm::cos(kPi * 2.0 * dy);
+    const double N = (double)kImages;
+    double g = f_dx(dx, cy, dy);
+    for (int k = 1; k < kImages; ++k) g = g + f_dx(dx - (double)k, cy, dy) + f_dx(dx + (double)k, cy, dy);
+    if (dx < 0) {
+        const double a = dx + N + 1.0;
+        return -f(a, cy, dy) - dx * f_dx(a, cy, dy) + f_dx(dx - N, cy, dy) * (1.0 + dx) + f(dx - N, cy, dy) +
+               f_dx(dx + N, cy, dy) + g;
+    }
+    const double a = dx - N - 1.0;
+    return f(a, cy, dy) + dx * f_dx(a, cy, dy) + f_dx(dx + N, cy, dy) * (1.0 - dx) - f(dx + N, cy, dy) +
+           f_dx(dx - N, cy, dy) + g;
+}
+
+
+// Production form of xy() (see header comment).
+SDLIBM_FN double xy_fast(double dx, double dy)
+{
+    const double cy = sdlibm::cos(kPi * 2.0 * dy);
+    const double a0 = dx * 2.0 * kPi;
+    const double ax = sdlibm::fabsd(a0);
+    // Near image: the reference's cosh, evaluated once.
+    double c0, t;
+    if (sdlibm::hiword(ax) < 0x3fd62e43u) {
+        const double m = sdlibm::expm1(ax);
+        const double w = 1.0 + m;
+        c0 = 1.0 + (m * m) / (w + w);
+        t = w;
+    } else {
+        t = sdlibm::exp(ax);
+        c0 = 0.5 * t + 0.5 / t;
+    }
+    const double near = (dx * dx + dy * dy > 1e-6) ? f_given_cosh(dx, c0, cy) : f_taylor(dx, dy);
+    // e^{+a0}, e^{-a0}
+    const double ti = 1.0 / t;
+    const double ep = a0 < 0 ? ti : t;
+    const double em = a0 < 0 ? t : ti;
+    const bool neg = dx < 0;
+    const double s = neg ? 1.0 : -1.0;
+    const double sdx = neg ? -dx : dx;
+    // cosh(2 pi (dx + j)) for j = +-1..+-5.
+    double g = near;
+    for (int k = 1; k < kImages; ++k) {
+        const double cm = 0.5 * (ep * sdlibm_exp_m2pik[k] + em * sdlibm_exp_2pik[k]);
+        const double cp = 0.5 * (ep * sdlibm_exp_2pik[k] + em * sdlibm_exp_m2pik[k]);
+        g = g + f_given_cosh(dx - (double)k, cm, cy) + f_given_cosh(dx + (double)k, cp, cy);
+    }
+    // Images at dx + 5s, dx - 4s, dx + 4s.
+    const double c5 = neg ? 0.5 * (ep * sdlibm_exp_2pik[5] + em * sdlibm_exp_m2pik[5]) : 0.5 * (ep * sdlibm_exp_m2pik[5] + em * sdlibm_exp_2pik[5]);
+    const double c4m = neg ? 0.5 * (ep * sdlibm_exp_m2pik[4] + em * sdlibm_exp_2pik[4]) : 0.5 * (ep * sdlibm_exp_2pik[4] + em * sdlibm_exp_m2pik[4]);
+    const double c4p = neg ? 0.5 * (ep * sdlibm_exp_2pik[4] + em * sdlibm_exp_m2pik[4]) : 0.5 * (ep * sdlibm_exp_m2pik[4] + em * sdlibm_exp_2pik[4]);
+    return sdx * f_given_cosh(dx + 5.0 * s, c5, cy) + (1.0 + s * dx) * f_given_cosh(dx - 4.0 * s, c4m, cy) +
+           f_given_cosh(dx + 4.0 * s, c4p, cy) + g;
+}
+
+}  // namespace sdfield

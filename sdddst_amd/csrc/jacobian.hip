@@ -1,0 +1,343 @@
+// K2: cutoff-limited Jacobian assembly (Simulation::calculateJacobian,
+// simulation.cpp:236-384) into the reference's CSC arrays Ap[N+1], Ai[nnz],
+// Ax[nnz] with the reference's exact sparsity pattern.
+//
+// Pattern.  Column j holds, with rows ascending: the rows i < j whose pair
+// passed the predicate and whose value is nonzero (simulation.cpp:260-268),
+// the diagonal j, and the rows i > j that pass the predicate
+// (simulation.cpp:327-345).  The predicate of the pair {lo < hi} is
+//     (sqrt(dx*dx + dy*dy) - rc)^2 < 36.8 * rc^2,
+// dx = wrap(x_hi - x_lo), dy = wrap(y_hi - y_lo), every operation rounded on
+// its own (no contraction), pow(., 2) == ., * . as an optimised build folds
+// it.  The pattern is therefore symmetric except when a lower value is
+// exactly zero; that case is detected and compacted afterwards.
+//
+// Stages (all on the context stream):
+//   count  one wavefront per column; lanes sweep the rows 64 at a time with
+//          a cheap r^2 pre-filter before the square root;
+//   scan   one workgroup, exclusive prefix sum into Ap (int64 checked);
+//   fill   one wavefront per column; ballot + prefix popcount write the rows
+//          in ascending order and record the diagonal position;
+//   values one wavefront per column over its strictly-lower rows: the
+//          reference's value h*b_i*b_j*tau_x*taper once per unique pair, also
+//          stored at the mirror (row j of column i, found by binary search);
+//   diag   one lane per column: the column sum in storage order
+//          (simulation.cpp:349-374), d_j;
+//   scale  one wavefront per column: A = row-scaled values (+1 on the
+//          diagonal, simulation.cpp:376-383) and the same matrix row-major
+//          (AxRow: row j of A on the index arrays of column j) for SpMV.
+//
+// Algorithmic HBM traffic of one assembly: 12*nnz + 40*N + 4 bytes.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "context.h"
+#include "field.h"
+
+namespace sdamd {
+
+namespace {
+
+struct CutParams {
+    double rc, rc2, thr, prefilter, inv_rc2;
+};
+
+// simulation.cpp:335 for the pair (lo, hi), dx = x_hi - x_lo.
+__device__ __forceinline__ bool pair_pred(double dx, double dy, const CutParams &cp, double &r2_out)
+{
+    const double r2 = dx * dx + dy * dy;
+    r2_out = r2;
+    if (!(r2 < cp.prefilter)) return false;
+    const double s = sdlibm::sqrt(r2) - cp.rc;
+    return s * s < cp.thr;
+}
+
+__device__ __forceinline__ bool pred_ij(const double *__restrict__ x, const double *__restrict__ y, int i, int j,
+                                        const CutParams &cp)
+{
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const double dx = sdfield::normalize(x[hi] - x[lo]);
+    const double dy = sdfield::normalize(y[hi] - y[lo]);
+    double r2;
+    return pair_pred(dx, dy, cp, r2);
+}
+
+__global__ __launch_bounds__(256) void k_jac_count(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                   CutParams cp, int *__restrict__ cnt)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    int c = 0;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const int i = i0 + lane;
+        if (i < n && i != j && pred_ij(x, y, i, j, cp)) ++c;
+    }
+    // wave reduction of the counts
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if (lane == 0) cnt[j] = c + 1;   // + diagonal
+}
+
+// Exclusive scan of n counts into Ap[0..n] by one workgroup of 1024 lanes.
+__global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cnt, int *__restrict__ Ap,
+                                               long long *__restrict__ total)
+{
+    __shared__ long long part[1024];
+    const int t = (int)threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int s0 = t * per, s1 = min(n, s0 + per);
+    long long acc = 0;
+    for (int k = s0; k < s1; ++k) acc += cnt[k];
+    part[t] = acc;
+    __syncthreads();
+    if (t == 0) {
+        long long run = 0;
+        for (int k = 0; k < 1024; ++k) {
+            const long long v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    long long run = part[t];
+    for (int k = s0; k < s1; ++k) {
+        Ap[k] = (int)run;
+        run += cnt[k];
+    }
+    if (t == 0) Ap[n] = (int)(*total);
+}
+
+__global__ __launch_bounds__(256) void k_jac_fill(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                  CutParams cp, const int *__restrict__ Ap, int *__restrict__ Ai,
+                                                  int *__restrict__ diag)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    int pos = Ap[j];
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const int i = i0 + lane;
+        const bool in = i < n && (i == j || pred_ij(x, y, i, j, cp));
+        const unsigned long long mask = __ballot(in);
+        if (in) {
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            Ai[pos + rank] = i;
+            if (i == j) diag[j] = pos + rank;
+        }
+        pos += __popcll(mask);
+    }
+}
+
+// Row j inside the strictly-upper part [lo, hi) of a column (ascending).
+__device__ __forceinline__ int find_row(const int *__restrict__ Ai, int lo, int hi, int row)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = Ai[mid];
+        if (v < row) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                    const double *__restrict__ b, CutParams cp, double h,
+                                                    const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                    const int *__restrict__ diag, double *__restrict__ Ax,
+                                                    int *__restrict__ flags)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    const int d = diag[j];
+    const int end = Ap[j + 1];
+    const double xj = x[j], yj = y[j], bj = b[j];
+    if (lane == 0) Ax[d] = -0.0 * h;    // no point defects: -tmp*h with tmp = 0 (simulation.cpp:325)
+    for (int k = d + 1 + lane; k < end; k += 64) {
+        const int i = Ai[k];
+        const double dx = sdfield::normalize(x[i] - xj);
+        const double dy = sdfield::normalize(y[i] - yj);
+        const double r2 = dx * dx + dy * dy;
+        double mult = 1;
+        if (r2 > cp.rc2) {
+            const double s = sdlibm::sqrt(r2) - cp.rc;
+            mult = sdlibm::exp(-(s * s) * cp.inv_rc2);
+        }
+        const double v = h * b[i] * bj * sdfield::xy_diff_x(dx, dy) * mult;
+        Ax[k] = v;
+        // mirror: row j of column i lies in its strictly-upper part
+        const int m = find_row(Ai, Ap[i], diag[i], j);
+        Ax[m] = v;
+        if (v == 0.0) atomicOr(flags, 1);
+    }
+}
+
+__global__ void k_jac_diag(int n, const int *__restrict__ Ap, const int *__restrict__ diag, double *__restrict__ Ax,
+                           double *__restrict__ dvec)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= n) return;
+    double subSum = 0;
+    const int e = Ap[j + 1];
+    for (int k = Ap[j]; k < e; ++k) subSum += Ax[k];
+    subSum *= -1.;
+    Ax[diag[j]] = subSum;
+    double dv;
+    if (subSum > 0) {
+        subSum = 1. / subSum;
+        subSum += 1.;
+        subSum *= subSum;
+        dv = 1. / subSum;
+    } else {
+        dv = 0.;
+    }
+    dvec[j] = dv;
+}
+
+__global__ __launch_bounds__(256) void k_jac_scale(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                   const int *__restrict__ diag, const double *__restrict__ dvec,
+                                                   double *__restrict__ Ax, double *__restrict__ AxRow)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    const double Dj = (1.0 + dvec[j]) * 0.5;
+    const int dpos = diag[j];
+    for (int k = Ap[j] + lane; k < Ap[j + 1]; k += 64) {
+        const double raw = Ax[k];
+        double a = raw * ((1.0 + dvec[Ai[k]]) * 0.5);
+        double r = raw * Dj;
+        if (k == dpos) {
+            a += 1.0;
+            r += 1.0;
+        }
+        Ax[k] = a;
+        if (AxRow) AxRow[k] = r;
+    }
+}
+
+}  // namespace
+
+static CutParams cut_params(const Context &c)
+{
+    CutParams cp;
+    cp.rc = c.cutoff;
+    cp.rc2 = c.cutoff_sqr;
+    cp.thr = 36.8 * c.cutoff_sqr;
+    cp.inv_rc2 = c.one_per_cutoff_sqr;
+    // r < (1 + sqrt(36.8)) rc is necessary for the predicate; the margin
+    // covers every rounding of the exact test.
+    const double rmax = (1.0 + 6.06630035524124) * c.cutoff;
+    cp.prefilter = rmax * rmax * (1.0 + 1e-9) + 1e-300;
+    if (!(cp.prefilter < 1e300)) cp.prefilter = __builtin_inf();
+    return cp;
+}
+
+// Rebuild without the upper entries whose mirrored value is exactly zero
+// (getElement returns 0 for them, simulation.cpp:263).  Host path: it runs
+// only when such a value occurred.
+static int compact_zero_upper(Context &c)
+{
+    const int n = c.n;
+    std::vector<int> Ap(n + 1), Ai(c.nnz), diag(n);
+    std::vector<double> Ax(c.nnz);
+    SD_HIP(hipMemcpyAsync(Ap.data(), c.Ap, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipMemcpyAsync(Ai.data(), c.Ai, sizeof(int) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipMemcpyAsync(Ax.data(), c.Ax, sizeof(double) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> nAp(n + 1), nAi, ndiag(n);
+    std::vector<double> nAx;
+    nAi.reserve(c.nnz);
+    nAx.reserve(c.nnz);
+    nAp[0] = 0;
+    for (int j = 0; j < n; ++j) {
+        for (int k = Ap[j]; k < Ap[j + 1]; ++k) {
+            const int i = Ai[k];
+            if (i < j && Ax[k] == 0.0) continue;
+            if (i == j) ndiag[j] = (int)nAi.size();
+            nAi.push_back(i);
+            nAx.push_back(Ax[k]);
+        }
+        nAp[j + 1] = (int)nAi.size();
+    }
+    c.nnz = (int64_t)nAi.size();
+    SD_HIP(hipMemcpyAsync(c.Ap, nAp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipMemcpyAsync(c.Ai, nAi.data(), sizeof(int) * c.nnz, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipMemcpyAsync(c.Ax, nAx.data(), sizeof(double) * c.nnz, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipMemcpyAsync(c.diag, ndiag.data(), sizeof(int) * n, hipMemcpyHostToDevice, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    return SDDDST_OK;
+}
+
+static int ensure_capacity(Context &c, int64_t need)
+{
+    if (need <= c.cap) return SDDDST_OK;
+    int64_t ncap = need + need / 8 + 1024;
+    if (c.Ai) SD_HIP(hipFree(c.Ai));
+    if (c.Ax) SD_HIP(hipFree(c.Ax));
+    if (c.AxRow) SD_HIP(hipFree(c.AxRow));
+    c.Ai = nullptr;
+    c.Ax = nullptr;
+    c.AxRow = nullptr;
+    c.cap = 0;
+    if (hipMalloc(&c.Ai, sizeof(int) * ncap) != hipSuccess || hipMalloc(&c.Ax, sizeof(double) * ncap) != hipSuccess ||
+        hipMalloc(&c.AxRow, sizeof(double) * ncap) != hipSuccess) {
+        set_error("device allocation of %lld Jacobian entries failed", (long long)ncap);
+        return SDDDST_ERR_NOMEM;
+    }
+    c.cap = ncap;
+    return SDDDST_OK;
+}
+
+int launch_jacobian(Context &c, int slot, double h)
+{
+    const int n = c.n;
+    SD_CHECK(c.pc == 0, SDDDST_ERR_UNSUPPORTED, "Jacobian with point defects is not supported yet");
+    const CutParams cp = cut_params(c);
+    const double *x = c.x[slot];
+    const int wblocks = (n + 3) / 4;
+    if (c.timing) SD_HIP(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.colcount);
+    SD_HIP(hipGetLastError());
+    long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, n, c.colcount, c.Ap, d_total);
+    SD_HIP(hipGetLastError());
+    long long total = 0;
+    SD_HIP(hipMemcpyAsync(&total, d_total, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
+    int rc = ensure_capacity(c, total);
+    if (rc) return rc;
+    c.nnz = total;
+    SD_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap, c.Ai, c.diag);
+    SD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_jac_values, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, c.b, cp, h, c.Ap, c.Ai, c.diag,
+                       c.Ax, c.d_flags);
+    SD_HIP(hipGetLastError());
+    int flag = 0;
+    SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    if (flag) {
+        rc = compact_zero_upper(c);
+        if (rc) return rc;
+    }
+    const int tblocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_jac_diag, dim3(tblocks), dim3(256), 0, c.stream, n, c.Ap, c.diag, c.Ax, c.dvec);
+    SD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec, c.Ax,
+                       c.AxRow);
+    SD_HIP(hipGetLastError());
+    if (c.timing) {
+        SD_HIP(hipEventRecord(c.ev1, c.stream));
+        SD_HIP(hipEventSynchronize(c.ev1));
+        SD_HIP(hipEventElapsedTime(&c.last_jacobian_ms, c.ev0, c.ev1));
+    }
+    c.jac_valid = true;
+    c.jac_h = h;
+    return SDDDST_OK;
+}
+
+}  // namespace sdamd

@@ -1,0 +1,294 @@
+// Restarted flexible GMRES in FP64 for A delta = g, with A in its CSR view
+// (AxRow on the CSC index arrays: the pattern is symmetric) and a right
+// preconditioner applied per direction (Jacobi by default, a sparse
+// approximate inverse when one is installed).  Acceptance: the true relative
+// residual ||g - A delta|| / ||g|| <= rel_tol, checked with an explicit SpMV
+// at the end of every cycle; non-convergence returns SDDDST_ERR_SOLVER and
+// the dispatcher falls back to the direct solver.
+//
+// Arnoldi uses classical Gram-Schmidt applied twice (CGS2): one kernel
+// computes all j+1 projections at once, so each iteration costs one SpMV,
+// two projection passes and one 8*(j+2)-byte device-to-host copy of the
+// Hessenberg column; the Givens rotations run on the host.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include <vector>
+
+#include "context.h"
+#include "solvers.h"
+
+namespace sdamd {
+
+namespace {
+
+constexpr int kRed = 256;   // workgroups of the reductions
+
+// y = A x, one wavefront per row.
+__global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                              const double *__restrict__ Av, const double *__restrict__ x,
+                                              double *__restrict__ y)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int r = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (r >= n) return;
+    double s = 0.0;
+    for (int k = Ap[r] + lane; k < Ap[r + 1]; k += 64) s += Av[k] * x[Ai[k]];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[r] = s;
+}
+
+// partial[b*(nv) + v] = sum over this block's rows of V[v][i] * w[i], v < nv
+__global__ __launch_bounds__(256) void k_proj_partial(int n, int nv, const double *__restrict__ V,
+                                                      const double *__restrict__ w, double *__restrict__ partial)
+{
+    __shared__ double s[256];
+    for (int v = 0; v < nv; ++v) {
+        double acc = 0.0;
+        for (int i = (int)(blockIdx.x * 256 + threadIdx.x); i < n; i += kRed * 256) acc += V[(size_t)v * n + i] * w[i];
+        s[threadIdx.x] = acc;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * nv + v] = s[0];
+        __syncthreads();
+    }
+}
+
+// h[v] = sum_b partial[b][v] (fixed order), one lane per v
+__global__ void k_proj_final(int nv, const double *__restrict__ partial, double *__restrict__ h, int accumulate)
+{
+    const int v = (int)threadIdx.x;
+    if (v >= nv) return;
+    double s = 0.0;
+    for (int b = 0; b < kRed; ++b) s += partial[(size_t)b * nv + v];
+    h[v] = accumulate ? h[v] + s : s;
+}
+
+// w -= sum_v h[v] V[v]   (h from the latest projection pass in hp)
+__global__ void k_orth(int n, int nv, const double *__restrict__ V, const double *__restrict__ hp,
+                       double *__restrict__ w)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double acc = w[i];
+    for (int v = 0; v < nv; ++v) acc -= hp[v] * V[(size_t)v * n + i];
+    w[i] = acc;
+}
+
+__global__ void k_scale_copy(int n, double a, const double *__restrict__ x, double *__restrict__ y)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) y[i] = a * x[i];
+}
+
+__global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const double *__restrict__ v,
+                               double *__restrict__ z)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) z[i] = dinv[i] * v[i];
+}
+
+__global__ void k_jacobi_setup(int n, const int *__restrict__ diag, const double *__restrict__ Ax,
+                               double *__restrict__ dinv)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) dinv[i] = 1.0 / Ax[diag[i]];
+}
+
+// x += sum_j y[j] Z[j]
+__global__ void k_combine(int n, int k, const double *__restrict__ Z, const double *__restrict__ yv,
+                          double *__restrict__ x)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double acc = x[i];
+    for (int j = 0; j < k; ++j) acc += yv[j] * Z[(size_t)j * n + i];
+    x[i] = acc;
+}
+
+// r = g - r (r holds A x on entry)
+__global__ void k_resid(int n, const double *__restrict__ g, double *__restrict__ r)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) r[i] = g[i] - r[i];
+}
+
+}  // namespace
+
+static inline int nb256(int n) { return (n + 255) / 256; }
+
+int Fgmres::init(Context &c)
+{
+    if (V && n == c.n && m == c.solver_restart) return SDDDST_OK;
+    release();
+    n = c.n;
+    m = c.solver_restart;
+    SD_HIP(hipMalloc(&V, sizeof(double) * (size_t)(m + 1) * n));
+    SD_HIP(hipMalloc(&Z, sizeof(double) * (size_t)m * n));
+    SD_HIP(hipMalloc(&w, sizeof(double) * n));
+    SD_HIP(hipMalloc(&dinv, sizeof(double) * n));
+    SD_HIP(hipMalloc(&red, sizeof(double) * (size_t)kRed * (m + 2)));
+    SD_HIP(hipMalloc(&hcol, sizeof(double) * 2 * (m + 2)));
+    SD_HIP(hipHostMalloc(&H, sizeof(double) * 2 * (m + 2), hipHostMallocDefault));
+    return SDDDST_OK;
+}
+
+void Fgmres::release()
+{
+    if (V) (void)hipFree(V);
+    if (Z) (void)hipFree(Z);
+    if (w) (void)hipFree(w);
+    if (dinv) (void)hipFree(dinv);
+    if (red) (void)hipFree(red);
+    if (hcol) (void)hipFree(hcol);
+    if (H) (void)hipHostFree(H);
+    V = Z = w = dinv = red = hcol = H = nullptr;
+}
+
+int Fgmres::factor(Context &c)
+{
+    int rc = init(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+// dots of w with the first nv rows of V into hcol[0..nv) (two passes: CGS2),
+// then hcol[nv] = ||w|| after orthogonalisation; copies to host H.
+static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
+{
+    const int n = f.n;
+    double *hp = f.hcol;             // this pass's projections
+    double *hsum = f.hcol + (f.m + 2); // accumulated
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_proj_partial, dim3(kRed), dim3(256), 0, c.stream, n, nv, f.V, f.w, f.red);
+        hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64 * ((nv + 63) / 64)), 0, c.stream, nv, f.red, hp, 0);
+        hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, hp, f.w);
+        hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64 * ((nv + 63) / 64)), 0, c.stream, nv, f.red, hsum,
+                           pass);
+        SD_HIP(hipGetLastError());
+    }
+    // ||w||^2
+    hipLaunchKernelGGL(k_proj_partial, dim3(kRed), dim3(256), 0, c.stream, n, 1, f.w, f.w, f.red);
+    hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64), 0, c.stream, 1, f.red, hsum + nv, 0);
+    SD_HIP(hipGetLastError());
+    SD_HIP(hipMemcpyAsync(hhost, hsum, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    hhost[nv] = sqrt(hhost[nv]);
+    return SDDDST_OK;
+}
+
+static int norm2(Context &c, Fgmres &f, const double *v, double *out)
+{
+    hipLaunchKernelGGL(k_proj_partial, dim3(kRed), dim3(256), 0, c.stream, f.n, 1, v, v, f.red);
+    hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64), 0, c.stream, 1, f.red, f.hcol, 0);
+    SD_HIP(hipGetLastError());
+    SD_HIP(hipMemcpyAsync(f.H, f.hcol, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    *out = sqrt(f.H[0]);
+    return SDDDST_OK;
+}
+
+int Fgmres::solve(Context &c)
+{
+    const int N = n;
+    const double tol = c.solver_tol;
+    int rc;
+    // delta = 0, r = g
+    SD_HIP(hipMemsetAsync(c.delta, 0, sizeof(double) * N, c.stream));
+    double gnorm;
+    rc = norm2(c, *this, c.g, &gnorm);
+    if (rc) return rc;
+    c.last_iters = 0;
+    if (gnorm == 0.0) {
+        c.last_rel_res = 0.0;
+        return SDDDST_OK;
+    }
+    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), hh(m + 2);
+    int total_it = 0;
+    double rel = 1.0;
+    const double *r0 = c.g;
+    for (int cycle = 0; total_it < c.solver_max_iter; ++cycle) {
+        // r = g - A delta
+        if (cycle > 0) {
+            hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow, c.delta, w);
+            hipLaunchKernelGGL(k_resid, dim3(nb256(N)), dim3(256), 0, c.stream, N, c.g, w);
+            SD_HIP(hipGetLastError());
+            r0 = w;
+        }
+        double beta;
+        rc = norm2(c, *this, r0, &beta);
+        if (rc) return rc;
+        rel = beta / gnorm;
+        if (rel <= tol) break;
+        hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / beta, r0, V);
+        SD_HIP(hipGetLastError());
+        std::fill(s.begin(), s.end(), 0.0);
+        s[0] = beta;
+        int k = 0;
+        for (; k < m && total_it < c.solver_max_iter; ++k, ++total_it) {
+            // z_k = M^-1 v_k ; w = A z_k
+            hipLaunchKernelGGL(k_jacobi_apply, dim3(nb256(N)), dim3(256), 0, c.stream, N, dinv, V + (size_t)k * N,
+                               Z + (size_t)k * N);
+            hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow,
+                               Z + (size_t)k * N, w);
+            SD_HIP(hipGetLastError());
+            rc = arnoldi_orth(c, *this, k + 1, hh.data());
+            if (rc) return rc;
+            const double hnext = hh[k + 1];
+            if (hnext > 0)
+                hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / hnext, w,
+                                   V + (size_t)(k + 1) * N);
+            // apply previous rotations to the new column
+            for (int i = 0; i < k; ++i) {
+                const double t = cs[i] * hh[i] + sn[i] * hh[i + 1];
+                hh[i + 1] = -sn[i] * hh[i] + cs[i] * hh[i + 1];
+                hh[i] = t;
+            }
+            const double denom = hypot(hh[k], hh[k + 1]);
+            cs[k] = denom > 0 ? hh[k] / denom : 1.0;
+            sn[k] = denom > 0 ? hh[k + 1] / denom : 0.0;
+            hh[k] = denom;
+            hh[k + 1] = 0.0;
+            s[k + 1] = -sn[k] * s[k];
+            s[k] = cs[k] * s[k];
+            for (int i = 0; i <= k; ++i) Hm[(size_t)i * m + k] = hh[i];
+            if (fabs(s[k + 1]) / gnorm <= 0.5 * tol || hnext == 0.0) {
+                ++k;
+                ++total_it;
+                break;
+            }
+        }
+        // back substitution H y = s
+        for (int i = k - 1; i >= 0; --i) {
+            double acc = s[i];
+            for (int j = i + 1; j < k; ++j) acc -= Hm[(size_t)i * m + j] * y[j];
+            y[i] = acc / Hm[(size_t)i * m + i];
+        }
+        SD_HIP(hipMemcpyAsync(hcol, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(k_combine, dim3(nb256(N)), dim3(256), 0, c.stream, N, k, Z, hcol, c.delta);
+        SD_HIP(hipGetLastError());
+        SD_HIP(hipStreamSynchronize(c.stream));
+    }
+    // true residual
+    hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow, c.delta, w);
+    hipLaunchKernelGGL(k_resid, dim3(nb256(N)), dim3(256), 0, c.stream, N, c.g, w);
+    SD_HIP(hipGetLastError());
+    double rn;
+    rc = norm2(c, *this, w, &rn);
+    if (rc) return rc;
+    c.last_iters = total_it;
+    c.last_rel_res = rn / gnorm;
+    if (!(c.last_rel_res <= tol)) {
+        set_error("FGMRES: relative residual %.3e after %d iterations", c.last_rel_res, total_it);
+        return SDDDST_ERR_SOLVER;
+    }
+    return SDDDST_OK;
+}
+
+}  // namespace sdamd

@@ -1,0 +1,38 @@
+// Device linear solvers for the Newton correction A delta = g.
+#pragma once
+
+#include <rocblas/rocblas.h>
+
+#include "context.h"
+
+namespace sdamd {
+
+struct DenseLu {
+    double *A = nullptr;
+    int *ipiv = nullptr;
+    int *info = nullptr;
+    rocblas_handle handle = nullptr;
+    int init(Context &c);
+    int factor(Context &c);
+    int solve(Context &c);
+    void release();
+};
+
+// Restarted flexible GMRES (FP64) on the CSR view of A with a
+// right preconditioner; see solver_fgmres.hip.
+struct Fgmres {
+    int n = 0, m = 0;
+    double *V = nullptr;     // (m+1) x n Krylov basis
+    double *Z = nullptr;     // m x n preconditioned directions
+    double *w = nullptr;     // n
+    double *dinv = nullptr;  // Jacobi: inverse diagonal of A
+    double *red = nullptr;   // reduction scratch
+    double *hcol = nullptr;  // device copy of one Hessenberg column
+    double *H = nullptr;     // host (m+1) x m Hessenberg (pinned)
+    int init(Context &c);
+    int factor(Context &c);
+    int solve(Context &c);
+    void release();
+};
+
+}  // namespace sdamd

@@ -1,0 +1,255 @@
+// K1: pair shear-stress speeds (Simulation::calculateSpeeds,
+// simulation.cpp:86-131) and the tolerance update folded into it
+// (PrecisionHandler::updateTolerance, precision_handler.cpp:72-87).
+//
+// Layout: x, y, b are SoA FP64 arrays of N.  The N(N-1)/2 unordered pairs
+// are cut into 64x64 tiles (bi <= bj) of the upper triangle; one wavefront
+// owns one tile.  Lane l holds target i = 64*bi + l in registers and walks
+// the 64 sources of block bj in a rotated order (lane l meets source
+// (l + s) mod 64 at step s), so every lane computes a distinct pair each step
+// and each pair term t = b_i b_j tau(x_i - x_j, y_i - y_j) is evaluated once:
+//   - the row sum for i (sum over j of +t) stays in a register;
+//   - the column sum for j (sum over i of -t) travels with the source: the
+//     accumulator rotates one lane per step (ds_bpermute), and after 64 steps
+//     lane l holds the column sum of source 64*bj + l.
+// Each tile writes its row sums into slot bj and its column sums into slot bi
+// of a [nb][N] partial buffer (the diagonal tile writes row + column into
+// slot bi), so every (slot, target) is written exactly once; the final
+// kernel adds the nb slots of a target in slot order.  The result is
+// deterministic (no atomics on sums) and independent of scheduling.
+//
+// The minimum r^2 per target (for the tolerance) is exact under any order
+// and is combined with 64-bit atomic min on the bit pattern (r^2 >= 0).
+//
+// The ordered kernel (SDDDST_FIELD_ORDERED) runs one lane per target and
+// sums in the reference's order: -t(i', i) for i' < i ascending, then
+// +t(i, j) for j > i ascending.  It evaluates every pair twice and serves
+// validation: with exact pair terms it reproduces the reference's speeds.
+#include <hip/hip_runtime.h>
+
+#include "context.h"
+#include "field.h"
+
+namespace sdamd {
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
+
+__device__ __forceinline__ double rotate_down1(double v, int lane)
+{
+    // lane l receives the value of lane (l + 1) mod 64
+    return __shfl(v, (lane + 1) & 63, 64);
+}
+
+template <int Mode>
+__device__ __forceinline__ double field_tau(double dx, double dy)
+{
+    if constexpr (Mode == SDDDST_FIELD_FAST) return sdfield::xy_fast(dx, dy);
+    else return sdfield::xy(dx, dy);
+}
+
+// tile id -> (bi, bj), 0 <= bi <= bj < nb, enumerated column by column.
+__device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
+{
+    int c = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((long)c * (c + 1) / 2 > t) --c;
+    while ((long)(c + 1) * (c + 2) / 2 <= t) ++c;
+    bj = c;
+    bi = t - c * (c + 1) / 2;
+}
+
+template <int Mode>
+__global__ __launch_bounds__(64) void k_pair_tiles(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                   const double *__restrict__ b, double *__restrict__ partial,
+                                                   unsigned long long *__restrict__ minr2)
+{
+    __shared__ double sx[kTile], sy[kTile], sb[kTile];
+    int bi, bj;
+    tile_coords((int)blockIdx.x, bi, bj);
+    const int l = (int)threadIdx.x;
+    const int i = bi * kTile + l;
+    const int jl = bj * kTile + l;
+    sx[l] = jl < n ? x[jl] : 0.0;
+    sy[l] = jl < n ? y[jl] : 0.0;
+    sb[l] = jl < n ? b[jl] : 0.0;
+    const double xi = i < n ? x[i] : 0.0;
+    const double yi = i < n ? y[i] : 0.0;
+    const double bI = i < n ? b[i] : 0.0;
+    __syncthreads();
+    const bool diag = (bi == bj);
+    double row = 0.0, col = 0.0;
+    double mrow = __builtin_inf(), mcol = __builtin_inf();
+    for (int s = 0; s < kTile; ++s) {
+        const int m = (l + s) & 63;
+        const int j = bj * kTile + m;
+        const bool valid = (i < n) && (j < n) && (!diag || l < m);
+        const double dx = sdfield::normalize(xi - sx[m]);
+        const double dy = sdfield::normalize(yi - sy[m]);
+        const double t = bI * sb[m] * field_tau<Mode>(dx, dy);
+        const double r2 = dx * dx + dy * dy;
+        if (valid) {
+            row += t;
+            col -= t;
+            mrow = fmin(mrow, r2);
+            mcol = fmin(mcol, r2);
+        }
+        col = rotate_down1(col, l);
+        mcol = rotate_down1(mcol, l);
+    }
+    if (diag) {
+        if (i < n) partial[(size_t)bi * n + i] = row + col;
+    } else {
+        if (i < n) partial[(size_t)bj * n + i] = row;
+        if (jl < n) partial[(size_t)bi * n + jl] = col;
+    }
+    if (i < n && mrow < __builtin_inf()) atomicMin(&minr2[i], (unsigned long long)__double_as_longlong(mrow));
+    if (!diag && jl < n && mcol < __builtin_inf())
+        atomicMin(&minr2[jl], (unsigned long long)__double_as_longlong(mcol));
+}
+
+// simulation.cpp:111-128: point-defect force on dislocation i and its
+// tolerance candidate (X2 + X2, simulation.cpp:119-126).
+__device__ __forceinline__ double pd_pair(double xi, double yi, double bI, double pxj, double pyj, double A, double K,
+                                          double &rSqr_out)
+{
+    const double dx = sdfield::normalize(xi - pxj);
+    const double dy = sdfield::normalize(yi - pyj);
+    const double pi = sdfield::kPi;
+    const double xSqr = (1.0 - sdlibm::cos(2.0 * pi * dx)) * 0.5 / pi / pi;
+    const double ySqr = (1.0 - sdlibm::cos(2.0 * pi * dy)) * 0.5 / pi / pi;
+    const double rSqr = xSqr + ySqr;
+    const double expXY = sdlibm::exp(-K * rSqr);
+    const double Xdx = sdlibm::sin(2.0 * pi * dx) * 0.5 / pi;
+    const double Xdy = sdlibm::sin(2.0 * pi * dy) * 0.5 / pi;
+    rSqr_out = rSqr;
+    return 2.0 * A * Xdx * Xdy * ((1. - expXY) / rSqr - K * expXY) / rSqr * bI;
+}
+
+// precision_handler.cpp:72-87 applied to the minimum candidate.
+__device__ __forceinline__ void tol_update(double &tol, double distanceSqr, double minPrecSqr)
+{
+    const double tmp = distanceSqr * 0.25 * 1e-2;
+    if (tmp < minPrecSqr && tmp < tol) tol = (tmp == 0.0) ? 1e-12 : tmp;
+}
+
+__global__ void k_speeds_finish(int n, int nb, int pc, const double *__restrict__ partial,
+                                const double *__restrict__ x, const double *__restrict__ y,
+                                const double *__restrict__ b, const double *__restrict__ px,
+                                const double *__restrict__ py, double A, double K, double sigma,
+                                const unsigned long long *__restrict__ minr2, double *__restrict__ tol,
+                                double minPrecSqr, int update_tol, double *__restrict__ v)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int s = 0; s < nb; ++s) acc += partial[(size_t)s * n + i];
+    double t = update_tol ? tol[i] : 0.0;
+    if (update_tol) {
+        const unsigned long long bits = minr2[i];
+        if (bits != kInfBits) tol_update(t, __longlong_as_double((long long)bits), minPrecSqr);
+    }
+    if (pc > 0) {
+        const double xi = x[i], yi = y[i], bI = b[i];
+        for (int j = 0; j < pc; ++j) {
+            double r2;
+            acc -= pd_pair(xi, yi, bI, px[j], py[j], A, K, r2);
+            if (update_tol) tol_update(t, r2, minPrecSqr);
+        }
+    }
+    acc += b[i] * sigma;
+    v[i] = acc;
+    if (update_tol) tol[i] = t;
+}
+
+// One lane per target in the reference's summation order (validation).
+__global__ void k_speeds_ordered(int n, int pc, const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ b, const double *__restrict__ px,
+                                 const double *__restrict__ py, double A, double K, double sigma,
+                                 double *__restrict__ tol, double minPrecSqr, int update_tol,
+                                 double *__restrict__ v)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const double xi = x[i], yi = y[i], bI = b[i];
+    double acc = 0.0;
+    double t = update_tol ? tol[i] : 0.0;
+    for (int j = 0; j < i; ++j) {
+        const double dx = sdfield::normalize(x[j] - xi);
+        const double dy = sdfield::normalize(y[j] - yi);
+        const double tmp = b[j] * bI * sdfield::xy(dx, dy);
+        if (update_tol) tol_update(t, dx * dx + dy * dy, minPrecSqr);
+        acc -= tmp;
+    }
+    for (int j = i + 1; j < n; ++j) {
+        const double dx = sdfield::normalize(xi - x[j]);
+        const double dy = sdfield::normalize(yi - y[j]);
+        const double tmp = bI * b[j] * sdfield::xy(dx, dy);
+        if (update_tol) tol_update(t, dx * dx + dy * dy, minPrecSqr);
+        acc += tmp;
+    }
+    for (int j = 0; j < pc; ++j) {
+        double r2;
+        acc -= pd_pair(xi, yi, bI, px[j], py[j], A, K, r2);
+        if (update_tol) tol_update(t, r2, minPrecSqr);
+    }
+    acc += bI * sigma;
+    v[i] = acc;
+    if (update_tol) tol[i] = t;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long val, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = val;
+}
+
+}  // namespace
+
+int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol)
+{
+    const int n = c.n;
+    const double minPrecSqr = c.p.precision * c.p.precision;
+    const int threads = 256;
+    const int blocks = (n + threads - 1) / threads;
+    if (c.timing) SD_HIP(hipEventRecord(c.ev0, c.stream));
+    if (c.p.field_mode == SDDDST_FIELD_ORDERED) {
+        hipLaunchKernelGGL(k_speeds_ordered, dim3(blocks), dim3(threads), 0, c.stream, n, c.pc, c.x[slot], c.y, c.b,
+                           c.px, c.py, c.p.A, c.p.KASQR, sigma, c.tol, minPrecSqr, update_tol, c.v[out_buf]);
+        SD_HIP(hipGetLastError());
+        if (c.timing) {
+            SD_HIP(hipEventRecord(c.ev1, c.stream));
+            SD_HIP(hipEventSynchronize(c.ev1));
+            SD_HIP(hipEventElapsedTime(&c.last_pair_kernel_ms, c.ev0, c.ev1));
+            c.last_speeds_ms = c.last_pair_kernel_ms;
+        }
+        return SDDDST_OK;
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(threads), 0, c.stream, c.minr2, kInfBits, n);
+    SD_HIP(hipGetLastError());
+    const int nb = c.nb;
+    const long tiles = (long)nb * (nb + 1) / 2;
+    SD_CHECK(tiles < (1L << 31), SDDDST_ERR_ARG, "too many tiles for N=%d", n);
+    hipEvent_t e_mid0 = c.ev0;
+    if (c.timing) SD_HIP(hipEventRecord(e_mid0, c.stream));
+    if (c.p.field_mode == SDDDST_FIELD_EXACT)
+        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
+                           c.x[slot], c.y, c.b, c.partial, c.minr2);
+    else
+        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
+                           c.x[slot], c.y, c.b, c.partial, c.minr2);
+    SD_HIP(hipGetLastError());
+    if (c.timing) SD_HIP(hipEventRecord(c.ev1, c.stream));
+    hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.partial, c.x[slot],
+                       c.y, c.b, c.px, c.py, c.p.A, c.p.KASQR, sigma, c.minr2, c.tol, minPrecSqr, update_tol,
+                       c.v[out_buf]);
+    SD_HIP(hipGetLastError());
+    if (c.timing) {
+        SD_HIP(hipEventSynchronize(c.ev1));
+        SD_HIP(hipEventElapsedTime(&c.last_pair_kernel_ms, e_mid0, c.ev1));
+    }
+    return SDDDST_OK;
+}
+
+}  // namespace sdamd

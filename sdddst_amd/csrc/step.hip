@@ -1,0 +1,182 @@
+// K3/K4: the O(N) kernels of one implicit step, all resident on the device.
+//   residual       Simulation::calculateG (simulation.cpp:189-192)
+//   newton update  Simulation::integrate (simulation.cpp:78-81)
+//   error control  Simulation::calculateXError + PrecisionHandler::updateError
+//                  / getNewStepSize / reset + max-stepsize clamp
+//                  (simulation.cpp:398-405, 516, 643-649;
+//                  precision_handler.cpp:62-119)
+//   sums           the log reductions of simulation.cpp:460-461, 508-509,
+//                  551, 557 and utility.h:92-100, in the reference's
+//                  sequential order so the logged values match bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "context.h"
+#include "field.h"
+
+namespace sdamd {
+
+namespace {
+
+__global__ void k_residual(int n, double h, const double *__restrict__ xn, const double *__restrict__ xo,
+                           const double *__restrict__ csp, const double *__restrict__ isp,
+                           const double *__restrict__ d, double *__restrict__ g)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    g[i] = xn[i] - (1.0 + d[i]) * 0.5 * h * csp[i] - xo[i] - (1.0 - d[i]) * 0.5 * h * isp[i];
+}
+
+__global__ void k_newton_update(int n, double *__restrict__ x, const double *__restrict__ delta)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    x[i] -= delta[i];
+}
+
+// precision_handler.cpp:103-119 + simulation.cpp:646-649
+__device__ double new_step_size(double maxRatio, double h, double hmax)
+{
+    double hn;
+    if (0.0 == maxRatio) {
+        hn = h * 2.0;
+    } else {
+        double tmp = 1. / sdlibm::sqrt(maxRatio);
+        tmp = sdlibm::pow_one_third(tmp);
+        if (tmp > 2.0) tmp = 2.0;
+        hn = 0.9 * h * tmp;
+    }
+    if (hmax > 0 && hmax < hn) hn = hmax;
+    return hn;
+}
+
+// One workgroup: max_i err_i^2/tol_i over err_i = |x_big - x_small2| > 0,
+// first index of the maximum (strict '>' in updateError keeps the first),
+// then the tolerance reset to prec^2 (PrecisionHandler::reset).
+__global__ __launch_bounds__(1024) void k_error_control(int n, const double *__restrict__ xbig,
+                                                        const double *__restrict__ xsmall2, double *__restrict__ tol,
+                                                        double minPrecSqr, double h, double hmax,
+                                                        DevScalars *__restrict__ out)
+{
+    __shared__ double smax[1024];
+    __shared__ int sidx[1024];
+    const int t = (int)threadIdx.x;
+    double best = 0.0;
+    int bidx = 0x7fffffff;
+    for (int i = t; i < n; i += 1024) {
+        const double e = __builtin_fabs(xbig[i] - xsmall2[i]);
+        if (0.0 < e) {
+            const double r = e * e / tol[i];
+            if (r > best) {   // NaN never wins, as in updateError
+                best = r;
+                bidx = i;
+            }
+        }
+        tol[i] = minPrecSqr;
+    }
+    smax[t] = best;
+    sidx[t] = bidx;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off) {
+            const double o = smax[t + off];
+            const int oi = sidx[t + off];
+            if (o > smax[t] || (o == smax[t] && oi < sidx[t])) {
+                smax[t] = o;
+                sidx[t] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double m = smax[0];
+        out->max_ratio = m;
+        out->selected = (m > 0.0) ? sidx[0] : 0;
+        out->accept = m < 1.0 ? 1 : 0;
+        out->h_next = new_step_size(m, h, hmax);
+    }
+}
+
+// Sequential sums in index order (std::accumulate semantics).
+__global__ void k_speed_sums(int n, const double *__restrict__ v, const double *__restrict__ b,
+                             DevScalars *__restrict__ out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double sa = 0.0, sq = 0.0, sbv = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double vi = v[i];
+        sa = sa + __builtin_fabs(vi);
+        sq = sq + vi * vi;
+        sbv += b[i] * vi;
+    }
+    out->sum_abs = sa;
+    out->sum_sq = sq;
+    out->sum_bv = sbv;
+}
+
+// utility.h:92-100
+__global__ void k_strain(int n, const double *__restrict__ xo, const double *__restrict__ xn,
+                         const double *__restrict__ b, DevScalars *__restrict__ out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double r = 0;
+    for (int i = 0; i < n; ++i) r += b[i] * (xn[i] - xo[i]);
+    out->strain = r;
+}
+
+__global__ void k_fill(double *p, double v, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+static inline int nblocks(int n, int t) { return (n + t - 1) / t; }
+
+int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp)
+{
+    hipLaunchKernelGGL(k_residual, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, h, c.x[new_slot],
+                       c.x[old_slot], c.v[csp], c.v[isp], c.dvec, c.g);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int launch_newton_update(Context &c, int slot)
+{
+    hipLaunchKernelGGL(k_newton_update, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.x[slot], c.delta);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int launch_error_control(Context &c, double h, double hmax)
+{
+    const double minPrecSqr = c.p.precision * c.p.precision;
+    hipLaunchKernelGGL(k_error_control, dim3(1), dim3(1024), 0, c.stream, c.n, c.x[SDDDST_SLOT_BIG_STEP],
+                       c.x[SDDDST_SLOT_SECOND_SMALL], c.tol, minPrecSqr, h, hmax, c.ds);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int launch_speed_sums(Context &c, int buf)
+{
+    hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(64), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int launch_strain(Context &c, int old_slot, int new_slot)
+{
+    hipLaunchKernelGGL(k_strain, dim3(1), dim3(64), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int launch_reset_tolerance(Context &c)
+{
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.tol,
+                       c.p.precision * c.p.precision, c.n);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+}  // namespace sdamd

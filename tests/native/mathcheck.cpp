@@ -1,0 +1,47 @@
+// Host build of the device math headers (libm_compat.h, field.h) for unit
+// tests that compare the library's elementary functions and per-pair field
+// values with the host C library and the oracle on millions of arguments.
+// Test infrastructure: nothing in the shipped library uses this file.
+#include "../../sdddst_amd/csrc/field.h"
+
+extern "C" {
+double mc_cos(double x) { return sdlibm::cos(x); }
+double mc_sin(double x) { return sdlibm::sin(x); }
+double mc_exp(double x) { return sdlibm::exp(x); }
+double mc_expm1(double x) { return sdlibm::expm1(x); }
+double mc_cosh(double x) { return sdlibm::cosh(x); }
+double mc_sinh(double x) { return sdlibm::sinh(x); }
+double mc_powi(double x, int n) { return sdlibm::powi(x, n); }
+double mc_normalize(double d) { return sdfield::normalize(d); }
+double mc_xy(double dx, double dy) { return sdfield::xy(dx, dy); }
+double mc_xy_fast(double dx, double dy) { return sdfield::xy_fast(dx, dy); }
+double mc_xy_diff_x(double dx, double dy) { return sdfield::xy_diff_x(dx, dy); }
+
+// Batched forms: fn = 0 cos, 1 sin, 2 exp, 3 expm1, 4 cosh, 5 sinh, 6 normalize
+void mc_batch1(int fn, long n, const double *x, double *out)
+{
+    for (long i = 0; i < n; ++i) {
+        double v = x[i], r = 0;
+        switch (fn) {
+        case 0: r = sdlibm::cos(v); break;
+        case 1: r = sdlibm::sin(v); break;
+        case 2: r = sdlibm::exp(v); break;
+        case 3: r = sdlibm::expm1(v); break;
+        case 4: r = sdlibm::cosh(v); break;
+        case 5: r = sdlibm::sinh(v); break;
+        case 6: r = sdfield::normalize(v); break;
+        }
+        out[i] = r;
+    }
+}
+
+// fn = 0 xy, 1 xy_fast, 2 xy_diff_x
+void mc_batch2(int fn, long n, const double *dx, const double *dy, double *out)
+{
+    for (long i = 0; i < n; ++i) {
+        if (fn == 0) out[i] = sdfield::xy(dx[i], dy[i]);
+        else if (fn == 1) out[i] = sdfield::xy_fast(dx[i], dy[i]);
+        else out[i] = sdfield::xy_diff_x(dx[i], dy[i]);
+    }
+}
+}

@@ -1,0 +1,162 @@
+"""GPU parity of the per-pair field, the device elementary functions, the
+pair-speed kernel (K1) and the Jacobian assembly (K2) against the oracle."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import sdddst_amd as S
+from sdddst_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(n, **kw):
+    return S.Context(n, **kw)
+
+
+@pytest.fixture(scope="module")
+def ctx256():
+    c = _ctx(256)
+    yield c
+    c.close()
+
+
+# -- device elementary functions -------------------------------------------
+
+@pytest.mark.parametrize("fn,lo,hi", [(0, -3.1416, 3.1416), (0, -0.02, 0.02), (1, -3.1416, 3.1416), (2, -40, 40),
+                                      (3, -35, 35), (3, -0.35, 0.35), (4, -35, 35), (4, -0.35, 0.35),
+                                      (5, -35, 35), (6, 0, 4), (8, -3, 3)])
+def test_device_libm_matches_host_bits(ctx256, fn, lo, hi):
+    rng = np.random.default_rng(100 + fn)
+    x = rng.uniform(lo, hi, 400_000)
+    dev = ctx256.libm_eval(fn, x)
+    host = O.libm_batch(fn, x)
+    mism = np.count_nonzero(dev.view(np.int64) != host.view(np.int64))
+    # sin(2 pi d) of the point-defect path: one measured non-reproduced case
+    # in 3e5 (TAYLOR_SIN region); everything else is bit-exact.
+    allowed = 2 if fn == 1 else 0
+    assert mism <= allowed, f"fn {fn}: {mism} bit mismatches"
+
+
+def test_device_pow_one_third(ctx256):
+    rng = np.random.default_rng(7)
+    t = np.concatenate([rng.uniform(0.05, 100, 200_000), np.exp(rng.uniform(-20, 20, 100_000))])
+    dev = ctx256.libm_eval(7, t)
+    host = O.libm_batch(7, t)
+    ulps = np.abs(dev.view(np.int64) - host.view(np.int64))
+    assert ulps.max() <= 1
+    assert np.count_nonzero(ulps) <= 0.005 * len(t)
+
+
+# -- per-pair field --------------------------------------------------------
+
+def _pairs(n, seed):
+    rng = np.random.default_rng(seed)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, n), rng.uniform(-2e-3, 2e-3, n // 4),
+                         np.array([0.0, -0.5, 0.49999999999999994, 1e-9, -1e-9])])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, n), rng.uniform(-2e-3, 2e-3, n // 4),
+                         np.array([0.3, 0.1, -0.2, 2e-4, -3e-4])])
+    return dx, dy
+
+
+def test_field_xy_exact_is_bit_identical(ctx256):
+    dx, dy = _pairs(200_000, 1)
+    dev = ctx256.field_eval(0, dx, dy)
+    ref, _ = O.xy_batch(dx, dy)
+    assert np.array_equal(dev.view(np.int64), ref.view(np.int64))
+
+
+def test_field_xy_diff_x(ctx256):
+    dx, dy = _pairs(100_000, 2)
+    dev = ctx256.field_eval(1, dx, dy)
+    _, ref = O.xy_batch(dx, dy)
+    same = dev.view(np.int64) == ref.view(np.int64)
+    assert same.mean() > 0.998
+    rel = np.abs(dev - ref) / np.maximum(np.abs(ref), 1.0)
+    assert rel.max() < 1e-13
+
+
+def test_field_xy_fast_abs_error(ctx256):
+    dx, dy = _pairs(200_000, 3)
+    dev = ctx256.field_eval(2, dx, dy)
+    ref, _ = O.xy_batch(dx, dy)
+    err = np.abs(dev - ref) / (np.abs(ref) + 1.0)
+    assert err.max() < 4e-15
+
+
+# -- K1 pair speeds --------------------------------------------------------
+
+def _speeds_dev(n, seed, mode, sigma=0.0, cutoff=1.0):
+    x, y, b = configs.dislocations(n, seed)
+    with _ctx(n, field_mode=mode, cutoff_multiplier=cutoff) as c:
+        c.upload_state(0, x, y, b)
+        v = c.speeds(0, S.V_INIT, sigma, update_tol=True)
+        tol = c.tolerance()
+    return x, y, b, v, tol
+
+
+def test_speeds_ordered_bit_identical_to_oracle():
+    x, y, b, v, tol = _speeds_dev(256, 1, S.FIELD_ORDERED)
+    vr, tr = O.speeds(x, y, b)
+    assert np.array_equal(v.view(np.int64), vr.view(np.int64))
+    assert np.array_equal(tol.view(np.int64), tr.view(np.int64))
+
+
+def test_speeds_ordered_bit_identical_reference_loop():
+    ref = O.reference()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) not present")
+    x, y, b, v, _ = _speeds_dev(512, 11, S.FIELD_ORDERED, sigma=0.02)
+    vr = np.empty(512)
+    ref.ref_speeds(512, O.dp(x), O.dp(y), O.dp(b), 0.02, O.dp(vr))
+    assert np.array_equal(v.view(np.int64), vr.view(np.int64))
+
+
+def _speed_parity(v, vr, x, y, b):
+    # per-component 1e-12 relative; a conditioning-aware bound for the
+    # components whose sum cancels: |dv| <= 1e-12 * sum_j |t_ij| scale
+    rel = np.abs(v - vr) / np.abs(vr)
+    return rel
+
+
+@pytest.mark.parametrize("mode", [S.FIELD_EXACT, S.FIELD_FAST])
+@pytest.mark.parametrize("n,seed", [(256, 1), (1000, 5), (2048, 9)])
+def test_speeds_tiled_vs_oracle(mode, n, seed):
+    x, y, b, v, tol = _speeds_dev(n, seed, mode, sigma=0.02)
+    vr, tr = O.speeds(x, y, b, sigma=0.02)
+    assert np.array_equal(tol.view(np.int64), tr.view(np.int64))
+    rel = np.abs(v - vr) / np.abs(vr)
+    assert np.median(rel) < 1e-14
+    assert np.mean(rel < 1e-12) > 0.999
+    assert np.linalg.norm(v - vr) / np.linalg.norm(vr) < 1e-13
+
+
+def test_speeds_sum_rule_and_determinism_n16384():
+    n = 16384
+    x, y, b = configs.dislocations(n, 2)
+    with _ctx(n, field_mode=S.FIELD_FAST, cutoff_multiplier=2.0) as c:
+        c.upload_state(0, x, y, b)
+        v1 = c.speeds(0, S.V_INIT, 0.02)
+        v2 = c.speeds(0, S.V_INIT2, 0.02)
+    assert np.array_equal(v1.view(np.int64), v2.view(np.int64))
+    # pair terms cancel exactly in exact arithmetic: sum v = sigma * sum b = 0
+    assert abs(v1.sum() - 0.02 * b.sum()) < 1e-9 * np.abs(v1).sum()
+
+
+# -- K2 Jacobian -----------------------------------------------------------
+
+@pytest.mark.parametrize("n,m,seed", [(256, 1.0, 1), (300, 1.0, 4), (1024, 2.0, 7), (64, 1.0, 3)])
+def test_jacobian_pattern_bit_exact(n, m, seed):
+    x, y, b = configs.dislocations(n, seed)
+    h = 1e-4
+    with _ctx(n, cutoff_multiplier=m) as c:
+        c.upload_state(0, x, y, b)
+        Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)
+    rAp, rAi, rAx, rdv, ridx, raw = O.jacobian(x, y, b, m, h)
+    assert np.array_equal(Ap, rAp)
+    assert np.array_equal(Ai, rAi)
+    assert np.array_equal(dg, ridx)
+    rel = np.abs(Ax - rAx) / np.maximum(np.abs(rAx), 1e-300)
+    assert np.mean(Ax.view(np.int64) == rAx.view(np.int64)) > 0.995
+    assert rel.max() < 1e-12
+    assert np.max(np.abs(dv - rdv)) < 1e-13

@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: successful implicit steps/s of the SDDDST implicit-step hot path
+on one MI355X, with the pair-speed kernel's FP64 roofline fraction and a
+bounded CPU baseline.
+
+Workload (default): N = 16384 edge dislocations, positions uniform in the
+unit periodic cell from seed 2, balanced +/-1 Burgers vectors, constant
+external stress 0.02, cutoff multiplier 2, precision 1e-6, initial step
+1e-6.  A "step" is one successful implicit step of sdddstCore::Simulation
+(three stages, Jacobian assemblies, Newton solves, the error controller and
+the log); failed attempts inside the timed region count against the rate.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver fgmres]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "successful implicit steps/s at N=16384, 1e-6 precision; GPU pairs/s vs FP64 peak"
+FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector, AMD public specification
+HBM_PEAK_GBS = 8000.0
+FLOPS_PER_PAIR = 745          # cost-model FP64 operations of one tau_xy pair term
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--solver", default="fgmres", choices=["fgmres", "dense-lu", "auto", "umfpack"])
+    p.add_argument("--field-mode", default="fast", choices=["fast", "exact"])
+    p.add_argument("--plain-steps", type=int, default=2,
+                   help="also time this many steps of the direct-solver path (0: skip)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    return p.parse_args()
+
+
+def make_sim(args, solver, x, y, b):
+    from sdddst_amd.simulation import Simulation
+    return Simulation(x, y, b, cutoff_multiplier=2.0, precision=1e-6, initial_step=1e-6, stress="constant",
+                      stress_value=0.02, calculate_strain=True, solver=solver, field_mode=args.field_mode,
+                      solver_tol=1e-10, solver_max_iter=400, solver_restart=40)
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_steps(sim, k, label):
+    """Wall time of k successful steps (failed attempts included)."""
+    s0 = sim.state()
+    ok = 0
+    t0 = time.perf_counter()
+    last = t0
+    while ok < k:
+        ok += 1 if sim.step() else 0
+        now = time.perf_counter()
+        if now - last > 10:
+            log(f"{label}: {ok}/{k} successful steps, {now - t0:.1f} s")
+            last = now
+    t1 = time.perf_counter()
+    s1 = sim.state()
+    return t1 - t0, s1.succesful_steps - s0.succesful_steps, s1.failed_steps - s0.failed_steps
+
+
+def cpu_baseline(x, y, b, seconds):
+    """Oracle (the CPU restatement of the reference, 1 thread) on a bounded
+    sample: the per-pair speed work of a row sample at full N, scaled to the
+    5 speed evaluations of one successful step with constant stress.  The
+    Jacobian assemblies and the sparse factorisations the reference also
+    performs per step are not timed (no UMFPACK on the host), so this is an
+    upper bound on the reference's steps/s."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib as O
+    n = len(x)
+    lib = O.oracle()
+    import ctypes
+    pairs_total = n * (n - 1) / 2
+    rows = 16
+    done_pairs = 0
+    elapsed = 0.0
+    i0 = 0
+    while elapsed < seconds and i0 < n:
+        i1 = min(n, i0 + rows)
+        cnt = ctypes.c_long()
+        t0 = time.perf_counter()
+        lib.orc_speeds_rows_sample(n, O.dp(x), O.dp(y), O.dp(b), i0, i1, ctypes.byref(cnt))
+        elapsed += time.perf_counter() - t0
+        done_pairs += cnt.value
+        i0 = i1 + n // 64      # spread the sample over the triangle
+    pairs_per_s = done_pairs / elapsed
+    step_s = 5 * pairs_total / pairs_per_s
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle pair terms of {done_pairs} pairs (row sample of N={n}) in {elapsed:.1f} s "
+                      f"= {pairs_per_s:.3e} pairs/s; x5 evaluations per step; assembly and solve not timed",
+            "pairs_per_s": pairs_per_s, "cpu": platform.processor() or platform.machine()}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(one process drives one MI355X; see the sharding design)", file=sys.stderr)
+        sys.exit(2)
+    from sdddst_amd import configs
+    from sdddst_amd._capi import kernel_stats, timing
+    n = args.n
+    x, y, b = configs.dislocations(n, 2)
+
+    log(f"workload n={n} solver={args.solver} field={args.field_mode}")
+    sim = make_sim(args, args.solver, x, y, b)
+    if args.warmup > 0:
+        timed_steps(sim, args.warmup, "warmup")
+    ctx = sim.device_context
+    timing(ctx, True)
+    wall, ok, failed = timed_steps(sim, args.steps, "timed")
+    log(f"timed: {ok} steps in {wall:.2f} s")
+    pair_cnt, pair_ms = kernel_stats(ctx, "pair")
+    sp_cnt, sp_ms = kernel_stats(ctx, "speeds")
+    jac_cnt, jac_ms = kernel_stats(ctx, "jacobian")
+    timing(ctx, False)
+    st = sim.state()
+    solver_name = sim.solver_name
+    sim.close()
+
+    steps_per_s = ok / wall
+    pairs = n * (n - 1) / 2
+    avg_pair_ms = pair_ms / max(pair_cnt, 1)
+    achieved_tflops = FLOPS_PER_PAIR * pairs / (avg_pair_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": ok,
+        "warmup": args.warmup,
+        "ms_per_step": wall / ok * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 2 positions, balanced +/-1 Burgers vectors)",
+        "config": {"workload": "n16384-m2-const0.02", "n": n, "cutoff_multiplier": 2.0, "precision": 1e-6,
+                   "stress": 0.02, "initial_step": 1e-6, "solver": solver_name, "field_mode": args.field_mode,
+                   "failed_steps_in_timed_region": failed},
+        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": achieved_tflops,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+                     "traffic": None, "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
+                     "avg_launch_ms": avg_pair_ms, "launches": pair_cnt},
+        "pairs_per_s": pairs / (avg_pair_ms * 1e-3),
+        "speeds_eval_ms": sp_ms / max(sp_cnt, 1),
+        "jacobian_ms": jac_ms / max(jac_cnt, 1),
+        "final_step_size": st.step_size,
+    }
+    if args.plain_steps > 0 and args.solver != "dense-lu":
+        log("direct-solver path")
+        sim2 = make_sim(args, "dense-lu", x, y, b)
+        timed_steps(sim2, 1, "plain warmup")
+        w2, ok2, f2 = timed_steps(sim2, args.plain_steps, "plain timed")
+        line["plain_direct_solver"] = {"solver": sim2.solver_name, "steps_per_s": ok2 / w2, "steps": ok2,
+                                       "failed": f2,
+                                       "speedup_of_headline": steps_per_s / (ok2 / w2)}
+        sim2.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(x, y, b, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+/*
+ * C entry points of the C++ simulator API (sdddstCore::Simulation and
+ * SimulationData, include/sdddst/simulation.h) for bindings in other
+ * languages: the role the reference's Boost.Python module PySdddstCore plays
+ * (bindings.cpp:140-206: SimulationData properties, Simulation.create / run
+ * / step / get_time).  Implemented in libsdddstCore.so on top of the C ABI
+ * of include/sdddst_amd.h.  Status codes and sdddst_last_error() as there.
+ */
+#ifndef SDDDST_CORE_CAPI_H
+#define SDDDST_CORE_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDDDST_PROTOCOL_ZERO 0
+#define SDDDST_PROTOCOL_FIXED_RATE 1
+#define SDDDST_PROTOCOL_SPRING 2
+#define SDDDST_PROTOCOL_CONSTANT 3
+
+#define SDDDST_LINSOLVE_AUTO 0
+#define SDDDST_LINSOLVE_UMFPACK 1
+#define SDDDST_LINSOLVE_DENSE_LU 2
+#define SDDDST_LINSOLVE_FGMRES 3
+
+typedef struct sdddst_sim sdddst_sim;
+
+/* SimulationData settings (simulation_data.h:84-254, project_parser.cpp:137-299). */
+typedef struct sdddst_sim_options {
+    double cutoff_multiplier;  /* cutOffMultiplier                         */
+    double precision;          /* prec                                     */
+    double initial_step;       /* stepSize at start                        */
+    double max_step;           /* maxStepSizeLimit; < 0: no limit          */
+    int protocol;              /* SDDDST_PROTOCOL_*                        */
+    double stress_param1;      /* rate (fixed-rate, spring) or the stress  */
+    double stress_param2;      /* spring constant                          */
+    int calculate_strain;      /* calculateStrainDuringSimulation          */
+    int order_parameter;       /* orderParameterCalculationIsOn            */
+    int linear_solver;         /* SDDDST_LINSOLVE_*                        */
+    double solver_tol;         /* FGMRES relative residual                 */
+    int solver_max_iter;
+    int solver_restart;
+    int field_mode;            /* SDDDST_FIELD_* of sdddst_amd.h          */
+    double A, KASQR;           /* point-defect constants (as used)         */
+    const char *log_path;      /* 12-column log file; NULL: none           */
+} sdddst_sim_options;
+
+/* Scalars of the state after the last step. */
+typedef struct sdddst_sim_state {
+    double sim_time, step_size, strain, energy, max_error_ratio_sqr, sum_avg_speed, cutoff;
+    long long succesful_steps, failed_steps;
+    int last_step_succeeded;
+    double last_step_wall_s;
+    int solver_last_iters, solver_fallbacks;
+    double solver_last_rel_residual;
+} sdddst_sim_state;
+
+void sdddst_sim_default_options(sdddst_sim_options *o);
+int sdddst_sim_create(int n, const double *x, const double *y, const double *b, int pc, const double *px,
+                      const double *py, const sdddst_sim_options *o, sdddst_sim **out);
+int sdddst_sim_create_from_files(const char *dconf, const char *pdconf, const sdddst_sim_options *o,
+                                 sdddst_sim **out);
+int sdddst_sim_destroy(sdddst_sim *s);
+/* Simulation::step (simulation.cpp:435-441). */
+int sdddst_sim_step(sdddst_sim *s, int *success);
+/* Simulation::run with a step-count limit of `successful` more steps. */
+int sdddst_sim_run(sdddst_sim *s, int successful);
+int sdddst_sim_set_step_size(sdddst_sim *s, double h);
+int sdddst_sim_get_state(sdddst_sim *s, sdddst_sim_state *out);
+int sdddst_sim_positions(sdddst_sim *s, double *x, int n);
+int sdddst_sim_flush_log(sdddst_sim *s);
+int sdddst_sim_write_dconf(sdddst_sim *s, const char *path);
+const char *sdddst_sim_solver_name(sdddst_sim *s);
+const char *sdddst_core_last_error(void);
+/* The device context the simulation runs on (for sdddst_timing etc.). */
+struct sdddst_ctx *sdddst_sim_device_context(sdddst_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

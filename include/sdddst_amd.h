@@ -17,10 +17,10 @@
  *   - One HIP stream per context; a context is not thread-safe.  Calls that
  *     return host data synchronise the context's stream.
  *   - Position slots follow SimulationData (simulation_data.h:69-127):
- *     0 dislocations, 1 bigStep, 2 firstSmall, 3 secondSmall.  Only x moves
- *     (simulation.cpp:80); y and b are shared by all slots.
+ *     0 dislocations, 1 bigStep, 2 firstSmall, 3 secondSmall, plus two
+ *     scratch slots.  Only x moves (simulation.cpp:80); y and b are shared.
  *   - Speed buffers: 0 initSpeed, 1 initSpeed2, 2 speed, 3 speed2
- *     (simulation_data.h:75-81).
+ *     (simulation_data.h:75-81), 4 scratch.
  */
 #ifndef SDDDST_AMD_H
 #define SDDDST_AMD_H
@@ -43,11 +43,16 @@ extern "C" {
 #define SDDDST_SLOT_BIG_STEP 1
 #define SDDDST_SLOT_FIRST_SMALL 2
 #define SDDDST_SLOT_SECOND_SMALL 3
+#define SDDDST_SLOT_SCRATCH_A 4    /* for host-vector API calls */
+#define SDDDST_SLOT_SCRATCH_B 5
+#define SDDDST_NUM_SLOTS 6
 
 #define SDDDST_V_INIT 0
 #define SDDDST_V_INIT2 1
 #define SDDDST_V_SPEED 2
 #define SDDDST_V_SPEED2 3
+#define SDDDST_V_SCRATCH 4
+#define SDDDST_NUM_VBUFS 5
 
 /* Field evaluation modes of the pair-speed kernel. */
 #define SDDDST_FIELD_FAST 0     /* near image as the reference, far images from one shared exp */
@@ -110,11 +115,11 @@ int sdddst_copy_speed(sdddst_ctx *ctx, int dst, int src);
 
 /* Simulation::calculateJacobian (simulation.cpp:236-384): the CSC of
  * A = I + diag((1+d)/2) * M for slot cfg_slot and step h, built on the
- * device with the reference's exact sparsity pattern.  Two-phase: with
- * Ap == NULL the matrix is built on the device and only *nnz is returned;
- * a second call with non-NULL pointers copies Ap[n+1], Ai[nnz], Ax[nnz],
- * dVec[n] and the diagonal positions diag[n] (SimulationData::indexes) to
- * the host without rebuilding (pass rebuild = 0), or rebuilds (rebuild = 1). */
+ * device with the reference's exact sparsity pattern.  rebuild = 1 builds
+ * it for (cfg_slot, h); rebuild = 0 reuses the last one.  Two-phase: with
+ * Ap == NULL only *nnz is returned; a call with non-NULL pointers copies
+ * Ap[n+1], Ai[nnz], Ax[nnz], and (each nullable) dVec[n] and the diagonal
+ * positions diag[n] (SimulationData::indexes) to the host. */
 int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, int *Ap, int *Ai, double *Ax,
                         double *dVec, int *diag, int64_t *nnz);
 
@@ -139,7 +144,8 @@ int sdddst_newton_update(sdddst_ctx *ctx, int slot, const double *delta_host);
  * getNewStepSize / reset + the max-stepsize clamp (simulation.cpp:398-405,
  * 516, 643-649; precision_handler.cpp:62-119), on the device:
  * max_ratio = max_i |x_big - x_small2|^2 / tol_i, accept = max_ratio < 1,
- * h_next = new step size (clamped to hmax when hmax > 0), tolerance reset. */
+ * h_next = new step size (clamped to hmax when hmax >= 0; pass a negative
+ * hmax for no limit), tolerance reset. */
 int sdddst_error_control(sdddst_ctx *ctx, double h, double hmax, int *accept, double *h_next, double *max_ratio,
                          int *selected_id);
 
@@ -162,8 +168,13 @@ int sdddst_field_eval(sdddst_ctx *ctx, int which, int64_t n, const double *dx, c
  * the device math against the host C library. */
 int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double *out);
 
-/* Timing of the last kernel launches (HIP events on the context stream). */
-int sdddst_last_kernel_ms(sdddst_ctx *ctx, const char *kernel, double *ms);
+/* Kernel timing with HIP events on the context stream (no host syncs while
+ * recording).  enable = 1 clears and starts recording, 0 stops.  kernel is
+ * "pair" (the pair-speed tile kernel), "speeds" (one whole speed evaluation)
+ * or "jacobian" (one whole assembly); count and total_ms cover the launches
+ * recorded since the last enable. */
+int sdddst_timing(sdddst_ctx *ctx, int enable);
+int sdddst_kernel_stats(sdddst_ctx *ctx, const char *kernel, int *count, double *total_ms);
 
 #ifdef __cplusplus
 }

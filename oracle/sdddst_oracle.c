@@ -943,3 +943,16 @@ void orc_xy_batch(long n, const double *dx, const double *dy, double *out, doubl
         if (out_dx) out_dx[i] = orc_xy_diff_x(dx[i], dy[i]);
     }
 }
+
+/* Per-target sum of |pair term| (the scale of the rounding of any ordering of
+ * the pair sum), for conditioning-aware speed comparisons. */
+void orc_speeds_abs(int n, const double *x, const double *y, const double *b, double *S)
+{
+    for (int i = 0; i < n; ++i) S[i] = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double t = fabs(b[i] * b[j] * orc_xy(orc_normalize(x[i] - x[j]), orc_normalize(y[i] - y[j])));
+            S[i] += t;
+            S[j] += t;
+        }
+}

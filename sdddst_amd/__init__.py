@@ -12,6 +12,8 @@ from ._capi import (  # noqa: F401
     SLOT_BIG_STEP,
     SLOT_DISLOCATIONS,
     SLOT_FIRST_SMALL,
+    SLOT_SCRATCH_A,
+    SLOT_SCRATCH_B,
     SLOT_SECOND_SMALL,
     SOLVER_DENSE_LU,
     SOLVER_FGMRES,
@@ -19,6 +21,7 @@ from ._capi import (  # noqa: F401
     V_INIT,
     V_INIT2,
     V_SPEED,
+    V_SCRATCH,
     V_SPEED2,
     Context,
     Params,

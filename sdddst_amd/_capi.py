@@ -17,8 +17,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsdddst_amd.so")
 
 OK = 0
-SLOT_DISLOCATIONS, SLOT_BIG_STEP, SLOT_FIRST_SMALL, SLOT_SECOND_SMALL = 0, 1, 2, 3
-V_INIT, V_INIT2, V_SPEED, V_SPEED2 = 0, 1, 2, 3
+SLOT_DISLOCATIONS, SLOT_BIG_STEP, SLOT_FIRST_SMALL, SLOT_SECOND_SMALL, SLOT_SCRATCH_A, SLOT_SCRATCH_B = range(6)
+V_INIT, V_INIT2, V_SPEED, V_SPEED2, V_SCRATCH = range(5)
 FIELD_FAST, FIELD_EXACT, FIELD_ORDERED = 0, 1, 2
 SOLVER_DENSE_LU, SOLVER_FGMRES, SOLVER_HOST = 0, 1, 2
 
@@ -84,7 +84,8 @@ _SIGNATURES = {
     "sdddst_synchronize": (ctypes.c_int, [_vp]),
     "sdddst_field_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]),
     "sdddst_libm_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp]),
-    "sdddst_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "sdddst_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "sdddst_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip, _dp]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -272,7 +273,7 @@ class Context:
         d = None if delta is None else _f64(delta)
         self._check(self.lib.sdddst_newton_update(self.h, slot, _dptr(d)))
 
-    def error_control(self, h: float, hmax: float = 0.0):
+    def error_control(self, h: float, hmax: float = -1.0):
         acc, hn, mr, sel = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         self._check(self.lib.sdddst_error_control(self.h, h, hmax, ctypes.byref(acc), ctypes.byref(hn),
                                                   ctypes.byref(mr), ctypes.byref(sel)))
@@ -305,10 +306,24 @@ class Context:
         return out
 
     def timing(self, on: bool):
-        ms = ctypes.c_double()
-        self._check(self.lib.sdddst_last_kernel_ms(self.h, b"enable" if on else b"disable", ctypes.byref(ms)))
+        timing(self.h, on)
 
-    def last_ms(self, kernel: str) -> float:
-        ms = ctypes.c_double()
-        self._check(self.lib.sdddst_last_kernel_ms(self.h, kernel.encode(), ctypes.byref(ms)))
-        return ms.value
+    def kernel_stats(self, kernel: str):
+        return kernel_stats(self.h, kernel)
+
+
+def timing(ctx_handle, on: bool):
+    lib = load_library()
+    rc = lib.sdddst_timing(ctx_handle, int(on))
+    if rc != OK:
+        raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
+
+
+def kernel_stats(ctx_handle, kernel: str):
+    """(launch count, total ms) of a kernel kind since timing was enabled."""
+    lib = load_library()
+    cnt, ms = ctypes.c_int(), ctypes.c_double()
+    rc = lib.sdddst_kernel_stats(ctx_handle, kernel.encode(), ctypes.byref(cnt), ctypes.byref(ms))
+    if rc != OK:
+        raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
+    return cnt.value, ms.value

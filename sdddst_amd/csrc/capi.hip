@@ -50,15 +50,19 @@ static int check_params(const sdddst_params *p)
 static void free_all(Context &c)
 {
     solver_free(c);
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < SDDDST_NUM_SLOTS; ++s)
         if (c.x[s]) (void)hipFree(c.x[s]);
+    for (int s = 0; s < SDDDST_NUM_VBUFS; ++s)
         if (c.v[s]) (void)hipFree(c.v[s]);
-    }
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
                     c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
+    for (Context::Timer *t : {&c.t_pair, &c.t_speeds, &c.t_jac}) {
+        for (hipEvent_t e : t->begin) (void)hipEventDestroy(e);
+        for (hipEvent_t e : t->end) (void)hipEventDestroy(e);
+    }
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -94,10 +98,10 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
     SD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     SD_HIP(hipEventCreate(&c.ev0));
     SD_HIP(hipEventCreate(&c.ev1));
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < SDDDST_NUM_SLOTS; ++s)
         if ((rc = dalloc(&c.x[s], n))) return rc;
+    for (int s = 0; s < SDDDST_NUM_VBUFS; ++s)
         if ((rc = dalloc(&c.v[s], n))) return rc;
-    }
     if ((rc = dalloc(&c.y, n)) || (rc = dalloc(&c.b, n)) || (rc = dalloc(&c.px, pc)) || (rc = dalloc(&c.py, pc)) ||
         (rc = dalloc(&c.tol, n)) || (rc = dalloc(&c.minr2, n)) || (rc = dalloc(&c.partial, (size_t)c.nb * n)) ||
         (rc = dalloc(&c.dvec, n)) || (rc = dalloc(&c.g, n)) || (rc = dalloc(&c.delta, n)) ||
@@ -113,7 +117,48 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
     return SDDDST_OK;
 }
 
-static int valid_slot(int s) { return s >= 0 && s < 4; }
+// Event pairs grow in chunks; elapsed times are read only in timer_collect.
+int timer_begin(Context &c, Context::Timer &t)
+{
+    if (t.used == (int)t.begin.size()) {
+        if (t.used >= 4096) {
+            int rc = timer_collect(c, t);
+            if (rc) return rc;
+        } else {
+            hipEvent_t a, b;
+            SD_HIP(hipEventCreate(&a));
+            SD_HIP(hipEventCreate(&b));
+            t.begin.push_back(a);
+            t.end.push_back(b);
+        }
+    }
+    SD_HIP(hipEventRecord(t.begin[t.used], c.stream));
+    return SDDDST_OK;
+}
+
+int timer_end(Context &c, Context::Timer &t)
+{
+    SD_HIP(hipEventRecord(t.end[t.used], c.stream));
+    t.used++;
+    return SDDDST_OK;
+}
+
+int timer_collect(Context &c, Context::Timer &t)
+{
+    if (t.used == 0) return SDDDST_OK;
+    SD_HIP(hipEventSynchronize(t.end[t.used - 1]));
+    for (int k = 0; k < t.used; ++k) {
+        float ms = 0;
+        SD_HIP(hipEventElapsedTime(&ms, t.begin[k], t.end[k]));
+        t.total_ms += ms;
+        t.count++;
+    }
+    t.used = 0;
+    return SDDDST_OK;
+}
+
+static int valid_slot(int s) { return s >= 0 && s < SDDDST_NUM_SLOTS; }
+static int valid_buf(int s) { return s >= 0 && s < SDDDST_NUM_VBUFS; }
 
 }  // namespace sdamd
 
@@ -234,7 +279,7 @@ int sdddst_speeds(sdddst_ctx *ctx, int cfg_slot, int out_buf, double sigma, int 
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(valid_slot(cfg_slot) && out_buf >= 0 && out_buf < 4, SDDDST_ERR_ARG, "bad slot or buffer");
+    SD_CHECK(valid_slot(cfg_slot) && valid_buf(out_buf), SDDDST_ERR_ARG, "bad slot or buffer");
     int rc = launch_speeds(c, cfg_slot, out_buf, sigma, update_tol);
     if (rc) return rc;
     if (v_out) {
@@ -264,7 +309,7 @@ int sdddst_download_speed(sdddst_ctx *ctx, int buf, double *v_out, int n)
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(buf >= 0 && buf < 4 && v_out && n == c.n, SDDDST_ERR_ARG, "bad arguments");
+    SD_CHECK(valid_buf(buf) && v_out && n == c.n, SDDDST_ERR_ARG, "bad arguments");
     SD_HIP(hipMemcpyAsync(v_out, c.v[buf], sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     return SDDDST_OK;
@@ -274,7 +319,7 @@ int sdddst_upload_speed(sdddst_ctx *ctx, int buf, const double *v, int n)
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(buf >= 0 && buf < 4 && v && n == c.n, SDDDST_ERR_ARG, "bad arguments");
+    SD_CHECK(valid_buf(buf) && v && n == c.n, SDDDST_ERR_ARG, "bad arguments");
     SD_HIP(hipMemcpyAsync(c.v[buf], v, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     return SDDDST_OK;
@@ -284,7 +329,7 @@ int sdddst_copy_speed(sdddst_ctx *ctx, int dst, int src)
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(dst >= 0 && dst < 4 && src >= 0 && src < 4, SDDDST_ERR_ARG, "bad buffer");
+    SD_CHECK(valid_buf(dst) && valid_buf(src), SDDDST_ERR_ARG, "bad buffer");
     if (dst != src)
         SD_HIP(hipMemcpyAsync(c.v[dst], c.v[src], sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.stream));
     return SDDDST_OK;
@@ -296,10 +341,11 @@ int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, in
     CTX_CHECK(ctx);
     Context &c = ctx->c;
     SD_CHECK(valid_slot(cfg_slot), SDDDST_ERR_ARG, "bad slot");
-    if (Ap == nullptr || rebuild || !c.jac_valid) {
+    if (rebuild) {
         int rc = launch_jacobian(c, cfg_slot, h);
         if (rc) return rc;
     }
+    SD_CHECK(c.jac_valid, SDDDST_ERR_STATE, "no Jacobian assembled");
     if (nnz) *nnz = c.nnz;
     if (Ap) {
         SD_CHECK(Ai && Ax, SDDDST_ERR_ARG, "Ai/Ax are NULL");
@@ -317,8 +363,7 @@ int sdddst_residual(sdddst_ctx *ctx, double h, int new_slot, int old_slot, int c
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(valid_slot(new_slot) && valid_slot(old_slot) && csp_buf >= 0 && csp_buf < 4 && isp_buf >= 0 &&
-                 isp_buf < 4,
+    SD_CHECK(valid_slot(new_slot) && valid_slot(old_slot) && valid_buf(csp_buf) && valid_buf(isp_buf),
              SDDDST_ERR_ARG, "bad slot or buffer");
     int rc = launch_residual(c, h, new_slot, old_slot, csp_buf, isp_buf);
     if (rc) return rc;
@@ -399,7 +444,7 @@ int sdddst_speed_sums(sdddst_ctx *ctx, int buf, double *sum_abs, double *sum_sq,
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(buf >= 0 && buf < 4, SDDDST_ERR_ARG, "bad buffer");
+    SD_CHECK(valid_buf(buf), SDDDST_ERR_ARG, "bad buffer");
     int rc = launch_speed_sums(c, buf);
     if (rc) return rc;
     SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
@@ -442,28 +487,34 @@ int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double
     return libm_eval(ctx->c, fn, n, x, out);
 }
 
-int sdddst_last_kernel_ms(sdddst_ctx *ctx, const char *kernel, double *ms)
+int sdddst_timing(sdddst_ctx *ctx, int enable)
 {
     CTX_CHECK(ctx);
     Context &c = ctx->c;
-    SD_CHECK(kernel && ms, SDDDST_ERR_ARG, "bad arguments");
-    if (strcmp(kernel, "enable") == 0) {
-        c.timing = true;
-        *ms = 0;
-        return SDDDST_OK;
+    SD_HIP(hipStreamSynchronize(c.stream));
+    for (Context::Timer *t : {&c.t_pair, &c.t_speeds, &c.t_jac}) {
+        t->used = 0;
+        t->total_ms = 0;
+        t->count = 0;
     }
-    if (strcmp(kernel, "disable") == 0) {
-        c.timing = false;
-        *ms = 0;
-        return SDDDST_OK;
-    }
-    if (strcmp(kernel, "pair") == 0) *ms = c.last_pair_kernel_ms;
-    else if (strcmp(kernel, "speeds") == 0) *ms = c.last_speeds_ms;
-    else if (strcmp(kernel, "jacobian") == 0) *ms = c.last_jacobian_ms;
-    else {
-        set_error("unknown kernel name '%s'", kernel);
-        return SDDDST_ERR_ARG;
-    }
+    c.timing = enable != 0;
+    return SDDDST_OK;
+}
+
+int sdddst_kernel_stats(sdddst_ctx *ctx, const char *kernel, int *count, double *total_ms)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(kernel && count && total_ms, SDDDST_ERR_ARG, "bad arguments");
+    Context::Timer *t = nullptr;
+    if (strcmp(kernel, "pair") == 0) t = &c.t_pair;
+    else if (strcmp(kernel, "speeds") == 0) t = &c.t_speeds;
+    else if (strcmp(kernel, "jacobian") == 0) t = &c.t_jac;
+    SD_CHECK(t, SDDDST_ERR_ARG, "unknown kernel name '%s'", kernel);
+    int rc = timer_collect(c, *t);
+    if (rc) return rc;
+    *count = t->count;
+    *total_ms = t->total_ms;
     return SDDDST_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/sdddst_amd.h"
 
@@ -52,11 +53,11 @@ struct Context {
     int device = 0;
 
     // positions: x per slot, y and b shared
-    double *x[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *x[SDDDST_NUM_SLOTS] = {};
     double *y = nullptr, *b = nullptr;
     double *px = nullptr, *py = nullptr;
     // speeds
-    double *v[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *v[SDDDST_NUM_VBUFS] = {};
     double *tol = nullptr;              // tolerance per dislocation
     unsigned long long *minr2 = nullptr; // per-call minimum r^2 bits
     double *partial = nullptr;          // [nb][n] pair partial sums
@@ -84,11 +85,21 @@ struct Context {
     Fgmres *gm = nullptr;
     int last_iters = 0, fallbacks = 0;
     double last_rel_res = 0;
-    // timing
+    // timing: recorded event pairs per kernel kind, read back on request
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_speeds_ms = 0, last_jacobian_ms = 0, last_pair_kernel_ms = 0;
     bool timing = false;
+    struct Timer {
+        std::vector<hipEvent_t> begin, end;
+        int used = 0;
+        double total_ms = 0;
+        int count = 0;
+    } t_pair, t_speeds, t_jac;
 };
+
+// Record the start / end of a timed region on the context stream.
+int timer_begin(Context &c, Context::Timer &t);
+int timer_end(Context &c, Context::Timer &t);
+int timer_collect(Context &c, Context::Timer &t);
 
 // cutoff of SimulationData::updateCutOff (simulation_data.cpp:208-218)
 void compute_cutoff(Context &c);

@@ -298,7 +298,8 @@ int launch_jacobian(Context &c, int slot, double h)
     const CutParams cp = cut_params(c);
     const double *x = c.x[slot];
     const int wblocks = (n + 3) / 4;
-    if (c.timing) SD_HIP(hipEventRecord(c.ev0, c.stream));
+    int rc;
+    if (c.timing && (rc = timer_begin(c, c.t_jac))) return rc;
     hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.colcount);
     SD_HIP(hipGetLastError());
     long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
@@ -308,7 +309,7 @@ int launch_jacobian(Context &c, int slot, double h)
     SD_HIP(hipMemcpyAsync(&total, d_total, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
-    int rc = ensure_capacity(c, total);
+    rc = ensure_capacity(c, total);
     if (rc) return rc;
     c.nnz = total;
     SD_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int), c.stream));
@@ -330,11 +331,7 @@ int launch_jacobian(Context &c, int slot, double h)
     hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec, c.Ax,
                        c.AxRow);
     SD_HIP(hipGetLastError());
-    if (c.timing) {
-        SD_HIP(hipEventRecord(c.ev1, c.stream));
-        SD_HIP(hipEventSynchronize(c.ev1));
-        SD_HIP(hipEventElapsedTime(&c.last_jacobian_ms, c.ev0, c.ev1));
-    }
+    if (c.timing && (rc = timer_end(c, c.t_jac))) return rc;
     c.jac_valid = true;
     c.jac_h = h;
     return SDDDST_OK;

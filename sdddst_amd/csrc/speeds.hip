@@ -213,17 +213,14 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     const double minPrecSqr = c.p.precision * c.p.precision;
     const int threads = 256;
     const int blocks = (n + threads - 1) / threads;
-    if (c.timing) SD_HIP(hipEventRecord(c.ev0, c.stream));
+    int rc;
+    if (c.timing && (rc = timer_begin(c, c.t_speeds))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_ORDERED) {
+        if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
         hipLaunchKernelGGL(k_speeds_ordered, dim3(blocks), dim3(threads), 0, c.stream, n, c.pc, c.x[slot], c.y, c.b,
                            c.px, c.py, c.p.A, c.p.KASQR, sigma, c.tol, minPrecSqr, update_tol, c.v[out_buf]);
         SD_HIP(hipGetLastError());
-        if (c.timing) {
-            SD_HIP(hipEventRecord(c.ev1, c.stream));
-            SD_HIP(hipEventSynchronize(c.ev1));
-            SD_HIP(hipEventElapsedTime(&c.last_pair_kernel_ms, c.ev0, c.ev1));
-            c.last_speeds_ms = c.last_pair_kernel_ms;
-        }
+        if (c.timing && ((rc = timer_end(c, c.t_pair)) || (rc = timer_end(c, c.t_speeds)))) return rc;
         return SDDDST_OK;
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(threads), 0, c.stream, c.minr2, kInfBits, n);
@@ -231,8 +228,7 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     const int nb = c.nb;
     const long tiles = (long)nb * (nb + 1) / 2;
     SD_CHECK(tiles < (1L << 31), SDDDST_ERR_ARG, "too many tiles for N=%d", n);
-    hipEvent_t e_mid0 = c.ev0;
-    if (c.timing) SD_HIP(hipEventRecord(e_mid0, c.stream));
+    if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
                            c.x[slot], c.y, c.b, c.partial, c.minr2);
@@ -240,15 +236,12 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
                            c.x[slot], c.y, c.b, c.partial, c.minr2);
     SD_HIP(hipGetLastError());
-    if (c.timing) SD_HIP(hipEventRecord(c.ev1, c.stream));
+    if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.partial, c.x[slot],
                        c.y, c.b, c.px, c.py, c.p.A, c.p.KASQR, sigma, c.minr2, c.tol, minPrecSqr, update_tol,
                        c.v[out_buf]);
     SD_HIP(hipGetLastError());
-    if (c.timing) {
-        SD_HIP(hipEventSynchronize(c.ev1));
-        SD_HIP(hipEventElapsedTime(&c.last_pair_kernel_ms, e_mid0, c.ev1));
-    }
+    if (c.timing && (rc = timer_end(c, c.t_speeds))) return rc;
     return SDDDST_OK;
 }
 

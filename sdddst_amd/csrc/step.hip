@@ -45,7 +45,7 @@ __device__ double new_step_size(double maxRatio, double h, double hmax)
         if (tmp > 2.0) tmp = 2.0;
         hn = 0.9 * h * tmp;
     }
-    if (hmax > 0 && hmax < hn) hn = hmax;
+    if (hmax >= 0 && hmax < hn) hn = hmax;
     return hn;
 }
 

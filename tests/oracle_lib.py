@@ -79,6 +79,7 @@ def oracle():
         "orc_sim_get_speeds": (None, [VP, DP, DP]),
         "orc_libm_batch": (None, [ctypes.c_int, L, DP, DP]),
         "orc_xy_batch": (None, [L, DP, DP, DP, DP]),
+        "orc_speeds_abs": (None, [ctypes.c_int, DP, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -162,6 +163,24 @@ def speeds(x, y, b, sigma=0.0, px=None, py=None, A=0.0, K=0.0, tol=None, prec=1e
     pc = 0 if px is None else len(px)
     oracle().orc_speeds(n, dp(x), dp(y), dp(b), pc, dp(px), dp(py), A, K, sigma, dp(v), dp(t), prec * prec)
     return v, t
+
+
+def speeds_abs(x, y, b):
+    """S_i = sum_j |t_ij| per target."""
+    n = len(x)
+    out = np.empty(n)
+    oracle().orc_speeds_abs(n, dp(x), dp(y), dp(b), dp(out))
+    return out
+
+
+def speed_parity(v, vr, S, n):
+    """Per-component speed parity: within 1e-12 relative, or, where the pair
+    sum cancels, within the rounding scale of a reordered n-term sum plus the
+    per-pair absolute error floor: 2^-52 * sqrt(n) * (S_i + n)."""
+    dv = np.abs(np.asarray(v) - np.asarray(vr))
+    rel_ok = dv <= 1e-12 * np.abs(vr)
+    cond_ok = dv <= 2.0 ** -52 * np.sqrt(n) * (S + n)
+    return rel_ok, rel_ok | cond_ok
 
 
 def jacobian(x, y, b, cutoff_multiplier, h, cap=None):

@@ -1,0 +1,89 @@
+"""The device elementary functions and field (sdddst_amd/csrc/libm_compat.h,
+field.h) built for the host (tests/native) against the host C library and
+the oracle, bit for bit, on CPU.  The GPU tests repeat the comparison with
+the device build."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+FN = {"cos": 0, "sin": 1, "exp": 2, "expm1": 3, "cosh": 4, "sinh": 5}
+
+
+def _mc1(fn, x):
+    mc = O.mathcheck()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    mc.mc_batch1(fn, len(x), O.dp(x), O.dp(out))
+    return out
+
+
+def _mc2(fn, dx, dy):
+    mc = O.mathcheck()
+    out = np.empty_like(dx)
+    mc.mc_batch2(fn, len(dx), O.dp(dx), O.dp(dy), O.dp(out))
+    return out
+
+
+@pytest.mark.parametrize("name,lo,hi", [("cos", -3.1416, 3.1416), ("cos", -0.03, 0.03), ("exp", -40, 40),
+                                        ("expm1", -0.35, 0.35), ("expm1", -35, 35), ("cosh", -35, 35),
+                                        ("cosh", -0.35, 0.35), ("sinh", -35, 35), ("sin", -3.1416, 3.1416)])
+def test_host_build_matches_glibc(name, lo, hi):
+    x = np.random.default_rng(hash(name) % 1000).uniform(lo, hi, 300_000)
+    got = _mc1(FN[name], x)
+    want = O.libm_batch(FN[name], x)
+    mism = np.count_nonzero(got.view(np.int64) != want.view(np.int64))
+    assert mism <= (2 if name == "sin" else 0)
+
+
+def test_thresholds_of_the_branches():
+    # dense sampling around the branch points of cosh / sinh / expm1 / cos
+    pts = []
+    for t in (0.34657359027997264, 1.0397207708399179, 22.0, 0.855469, 2.426265, 0.126, 2 ** -27, 2 ** -28):
+        pts.append(t + np.arange(-2000, 2000) * np.spacing(t))
+    x = np.concatenate(pts + [-p for p in pts])
+    for name in ("cosh", "sinh", "expm1", "exp", "cos"):
+        xs = x[np.abs(x) < 4.0] if name == "cos" else x     # cos is used for |x| <= pi
+        got = _mc1(FN[name], xs)
+        x_ = xs
+        want = O.libm_batch(FN[name], x_)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
+
+
+def test_normalize_equals_loop():
+    x = np.concatenate([np.random.default_rng(1).uniform(-6, 6, 100_000),
+                        [0.5, -0.5, 1.5, -1.5, 2.5, 0.49999999999999994, -0.5000000000000001, 0.0, -0.0, 3.0]])
+    got = _mc1(6, x)
+    want = O.libm_batch(8, x)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+
+
+def test_field_exact_bit_identical_to_oracle():
+    rng = np.random.default_rng(9)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, 80_000), rng.uniform(-2e-3, 2e-3, 20_000)])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, 80_000), rng.uniform(-2e-3, 2e-3, 20_000)])
+    got = _mc2(0, dx, dy)
+    want, _ = O.xy_batch(dx, dy)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+
+
+def test_field_fast_and_derivative():
+    rng = np.random.default_rng(10)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, 50_000), rng.uniform(-2e-3, 2e-3, 10_000)])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, 50_000), rng.uniform(-2e-3, 2e-3, 10_000)])
+    xy, xyd = O.xy_batch(dx, dy)
+    fast = _mc2(1, dx, dy)
+    assert np.max(np.abs(fast - xy) / (np.abs(xy) + 1.0)) < 4e-15
+    d = _mc2(2, dx, dy)
+    assert np.mean(d.view(np.int64) == xyd.view(np.int64)) > 0.998
+    assert np.max(np.abs(d - xyd) / np.maximum(np.abs(xyd), 1.0)) < 1e-13
+
+
+def test_integer_powers_correctly_rounded():
+    import mpmath
+    mpmath.mp.prec = 200
+    mc = O.mathcheck()
+    rng = np.random.default_rng(2)
+    for v in rng.uniform(-7e-3, 7e-3, 300):
+        for n in (3, 4, 5, 6, 7):
+            assert mc.mc_powi(float(v), n) == float(mpmath.mpf(float(v)) ** n)

@@ -1,0 +1,202 @@
+"""GPU parity of the implicit step: residual/update/controller kernels, the
+device solvers, and whole fixed-step and adaptive trajectories of
+sdddstCore::Simulation against the oracle's stepping driver."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import sdddst_amd as S
+from sdddst_amd import configs
+from sdddst_amd.simulation import Simulation, read_log
+
+pytestmark = pytest.mark.gpu
+
+
+def test_error_controller_matches_oracle():
+    n = 1000
+    rng = np.random.default_rng(5)
+    xb = rng.uniform(-0.5, 0.5, n)
+    xs = xb + rng.normal(0, 1e-7, n)
+    xs[::7] = xb[::7]          # zero errors are skipped
+    prec = 1e-6
+    with S.Context(n, precision=prec) as c:
+        c.upload_state(S.SLOT_BIG_STEP, xb, np.zeros(n), np.ones(n))
+        c.upload_state(S.SLOT_SECOND_SMALL, xs)
+        c.reset_tolerance()
+        acc, hn, mr, sel = c.error_control(1e-4, -1.0)
+        tol_after = c.tolerance()
+    tol = np.full(n, prec * prec)
+    import ctypes
+    sel_o = ctypes.c_int()
+    mr_o = O.oracle().orc_max_error_ratio(n, O.dp(xb), O.dp(xs), O.dp(tol), ctypes.byref(sel_o))
+    assert mr == mr_o and sel == sel_o.value
+    assert acc == (mr_o < 1.0)
+    hn_o = O.oracle().orc_new_step_size(mr_o, 1e-4)
+    assert abs(hn - hn_o) <= np.spacing(hn_o)
+    assert np.all(tol_after == prec * prec)
+
+
+def test_error_controller_clamp_and_zero():
+    n = 64
+    x = np.linspace(-0.4, 0.4, n)
+    with S.Context(n) as c:
+        c.upload_state(S.SLOT_BIG_STEP, x, np.zeros(n), np.ones(n))
+        c.upload_state(S.SLOT_SECOND_SMALL, x)
+        acc, hn, mr, sel = c.error_control(1e-3, -1.0)
+        assert acc and mr == 0.0 and hn == 2e-3        # precision_handler.cpp:105-108
+        acc, hn, mr, sel = c.error_control(1e-3, 1.5e-3)
+        assert hn == 1.5e-3                            # simulation.cpp:646-649
+
+
+def _jacobian_ctx(n, m, seed, h):
+    x, y, b = configs.dislocations(n, seed)
+    c = S.Context(n, cutoff_multiplier=m)
+    c.upload_state(0, x, y, b)
+    c.jacobian_nnz(0, h)
+    return c, x, y, b
+
+
+def test_fgmres_matches_dense_lu():
+    n = 1024
+    c, x, y, b = _jacobian_ctx(n, 1.0, 3, 2e-4)
+    rng = np.random.default_rng(1)
+    g = rng.normal(size=n)
+    # g enters through the residual kernel: v buffers zero, x_new - x_old = g
+    c.upload_speed(S.V_INIT, np.zeros(n))
+    c.upload_speed(S.V_SPEED, np.zeros(n))
+    c.upload_state(S.SLOT_SCRATCH_A, x + g)
+    c.upload_state(S.SLOT_SCRATCH_B, x)
+    gd = c.residual(1e-4, S.SLOT_SCRATCH_A, S.SLOT_SCRATCH_B, S.V_SPEED, S.V_INIT, download=True)
+    c.solver_select(S.SOLVER_DENSE_LU)
+    c.solver_factor()
+    d_lu = c.solver_solve(download=True)
+    c.solver_select(S.SOLVER_FGMRES, 1e-10, 200, 30)
+    c.solver_factor()
+    d_gm = c.solver_solve(download=True)
+    it, rr, fb = c.solver_stats()
+    assert rr <= 1e-10 and fb == 0 and it > 0
+    Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, 2e-4, rebuild=False)
+    # residual of the LU solution in the host CSC
+    A = np.zeros((n, n))
+    for j in range(n):
+        A[Ai[Ap[j]:Ap[j + 1]], j] = Ax[Ap[j]:Ap[j + 1]]
+    assert np.linalg.norm(A @ d_lu - gd) / np.linalg.norm(gd) < 1e-13
+    assert np.linalg.norm(A @ d_gm - gd) / np.linalg.norm(gd) < 2e-10
+    assert np.linalg.norm(d_gm - d_lu) / np.linalg.norm(d_lu) < 1e-8
+    c.close()
+
+
+def test_fgmres_nonconvergence_falls_back_to_direct():
+    n = 512
+    c, x, y, b = _jacobian_ctx(n, 1.0, 8, 5e-3)
+    c.upload_speed(S.V_INIT, np.zeros(n))
+    c.upload_state(S.SLOT_SCRATCH_A, x + 1e-3)
+    c.upload_state(S.SLOT_SCRATCH_B, x)
+    c.residual(1e-4, S.SLOT_SCRATCH_A, S.SLOT_SCRATCH_B, S.V_INIT, S.V_INIT)
+    c.solver_select(S.SOLVER_FGMRES, 1e-14, 1, 1)     # one iteration cannot reach 1e-14
+    c.solver_factor()
+    d = c.solver_solve(download=True)
+    it, rr, fb = c.solver_stats()
+    assert fb == 1
+    c.solver_select(S.SOLVER_DENSE_LU)
+    c.solver_factor()
+    d_lu = c.solver_solve(download=True)
+    assert np.array_equal(d, d_lu)
+    c.close()
+
+
+# -- whole steps -----------------------------------------------------------
+
+def _pair(n, seed, m, stress, sval, h0, mode, tmp_path, solver="dense-lu"):
+    x, y, b = configs.dislocations(n, seed)
+    o = O.OracleSim(x, y, b, cutoff_multiplier=m, prec=1e-6, h0=h0, stress=stress, p1=sval,
+                    calculate_strain=True, order_parameter=True)
+    log = os.path.join(str(tmp_path), "gpu.log")
+    g = Simulation(x, y, b, cutoff_multiplier=m, precision=1e-6, initial_step=h0, stress=stress,
+                   stress_value=sval, calculate_strain=True, order_parameter=True, solver=solver,
+                   field_mode=mode, log_path=log)
+    return o, g, log
+
+
+@pytest.mark.parametrize("mode", ["ordered", "exact", "fast"])
+def test_fixed_step_trajectory(mode, tmp_path):
+    o, g, log = _pair(256, 1, 1.0, "zero", 0.0, 1e-6, mode, tmp_path)
+    h = 2e-6
+    for k in range(8):
+        o.set_step_size(h)
+        g.set_step_size(h)
+        ro = o.step()
+        rg = g.step()
+        assert ro == rg
+    xo, xg = o.x(), g.positions()
+    assert np.max(np.abs(xg - xo)) < 1e-9
+    g.flush_log()
+    lg, lo = read_log(log), o.log()
+    assert lg.shape == lo.shape
+    cols = [0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11]
+    a, r = lg[:, cols], lo[:, cols]
+    same_nan = np.isnan(a) == np.isnan(r)
+    assert same_nan.all()
+    fin = ~np.isnan(r)
+    rel = np.abs(a[fin] - r[fin]) / np.maximum(np.abs(r[fin]), 1e-300)
+    # the log prints 17 significant digits: max-error ratio (col 3) is the
+    # only value sensitive at the 1e-9 level to last-bit differences
+    assert rel.max() < 1e-6
+    o.close()
+    g.close()
+
+
+def test_adaptive_run_config1(tmp_path):
+    # N=256, m=1, zero stress, precision 1e-6: 100 successful steps
+    o, g, log = _pair(256, 1, 1.0, "zero", 0.0, 1e-6, "exact", tmp_path)
+    while o.succesful_steps < 100:
+        o.step()
+    g.run(100)
+    so = (o.time, o.succesful_steps, o.failed_steps)
+    st = g.state()
+    assert st.succesful_steps == 100
+    assert st.failed_steps == so[2]
+    assert abs(st.sim_time - so[0]) <= 1e-9 * abs(so[0])
+    assert np.max(np.abs(g.positions() - o.x())) < 1e-8
+    o.close()
+    g.close()
+
+
+def test_constant_stress_shortcut_is_exact(tmp_path):
+    # constant stress takes the zero-stress shortcut (one speed evaluation
+    # less per integrate); the oracle evaluates it and must agree
+    o, g, log = _pair(256, 4, 1.0, "constant", 0.02, 1e-6, "ordered", tmp_path)
+    for k in range(5):
+        o.step()
+        g.step()
+    assert np.max(np.abs(g.positions() - o.x())) < 1e-12
+    o.close()
+    g.close()
+
+
+def test_fixed_rate_protocol(tmp_path):
+    o, g, log = _pair(128, 6, 1.0, "fixed-rate", 1e3, 1e-6, "exact", tmp_path)
+    for k in range(6):
+        o.step()
+        g.step()
+    assert abs(g.get_time() - o.time) <= 1e-12 * abs(o.time)
+    assert np.max(np.abs(g.positions() - o.x())) < 1e-9
+    o.close()
+    g.close()
+
+
+def test_fgmres_run_matches_direct(tmp_path):
+    x, y, b = configs.dislocations(2048, 12)
+    kw = dict(cutoff_multiplier=2.0, precision=1e-6, initial_step=1e-6, stress="constant", stress_value=0.02)
+    g1 = Simulation(x, y, b, solver="fgmres", **kw)
+    g2 = Simulation(x, y, b, solver="dense-lu", **kw)
+    g1.run(5)
+    g2.run(5)
+    s1, s2 = g1.state(), g2.state()
+    assert s1.failed_steps == s2.failed_steps
+    assert abs(s1.sim_time - s2.sim_time) <= 1e-9 * s2.sim_time
+    assert np.max(np.abs(g1.positions() - g2.positions())) < 1e-9
+    g1.close()
+    g2.close()

@@ -41,6 +41,7 @@ typedef struct sdddst_sim_options {
     int solver_max_iter;
     int solver_restart;
     int field_mode;            /* SDDDST_FIELD_* of sdddst_amd.h          */
+    int preconditioner;        /* SDDDST_PRECOND_* of sdddst_amd.h        */
     double A, KASQR;           /* point-defect constants (as used)         */
     const char *log_path;      /* 12-column log file; NULL: none           */
 } sdddst_sim_options;
@@ -53,6 +54,7 @@ typedef struct sdddst_sim_state {
     double last_step_wall_s;
     int solver_last_iters, solver_fallbacks;
     double solver_last_rel_residual;
+    long long solver_total_iters, solver_total_solves;
 } sdddst_sim_state;
 
 void sdddst_sim_default_options(sdddst_sim_options *o);

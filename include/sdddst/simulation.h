@@ -62,7 +62,7 @@ public:
 
 private:
     void integrateSlots(double stepsize, int newSlot, int oldSlot, bool useSpeed2, bool calculateInitSpeed,
-                        StressProtocolStepType origin, StressProtocolStepType end);
+                        StressProtocolStepType origin, StressProtocolStepType end, bool samePositions = false);
     void calculateGSlots(double stepsize, int newSlot, int oldSlot, bool useSpeed2, bool calculateInitSpeed,
                          bool useInitSpeedForFirstStep, StressProtocolStepType origin, StressProtocolStepType end);
     void speedsSlot(int slot, int buf, bool updateTolerance);

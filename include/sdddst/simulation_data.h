@@ -124,6 +124,7 @@ public:
     double solverRelTolerance;    // FGMRES acceptance ||g - A d|| / ||g||
     int solverMaxIterations;
     int solverRestart;
+    int preconditioner;           // SDDDST_PRECOND_* of include/sdddst_amd.h
     int fieldMode;                // SDDDST_FIELD_* of include/sdddst_amd.h
 
 private:

@@ -64,6 +64,11 @@ extern "C" {
 #define SDDDST_SOLVER_FGMRES 1     /* device FGMRES, FP64, relative residual tolerance        */
 #define SDDDST_SOLVER_HOST 2       /* host hand-off: CSC to the caller (UMFPACK)              */
 
+/* FGMRES right preconditioners. */
+#define SDDDST_PRECOND_JACOBI 0   /* 1 / A_ii                                              */
+#define SDDDST_PRECOND_PAIR 1     /* exact inverse of 2x2 blocks of mutually strongest pairs */
+#define SDDDST_PRECOND_LEARNED 2  /* sparse approximate inverse on A's pattern from the net  */
+
 typedef struct sdddst_ctx sdddst_ctx;
 
 /* Scalars of SimulationData that the kernels need
@@ -116,7 +121,10 @@ int sdddst_copy_speed(sdddst_ctx *ctx, int dst, int src);
 /* Simulation::calculateJacobian (simulation.cpp:236-384): the CSC of
  * A = I + diag((1+d)/2) * M for slot cfg_slot and step h, built on the
  * device with the reference's exact sparsity pattern.  rebuild = 1 builds
- * it for (cfg_slot, h); rebuild = 0 reuses the last one.  Two-phase: with
+ * it for (cfg_slot, h); rebuild = 2 keeps the sparsity pattern and the
+ * per-pair field derivatives of the last rebuild = 1 (the caller asserts the
+ * positions are the same, as for stages I and II of a step) and recomputes
+ * the h-dependent values; rebuild = 0 reuses the last matrix.  Two-phase: with
  * Ap == NULL only *nnz is returned; a call with non-NULL pointers copies
  * Ap[n+1], Ai[nnz], Ax[nnz], and (each nullable) dVec[n] and the diagonal
  * positions diag[n] (SimulationData::indexes) to the host. */
@@ -132,9 +140,11 @@ int sdddst_residual(sdddst_ctx *ctx, double h, int new_slot, int old_slot, int c
  * simulation.cpp:386-396, when the solver is a device solver).
  * factor: once per Jacobian; solve: delta on the device, delta_out nullable. */
 int sdddst_solver_select(sdddst_ctx *ctx, int solver, double rel_tol, int max_iter, int restart);
+int sdddst_set_preconditioner(sdddst_ctx *ctx, int precond);
 int sdddst_solver_factor(sdddst_ctx *ctx);
 int sdddst_solver_solve(sdddst_ctx *ctx, double *delta_out);
-int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks);
+int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks,
+                        int64_t *total_iters, int64_t *total_solves);
 
 /* Newton update x[slot] -= delta (simulation.cpp:78-81); delta_host NULL
  * uses the device delta of the last solve. */

@@ -9,6 +9,9 @@ from ._capi import (  # noqa: F401
     FIELD_EXACT,
     FIELD_FAST,
     FIELD_ORDERED,
+    PRECOND_JACOBI,
+    PRECOND_LEARNED,
+    PRECOND_PAIR,
     SLOT_BIG_STEP,
     SLOT_DISLOCATIONS,
     SLOT_FIRST_SMALL,

@@ -21,6 +21,7 @@ SLOT_DISLOCATIONS, SLOT_BIG_STEP, SLOT_FIRST_SMALL, SLOT_SECOND_SMALL, SLOT_SCRA
 V_INIT, V_INIT2, V_SPEED, V_SPEED2, V_SCRATCH = range(5)
 FIELD_FAST, FIELD_EXACT, FIELD_ORDERED = 0, 1, 2
 SOLVER_DENSE_LU, SOLVER_FGMRES, SOLVER_HOST = 0, 1, 2
+PRECOND_JACOBI, PRECOND_PAIR, PRECOND_LEARNED = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -71,9 +72,10 @@ _SIGNATURES = {
         [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
     ),
     "sdddst_solver_select": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "sdddst_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "sdddst_solver_factor": (ctypes.c_int, [_vp]),
     "sdddst_solver_solve": (ctypes.c_int, [_vp, _dp]),
-    "sdddst_solver_stats": (ctypes.c_int, [_vp, _ip, _dp, _ip]),
+    "sdddst_solver_stats": (ctypes.c_int, [_vp, _ip, _dp, _ip, _i64p, _i64p]),
     "sdddst_newton_update": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "sdddst_error_control": (
         ctypes.c_int,
@@ -256,6 +258,9 @@ class Context:
     def solver_select(self, solver: int, rel_tol: float = 0.0, max_iter: int = 0, restart: int = 0):
         self._check(self.lib.sdddst_solver_select(self.h, solver, rel_tol, max_iter, restart))
 
+    def set_preconditioner(self, precond: int):
+        self._check(self.lib.sdddst_set_preconditioner(self.h, precond))
+
     def solver_factor(self):
         self._check(self.lib.sdddst_solver_factor(self.h))
 
@@ -266,7 +271,8 @@ class Context:
 
     def solver_stats(self):
         it, rr, fb = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
-        self._check(self.lib.sdddst_solver_stats(self.h, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(fb)))
+        self._check(self.lib.sdddst_solver_stats(self.h, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(fb),
+                                                 None, None))
         return it.value, rr.value, fb.value
 
     def newton_update(self, slot: int, delta=None):

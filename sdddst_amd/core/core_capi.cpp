@@ -45,6 +45,7 @@ void configure(SimulationData &d, const sdddst_sim_options &o)
     d.A = o.A;
     d.KASQR = o.KASQR;
     d.fieldMode = o.field_mode;
+    d.preconditioner = o.preconditioner;
     d.solverRelTolerance = o.solver_tol;
     d.solverMaxIterations = o.solver_max_iter;
     d.solverRestart = o.solver_restart;
@@ -92,6 +93,7 @@ void sdddst_sim_default_options(sdddst_sim_options *o)
     o->solver_max_iter = 300;
     o->solver_restart = 40;
     o->field_mode = SDDDST_FIELD_FAST;
+    o->preconditioner = SDDDST_PRECOND_PAIR;
     o->A = 1e-4 * 16.0;
     o->KASQR = 1.65 * 1.65 * 1e6 / 256.0;
     o->log_path = nullptr;
@@ -190,7 +192,10 @@ int sdddst_sim_get_state(sdddst_sim *s, sdddst_sim_state *o)
     o->last_step_wall_s = s->lastWall;
     int it = 0, fb = 0;
     double rr = 0;
-    sdddst_solver_stats(s->sim->deviceContext(), &it, &rr, &fb);
+    int64_t ti = 0, ts = 0;
+    sdddst_solver_stats(s->sim->deviceContext(), &it, &rr, &fb, &ti, &ts);
+    o->solver_total_iters = ti;
+    o->solver_total_solves = ts;
     o->solver_last_iters = it;
     o->solver_fallbacks = fb;
     o->solver_last_rel_residual = rr;

@@ -80,6 +80,7 @@ Simulation::Simulation(std::shared_ptr<SimulationData> _sD)
     if (solver == LinearSolver::FGMRES) kind = SDDDST_SOLVER_FGMRES;
     check(sdddst_solver_select(ctx, kind, sD->solverRelTolerance, sD->solverMaxIterations, sD->solverRestart),
           "solver select");
+    check(sdddst_set_preconditioner(ctx, sD->preconditioner), "preconditioner");
 }
 
 Simulation::~Simulation()
@@ -191,10 +192,14 @@ void Simulation::calculateGSlots(double stepsize, int newSlot, int oldSlot, bool
 }
 
 // simulation.cpp:62-84 on device slots.
+// samePositions: the Jacobian is assembled at the positions of the previous
+// assembly (stage II starts where stage I started), so the sparsity pattern
+// and the field derivatives are reused and only the h-dependent values change.
 void Simulation::integrateSlots(double stepsize, int newSlot, int oldSlot, bool useSpeed2, bool calculateInitSpeed,
-                                StressProtocolStepType origin, StressProtocolStepType end)
+                                StressProtocolStepType origin, StressProtocolStepType end, bool samePositions)
 {
-    check(sdddst_jacobian_csc(ctx, newSlot, stepsize, 1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
+    check(sdddst_jacobian_csc(ctx, newSlot, stepsize, samePositions ? 2 : 1, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr),
           "jacobian");
     calculateSparseFormForJacobian();
     for (size_t i = 0; i < sD->ic; i++) {
@@ -414,7 +419,7 @@ void Simulation::stepStageI()
 void Simulation::stepStageII()
 {
     check(sdddst_copy_slot(ctx, FS, DISL), "copy");
-    integrateSlots(0.5 * sD->stepSize, FS, DISL, false, false, Original, EndOfFirstSmallStep);
+    integrateSlots(0.5 * sD->stepSize, FS, DISL, false, false, Original, EndOfFirstSmallStep, true);
 }
 
 // simulation.cpp:501-650

@@ -29,7 +29,8 @@ SimulationData::SimulationData()
       speedThresholdForCutoffChange(0), isSpeedThresholdForCutoffChange(false), currentStorageSize(0),
       sumAvgSpeed(0), eVAnalResultFile(""), remainingFinalSteps(0), inFinal(false), calculateDerivativeEVAnal(false),
       numberOfEigenVecToWrite(10), writeCorrelMatrices(0), linearSolver(LinearSolver::Auto),
-      solverRelTolerance(1e-10), solverMaxIterations(300), solverRestart(40), fieldMode(SDDDST_FIELD_FAST),
+      solverRelTolerance(1e-10), solverMaxIterations(300), solverRestart(40), preconditioner(SDDDST_PRECOND_PAIR),
+      fieldMode(SDDDST_FIELD_FAST),
       dislocationDataIsLoaded(false)
 {
 }

@@ -55,7 +55,7 @@ static void free_all(Context &c)
     for (int s = 0; s < SDDDST_NUM_VBUFS; ++s)
         if (c.v[s]) (void)hipFree(c.v[s]);
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
-                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds};
+                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
@@ -205,6 +205,7 @@ int sdddst_set_params(sdddst_ctx *ctx, const sdddst_params *params)
     ctx->c.p = *params;
     compute_cutoff(ctx->c);
     ctx->c.jac_valid = false;
+    ctx->c.pattern_valid = false;
     return SDDDST_OK;
 }
 
@@ -227,7 +228,9 @@ int sdddst_upload_state(sdddst_ctx *ctx, int slot, const double *x, const double
     if (y) SD_HIP(hipMemcpyAsync(c.y, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     if (b) SD_HIP(hipMemcpyAsync(c.b, b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
-    c.jac_valid = false;
+    // The assembled matrix stays valid.  Reusing its pattern (rebuild = 2) is
+    // the caller's assertion about x; new y or b rule it out.
+    if (y != nullptr || b != nullptr) c.pattern_valid = false;
     return SDDDST_OK;
 }
 
@@ -341,8 +344,9 @@ int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, in
     CTX_CHECK(ctx);
     Context &c = ctx->c;
     SD_CHECK(valid_slot(cfg_slot), SDDDST_ERR_ARG, "bad slot");
+    SD_CHECK(rebuild >= 0 && rebuild <= 2, SDDDST_ERR_ARG, "rebuild must be 0, 1 or 2");
     if (rebuild) {
-        int rc = launch_jacobian(c, cfg_slot, h);
+        int rc = launch_jacobian(c, cfg_slot, h, rebuild == 2);
         if (rc) return rc;
     }
     SD_CHECK(c.jac_valid, SDDDST_ERR_STATE, "no Jacobian assembled");
@@ -387,6 +391,15 @@ int sdddst_solver_select(sdddst_ctx *ctx, int solver, double rel_tol, int max_it
     return SDDDST_OK;
 }
 
+int sdddst_set_preconditioner(sdddst_ctx *ctx, int precond)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(precond >= SDDDST_PRECOND_JACOBI && precond <= SDDDST_PRECOND_LEARNED, SDDDST_ERR_ARG,
+             "unknown preconditioner %d", precond);
+    ctx->c.precond = precond;
+    return SDDDST_OK;
+}
+
 int sdddst_solver_factor(sdddst_ctx *ctx)
 {
     CTX_CHECK(ctx);
@@ -406,9 +419,12 @@ int sdddst_solver_solve(sdddst_ctx *ctx, double *delta_out)
     return SDDDST_OK;
 }
 
-int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks)
+int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks,
+                        int64_t *total_iters, int64_t *total_solves)
 {
     CTX_CHECK(ctx);
+    if (total_iters) *total_iters = ctx->c.total_iters;
+    if (total_solves) *total_solves = ctx->c.total_solves;
     if (last_iters) *last_iters = ctx->c.last_iters;
     if (last_rel_residual) *last_rel_residual = ctx->c.last_rel_res;
     if (fallbacks) *fallbacks = ctx->c.fallbacks;

@@ -69,6 +69,11 @@ struct Context {
     double *Ax = nullptr;               // final values
     double *AxRow = nullptr;            // CSR-of-A values (same index arrays): for SpMV
     int64_t nnz = 0, cap = 0;
+    double *jtau = nullptr, *jmult = nullptr;   // tau_x and taper per strictly-lower entry
+    int *jmirror = nullptr;                     // position of the mirror entry
+    unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
+    bool pattern_valid = false;
+    int pattern_slot = -1;                      // slot whose positions built the pattern
     bool jac_valid = false;
     double jac_h = 0;
     int *d_flags = nullptr;             // device flags: [0] zero-valued upper entries found
@@ -81,9 +86,12 @@ struct Context {
     int solver = SDDDST_SOLVER_DENSE_LU;
     double solver_tol = 1e-10;
     int solver_max_iter = 200, solver_restart = 30;
+    int precond = SDDDST_PRECOND_PAIR;
+    struct Spai *spai = nullptr;       // learned preconditioner state
     DenseLu *lu = nullptr;
     Fgmres *gm = nullptr;
     int last_iters = 0, fallbacks = 0;
+    int64_t total_iters = 0, total_solves = 0;
     double last_rel_res = 0;
     // timing: recorded event pairs per kernel kind, read back on request
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -106,7 +114,7 @@ void compute_cutoff(Context &c);
 
 // kernel launchers (return SDDDST status)
 int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol);
-int launch_jacobian(Context &c, int slot, double h);
+int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern);
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
 int launch_newton_update(Context &c, int slot);
 int launch_error_control(Context &c, double h, double hmax);

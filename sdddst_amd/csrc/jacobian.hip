@@ -14,13 +14,19 @@
 //
 // Stages (all on the context stream):
 //   count  one wavefront per column; lanes sweep the rows 64 at a time with
-//          a cheap r^2 pre-filter before the square root;
+//          a cheap r^2 pre-filter before the square root; the ballot of each
+//          sweep is the column's membership bitmask word, stored, so the
+//          predicate runs once per ordered pair;
 //   scan   one workgroup, exclusive prefix sum into Ap (int64 checked);
-//   fill   one wavefront per column; ballot + prefix popcount write the rows
-//          in ascending order and record the diagonal position;
-//   values one wavefront per column over its strictly-lower rows: the
-//          reference's value h*b_i*b_j*tau_x*taper once per unique pair, also
-//          stored at the mirror (row j of column i, found by binary search);
+//   fill   one wavefront per column, from the bitmask: prefix popcounts give
+//          each row its slot, rows land in ascending order, the diagonal
+//          position is recorded;
+//   values one wavefront per column over its strictly-lower rows: tau_x and
+//          the taper once per unique pair, kept with the position of the
+//          mirror entry (row j of column i, found by binary search);
+//   raw    h*b_i*b_j*tau_x*taper written at the entry and its mirror.  Stage
+//          II of a step assembles at the same positions as stage I with h/2:
+//          it reuses pattern, tau_x and taper and reruns raw, diag, scale;
 //   diag   one lane per column: the column sum in storage order
 //          (simulation.cpp:349-374), d_j;
 //   scale  one wavefront per column: A = row-scaled values (+1 on the
@@ -53,30 +59,36 @@ __device__ __forceinline__ bool pair_pred(double dx, double dy, const CutParams 
     return s * s < cp.thr;
 }
 
-__device__ __forceinline__ bool pred_ij(const double *__restrict__ x, const double *__restrict__ y, int i, int j,
-                                        const CutParams &cp)
-{
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    const double dx = sdfield::normalize(x[hi] - x[lo]);
-    const double dy = sdfield::normalize(y[hi] - y[lo]);
-    double r2;
-    return pair_pred(dx, dy, cp, r2);
-}
-
-__global__ __launch_bounds__(256) void k_jac_count(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                   CutParams cp, int *__restrict__ cnt)
+__global__ __launch_bounds__(256) void k_jac_count(int n, int words, const double *__restrict__ x,
+                                                   const double *__restrict__ y, CutParams cp,
+                                                   int *__restrict__ cnt, unsigned long long *__restrict__ mask)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
+    const double xj = x[j], yj = y[j];
+    unsigned long long *mj = mask + (size_t)j * words;
     int c = 0;
-    for (int i0 = 0; i0 < n; i0 += 64) {
-        const int i = i0 + lane;
-        if (i < n && i != j && pred_ij(x, y, i, j, cp)) ++c;
+    for (int w = 0; w < words; ++w) {
+        const int i = w * 64 + lane;
+        bool in = false;
+        if (i < n) {
+            if (i == j) {
+                in = true;
+            } else {
+                const double xi = x[i], yi = y[i];
+                // dx = x_hi - x_lo for the pair {i, j}
+                const double dx = sdfield::normalize(i > j ? xi - xj : xj - xi);
+                const double dy = sdfield::normalize(i > j ? yi - yj : yj - yi);
+                double r2;
+                in = pair_pred(dx, dy, cp, r2);
+            }
+        }
+        const unsigned long long m = __ballot(in);
+        if (lane == 0) mj[w] = m;
+        c += __popcll(m);
     }
-    // wave reduction of the counts
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if (lane == 0) cnt[j] = c + 1;   // + diagonal
+    if (lane == 0) cnt[j] = c;
 }
 
 // Exclusive scan of n counts into Ap[0..n] by one workgroup of 1024 lanes.
@@ -109,24 +121,36 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cn
     if (t == 0) Ap[n] = (int)(*total);
 }
 
-__global__ __launch_bounds__(256) void k_jac_fill(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                  CutParams cp, const int *__restrict__ Ap, int *__restrict__ Ai,
+__global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsigned long long *__restrict__ mask,
+                                                  const int *__restrict__ Ap, int *__restrict__ Ai,
                                                   int *__restrict__ diag)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    int pos = Ap[j];
-    for (int i0 = 0; i0 < n; i0 += 64) {
-        const int i = i0 + lane;
-        const bool in = i < n && (i == j || pred_ij(x, y, i, j, cp));
-        const unsigned long long mask = __ballot(in);
-        if (in) {
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            Ai[pos + rank] = i;
-            if (i == j) diag[j] = pos + rank;
+    const unsigned long long *mj = mask + (size_t)j * words;
+    int base = Ap[j];
+    for (int w0 = 0; w0 < words; w0 += 64) {
+        const int w = w0 + lane;
+        const unsigned long long m = w < words ? mj[w] : 0ull;
+        const int pc = __popcll(m);
+        // exclusive prefix of pc across the wave
+        int incl = pc;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
         }
-        pos += __popcll(mask);
+        int pos = base + incl - pc;
+        unsigned long long bits = m;
+        while (bits) {
+            const int bit = __ffsll((long long)bits) - 1;
+            bits &= bits - 1;
+            const int i = w * 64 + bit;
+            Ai[pos] = i;
+            if (i == j) diag[j] = pos;
+            ++pos;
+        }
+        base += __shfl(incl, 63, 64);
     }
 }
 
@@ -142,34 +166,56 @@ __device__ __forceinline__ int find_row(const int *__restrict__ Ai, int lo, int 
     return lo;
 }
 
+// tau_x and the taper of every strictly-lower entry (row i > column j) and
+// the position of its mirror (row j in column i).
 __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                    const double *__restrict__ b, CutParams cp, double h,
-                                                    const int *__restrict__ Ap, const int *__restrict__ Ai,
-                                                    const int *__restrict__ diag, double *__restrict__ Ax,
-                                                    int *__restrict__ flags)
+                                                    CutParams cp, const int *__restrict__ Ap,
+                                                    const int *__restrict__ Ai, const int *__restrict__ diag,
+                                                    double *__restrict__ tau, double *__restrict__ mult,
+                                                    int *__restrict__ mirror)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
     const int d = diag[j];
     const int end = Ap[j + 1];
-    const double xj = x[j], yj = y[j], bj = b[j];
-    if (lane == 0) Ax[d] = -0.0 * h;    // no point defects: -tmp*h with tmp = 0 (simulation.cpp:325)
+    const double xj = x[j], yj = y[j];
     for (int k = d + 1 + lane; k < end; k += 64) {
         const int i = Ai[k];
         const double dx = sdfield::normalize(x[i] - xj);
         const double dy = sdfield::normalize(y[i] - yj);
         const double r2 = dx * dx + dy * dy;
-        double mult = 1;
+        double m = 1;
         if (r2 > cp.rc2) {
             const double s = sdlibm::sqrt(r2) - cp.rc;
-            mult = sdlibm::exp(-(s * s) * cp.inv_rc2);
+            m = sdlibm::exp(-(s * s) * cp.inv_rc2);
         }
-        const double v = h * b[i] * bj * sdfield::xy_diff_x(dx, dy) * mult;
+        tau[k] = sdfield::xy_diff_x(dx, dy);
+        mult[k] = m;
+        mirror[k] = find_row(Ai, Ap[i], diag[i], j);
+    }
+}
+
+// simulation.cpp:325, 343: raw value h*b_i*b_j*tau_x*taper at the entry and
+// at its mirror (simulation.cpp:260-268); the diagonal holds -0*h (no point
+// defects).  Exact zeros are flagged: their mirrors must not exist.
+__global__ __launch_bounds__(256) void k_jac_raw(int n, const double *__restrict__ b, double h,
+                                                 const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                 const int *__restrict__ diag, const double *__restrict__ tau,
+                                                 const double *__restrict__ mult, const int *__restrict__ mirror,
+                                                 double *__restrict__ Ax, int *__restrict__ flags)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    const int d = diag[j];
+    const int end = Ap[j + 1];
+    const double bj = b[j];
+    if (lane == 0) Ax[d] = -0.0 * h;
+    for (int k = d + 1 + lane; k < end; k += 64) {
+        const double v = h * b[Ai[k]] * bj * tau[k] * mult[k];
         Ax[k] = v;
-        // mirror: row j of column i lies in its strictly-upper part
-        const int m = find_row(Ai, Ap[i], diag[i], j);
-        Ax[m] = v;
+        Ax[mirror[k]] = v;
         if (v == 0.0) atomicOr(flags, 1);
     }
 }
@@ -216,6 +262,12 @@ __global__ __launch_bounds__(256) void k_jac_scale(int n, const int *__restrict_
         Ax[k] = a;
         if (AxRow) AxRow[k] = r;
     }
+}
+
+__global__ void k_fill_int(int *p, int v, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
 }
 
 }  // namespace
@@ -275,32 +327,42 @@ static int ensure_capacity(Context &c, int64_t need)
 {
     if (need <= c.cap) return SDDDST_OK;
     int64_t ncap = need + need / 8 + 1024;
-    if (c.Ai) SD_HIP(hipFree(c.Ai));
-    if (c.Ax) SD_HIP(hipFree(c.Ax));
-    if (c.AxRow) SD_HIP(hipFree(c.AxRow));
-    c.Ai = nullptr;
-    c.Ax = nullptr;
-    c.AxRow = nullptr;
+    void **bufs[] = {(void **)&c.Ai, (void **)&c.Ax, (void **)&c.AxRow, (void **)&c.jtau, (void **)&c.jmult,
+                     (void **)&c.jmirror};
+    for (void **p : bufs) {
+        if (*p) SD_HIP(hipFree(*p));
+        *p = nullptr;
+    }
     c.cap = 0;
-    if (hipMalloc(&c.Ai, sizeof(int) * ncap) != hipSuccess || hipMalloc(&c.Ax, sizeof(double) * ncap) != hipSuccess ||
-        hipMalloc(&c.AxRow, sizeof(double) * ncap) != hipSuccess) {
-        set_error("device allocation of %lld Jacobian entries failed", (long long)ncap);
-        return SDDDST_ERR_NOMEM;
+    const size_t sizes[] = {sizeof(int), sizeof(double), sizeof(double), sizeof(double), sizeof(double), sizeof(int)};
+    for (int k = 0; k < 6; ++k) {
+        if (hipMalloc(bufs[k], sizes[k] * ncap) != hipSuccess) {
+            *bufs[k] = nullptr;
+            set_error("device allocation of %lld Jacobian entries failed", (long long)ncap);
+            return SDDDST_ERR_NOMEM;
+        }
     }
     c.cap = ncap;
     return SDDDST_OK;
 }
 
-int launch_jacobian(Context &c, int slot, double h)
+// Pattern, tau_x, taper and mirror positions for the positions in slot.
+static int build_pattern(Context &c, int slot)
 {
     const int n = c.n;
-    SD_CHECK(c.pc == 0, SDDDST_ERR_UNSUPPORTED, "Jacobian with point defects is not supported yet");
     const CutParams cp = cut_params(c);
     const double *x = c.x[slot];
     const int wblocks = (n + 3) / 4;
-    int rc;
-    if (c.timing && (rc = timer_begin(c, c.t_jac))) return rc;
-    hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.colcount);
+    const int words = (n + 63) / 64;
+    if (!c.jmask) {
+        if (hipMalloc(&c.jmask, sizeof(unsigned long long) * (size_t)n * words) != hipSuccess) {
+            c.jmask = nullptr;
+            set_error("device allocation of the %d x %d membership bitmask failed", n, words);
+            return SDDDST_ERR_NOMEM;
+        }
+    }
+    hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, words, x, c.y, cp, c.colcount,
+                       c.jmask);
     SD_HIP(hipGetLastError());
     long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, n, c.colcount, c.Ap, d_total);
@@ -309,14 +371,35 @@ int launch_jacobian(Context &c, int slot, double h)
     SD_HIP(hipMemcpyAsync(&total, d_total, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
-    rc = ensure_capacity(c, total);
+    int rc = ensure_capacity(c, total);
     if (rc) return rc;
     c.nnz = total;
-    SD_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int), c.stream));
-    hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap, c.Ai, c.diag);
+    hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag);
     SD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_jac_values, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, c.b, cp, h, c.Ap, c.Ai, c.diag,
-                       c.Ax, c.d_flags);
+    hipLaunchKernelGGL(k_jac_values, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap, c.Ai, c.diag,
+                       c.jtau, c.jmult, c.jmirror);
+    SD_HIP(hipGetLastError());
+    c.pattern_valid = true;
+    c.pattern_slot = slot;
+    return SDDDST_OK;
+}
+
+// reuse_pattern: the positions are those of the last build (stage II of a
+// step); only the h-dependent passes run.
+int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
+{
+    const int n = c.n;
+    SD_CHECK(c.pc == 0, SDDDST_ERR_UNSUPPORTED, "Jacobian with point defects is not supported yet");
+    const int wblocks = (n + 3) / 4;
+    int rc;
+    if (c.timing && (rc = timer_begin(c, c.t_jac))) return rc;
+    if (!reuse_pattern || !c.pattern_valid) {
+        rc = build_pattern(c, slot);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
+    hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag, c.jtau,
+                       c.jmult, c.jmirror, c.Ax, c.d_flags);
     SD_HIP(hipGetLastError());
     int flag = 0;
     SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -324,6 +407,7 @@ int launch_jacobian(Context &c, int slot, double h)
     if (flag) {
         rc = compact_zero_upper(c);
         if (rc) return rc;
+        c.pattern_valid = false;   // positions of the kept entries moved
     }
     const int tblocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_jac_diag, dim3(tblocks), dim3(256), 0, c.stream, n, c.Ap, c.diag, c.Ax, c.dvec);

@@ -29,6 +29,8 @@ int solver_solve(Context &c)
     if (c.solver == SDDDST_SOLVER_FGMRES) {
         SD_CHECK(c.gm, SDDDST_ERR_STATE, "solve before factor");
         int rc = c.gm->solve(c);
+        c.total_iters += c.last_iters;
+        c.total_solves += 1;
         if (rc == SDDDST_ERR_SOLVER) {
             // no convergence: fall back to the direct solver
             c.fallbacks++;

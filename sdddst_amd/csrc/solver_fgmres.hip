@@ -23,7 +23,8 @@ namespace sdamd {
 
 namespace {
 
-constexpr int kRed = 256;   // workgroups of the reductions
+constexpr int kRed = 64;    // workgroups of the reductions
+constexpr int kMaxV = 64;   // most vectors one dots launch handles (restart + 1)
 
 // y = A x, one wavefront per row.
 __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
@@ -39,33 +40,40 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap,
     if (lane == 0) y[r] = s;
 }
 
-// partial[b*(nv) + v] = sum over this block's rows of V[v][i] * w[i], v < nv
-__global__ __launch_bounds__(256) void k_proj_partial(int n, int nv, const double *__restrict__ V,
-                                                      const double *__restrict__ w, double *__restrict__ partial)
+// out[v] (+)= sum_i V[v][i] * w[i] for v < nv in one launch: every block
+// reduces its rows per v (wave shuffles, then LDS across the 4 waves) into
+// partial[block][v]; the last block to finish (atomic ticket) adds the
+// partials in block order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_dots(int n, int nv, const double *__restrict__ V,
+                                              const double *__restrict__ w, double *__restrict__ partial,
+                                              unsigned *__restrict__ ticket, double *__restrict__ out, int accumulate)
 {
-    __shared__ double s[256];
+    __shared__ double wsum[4][kMaxV];
+    __shared__ bool last;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    const int i0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     for (int v = 0; v < nv; ++v) {
         double acc = 0.0;
-        for (int i = (int)(blockIdx.x * 256 + threadIdx.x); i < n; i += kRed * 256) acc += V[(size_t)v * n + i] * w[i];
-        s[threadIdx.x] = acc;
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-            if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * nv + v] = s[0];
-        __syncthreads();
+        for (int i = i0; i < n; i += stride) acc += V[(size_t)v * n + i] * w[i];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) wsum[wave][v] = acc;
     }
-}
-
-// h[v] = sum_b partial[b][v] (fixed order), one lane per v
-__global__ void k_proj_final(int nv, const double *__restrict__ partial, double *__restrict__ h, int accumulate)
-{
-    const int v = (int)threadIdx.x;
-    if (v >= nv) return;
-    double s = 0.0;
-    for (int b = 0; b < kRed; ++b) s += partial[(size_t)b * nv + v];
-    h[v] = accumulate ? h[v] + s : s;
+    __syncthreads();
+    for (int v = (int)threadIdx.x; v < nv; v += 256)
+        partial[(size_t)blockIdx.x * nv + v] = ((wsum[0][v] + wsum[1][v]) + wsum[2][v]) + wsum[3][v];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int v = (int)threadIdx.x; v < nv; v += 256) {
+        double sum = 0.0;
+        for (int b = 0; b < (int)gridDim.x; ++b) sum += __builtin_nontemporal_load(&partial[(size_t)b * nv + v]);
+        out[v] = accumulate ? out[v] + sum : sum;
+    }
+    if (threadIdx.x == 0) *ticket = 0u;
 }
 
 // w -= sum_v h[v] V[v]   (h from the latest projection pass in hp)
@@ -97,6 +105,94 @@ __global__ void k_jacobi_setup(int n, const int *__restrict__ diag, const double
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) dinv[i] = 1.0 / Ax[diag[i]];
+}
+
+// Pair-block Jacobi.  Row i of A lives at positions Ap[i]..Ap[i+1] with
+// A_ij = AxRow[k] and A_ji = Ax[k] (column i of the CSC holds row i's
+// transpose partners).  The partner of i is the j maximising the coupling
+// |A_ij A_ji| / |A_ii A_jj|; one wavefront per row.
+__global__ __launch_bounds__(256) void k_pair_partner(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                      const int *__restrict__ diag, const double *__restrict__ Ax,
+                                                      const double *__restrict__ AxRow, int *__restrict__ partner,
+                                                      double *__restrict__ strength)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int i = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= n) return;
+    const double aii = __builtin_fabs(AxRow[diag[i]]);
+    double best = -1.0;
+    int bj = -1;
+    for (int k = Ap[i] + lane; k < Ap[i + 1]; k += 64) {
+        const int j = Ai[k];
+        if (j == i) continue;
+        const double ajj = __builtin_fabs(AxRow[diag[j]]);
+        const double s = __builtin_fabs(AxRow[k] * Ax[k]) / (aii * ajj);
+        if (s > best || (s == best && j < bj)) {
+            best = s;
+            bj = j;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int oj = __shfl_down(bj, off, 64);
+        if (ob > best || (ob == best && oj >= 0 && (bj < 0 || oj < bj))) {
+            best = ob;
+            bj = oj;
+        }
+    }
+    if (lane == 0) {
+        partner[i] = bj;
+        strength[i] = best;
+    }
+}
+
+// Mutual partners with coupling above the threshold form a 2x2 block; the
+// row of the block inverse is stored as (binv_ii, binv_ij, j), else 1/A_ii.
+__global__ __launch_bounds__(256) void k_pair_blocks(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                     const int *__restrict__ diag, const double *__restrict__ Ax,
+                                                     const double *__restrict__ AxRow, const int *__restrict__ partner,
+                                                     const double *__restrict__ strength, double threshold,
+                                                     double *__restrict__ binv, int *__restrict__ bcol)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const double aii = AxRow[diag[i]];
+    const int j = partner[i];
+    if (j >= 0 && partner[j] == i && strength[i] > threshold) {
+        // A_ij and A_ji: position of j in row i
+        int lo = Ap[i], hi = Ap[i + 1];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Ai[mid] < j) lo = mid + 1;
+            else hi = mid;
+        }
+        const double aij = AxRow[lo], aji = Ax[lo], ajj = AxRow[diag[j]];
+        const double det = aii * ajj - aij * aji;
+        binv[2 * i] = ajj / det;
+        binv[2 * i + 1] = -aij / det;
+        bcol[i] = j;
+    } else {
+        binv[2 * i] = 1.0 / aii;
+        binv[2 * i + 1] = 0.0;
+        bcol[i] = -1;
+    }
+}
+
+__global__ void k_pair_apply(int n, const double *__restrict__ binv, const int *__restrict__ bcol,
+                             const double *__restrict__ v, double *__restrict__ z)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int j = bcol[i];
+    double r = binv[2 * i] * v[i];
+    if (j >= 0) r += binv[2 * i + 1] * v[j];
+    z[i] = r;
+}
+
+// hsum[v] = (accumulate ? hsum[v] : 0) + hp[v]
+__global__ void k_add(int nv, const double *__restrict__ hp, double *__restrict__ hsum, int accumulate)
+{
+    for (int v = (int)threadIdx.x; v < nv; v += 64) hsum[v] = accumulate ? hsum[v] + hp[v] : hp[v];
 }
 
 // x += sum_j y[j] Z[j]
@@ -131,7 +227,14 @@ int Fgmres::init(Context &c)
     SD_HIP(hipMalloc(&Z, sizeof(double) * (size_t)m * n));
     SD_HIP(hipMalloc(&w, sizeof(double) * n));
     SD_HIP(hipMalloc(&dinv, sizeof(double) * n));
+    SD_HIP(hipMalloc(&binv, sizeof(double) * 2 * n));
+    SD_HIP(hipMalloc(&strength, sizeof(double) * n));
+    SD_HIP(hipMalloc(&partner, sizeof(int) * n));
+    SD_HIP(hipMalloc(&bcol, sizeof(int) * n));
+    SD_CHECK(m + 1 <= kMaxV, SDDDST_ERR_ARG, "FGMRES restart %d exceeds %d", m, kMaxV - 1);
     SD_HIP(hipMalloc(&red, sizeof(double) * (size_t)kRed * (m + 2)));
+    SD_HIP(hipMalloc(&ticket, sizeof(unsigned)));
+    SD_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), c.stream));
     SD_HIP(hipMalloc(&hcol, sizeof(double) * 2 * (m + 2)));
     SD_HIP(hipHostMalloc(&H, sizeof(double) * 2 * (m + 2), hipHostMallocDefault));
     return SDDDST_OK;
@@ -146,36 +249,58 @@ void Fgmres::release()
     if (red) (void)hipFree(red);
     if (hcol) (void)hipFree(hcol);
     if (H) (void)hipHostFree(H);
+    if (ticket) (void)hipFree(ticket);
+    for (void *p : {(void *)binv, (void *)strength, (void *)partner, (void *)bcol})
+        if (p) (void)hipFree(p);
+    binv = strength = nullptr;
+    partner = bcol = nullptr;
     V = Z = w = dinv = red = hcol = H = nullptr;
+    ticket = nullptr;
 }
 
 int Fgmres::factor(Context &c)
 {
     int rc = init(c);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
+    if (c.precond == SDDDST_PRECOND_JACOBI) {
+        hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
+    } else {
+        hipLaunchKernelGGL(k_pair_partner, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
+                           c.AxRow, partner, strength);
+        hipLaunchKernelGGL(k_pair_blocks, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
+                           c.AxRow, partner, strength, 1e-3, binv, bcol);
+    }
+    SD_HIP(hipGetLastError());
+    if (c.precond == SDDDST_PRECOND_LEARNED) return spai_factor(c);
+    return SDDDST_OK;
+}
+
+int Fgmres::apply(Context &c, const double *v, double *z)
+{
+    if (c.precond == SDDDST_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb256(n)), dim3(256), 0, c.stream, n, dinv, v, z);
+    else if (c.precond == SDDDST_PRECOND_LEARNED)
+        return spai_apply(c, v, z);
+    else
+        hipLaunchKernelGGL(k_pair_apply, dim3(nb256(n)), dim3(256), 0, c.stream, n, binv, bcol, v, z);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
-// dots of w with the first nv rows of V into hcol[0..nv) (two passes: CGS2),
-// then hcol[nv] = ||w|| after orthogonalisation; copies to host H.
+// CGS2: two passes of (h = V^T w, w -= V h) accumulated into hsum, then
+// hsum[nv] = ||w||; one copy of the column to the host.
 static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
 {
     const int n = f.n;
-    double *hp = f.hcol;             // this pass's projections
-    double *hsum = f.hcol + (f.m + 2); // accumulated
+    double *hp = f.hcol;                // this pass's projections
+    double *hsum = f.hcol + (f.m + 2);  // accumulated
     for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_proj_partial, dim3(kRed), dim3(256), 0, c.stream, n, nv, f.V, f.w, f.red);
-        hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64 * ((nv + 63) / 64)), 0, c.stream, nv, f.red, hp, 0);
+        hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, n, nv, f.V, f.w, f.red, f.ticket, hp, 0);
         hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, hp, f.w);
-        hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64 * ((nv + 63) / 64)), 0, c.stream, nv, f.red, hsum,
-                           pass);
+        hipLaunchKernelGGL(k_add, dim3(1), dim3(64), 0, c.stream, nv, hp, hsum, pass);
         SD_HIP(hipGetLastError());
     }
-    // ||w||^2
-    hipLaunchKernelGGL(k_proj_partial, dim3(kRed), dim3(256), 0, c.stream, n, 1, f.w, f.w, f.red);
-    hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64), 0, c.stream, 1, f.red, hsum + nv, 0);
+    hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, n, 1, f.w, f.w, f.red, f.ticket, hsum + nv, 0);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(hhost, hsum, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
@@ -185,8 +310,7 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
 
 static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
-    hipLaunchKernelGGL(k_proj_partial, dim3(kRed), dim3(256), 0, c.stream, f.n, 1, v, v, f.red);
-    hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(64), 0, c.stream, 1, f.red, f.hcol, 0);
+    hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, 1, v, v, f.red, f.ticket, f.hcol, 0);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(f.H, f.hcol, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
@@ -233,8 +357,8 @@ int Fgmres::solve(Context &c)
         int k = 0;
         for (; k < m && total_it < c.solver_max_iter; ++k, ++total_it) {
             // z_k = M^-1 v_k ; w = A z_k
-            hipLaunchKernelGGL(k_jacobi_apply, dim3(nb256(N)), dim3(256), 0, c.stream, N, dinv, V + (size_t)k * N,
-                               Z + (size_t)k * N);
+            rc = apply(c, V + (size_t)k * N, Z + (size_t)k * N);
+            if (rc) return rc;
             hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow,
                                Z + (size_t)k * N, w);
             SD_HIP(hipGetLastError());

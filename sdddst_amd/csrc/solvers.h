@@ -29,10 +29,19 @@ struct Fgmres {
     double *red = nullptr;   // reduction scratch
     double *hcol = nullptr;  // device copy of one Hessenberg column
     double *H = nullptr;     // host (m+1) x m Hessenberg (pinned)
+    unsigned *ticket = nullptr;  // last-block counter of the fused reductions
+    double *binv = nullptr;      // pair-block inverse rows (2 per row)
+    double *strength = nullptr;
+    int *partner = nullptr, *bcol = nullptr;
     int init(Context &c);
     int factor(Context &c);
+    int apply(Context &c, const double *v, double *z);   // z = M^-1 v
     int solve(Context &c);
     void release();
 };
+
+// Learned sparse approximate inverse (spai.hip).
+int spai_factor(Context &c);
+int spai_apply(Context &c, const double *v, double *z);
 
 }  // namespace sdamd

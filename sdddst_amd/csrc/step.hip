@@ -96,31 +96,61 @@ __global__ __launch_bounds__(1024) void k_error_control(int n, const double *__r
     }
 }
 
-// Sequential sums in index order (std::accumulate semantics).
-__global__ void k_speed_sums(int n, const double *__restrict__ v, const double *__restrict__ b,
-                             DevScalars *__restrict__ out)
+// Sequential sums in index order (std::accumulate semantics).  The
+// workgroup stages 2048 values at a time in LDS with coalesced loads; one
+// lane then runs the three dependent chains from LDS.
+constexpr int kSumChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_speed_sums(int n, const double *__restrict__ v,
+                                                    const double *__restrict__ b, DevScalars *__restrict__ out)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    __shared__ double sv[kSumChunk], sb[kSumChunk];
     double sa = 0.0, sq = 0.0, sbv = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const double vi = v[i];
-        sa = sa + __builtin_fabs(vi);
-        sq = sq + vi * vi;
-        sbv += b[i] * vi;
+    for (int base = 0; base < n; base += kSumChunk) {
+        const int cnt = min(kSumChunk, n - base);
+        for (int k = (int)threadIdx.x; k < cnt; k += 256) {
+            sv[k] = v[base + k];
+            sb[k] = b[base + k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll 8
+            for (int k = 0; k < cnt; ++k) {
+                const double vi = sv[k];
+                sa = sa + __builtin_fabs(vi);
+                sq = sq + vi * vi;
+                sbv += sb[k] * vi;
+            }
+        }
+        __syncthreads();
     }
-    out->sum_abs = sa;
-    out->sum_sq = sq;
-    out->sum_bv = sbv;
+    if (threadIdx.x == 0) {
+        out->sum_abs = sa;
+        out->sum_sq = sq;
+        out->sum_bv = sbv;
+    }
 }
 
-// utility.h:92-100
-__global__ void k_strain(int n, const double *__restrict__ xo, const double *__restrict__ xn,
-                         const double *__restrict__ b, DevScalars *__restrict__ out)
+// utility.h:92-100, staged the same way.
+__global__ __launch_bounds__(256) void k_strain(int n, const double *__restrict__ xo, const double *__restrict__ xn,
+                                                const double *__restrict__ b, DevScalars *__restrict__ out)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    __shared__ double sd[kSumChunk], sb[kSumChunk];
     double r = 0;
-    for (int i = 0; i < n; ++i) r += b[i] * (xn[i] - xo[i]);
-    out->strain = r;
+    for (int base = 0; base < n; base += kSumChunk) {
+        const int cnt = min(kSumChunk, n - base);
+        for (int k = (int)threadIdx.x; k < cnt; k += 256) {
+            sd[k] = xn[base + k] - xo[base + k];
+            sb[k] = b[base + k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll 8
+            for (int k = 0; k < cnt; ++k) r += sb[k] * sd[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out->strain = r;
 }
 
 __global__ void k_fill(double *p, double v, int n)
@@ -159,14 +189,14 @@ int launch_error_control(Context &c, double h, double hmax)
 
 int launch_speed_sums(Context &c, int buf)
 {
-    hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(64), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
+    hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(256), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
 int launch_strain(Context &c, int old_slot, int new_slot)
 {
-    hipLaunchKernelGGL(k_strain, dim3(1), dim3(64), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
+    hipLaunchKernelGGL(k_strain, dim3(1), dim3(256), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }

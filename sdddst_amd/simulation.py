@@ -18,6 +18,7 @@ CORE_PATH = os.path.join(_HERE, "libsdddstCore.so")
 PROTOCOLS = {"zero": 0, "fixed-rate": 1, "spring": 2, "constant": 3}
 SOLVERS = {"auto": 0, "umfpack": 1, "dense-lu": 2, "fgmres": 3}
 FIELD_MODES = {"fast": 0, "exact": 1, "ordered": 2}
+PRECONDITIONERS = {"jacobi": 0, "pair": 1, "learned": 2}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -38,6 +39,7 @@ class SimOptions(ctypes.Structure):
         ("solver_max_iter", ctypes.c_int),
         ("solver_restart", ctypes.c_int),
         ("field_mode", ctypes.c_int),
+        ("preconditioner", ctypes.c_int),
         ("A", ctypes.c_double),
         ("KASQR", ctypes.c_double),
         ("log_path", ctypes.c_char_p),
@@ -60,6 +62,8 @@ class SimState(ctypes.Structure):
         ("solver_last_iters", ctypes.c_int),
         ("solver_fallbacks", ctypes.c_int),
         ("solver_last_rel_residual", ctypes.c_double),
+        ("solver_total_iters", ctypes.c_longlong),
+        ("solver_total_solves", ctypes.c_longlong),
     ]
 
 
@@ -123,6 +127,8 @@ class StepState:
     solver_last_iters: int
     solver_fallbacks: int
     solver_last_rel_residual: float
+    solver_total_iters: int
+    solver_total_solves: int
 
 
 class Simulation:
@@ -131,7 +137,7 @@ class Simulation:
     def __init__(self, x, y, b, *, cutoff_multiplier=1.0, precision=1e-6, initial_step=1e-6, max_step=None,
                  stress="zero", stress_value=0.0, spring_constant=1.0, calculate_strain=True,
                  order_parameter=False, solver="auto", solver_tol=1e-10, solver_max_iter=300, solver_restart=40,
-                 field_mode="fast", px=None, py=None, A=1e-4 * 16.0, KASQR=1.65 * 1.65 * 1e6 / 256.0,
+                 field_mode="fast", preconditioner="pair", px=None, py=None, A=1e-4 * 16.0, KASQR=1.65 * 1.65 * 1e6 / 256.0,
                  log_path: Optional[str] = None):
         self.lib = load_core()
         o = default_options()
@@ -149,6 +155,7 @@ class Simulation:
         o.solver_max_iter = solver_max_iter
         o.solver_restart = solver_restart
         o.field_mode = FIELD_MODES[field_mode]
+        o.preconditioner = PRECONDITIONERS[preconditioner]
         o.A = A
         o.KASQR = KASQR
         self._log = log_path.encode() if log_path else None
@@ -187,7 +194,8 @@ class Simulation:
         self._check(self.lib.sdddst_sim_get_state(self.h, ctypes.byref(s)))
         return StepState(s.sim_time, s.step_size, s.strain, s.energy, s.max_error_ratio_sqr, s.sum_avg_speed,
                          s.cutoff, s.succesful_steps, s.failed_steps, bool(s.last_step_succeeded),
-                         s.last_step_wall_s, s.solver_last_iters, s.solver_fallbacks, s.solver_last_rel_residual)
+                         s.last_step_wall_s, s.solver_last_iters, s.solver_fallbacks, s.solver_last_rel_residual,
+                         s.solver_total_iters, s.solver_total_solves)
 
     def get_time(self) -> float:
         return self.state().sim_time

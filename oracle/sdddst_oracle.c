@@ -388,6 +388,12 @@ static double pd_diag_term(double dx, double dy, double bj, double A, double K, 
     return bj * (t1 + t2 + t3) * multiplier;
 }
 
+/* ctypes entry: the point-defect term of simulation.cpp:288-322 for b, taper. */
+double orc_pd_diag_term(double dx, double dy, double bj, double A, double K, double multiplier)
+{
+    return pd_diag_term(dx, dy, bj, A, K, multiplier);
+}
+
 /* Pass 1 of calculateJacobian (simulation.cpp:240-347).  Fills Ap, Ai, Ax
  * (raw values, diag = -pd*h).  Returns nnz, or -1 if cap is exceeded. */
 static long jac_pass1(const orc_sys *s, const orc_cut *c, double stepsize, long cap, int *Ap, int *Ai, double *Ax)
@@ -915,7 +921,8 @@ void orc_sim_get_speeds(const orc_sim *s, double *initSpeed, double *initSpeed2)
 /* ---------------------------------------------------------------------- */
 
 /* Host C library over an array: fn 0 cos, 1 sin, 2 exp, 3 expm1, 4 cosh,
- * 5 sinh, 6 sqrt, 7 pow(x, 1./3.), 8 normalize. */
+ * 5 sinh, 6 sqrt, 7 pow(x, 1./3.), 8 normalize, 9 pow(M_E, x),
+ * 10 pow(x, -2). */
 void orc_libm_batch(int fn, long n, const double *x, double *out)
 {
     for (long i = 0; i < n; ++i) {
@@ -930,6 +937,8 @@ void orc_libm_batch(int fn, long n, const double *x, double *out)
         case 6: r = sqrt(v); break;
         case 7: r = pow(v, 1. / 3.); break;
         case 8: r = orc_normalize(v); break;
+        case 9: r = pow(M_E, v); break;
+        case 10: r = pow(v, -0.2e1); break;
         default: r = 0; break;
         }
         out[i] = r;

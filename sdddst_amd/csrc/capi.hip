@@ -55,7 +55,7 @@ static void free_all(Context &c)
     for (int s = 0; s < SDDDST_NUM_VBUFS; ++s)
         if (c.v[s]) (void)hipFree(c.v[s]);
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
-                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask};
+                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
@@ -106,7 +106,7 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
         (rc = dalloc(&c.tol, n)) || (rc = dalloc(&c.minr2, n)) || (rc = dalloc(&c.partial, (size_t)c.nb * n)) ||
         (rc = dalloc(&c.dvec, n)) || (rc = dalloc(&c.g, n)) || (rc = dalloc(&c.delta, n)) ||
         (rc = dalloc(&c.Ap, (size_t)n + 1)) || (rc = dalloc(&c.diag, n)) || (rc = dalloc(&c.colcount, n)) ||
-        (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, 1)))
+        (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, 1)) || (rc = dalloc(&c.jpd, n)))
         return rc;
     SD_HIP(hipHostMalloc((void **)&c.hs, sizeof(DevScalars), hipHostMallocDefault));
     SD_HIP(hipMemsetAsync(c.dvec, 0, sizeof(double) * n, c.stream));

@@ -71,6 +71,7 @@ struct Context {
     int64_t nnz = 0, cap = 0;
     double *jtau = nullptr, *jmult = nullptr;   // tau_x and taper per strictly-lower entry
     int *jmirror = nullptr;                     // position of the mirror entry
+    double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
     bool pattern_valid = false;
     int pattern_slot = -1;                      // slot whose positions built the pattern

@@ -172,4 +172,25 @@ SDLIBM_FN double xy_fast(double dx, double dy)
            f_given_cosh(dx + 4.0 * s, c4p, cy) + g;
 }
 
+// simulation.cpp:288-322: x-derivative of the point-defect force on a
+// dislocation at (dx, dy) from a defect (the Maple expression, grouped as
+// three terms that keep its evaluation order), without b and the taper.
+SDLIBM_FN double pd_force_dx(double dx, double dy, double A, double K)
+{
+    const double Pm2 = 0x1.9f02f6222c720p-4;   // pow(M_PI, -2) of the double M_PI, rounded
+    const double Pm3 = 0x1.08345eb45d77fp-5;   // pow(M_PI, -3) of the double M_PI, rounded
+    const double cx = sdlibm::cos(0.2e1 * kPi * dx), cyy = sdlibm::cos(0.2e1 * kPi * dy);
+    const double sx = sdlibm::sin(0.2e1 * kPi * dx), sy = sdlibm::sin(0.2e1 * kPi * dy);
+    const double R = (0.1e1 - cx) * Pm2 / 0.2e1 + (0.1e1 - cyy) * Pm2 / 0.2e1;
+    const double E = sdlibm::pow_e(-K * R);
+    const double logMe = 1.0;                   // log(M_E) rounds to 1
+    const double t1 = -A * cx / kPi * sy * ((0.1e1 - E) / R - K * E) / R;
+    const double t2 = -A * sx * Pm2 * sy *
+                      (E * K * sx / kPi * logMe / R - (0.1e1 - E) * sdlibm::pow_m2(R) * sx / kPi +
+                       K * K * E * sx / kPi * logMe) /
+                      R / 0.2e1;
+    const double t3 = A * (sx * sx) * Pm3 * sy * ((0.1e1 - E) / R - K * E) * sdlibm::pow_m2(R) / 0.2e1;
+    return t1 + t2 + t3;
+}
+
 }  // namespace sdfield

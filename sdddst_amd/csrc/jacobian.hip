@@ -196,14 +196,42 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
     }
 }
 
+// simulation.cpp:269-324: the point-defect sum tmp of column j (defects in
+// ascending order, tmp -= b_j * dF/dx * taper), h-independent.
+__global__ void k_jac_pd(int n, int pc, const double *__restrict__ x, const double *__restrict__ y,
+                         const double *__restrict__ b, const double *__restrict__ px,
+                         const double *__restrict__ py, double A, double K, CutParams cp,
+                         double *__restrict__ pdtmp)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= n) return;
+    const double xj = x[j], yj = y[j], bj = b[j];
+    double tmp = 0;
+    for (int l = 0; l < pc; ++l) {
+        const double dx = sdfield::normalize(xj - px[l]);
+        const double dy = sdfield::normalize(yj - py[l]);
+        double r2;
+        if (pair_pred(dx, dy, cp, r2)) {
+            double multiplier = 1;
+            if (r2 > cp.rc2) {
+                const double s = sdlibm::sqrt(r2) - cp.rc;
+                multiplier = sdlibm::exp(-(s * s) * cp.inv_rc2);
+            }
+            tmp -= bj * sdfield::pd_force_dx(dx, dy, A, K) * multiplier;
+        }
+    }
+    pdtmp[j] = tmp;
+}
+
 // simulation.cpp:325, 343: raw value h*b_i*b_j*tau_x*taper at the entry and
-// at its mirror (simulation.cpp:260-268); the diagonal holds -0*h (no point
-// defects).  Exact zeros are flagged: their mirrors must not exist.
+// at its mirror (simulation.cpp:260-268); the diagonal holds -tmp*h.  Exact
+// zeros are flagged: their mirrors must not exist.
 __global__ __launch_bounds__(256) void k_jac_raw(int n, const double *__restrict__ b, double h,
                                                  const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                  const int *__restrict__ diag, const double *__restrict__ tau,
                                                  const double *__restrict__ mult, const int *__restrict__ mirror,
-                                                 double *__restrict__ Ax, int *__restrict__ flags)
+                                                 const double *__restrict__ pdtmp, double *__restrict__ Ax,
+                                                 int *__restrict__ flags)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -211,7 +239,7 @@ __global__ __launch_bounds__(256) void k_jac_raw(int n, const double *__restrict
     const int d = diag[j];
     const int end = Ap[j + 1];
     const double bj = b[j];
-    if (lane == 0) Ax[d] = -0.0 * h;
+    if (lane == 0) Ax[d] = -(pdtmp ? pdtmp[j] : 0.0) * h;
     for (int k = d + 1 + lane; k < end; k += 64) {
         const double v = h * b[Ai[k]] * bj * tau[k] * mult[k];
         Ax[k] = v;
@@ -379,6 +407,11 @@ static int build_pattern(Context &c, int slot)
     hipLaunchKernelGGL(k_jac_values, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap, c.Ai, c.diag,
                        c.jtau, c.jmult, c.jmirror);
     SD_HIP(hipGetLastError());
+    if (c.pc > 0) {
+        hipLaunchKernelGGL(k_jac_pd, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.pc, x, c.y, c.b, c.px,
+                           c.py, c.p.A, c.p.KASQR, cp, c.jpd);
+        SD_HIP(hipGetLastError());
+    }
     c.pattern_valid = true;
     c.pattern_slot = slot;
     return SDDDST_OK;
@@ -389,7 +422,6 @@ static int build_pattern(Context &c, int slot)
 int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
 {
     const int n = c.n;
-    SD_CHECK(c.pc == 0, SDDDST_ERR_UNSUPPORTED, "Jacobian with point defects is not supported yet");
     const int wblocks = (n + 3) / 4;
     int rc;
     if (c.timing && (rc = timer_begin(c, c.t_jac))) return rc;
@@ -399,7 +431,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     }
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
     hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag, c.jtau,
-                       c.jmult, c.jmirror, c.Ax, c.d_flags);
+                       c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Ax, c.d_flags);
     SD_HIP(hipGetLastError());
     int flag = 0;
     SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -119,30 +119,82 @@ SDLIBM_FN double expm1(double x)
 }
 
 // ------------------------------------------------------------------ exp --
-// Valid for |x| < 512 (the field needs |x| < 40); larger |x| saturates.
-SDLIBM_FN double exp(double x)
+namespace detail {
+// |x| in [512, 1024): the scale's exponent is out of range; shift it and
+// rescale (glibc's specialcase; unlike the main path, not fused).
+SDLIBM_FN double exp_special(double tmp, uint64_t sbits, uint64_t ki)
+{
+    if ((ki & 0x80000000u) == 0) {
+        sbits -= 1009ull << 52;
+        const double scale = asf64(sbits);
+        return 0x1p1009 * fmad(scale, tmp, scale);
+    }
+    sbits += 1022ull << 52;
+    const double scale = asf64(sbits);
+    double y = scale + scale * tmp;
+    if (y < 1.0) {
+        double lo = scale - y + scale * tmp;
+        const double hi = 1.0 + y;
+        lo = 1.0 - hi + y + lo;
+        y = (hi + lo) - 1.0;
+        if (y == 0.0) y = 0.0;
+    }
+    return 0x1p-1022 * y;
+}
+}  // namespace detail
+
+// exp(x + xt) for a tail |xt| << ulp(x)-scale: the tail joins the reduced
+// argument.  exp(x) is exp_tail(x, 0.0), bit for bit.
+SDLIBM_FN double exp_tail(double x, double xt)
 {
     const double InvLn2N = 0x1.71547652b82fep0 * 128, NegLn2hiN = -0x1.62e42fefa0000p-8,
                  NegLn2loN = -0x1.cf79abc9e3b3ap-47, Shift = 0x1.8p52;
     const double C2 = 0x1.ffffffffffdbdp-2, C3 = 0x1.555555555543cp-3, C4 = 0x1.55555cf172b91p-5,
                  C5 = 0x1.1111167a4d017p-7;
-    const uint32_t abstop = (uint32_t)(asu64(x) >> 52) & 0x7ff;
+    uint32_t abstop = (uint32_t)(asu64(x) >> 52) & 0x7ff;
     if (abstop - 0x3c9u >= 0x408u - 0x3c9u) {      // |x| < 2^-54 or |x| >= 512
         if (abstop - 0x3c9u >= 0x80000000u) return 1.0 + x;
-        return x > 0 ? __builtin_inf() : 0.0;
+        if (abstop >= 0x409u) {                    // |x| >= 1024
+            if (asu64(x) == asu64(-__builtin_inf())) return 0.0;
+            if (abstop >= 0x7ffu) return 1.0 + x;
+            return (asu64(x) >> 63) ? 0.0 : __builtin_inf();
+        }
+        abstop = 0;                                 // large |x|: special case below
     }
     double kd = fmad(InvLn2N, x, Shift);
     const uint64_t ki = asu64(kd);
     kd -= Shift;
-    const double r = fmad(kd, NegLn2loN, fmad(kd, NegLn2hiN, x));
+    const double r = fmad(kd, NegLn2loN, fmad(kd, NegLn2hiN, x)) + xt;
     const uint64_t idx = 2 * (ki % 128);
     const uint64_t top = ki << 45;
     const double tail = asf64(sdlibm_exp2_tab[idx]);
     const uint64_t sbits = sdlibm_exp2_tab[idx + 1] + top;
     const double r2 = r * r;
     const double tmp = fmad(r2 * r2, fmad(r, C5, C4), fmad(r2, fmad(r, C3, C2), tail + r));
+    if (abstop == 0) return detail::exp_special(tmp, sbits, ki);
     const double scale = asf64(sbits);
     return fmad(scale, tmp, scale);
+}
+
+SDLIBM_FN double exp(double x) { return exp_tail(x, 0.0); }
+
+// pow(M_E, y) with M_E rounded to double: exp(y * ln(M_E)) with
+// ln(M_E) = 1 + c, c = -5.318e-17, i.e. exp of y plus the tail y*c.
+SDLIBM_FN double pow_e(double y)
+{
+    const double c = -0x1.ea8556644e4cdp-55;
+    return exp_tail(y, y * c);
+}
+
+// pow(x, -2.0) = 1 / x^2 with x^2 kept in double-double, rounded once.
+SDLIBM_FN double pow_m2(double x)
+{
+    const double h = x * x;
+    const double l = fmad(x, x, -h);
+    const double q = 1.0 / h;
+    // 1/(h + l) = q * (1 - (h q - 1) - l q) to first order
+    const double e = fmad(h, q, -1.0);
+    return fmad(q, -(e + l * q), q);
 }
 
 // ----------------------------------------------------------- cosh, sinh --

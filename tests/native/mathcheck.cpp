@@ -16,6 +16,7 @@ double mc_normalize(double d) { return sdfield::normalize(d); }
 double mc_xy(double dx, double dy) { return sdfield::xy(dx, dy); }
 double mc_xy_fast(double dx, double dy) { return sdfield::xy_fast(dx, dy); }
 double mc_xy_diff_x(double dx, double dy) { return sdfield::xy_diff_x(dx, dy); }
+double mc_pd_force_dx(double dx, double dy, double A, double K) { return sdfield::pd_force_dx(dx, dy, A, K); }
 
 // Batched forms: fn = 0 cos, 1 sin, 2 exp, 3 expm1, 4 cosh, 5 sinh, 6 normalize
 void mc_batch1(int fn, long n, const double *x, double *out)
@@ -30,6 +31,8 @@ void mc_batch1(int fn, long n, const double *x, double *out)
         case 4: r = sdlibm::cosh(v); break;
         case 5: r = sdlibm::sinh(v); break;
         case 6: r = sdfield::normalize(v); break;
+        case 9: r = sdlibm::pow_e(v); break;
+        case 10: r = sdlibm::pow_m2(v); break;
         }
         out[i] = r;
     }

@@ -80,6 +80,7 @@ def oracle():
         "orc_libm_batch": (None, [ctypes.c_int, L, DP, DP]),
         "orc_xy_batch": (None, [L, DP, DP, DP, DP]),
         "orc_speeds_abs": (None, [ctypes.c_int, DP, DP, DP, DP]),
+        "orc_pd_diag_term": (D, [D, D, D, D, D, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -183,8 +184,9 @@ def speed_parity(v, vr, S, n):
     return rel_ok, rel_ok | cond_ok
 
 
-def jacobian(x, y, b, cutoff_multiplier, h, cap=None):
+def jacobian(x, y, b, cutoff_multiplier, h, cap=None, px=None, py=None, A=0.0, K=0.0):
     n = len(x)
+    pc = 0 if px is None else len(px)
     cap = cap or min(n * n, 400 * n + 4096)
     while True:
         Ap = np.zeros(n + 1, dtype=np.int32)
@@ -193,7 +195,7 @@ def jacobian(x, y, b, cutoff_multiplier, h, cap=None):
         dv = np.zeros(n)
         idx = np.zeros(n, dtype=np.int32)
         raw = np.zeros(cap)
-        nnz = oracle().orc_jacobian(n, dp(x), dp(y), dp(b), 0, None, None, 0.0, 0.0, cutoff_multiplier, h, cap,
+        nnz = oracle().orc_jacobian(n, dp(x), dp(y), dp(b), pc, dp(px), dp(py), A, K, cutoff_multiplier, h, cap,
                                     ip(Ap), ip(Ai), dp(Ax), dp(dv), ip(idx), dp(raw))
         if nnz >= 0:
             return Ap, Ai[:nnz].copy(), Ax[:nnz].copy(), dv, idx, raw[:nnz].copy()

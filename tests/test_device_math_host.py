@@ -87,3 +87,21 @@ def test_integer_powers_correctly_rounded():
     for v in rng.uniform(-7e-3, 7e-3, 300):
         for n in (3, 4, 5, 6, 7):
             assert mc.mc_powi(float(v), n) == float(mpmath.mpf(float(v)) ** n)
+
+
+def test_point_defect_term_host_build():
+    mc = O.mathcheck()
+    mc.mc_pd_force_dx.restype = __import__("ctypes").c_double
+    mc.mc_pd_force_dx.argtypes = [__import__("ctypes").c_double] * 4
+    rng = np.random.default_rng(6)
+    n = 65536
+    A, K = 1e-4 * 16.0 / np.sqrt(n), 1.65 * 1.65 * 1e6 / 256.0 * n
+    worst = 0.0
+    same = 0
+    for _ in range(3000):
+        dx, dy = rng.uniform(-0.02, 0.02, 2)
+        a = mc.mc_pd_force_dx(dx, dy, A, K)
+        r = O.oracle().orc_pd_diag_term(dx, dy, 1.0, A, K, 1.0)
+        same += a == r
+        worst = max(worst, abs(a - r) / max(abs(r), 1e-300))
+    assert worst < 1e-14 and same > 0.97 * 3000

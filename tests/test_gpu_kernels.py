@@ -162,3 +162,44 @@ def test_jacobian_pattern_bit_exact(n, m, seed):
     assert np.mean(Ax.view(np.int64) == rAx.view(np.int64)) > 0.995
     assert rel.max() < 1e-12
     assert np.max(np.abs(dv - rdv)) < 1e-13
+
+
+# -- point defects (config-5 style; A and KASQR scaled as the CLI does) --------
+
+def _pd_setup(n, pc, seed):
+    x, y, b = configs.dislocations(n, seed)
+    px, py = configs.point_defects(n, pc, seed)
+    A, K = 1e-4 * 16.0 / np.sqrt(n), 1.65 * 1.65 * 1e6 / 256.0 * n
+    return x, y, b, px, py, A, K
+
+
+@pytest.mark.parametrize("mode", [S.FIELD_ORDERED, S.FIELD_EXACT])
+def test_speeds_with_point_defects(mode):
+    n, pc = 512, 96
+    x, y, b, px, py, A, K = _pd_setup(n, pc, 50)
+    with S.Context(n, pc=pc, field_mode=mode, A=A, KASQR=K) as c:
+        c.upload_state(0, x, y, b)
+        c.upload_points(px, py)
+        v = c.speeds(0, S.V_INIT, 0.01)
+        tol = c.tolerance()
+    vr, tr = O.speeds(x, y, b, sigma=0.01, px=px, py=py, A=A, K=K)
+    assert np.array_equal(tol.view(np.int64), tr.view(np.int64))
+    if mode == S.FIELD_ORDERED:
+        assert np.array_equal(v.view(np.int64), vr.view(np.int64))
+    else:
+        strict, ok = O.speed_parity(v, vr, O.speeds_abs(x, y, b), n)
+        assert ok.all() and strict.mean() >= 0.995
+
+
+def test_jacobian_with_point_defects():
+    n, pc, h = 400, 64, 1e-4
+    x, y, b, px, py, A, K = _pd_setup(n, pc, 51)
+    with S.Context(n, pc=pc, cutoff_multiplier=2.0, A=A, KASQR=K) as c:
+        c.upload_state(0, x, y, b)
+        c.upload_points(px, py)
+        Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)
+    rAp, rAi, rAx, rdv, ridx, raw = O.jacobian(x, y, b, 2.0, h, px=px, py=py, A=A, K=K)
+    assert np.array_equal(Ap, rAp) and np.array_equal(Ai, rAi) and np.array_equal(dg, ridx)
+    rel = np.abs(Ax - rAx) / np.maximum(np.abs(rAx), 1e-300)
+    assert rel.max() < 1e-11
+    assert np.max(np.abs(dv - rdv)) < 1e-12

@@ -267,3 +267,22 @@ def test_fgmres_preconditioners(precond):
         assert fb == 0 and rr <= 1e-10
     assert np.linalg.norm(d - d_lu) <= 1e-6 * np.linalg.norm(d_lu)
     c.close()
+
+
+def test_point_defects_fixed_rate_run(tmp_path):
+    # config-5 style at small N: point defects, fixed-rate stress ramp
+    n, pc = 256, 48
+    x, y, b = configs.dislocations(n, 61)
+    px, py = configs.point_defects(n, pc, 61)
+    A, K = 1e-4 * 16.0 / np.sqrt(n), 1.65 * 1.65 * 1e6 / 256.0 * n
+    o = O.OracleSim(x, y, b, cutoff_multiplier=2.0, prec=1e-6, h0=1e-6, stress="fixed-rate", p1=50.0,
+                    calculate_strain=True, px=px, py=py, A=A, K=K)
+    g = Simulation(x, y, b, cutoff_multiplier=2.0, precision=1e-6, initial_step=1e-6, stress="fixed-rate",
+                   stress_value=50.0, calculate_strain=True, solver="dense-lu", field_mode="exact", px=px, py=py,
+                   A=A, KASQR=K)
+    for _ in range(6):
+        assert o.step() == g.step()
+    assert abs(g.get_time() - o.time) <= 1e-12 * abs(o.time)
+    assert np.max(np.abs(g.positions() - o.x())) < 1e-9
+    o.close()
+    g.close()

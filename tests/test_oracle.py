@@ -337,3 +337,38 @@ def test_config_generator():
     assert len(sizes) == 256 and min(sizes) >= 64 and max(sizes) <= 2048
     # SplitMix64 known answer for seed 0 (first output)
     assert int(configs.splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
+
+
+# -- point defects (simulation.cpp:111-128, 269-325) ---------------------------
+
+def _pd_speed_mp(dx, dy, A, K):
+    X = lambda t: mpmath.sin(2 * mpmath.pi * t) * mpmath.mpf("0.5") / mpmath.pi  # noqa: E731
+    X2 = lambda t: (1 - mpmath.cos(2 * mpmath.pi * t)) * mpmath.mpf("0.5") / mpmath.pi ** 2  # noqa: E731
+    R = X2(dx) + X2(dy)
+    E = mpmath.exp(-K * R)
+    return -2 * A * X(dx) * X(dy) * ((1 - E) / R - K * E) / R
+
+
+@pytest.mark.parametrize("dx,dy", [(0.003, 0.002), (-0.01, 0.004), (0.001, -0.0015), (0.02, 0.01)])
+def test_point_defect_jacobian_term_is_force_derivative(dx, dy):
+    # the Maple expression of the Jacobian diagonal is d/dx of the point-defect
+    # speed term (per unit Burgers vector, no taper)
+    n = 4096
+    A, K = 1e-4 * 16.0 / np.sqrt(n), 1.65 * 1.65 * 1e6 / 256.0 * n
+    d = float(mpmath.diff(lambda t: _pd_speed_mp(t, mpmath.mpf(dy), A, K), mpmath.mpf(dx)))
+    v = O.oracle().orc_pd_diag_term(dx, dy, 1.0, A, K, 1.0)
+    assert abs(v - d) <= 1e-9 * max(abs(d), 1e-12)
+
+
+def test_point_defect_speed_term():
+    # speeds with defects minus speeds without = the defect force, summed
+    n, pc = 100, 40
+    x, y, b = configs.dislocations(n, 41)
+    px, py = configs.point_defects(n, pc, 41)
+    A, K = 1e-4 * 16.0 / np.sqrt(n), 1.65 * 1.65 * 1e6 / 256.0 * n
+    v1, _ = O.speeds(x, y, b, px=px, py=py, A=A, K=K)
+    v0, _ = O.speeds(x, y, b)
+    for i in range(0, n, 9):
+        f = sum(float(_pd_speed_mp(mpmath.mpf(O.oracle().orc_normalize(x[i] - px[l])),
+                                   mpmath.mpf(O.oracle().orc_normalize(y[i] - py[l])), A, K)) for l in range(pc))
+        assert abs((v1[i] - v0[i]) - f * b[i]) <= 1e-9 * (abs(f) + 1e-6)

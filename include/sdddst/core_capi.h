@@ -44,6 +44,9 @@ typedef struct sdddst_sim_options {
     int preconditioner;        /* SDDDST_PRECOND_* of sdddst_amd.h        */
     double A, KASQR;           /* point-defect constants (as used)         */
     const char *log_path;      /* 12-column log file; NULL: none           */
+    const char *spai_weights;  /* learned-preconditioner weights (raw
+                                  float32); NULL: data/spai_gnn_h32.f32
+                                  beside libsdddstCore.so                  */
 } sdddst_sim_options;
 
 /* Scalars of the state after the last step. */
@@ -55,6 +58,7 @@ typedef struct sdddst_sim_state {
     int solver_last_iters, solver_fallbacks;
     double solver_last_rel_residual;
     long long solver_total_iters, solver_total_solves;
+    int solver_precond_fallbacks;  /* learned solves retried with pair blocks */
 } sdddst_sim_state;
 
 void sdddst_sim_default_options(sdddst_sim_options *o);

@@ -71,6 +71,7 @@ private:
     void syncPositionsToHost(int slot, std::vector<Dislocation> &out);
     void uploadPositions(int slot, const std::vector<Dislocation> &in);
     void pushParams();
+    void loadSpaiWeights();
     void check(int status, const char *what);
     void writeLogLine(const std::string &line);
 

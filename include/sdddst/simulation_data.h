@@ -125,6 +125,7 @@ public:
     int solverMaxIterations;
     int solverRestart;
     int preconditioner;           // SDDDST_PRECOND_* of include/sdddst_amd.h
+    std::string spaiWeightsPath;  // learned preconditioner weights; empty: the shipped file
     int fieldMode;                // SDDDST_FIELD_* of include/sdddst_amd.h
 
 private:

@@ -145,6 +145,37 @@ int sdddst_solver_factor(sdddst_ctx *ctx);
 int sdddst_solver_solve(sdddst_ctx *ctx, double *delta_out);
 int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_residual, int *fallbacks,
                         int64_t *total_iters, int64_t *total_solves);
+/* Learned-preconditioned solves that did not converge within one restart
+ * cycle and were solved again with the pair-block preconditioner (no
+ * counterpart in the reference). */
+int sdddst_solver_precond_fallbacks(sdddst_ctx *ctx, int *count);
+
+/* ---- Learned preconditioner (SDDDST_PRECOND_LEARNED) ------------------
+ * No counterpart in the reference, which solves A delta = g with UMFPACK
+ * and no preconditioner (simulation.cpp:386-396).  M = B^-1 + C: B^-1 the
+ * pair-block inverse, C on the within-cutoff core of A's pattern with
+ * values from a two-layer message-passing network (hidden width 32, FP32
+ * weights; sdddst_spai_param_count floats).
+ *
+ * set/get_weights: install or read the network (needed before a factor
+ *   with the learned preconditioner).
+ * train_init: weights from the seeded initialiser, Adam state cleared.
+ * train_step: on the current Jacobian, K = probes Rademacher probes from
+ *   seed: loss L = mean_i log(1 + q_i/eps), q_i = mean_k (A M w_k - w_k)_i^2,
+ *   its gradient (grad: sdddst_spai_param_count doubles, nullable) and,
+ *   when lr > 0, one Adam step (beta 0.9 / 0.999) of learning rate lr.
+ * graph: two-phase download of the last evaluated core graph (Cp[n+1],
+ *   Ci[ne]), C's values (Ce[ne], Cd[n]), the network outputs gamma_e[ne],
+ *   gamma_d[n] and the pair-block inverse (bd, bo, bcol); any pointer may be
+ *   NULL, n_edges receives ne. */
+int sdddst_spai_param_count(void);
+int sdddst_spai_set_weights(sdddst_ctx *ctx, const float *w, int count);
+int sdddst_spai_get_weights(sdddst_ctx *ctx, float *w, int count);
+int sdddst_spai_train_init(sdddst_ctx *ctx, uint64_t seed);
+int sdddst_spai_train_step(sdddst_ctx *ctx, int probes, uint64_t seed, double eps, double lr, double *loss,
+                           double *grad);
+int sdddst_spai_graph(sdddst_ctx *ctx, int64_t *n_edges, int *Cp, int *Ci, double *Ce, double *Cd, float *gamma_e,
+                      float *gamma_d, double *bd, double *bo, int *bcol);
 
 /* Newton update x[slot] -= delta (simulation.cpp:78-81); delta_host NULL
  * uses the device delta of the last solve. */

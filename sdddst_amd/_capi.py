@@ -26,6 +26,7 @@ PRECOND_JACOBI, PRECOND_PAIR, PRECOND_LEARNED = 0, 1, 2
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_fp = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -76,6 +77,16 @@ _SIGNATURES = {
     "sdddst_solver_factor": (ctypes.c_int, [_vp]),
     "sdddst_solver_solve": (ctypes.c_int, [_vp, _dp]),
     "sdddst_solver_stats": (ctypes.c_int, [_vp, _ip, _dp, _ip, _i64p, _i64p]),
+    "sdddst_solver_precond_fallbacks": (ctypes.c_int, [_vp, _ip]),
+    "sdddst_spai_param_count": (ctypes.c_int, []),
+    "sdddst_spai_set_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
+    "sdddst_spai_get_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
+    "sdddst_spai_train_init": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "sdddst_spai_train_step": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, _dp, _dp],
+    ),
+    "sdddst_spai_graph": (ctypes.c_int, [_vp, _i64p, _ip, _ip, _dp, _dp, _fp, _fp, _dp, _dp, _ip]),
     "sdddst_newton_update": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "sdddst_error_control": (
         ctypes.c_int,
@@ -124,6 +135,10 @@ def _iptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(_ip)
 
 
+def _fptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
 def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -148,9 +163,22 @@ class Context:
             msg = self.lib.sdddst_last_error().decode(errors="replace")
             raise SdddstError(f"status {rc}: {msg}")
 
+    @classmethod
+    def borrow(cls, handle, n: int, pc: int = 0) -> "Context":
+        """Non-owning view of a context created elsewhere (the simulator's,
+        Simulation.device_context()); closing it leaves the context alive."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.n, self.pc = int(n), int(pc)
+        self.params = None
+        self.h = handle
+        self._borrowed = True
+        return self
+
     def close(self):
         if getattr(self, "h", None):
-            self.lib.sdddst_ctx_destroy(self.h)
+            if not getattr(self, "_borrowed", False):
+                self.lib.sdddst_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -274,6 +302,49 @@ class Context:
         self._check(self.lib.sdddst_solver_stats(self.h, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(fb),
                                                  None, None))
         return it.value, rr.value, fb.value
+
+    def precond_fallbacks(self) -> int:
+        v = ctypes.c_int()
+        self._check(self.lib.sdddst_solver_precond_fallbacks(self.h, ctypes.byref(v)))
+        return v.value
+
+    # -- learned preconditioner ------------------------------------------
+    def spai_param_count(self) -> int:
+        return int(self.lib.sdddst_spai_param_count())
+
+    def spai_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.sdddst_spai_set_weights(self.h, _fptr(w), len(w)))
+
+    def spai_get_weights(self) -> np.ndarray:
+        w = np.empty(self.spai_param_count(), dtype=np.float32)
+        self._check(self.lib.sdddst_spai_get_weights(self.h, _fptr(w), len(w)))
+        return w
+
+    def spai_train_init(self, seed: int):
+        self._check(self.lib.sdddst_spai_train_init(self.h, seed))
+
+    def spai_train_step(self, probes: int, seed: int, lr: float = 0.0, eps: float = 1e-4, want_grad: bool = False):
+        """Loss and gradient on the current Jacobian, then one Adam step
+        when lr > 0: returns (loss, grad or None)."""
+        loss = ctypes.c_double()
+        g = np.empty(self.spai_param_count()) if want_grad else None
+        self._check(self.lib.sdddst_spai_train_step(self.h, probes, seed, eps, lr, ctypes.byref(loss), _dptr(g)))
+        return loss.value, g
+
+    def spai_graph(self):
+        """Last evaluated core graph and preconditioner values as a dict."""
+        ne = ctypes.c_int64()
+        self._check(self.lib.sdddst_spai_graph(self.h, ctypes.byref(ne), None, None, None, None, None, None, None,
+                                               None, None))
+        n, m = self.n, ne.value
+        out = dict(Cp=np.empty(n + 1, np.int32), Ci=np.empty(m, np.int32), Ce=np.empty(m), Cd=np.empty(n),
+                   gamma_e=np.empty(m, np.float32), gamma_d=np.empty(n, np.float32), bd=np.empty(n), bo=np.empty(n),
+                   bcol=np.empty(n, np.int32))
+        self._check(self.lib.sdddst_spai_graph(
+            self.h, ctypes.byref(ne), _iptr(out["Cp"]), _iptr(out["Ci"]), _dptr(out["Ce"]), _dptr(out["Cd"]),
+            _fptr(out["gamma_e"]), _fptr(out["gamma_d"]), _dptr(out["bd"]), _dptr(out["bo"]), _iptr(out["bcol"])))
+        return out
 
     def newton_update(self, slot: int, delta=None):
         d = None if delta is None else _f64(delta)

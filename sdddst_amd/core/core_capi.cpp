@@ -71,6 +71,7 @@ void configure(SimulationData &d, const sdddst_sim_options &o)
         d.externalStressProtocol.reset(new StressProtocol());
     }
     if (o.log_path && *o.log_path) d.standardOutputLog = std::ofstream(o.log_path);
+    if (o.spai_weights && *o.spai_weights) d.spaiWeightsPath = o.spai_weights;
 }
 }  // namespace
 
@@ -97,6 +98,7 @@ void sdddst_sim_default_options(sdddst_sim_options *o)
     o->A = 1e-4 * 16.0;
     o->KASQR = 1.65 * 1.65 * 1e6 / 256.0;
     o->log_path = nullptr;
+    o->spai_weights = nullptr;
 }
 
 int sdddst_sim_create(int n, const double *x, const double *y, const double *b, int pc, const double *px,
@@ -199,6 +201,9 @@ int sdddst_sim_get_state(sdddst_sim *s, sdddst_sim_state *o)
     o->solver_last_iters = it;
     o->solver_fallbacks = fb;
     o->solver_last_rel_residual = rr;
+    int pf = 0;
+    sdddst_solver_precond_fallbacks(s->sim->deviceContext(), &pf);
+    o->solver_precond_fallbacks = pf;
     return SDDDST_OK;
 }
 

@@ -7,6 +7,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
+
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -14,6 +17,7 @@
 
 #include "../../include/sdddst/StressProtocols/constant_stress_protocol.h"
 #include "../../include/sdddst/StressProtocols/spring_protocol.h"
+#include "../../include/sdddst/core_capi.h"
 #include "../../include/sdddst/simulation.h"
 #include "../../include/sdddst/utility.h"
 #include "../../include/sdddst_amd.h"
@@ -81,6 +85,31 @@ Simulation::Simulation(std::shared_ptr<SimulationData> _sD)
     check(sdddst_solver_select(ctx, kind, sD->solverRelTolerance, sD->solverMaxIterations, sD->solverRestart),
           "solver select");
     check(sdddst_set_preconditioner(ctx, sD->preconditioner), "preconditioner");
+    if (sD->preconditioner == SDDDST_PRECOND_LEARNED) loadSpaiWeights();
+}
+
+// Raw little-endian float32 weights of the learned preconditioner; the
+// default file ships in data/ beside this library.
+void Simulation::loadSpaiWeights()
+{
+    std::string path = sD->spaiWeightsPath;
+    if (path.empty()) {
+        Dl_info info;
+        if (!dladdr((void *)&sdddst_sim_default_options, &info) || !info.dli_fname)
+            throw std::runtime_error("cannot locate libsdddstCore.so for the default preconditioner weights");
+        std::string lib = info.dli_fname;
+        const size_t slash = lib.find_last_of('/');
+        path = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/data/spai_gnn_h32.f32";
+    }
+    const int count = sdddst_spai_param_count();
+    std::vector<float> w(count);
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open preconditioner weights " + path);
+    f.read(reinterpret_cast<char *>(w.data()), sizeof(float) * count);
+    if (f.gcount() != (std::streamsize)(sizeof(float) * count) || f.peek() != std::char_traits<char>::eof())
+        throw std::runtime_error("preconditioner weights " + path + " must hold exactly " + std::to_string(count) +
+                                 " float32 values");
+    check(sdddst_spai_set_weights(ctx, w.data(), count), "preconditioner weights");
 }
 
 Simulation::~Simulation()

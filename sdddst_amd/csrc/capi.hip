@@ -50,12 +50,13 @@ static int check_params(const sdddst_params *p)
 static void free_all(Context &c)
 {
     solver_free(c);
+    spai_free(c);
     for (int s = 0; s < SDDDST_NUM_SLOTS; ++s)
         if (c.x[s]) (void)hipFree(c.x[s]);
     for (int s = 0; s < SDDDST_NUM_VBUFS; ++s)
         if (c.v[s]) (void)hipFree(c.v[s]);
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
-                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd};
+                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
@@ -429,6 +430,64 @@ int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_resid
     if (last_rel_residual) *last_rel_residual = ctx->c.last_rel_res;
     if (fallbacks) *fallbacks = ctx->c.fallbacks;
     return SDDDST_OK;
+}
+
+int sdddst_solver_precond_fallbacks(sdddst_ctx *ctx, int *count)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(count, SDDDST_ERR_ARG, "count is required");
+    *count = ctx->c.precond_fallbacks;
+    return SDDDST_OK;
+}
+
+int sdddst_spai_param_count(void) { return spai_param_count(); }
+
+int sdddst_spai_set_weights(sdddst_ctx *ctx, const float *w, int count)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(w && count == spai_param_count(), SDDDST_ERR_ARG, "expected %d weights, got %d", spai_param_count(),
+             count);
+    for (int t = 0; t < count; ++t) SD_CHECK(std::isfinite(w[t]), SDDDST_ERR_ARG, "weight %d is not finite", t);
+    return spai_set_weights(ctx->c, w);
+}
+
+int sdddst_spai_get_weights(sdddst_ctx *ctx, float *w, int count)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(w && count == spai_param_count(), SDDDST_ERR_ARG, "expected room for %d weights, got %d",
+             spai_param_count(), count);
+    return spai_get_weights(ctx->c, w);
+}
+
+int sdddst_spai_train_init(sdddst_ctx *ctx, uint64_t seed)
+{
+    CTX_CHECK(ctx);
+    return spai_train_init(ctx->c, seed);
+}
+
+int sdddst_spai_train_step(sdddst_ctx *ctx, int probes, uint64_t seed, double eps, double lr, double *loss,
+                           double *grad)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(probes >= 1 && probes <= 64, SDDDST_ERR_ARG, "probes must be in [1, 64]");
+    SD_CHECK(eps > 0 && std::isfinite(eps), SDDDST_ERR_ARG, "eps must be positive");
+    SD_CHECK(std::isfinite(lr), SDDDST_ERR_ARG, "learning rate must be finite");
+    SD_CHECK(c.jac_valid, SDDDST_ERR_STATE, "no Jacobian assembled");
+    SD_CHECK(c.spai, SDDDST_ERR_STATE, "no weights installed (sdddst_spai_train_init or set_weights)");
+    if (!c.gm) c.gm = new Fgmres();
+    int rc = c.gm->pair_factor(c);
+    if (rc) return rc;
+    return spai_train_step(c, *c.gm, probes, seed, eps, lr, loss, grad);
+}
+
+int sdddst_spai_graph(sdddst_ctx *ctx, int64_t *n_edges, int *Cp, int *Ci, double *Ce, double *Cd, float *gamma_e,
+                      float *gamma_d, double *bd, double *bo, int *bcol)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(n_edges, SDDDST_ERR_ARG, "n_edges is required");
+    SD_CHECK(ctx->c.gm, SDDDST_ERR_STATE, "no learned preconditioner evaluated");
+    return spai_graph(ctx->c, *ctx->c.gm, n_edges, Cp, Ci, Ce, Cd, gamma_e, gamma_d, bd, bo, bcol);
 }
 
 int sdddst_newton_update(sdddst_ctx *ctx, int slot, const double *delta_host)

@@ -71,11 +71,14 @@ struct Context {
     int64_t nnz = 0, cap = 0;
     double *jtau = nullptr, *jmult = nullptr;   // tau_x and taper per strictly-lower entry
     int *jmirror = nullptr;                     // position of the mirror entry
+    unsigned char *jcore = nullptr;             // 1 where the pair lies within the cutoff (untapered)
     double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
     bool pattern_valid = false;
     int pattern_slot = -1;                      // slot whose positions built the pattern
     bool jac_valid = false;
+    bool jac_symmetric = true;
+    bool direct_once = false;                   // this Jacobian goes to the direct solver                  // pattern symmetric (no zero-value compaction)
     double jac_h = 0;
     int *d_flags = nullptr;             // device flags: [0] zero-valued upper entries found
     void *scan_tmp = nullptr;
@@ -88,6 +91,8 @@ struct Context {
     double solver_tol = 1e-10;
     int solver_max_iter = 200, solver_restart = 30;
     int precond = SDDDST_PRECOND_PAIR;
+    int precond_override = -1;          // >= 0: preconditioner of the current retry
+    int precond_fallbacks = 0;          // learned solves retried with the pair blocks
     struct Spai *spai = nullptr;       // learned preconditioner state
     DenseLu *lu = nullptr;
     Fgmres *gm = nullptr;
@@ -124,9 +129,11 @@ int launch_strain(Context &c, int old_slot, int new_slot);
 int launch_reset_tolerance(Context &c);
 int field_eval(Context &c, int which, int64_t n, const double *dx, const double *dy, double *out);
 int libm_eval(Context &c, int fn, int64_t n, const double *x, double *out);
+int launch_scan(Context &c, int n, const int *cnt, int *out, long long *d_total);
 int solver_factor(Context &c);
 int solver_solve(Context &c);
 void solver_free(Context &c);
+void spai_free(Context &c);
 
 }  // namespace sdamd
 

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
                                                     CutParams cp, const int *__restrict__ Ap,
                                                     const int *__restrict__ Ai, const int *__restrict__ diag,
                                                     double *__restrict__ tau, double *__restrict__ mult,
-                                                    int *__restrict__ mirror)
+                                                    int *__restrict__ mirror, unsigned char *__restrict__ core)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -192,7 +192,11 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
         }
         tau[k] = sdfield::xy_diff_x(dx, dy);
         mult[k] = m;
-        mirror[k] = find_row(Ai, Ap[i], diag[i], j);
+        const int mk = find_row(Ai, Ap[i], diag[i], j);
+        mirror[k] = mk;
+        const unsigned char inside = r2 > cp.rc2 ? 0 : 1;
+        core[k] = inside;
+        core[mk] = inside;
     }
 }
 
@@ -315,6 +319,13 @@ static CutParams cut_params(const Context &c)
     return cp;
 }
 
+int launch_scan(Context &c, int n, const int *cnt, int *out, long long *d_total)
+{
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, n, cnt, out, d_total);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
 // Rebuild without the upper entries whose mirrored value is exactly zero
 // (getElement returns 0 for them, simulation.cpp:263).  Host path: it runs
 // only when such a value occurred.
@@ -355,15 +366,16 @@ static int ensure_capacity(Context &c, int64_t need)
 {
     if (need <= c.cap) return SDDDST_OK;
     int64_t ncap = need + need / 8 + 1024;
-    void **bufs[] = {(void **)&c.Ai, (void **)&c.Ax, (void **)&c.AxRow, (void **)&c.jtau, (void **)&c.jmult,
-                     (void **)&c.jmirror};
+    void **bufs[] = {(void **)&c.Ai,   (void **)&c.Ax,      (void **)&c.AxRow, (void **)&c.jtau,
+                     (void **)&c.jmult, (void **)&c.jmirror, (void **)&c.jcore};
     for (void **p : bufs) {
         if (*p) SD_HIP(hipFree(*p));
         *p = nullptr;
     }
     c.cap = 0;
-    const size_t sizes[] = {sizeof(int), sizeof(double), sizeof(double), sizeof(double), sizeof(double), sizeof(int)};
-    for (int k = 0; k < 6; ++k) {
+    const size_t sizes[] = {sizeof(int),    sizeof(double), sizeof(double),       sizeof(double),
+                            sizeof(double), sizeof(int),    sizeof(unsigned char)};
+    for (int k = 0; k < 7; ++k) {
         if (hipMalloc(bufs[k], sizes[k] * ncap) != hipSuccess) {
             *bufs[k] = nullptr;
             set_error("device allocation of %lld Jacobian entries failed", (long long)ncap);
@@ -393,19 +405,19 @@ static int build_pattern(Context &c, int slot)
                        c.jmask);
     SD_HIP(hipGetLastError());
     long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, n, c.colcount, c.Ap, d_total);
-    SD_HIP(hipGetLastError());
+    int rc = launch_scan(c, n, c.colcount, c.Ap, d_total);
+    if (rc) return rc;
     long long total = 0;
     SD_HIP(hipMemcpyAsync(&total, d_total, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
-    int rc = ensure_capacity(c, total);
+    rc = ensure_capacity(c, total);
     if (rc) return rc;
     c.nnz = total;
     hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag);
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_jac_values, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap, c.Ai, c.diag,
-                       c.jtau, c.jmult, c.jmirror);
+                       c.jtau, c.jmult, c.jmirror, c.jcore);
     SD_HIP(hipGetLastError());
     if (c.pc > 0) {
         hipLaunchKernelGGL(k_jac_pd, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.pc, x, c.y, c.b, c.px,
@@ -436,6 +448,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     int flag = 0;
     SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
+    c.jac_symmetric = flag == 0;
     if (flag) {
         rc = compact_zero_upper(c);
         if (rc) return rc;

@@ -14,6 +14,7 @@
 
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "context.h"
@@ -258,29 +259,39 @@ void Fgmres::release()
     ticket = nullptr;
 }
 
+int Fgmres::pair_factor(Context &c)
+{
+    int rc = init(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pair_partner, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
+                       c.AxRow, partner, strength);
+    hipLaunchKernelGGL(k_pair_blocks, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax, c.AxRow,
+                       partner, strength, 1e-3, binv, bcol);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
 int Fgmres::factor(Context &c)
 {
     int rc = init(c);
     if (rc) return rc;
     if (c.precond == SDDDST_PRECOND_JACOBI) {
         hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
-    } else {
-        hipLaunchKernelGGL(k_pair_partner, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
-                           c.AxRow, partner, strength);
-        hipLaunchKernelGGL(k_pair_blocks, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
-                           c.AxRow, partner, strength, 1e-3, binv, bcol);
+        SD_HIP(hipGetLastError());
+        return SDDDST_OK;
     }
-    SD_HIP(hipGetLastError());
-    if (c.precond == SDDDST_PRECOND_LEARNED) return spai_factor(c);
+    if ((rc = pair_factor(c))) return rc;
+    if (c.precond == SDDDST_PRECOND_LEARNED) return spai_factor(c, *this);
     return SDDDST_OK;
 }
 
 int Fgmres::apply(Context &c, const double *v, double *z)
 {
-    if (c.precond == SDDDST_PRECOND_JACOBI)
+    const int precond = c.precond_override >= 0 ? c.precond_override : c.precond;
+    if (precond == SDDDST_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_jacobi_apply, dim3(nb256(n)), dim3(256), 0, c.stream, n, dinv, v, z);
-    else if (c.precond == SDDDST_PRECOND_LEARNED)
-        return spai_apply(c, v, z);
+    else if (precond == SDDDST_PRECOND_LEARNED)
+        return spai_apply(c, *this, v, z);
     else
         hipLaunchKernelGGL(k_pair_apply, dim3(nb256(n)), dim3(256), 0, c.stream, n, binv, bcol, v, z);
     SD_HIP(hipGetLastError());
@@ -322,6 +333,10 @@ int Fgmres::solve(Context &c)
 {
     const int N = n;
     const double tol = c.solver_tol;
+    // a learned-preconditioned solve gets one restart cycle; the caller
+    // retries with the pair blocks when it does not converge in it
+    const int precond = c.precond_override >= 0 ? c.precond_override : c.precond;
+    const int max_it = precond == SDDDST_PRECOND_LEARNED ? std::min(c.solver_max_iter, m) : c.solver_max_iter;
     int rc;
     // delta = 0, r = g
     SD_HIP(hipMemsetAsync(c.delta, 0, sizeof(double) * N, c.stream));
@@ -337,7 +352,7 @@ int Fgmres::solve(Context &c)
     int total_it = 0;
     double rel = 1.0;
     const double *r0 = c.g;
-    for (int cycle = 0; total_it < c.solver_max_iter; ++cycle) {
+    for (int cycle = 0; total_it < max_it; ++cycle) {
         // r = g - A delta
         if (cycle > 0) {
             hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow, c.delta, w);
@@ -355,7 +370,7 @@ int Fgmres::solve(Context &c)
         std::fill(s.begin(), s.end(), 0.0);
         s[0] = beta;
         int k = 0;
-        for (; k < m && total_it < c.solver_max_iter; ++k, ++total_it) {
+        for (; k < m && total_it < max_it; ++k, ++total_it) {
             // z_k = M^-1 v_k ; w = A z_k
             rc = apply(c, V + (size_t)k * N, Z + (size_t)k * N);
             if (rc) return rc;

@@ -35,13 +35,23 @@ struct Fgmres {
     int *partner = nullptr, *bcol = nullptr;
     int init(Context &c);
     int factor(Context &c);
+    int pair_factor(Context &c);                         // pair-block inverse of A
     int apply(Context &c, const double *v, double *z);   // z = M^-1 v
     int solve(Context &c);
     void release();
 };
 
-// Learned sparse approximate inverse (spai.hip).
-int spai_factor(Context &c);
-int spai_apply(Context &c, const double *v, double *z);
+// Learned sparse approximate inverse (spai.hip) on top of the pair-block
+// inverse held by f.
+int spai_param_count();
+int spai_factor(Context &c, const Fgmres &f);
+int spai_apply(Context &c, const Fgmres &f, const double *v, double *z);
+int spai_set_weights(Context &c, const float *w);
+int spai_get_weights(Context &c, float *w);
+int spai_train_init(Context &c, unsigned long long seed);
+int spai_train_step(Context &c, Fgmres &f, int k, unsigned long long seed, double eps, double lr, double *loss,
+                    double *grad);
+int spai_graph(Context &c, const Fgmres &f, int64_t *ne, int *Cp, int *Ci, double *Ce, double *Cd, float *ge,
+               float *gd, double *bd, double *bo, int *bcol);
 
 }  // namespace sdamd

@@ -43,6 +43,7 @@ class SimOptions(ctypes.Structure):
         ("A", ctypes.c_double),
         ("KASQR", ctypes.c_double),
         ("log_path", ctypes.c_char_p),
+        ("spai_weights", ctypes.c_char_p),
     ]
 
 
@@ -64,6 +65,7 @@ class SimState(ctypes.Structure):
         ("solver_last_rel_residual", ctypes.c_double),
         ("solver_total_iters", ctypes.c_longlong),
         ("solver_total_solves", ctypes.c_longlong),
+        ("solver_precond_fallbacks", ctypes.c_int),
     ]
 
 
@@ -129,6 +131,7 @@ class StepState:
     solver_last_rel_residual: float
     solver_total_iters: int
     solver_total_solves: int
+    solver_precond_fallbacks: int
 
 
 class Simulation:
@@ -138,7 +141,7 @@ class Simulation:
                  stress="zero", stress_value=0.0, spring_constant=1.0, calculate_strain=True,
                  order_parameter=False, solver="auto", solver_tol=1e-10, solver_max_iter=300, solver_restart=40,
                  field_mode="fast", preconditioner="pair", px=None, py=None, A=1e-4 * 16.0, KASQR=1.65 * 1.65 * 1e6 / 256.0,
-                 log_path: Optional[str] = None):
+                 log_path: Optional[str] = None, spai_weights: Optional[str] = None):
         self.lib = load_core()
         o = default_options()
         o.cutoff_multiplier = cutoff_multiplier
@@ -160,6 +163,8 @@ class Simulation:
         o.KASQR = KASQR
         self._log = log_path.encode() if log_path else None
         o.log_path = self._log
+        self._weights = spai_weights.encode() if spai_weights else None
+        o.spai_weights = self._weights
         self.n = len(x)
         xs, ys, bs = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, b))
         pc = 0 if px is None else len(px)
@@ -195,7 +200,7 @@ class Simulation:
         return StepState(s.sim_time, s.step_size, s.strain, s.energy, s.max_error_ratio_sqr, s.sum_avg_speed,
                          s.cutoff, s.succesful_steps, s.failed_steps, bool(s.last_step_succeeded),
                          s.last_step_wall_s, s.solver_last_iters, s.solver_fallbacks, s.solver_last_rel_residual,
-                         s.solver_total_iters, s.solver_total_solves)
+                         s.solver_total_iters, s.solver_total_solves, s.solver_precond_fallbacks)
 
     def get_time(self) -> float:
         return self.state().sim_time

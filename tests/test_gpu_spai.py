@@ -1,0 +1,148 @@
+"""Learned preconditioner on the device against the CPU restatement
+(oracle/spai_gnn.py): core graph and pair-block inverse bit-exact, network
+outputs, loss and weight gradient within FP32 tolerances, training lowers
+the loss, and FGMRES with the learned preconditioner solves to 1e-10."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import spai_gnn as G  # noqa: E402
+
+import sdddst_amd as S  # noqa: E402
+from sdddst_amd import configs  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_system(n, m, seed, h):
+    x, y, b = configs.dislocations(n, seed)
+    c = S.Context(n, cutoff_multiplier=m)
+    c.upload_state(0, x, y, b)
+    Ap, Ai, Ax, dv, diag = c.jacobian_csc(0, h)
+    rc, _ = c.cutoff()
+    return c, G.system_from_jacobian(x, y, Ap, Ai, Ax, diag, rc)
+
+
+@pytest.mark.parametrize("n,m,seed,h", [(1024, 2.0, 31, 2e-5), (700, 4.0, 32, 5e-6)])
+def test_device_network_and_gradient_match_oracle(n, m, seed, h):
+    c, s = _device_system(n, m, seed, h)
+    with c:
+        rng = np.random.default_rng(seed)
+        th = (G.init_params(7) + 0.05 * rng.standard_normal(G.NPARAM)).astype(np.float32)
+        c.spai_train_init(7)
+        assert np.array_equal(c.spai_get_weights(), G.init_params(7).astype(np.float32))
+        c.spai_set_weights(th)
+        loss, grad = c.spai_train_step(8, 1234, lr=0.0, want_grad=True)
+        gr = c.spai_graph()
+    theta = th.astype(np.float64)
+    # pair-block inverse and core graph: bit-exact
+    assert np.array_equal(gr["bcol"], s.bcol)
+    assert np.array_equal(gr["bd"].view(np.int64), s.bd.view(np.int64))
+    assert np.array_equal(gr["bo"].view(np.int64), s.bo.view(np.int64))
+    assert np.array_equal(gr["Cp"], s.Cp) and np.array_equal(gr["Ci"], s.ci)
+    assert (s.bcol >= 0).sum() > 0 and len(s.ci) > 0
+    # network outputs (FP32 on the device, FP64 in the oracle)
+    L, g, ge, gd = G.loss_and_grad(theta, s, 8, 1234)
+    scale = max(np.abs(ge).max(), np.abs(gd).max())
+    assert np.max(np.abs(gr["gamma_e"] - ge)) <= 2e-5 * scale
+    assert np.max(np.abs(gr["gamma_d"] - gd)) <= 2e-5 * scale
+    assert abs(loss - L) <= 1e-5 * abs(L)
+    assert np.linalg.norm(grad - g) <= 1e-3 * np.linalg.norm(g)
+    big = np.abs(g) > 1e-2 * np.abs(g).max()
+    assert np.max(np.abs(grad[big] - g[big]) / np.abs(g[big])) < 2e-2
+
+
+def test_device_training_lowers_the_loss():
+    c, s = _device_system(1024, 2.0, 33, 2e-5)
+    with c:
+        c.spai_train_init(5)
+        l0, _ = c.spai_train_step(8, 1, lr=0.0)
+        for t in range(40):
+            c.spai_train_step(8, 100 + t, lr=3e-3)
+        l1, _ = c.spai_train_step(8, 1, lr=0.0)
+    assert l1 < 0.7 * l0, (l0, l1)
+
+
+def test_learned_fgmres_matches_direct_solve():
+    n, m, h = 1024, 2.0, 2e-5
+    x, y, b = configs.dislocations(n, 34)
+    rng = np.random.default_rng(3)
+    th = (G.init_params(9) + 0.02 * rng.standard_normal(G.NPARAM)).astype(np.float32)
+    out = {}
+    for solver, precond in ((S.SOLVER_DENSE_LU, S.PRECOND_PAIR), (S.SOLVER_FGMRES, S.PRECOND_LEARNED),
+                            (S.SOLVER_FGMRES, S.PRECOND_PAIR)):
+        with S.Context(n, cutoff_multiplier=m) as c:
+            c.upload_state(0, x, y, b)
+            c.speeds(0, S.V_INIT, 0.0)
+            c.speeds(0, S.V_INIT2, 0.0)
+            c.jacobian_csc(0, h)
+            c.spai_set_weights(th)
+            c.solver_select(solver, 1e-10, 300, 40)
+            c.set_preconditioner(precond)
+            g = c.residual(h, 0, 0, S.V_INIT, S.V_INIT2)
+            c.solver_factor()
+            d = c.solver_solve(download=True)
+            it, rr, fb = c.solver_stats()
+            out[(solver, precond)] = (d, it, rr, fb)
+    d0 = out[(S.SOLVER_DENSE_LU, S.PRECOND_PAIR)][0]
+    d1, it1, rr1, fb1 = out[(S.SOLVER_FGMRES, S.PRECOND_LEARNED)]
+    assert fb1 == 0 and rr1 <= 1e-10 and it1 > 0
+    assert np.max(np.abs(d1 - d0)) <= 1e-8 * np.max(np.abs(d0))
+    with pytest.raises(S.SdddstError):
+        with S.Context(64) as c:
+            x2, y2, b2 = configs.dislocations(64, 1)
+            c.upload_state(0, x2, y2, b2)
+            c.jacobian_csc(0, 1e-6)
+            c.solver_select(S.SOLVER_FGMRES, 1e-10, 100, 20)
+            c.set_preconditioner(S.PRECOND_LEARNED)
+            c.solver_factor()      # no weights installed
+
+
+def _solve_iters(n, m, x, y, b, h, precond, weights):
+    with S.Context(n, cutoff_multiplier=m) as c:
+        c.upload_state(0, x, y, b)
+        c.speeds(0, S.V_INIT, 0.02)
+        c.speeds(0, S.V_INIT2, 0.02)
+        c.jacobian_csc(0, h)
+        c.spai_set_weights(weights)
+        c.solver_select(S.SOLVER_FGMRES, 1e-10, 300, 40)
+        c.set_preconditioner(precond)
+        c.residual(h, 0, 0, S.V_INIT, S.V_INIT2)
+        c.solver_factor()
+        c.solver_solve()
+        it, rr, fb = c.solver_stats()
+        return it, rr, fb, c.precond_fallbacks()
+
+
+@pytest.mark.parametrize("m,seed", [(2.0, 7001), (4.0, 7002)])
+def test_trained_network_gains_over_pair_blocks(m, seed):
+    """The shipped network against the method without it (gamma = 0: the
+    pair-block preconditioner) on the Newton systems a simulation at N=2048
+    emits (seeds not used in training), at FGMRES relative residual 1e-10."""
+    from sdddst_amd.simulation import Simulation
+    from sdddst_amd.spai_train import DEFAULT_WEIGHTS, load_weights
+    w = load_weights(DEFAULT_WEIGHTS)
+    n = 2048
+    x, y, b = configs.dislocations(n, seed)
+    sim = Simulation(x, y, b, cutoff_multiplier=m, precision=1e-6, initial_step=1e-6, stress="constant",
+                     stress_value=0.02, solver="fgmres", preconditioner="pair")
+    snaps = []
+    while sim.state().succesful_steps < 200:
+        st = sim.state()
+        if st.succesful_steps % 25 == 0 and (not snaps or snaps[-1][0] != st.succesful_steps):
+            snaps.append((st.succesful_steps, sim.positions().copy(), st.step_size))
+        sim.step()
+    sim.close()
+    pair, learned = [], []
+    for _, xs, h in snaps:
+        it_p, rr_p, fb_p, _ = _solve_iters(n, m, xs, y, b, h, S.PRECOND_PAIR, w)
+        it_l, rr_l, fb_l, pf_l = _solve_iters(n, m, xs, y, b, h, S.PRECOND_LEARNED, w)
+        assert fb_p == 0 and fb_l == 0 and pf_l == 0 and rr_l <= 1e-10
+        pair.append(it_p)
+        learned.append(it_l)
+    print(f"m={m}: FGMRES iterations pair {pair} learned {learned}")
+    assert sum(learned) <= 0.85 * sum(pair)
+    assert all(l <= p + 1 for l, p in zip(learned, pair))

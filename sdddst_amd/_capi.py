@@ -124,7 +124,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 
 def exported_symbols():
-    return list(_SIGNATURES)
+    """Every entry point bound from Python (this module and ensemble.py)."""
+    from .ensemble import _SIGS as ensemble_sigs
+    return list(_SIGNATURES) + list(ensemble_sigs)
 
 
 def _dptr(a: Optional[np.ndarray]):

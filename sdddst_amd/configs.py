@@ -91,3 +91,17 @@ def read_dconf(path: str):
     """SimulationData::readDislocationDataFromFile (simulation_data.cpp:119-150)."""
     vals = np.loadtxt(path, dtype=np.float64, ndmin=2)
     return vals[:, 0].copy(), vals[:, 1].copy(), vals[:, 2].copy()
+
+
+def ensemble_workload(count: int = 256, seed: int = 3, lo: int = 64, hi: int = 2048):
+    """The batched-ensemble workload: sizes from ensemble_sizes and, for
+    realization r, positions from dislocations(n_r, seed * 1000 + r);
+    returns (sizes, x, y, b) with the realizations concatenated."""
+    sizes = ensemble_sizes(count, seed, lo, hi)
+    xs, ys, bs = [], [], []
+    for r, n in enumerate(sizes):
+        x, y, b = dislocations(n, seed * 1000 + r)
+        xs.append(x)
+        ys.append(y)
+        bs.append(b)
+    return sizes, np.concatenate(xs), np.concatenate(ys), np.concatenate(bs)

@@ -40,58 +40,26 @@
 
 #include "context.h"
 #include "field.h"
+#include "jac_core.h"
 
 namespace sdamd {
 
 namespace {
 
-struct CutParams {
-    double rc, rc2, thr, prefilter, inv_rc2;
-};
-
-// simulation.cpp:335 for the pair (lo, hi), dx = x_hi - x_lo.
-__device__ __forceinline__ bool pair_pred(double dx, double dy, const CutParams &cp, double &r2_out)
-{
-    const double r2 = dx * dx + dy * dy;
-    r2_out = r2;
-    if (!(r2 < cp.prefilter)) return false;
-    const double s = sdlibm::sqrt(r2) - cp.rc;
-    return s * s < cp.thr;
-}
+using jac::CutParams;
 
 __global__ __launch_bounds__(256) void k_jac_count(int n, int words, const double *__restrict__ x,
                                                    const double *__restrict__ y, CutParams cp,
                                                    int *__restrict__ cnt, unsigned long long *__restrict__ mask)
 {
-    const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    const double xj = x[j], yj = y[j];
-    unsigned long long *mj = mask + (size_t)j * words;
-    int c = 0;
-    for (int w = 0; w < words; ++w) {
-        const int i = w * 64 + lane;
-        bool in = false;
-        if (i < n) {
-            if (i == j) {
-                in = true;
-            } else {
-                const double xi = x[i], yi = y[i];
-                // dx = x_hi - x_lo for the pair {i, j}
-                const double dx = sdfield::normalize(i > j ? xi - xj : xj - xi);
-                const double dy = sdfield::normalize(i > j ? yi - yj : yj - yi);
-                double r2;
-                in = pair_pred(dx, dy, cp, r2);
-            }
-        }
-        const unsigned long long m = __ballot(in);
-        if (lane == 0) mj[w] = m;
-        c += __popcll(m);
-    }
-    if (lane == 0) cnt[j] = c;
+    jac::count_column(j, n, words, x, y, cp, cnt + j, mask + (size_t)j * words);
 }
 
-// Exclusive scan of n counts into Ap[0..n] by one workgroup of 1024 lanes.
+// Exclusive scan of n counts into Ap[0..n] by one workgroup of 1024 lanes:
+// per-lane sums of contiguous chunks, a work-efficient LDS scan of the 1024
+// partials, then the per-lane chunk scans.
 __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cnt, int *__restrict__ Ap,
                                                long long *__restrict__ total)
 {
@@ -103,67 +71,31 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cn
     for (int k = s0; k < s1; ++k) acc += cnt[k];
     part[t] = acc;
     __syncthreads();
-    if (t == 0) {
-        long long run = 0;
-        for (int k = 0; k < 1024; ++k) {
-            const long long v = part[k];
-            part[k] = run;
-            run += v;
-        }
-        *total = run;
+    // inclusive Hillis-Steele scan of the 1024 partials
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
     }
-    __syncthreads();
-    long long run = part[t];
+    long long run = part[t] - acc;   // exclusive prefix of this lane's chunk
     for (int k = s0; k < s1; ++k) {
         Ap[k] = (int)run;
         run += cnt[k];
     }
-    if (t == 0) Ap[n] = (int)(*total);
+    if (t == 1023) {
+        *total = part[1023];
+        Ap[n] = (int)part[1023];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsigned long long *__restrict__ mask,
                                                   const int *__restrict__ Ap, int *__restrict__ Ai,
                                                   int *__restrict__ diag)
 {
-    const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    const unsigned long long *mj = mask + (size_t)j * words;
-    int base = Ap[j];
-    for (int w0 = 0; w0 < words; w0 += 64) {
-        const int w = w0 + lane;
-        const unsigned long long m = w < words ? mj[w] : 0ull;
-        const int pc = __popcll(m);
-        // exclusive prefix of pc across the wave
-        int incl = pc;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        int pos = base + incl - pc;
-        unsigned long long bits = m;
-        while (bits) {
-            const int bit = __ffsll((long long)bits) - 1;
-            bits &= bits - 1;
-            const int i = w * 64 + bit;
-            Ai[pos] = i;
-            if (i == j) diag[j] = pos;
-            ++pos;
-        }
-        base += __shfl(incl, 63, 64);
-    }
-}
-
-// Row j inside the strictly-upper part [lo, hi) of a column (ascending).
-__device__ __forceinline__ int find_row(const int *__restrict__ Ai, int lo, int hi, int row)
-{
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = Ai[mid];
-        if (v < row) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+    jac::fill_column(j, words, mask + (size_t)j * words, Ap[j], Ai, diag + j);
 }
 
 // tau_x and the taper of every strictly-lower entry (row i > column j) and
@@ -174,30 +106,9 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
                                                     double *__restrict__ tau, double *__restrict__ mult,
                                                     int *__restrict__ mirror, unsigned char *__restrict__ core)
 {
-    const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    const int d = diag[j];
-    const int end = Ap[j + 1];
-    const double xj = x[j], yj = y[j];
-    for (int k = d + 1 + lane; k < end; k += 64) {
-        const int i = Ai[k];
-        const double dx = sdfield::normalize(x[i] - xj);
-        const double dy = sdfield::normalize(y[i] - yj);
-        const double r2 = dx * dx + dy * dy;
-        double m = 1;
-        if (r2 > cp.rc2) {
-            const double s = sdlibm::sqrt(r2) - cp.rc;
-            m = sdlibm::exp(-(s * s) * cp.inv_rc2);
-        }
-        tau[k] = sdfield::xy_diff_x(dx, dy);
-        mult[k] = m;
-        const int mk = find_row(Ai, Ap[i], diag[i], j);
-        mirror[k] = mk;
-        const unsigned char inside = r2 > cp.rc2 ? 0 : 1;
-        core[k] = inside;
-        core[mk] = inside;
-    }
+    jac::values_column(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core);
 }
 
 // simulation.cpp:269-324: the point-defect sum tmp of column j (defects in
@@ -215,7 +126,7 @@ __global__ void k_jac_pd(int n, int pc, const double *__restrict__ x, const doub
         const double dx = sdfield::normalize(xj - px[l]);
         const double dy = sdfield::normalize(yj - py[l]);
         double r2;
-        if (pair_pred(dx, dy, cp, r2)) {
+        if (jac::pair_pred(dx, dy, cp, r2)) {
             double multiplier = 1;
             if (r2 > cp.rc2) {
                 const double s = sdlibm::sqrt(r2) - cp.rc;
@@ -227,9 +138,6 @@ __global__ void k_jac_pd(int n, int pc, const double *__restrict__ x, const doub
     pdtmp[j] = tmp;
 }
 
-// simulation.cpp:325, 343: raw value h*b_i*b_j*tau_x*taper at the entry and
-// at its mirror (simulation.cpp:260-268); the diagonal holds -tmp*h.  Exact
-// zeros are flagged: their mirrors must not exist.
 __global__ __launch_bounds__(256) void k_jac_raw(int n, const double *__restrict__ b, double h,
                                                  const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                  const int *__restrict__ diag, const double *__restrict__ tau,
@@ -237,19 +145,9 @@ __global__ __launch_bounds__(256) void k_jac_raw(int n, const double *__restrict
                                                  const double *__restrict__ pdtmp, double *__restrict__ Ax,
                                                  int *__restrict__ flags)
 {
-    const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    const int d = diag[j];
-    const int end = Ap[j + 1];
-    const double bj = b[j];
-    if (lane == 0) Ax[d] = -(pdtmp ? pdtmp[j] : 0.0) * h;
-    for (int k = d + 1 + lane; k < end; k += 64) {
-        const double v = h * b[Ai[k]] * bj * tau[k] * mult[k];
-        Ax[k] = v;
-        Ax[mirror[k]] = v;
-        if (v == 0.0) atomicOr(flags, 1);
-    }
+    jac::raw_column(j, b, h, Ap, Ai, diag, tau, mult, mirror, pdtmp ? pdtmp[j] : 0.0, Ax, flags);
 }
 
 __global__ void k_jac_diag(int n, const int *__restrict__ Ap, const int *__restrict__ diag, double *__restrict__ Ax,
@@ -257,43 +155,16 @@ __global__ void k_jac_diag(int n, const int *__restrict__ Ap, const int *__restr
 {
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j >= n) return;
-    double subSum = 0;
-    const int e = Ap[j + 1];
-    for (int k = Ap[j]; k < e; ++k) subSum += Ax[k];
-    subSum *= -1.;
-    Ax[diag[j]] = subSum;
-    double dv;
-    if (subSum > 0) {
-        subSum = 1. / subSum;
-        subSum += 1.;
-        subSum *= subSum;
-        dv = 1. / subSum;
-    } else {
-        dv = 0.;
-    }
-    dvec[j] = dv;
+    jac::diag_column(j, Ap, diag, Ax, dvec);
 }
 
 __global__ __launch_bounds__(256) void k_jac_scale(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                    const int *__restrict__ diag, const double *__restrict__ dvec,
                                                    double *__restrict__ Ax, double *__restrict__ AxRow)
 {
-    const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    const double Dj = (1.0 + dvec[j]) * 0.5;
-    const int dpos = diag[j];
-    for (int k = Ap[j] + lane; k < Ap[j + 1]; k += 64) {
-        const double raw = Ax[k];
-        double a = raw * ((1.0 + dvec[Ai[k]]) * 0.5);
-        double r = raw * Dj;
-        if (k == dpos) {
-            a += 1.0;
-            r += 1.0;
-        }
-        Ax[k] = a;
-        if (AxRow) AxRow[k] = r;
-    }
+    jac::scale_column(j, Ap, Ai, diag, dvec, Ax, AxRow);
 }
 
 __global__ void k_fill_int(int *p, int v, int n)
@@ -306,17 +177,7 @@ __global__ void k_fill_int(int *p, int v, int n)
 
 static CutParams cut_params(const Context &c)
 {
-    CutParams cp;
-    cp.rc = c.cutoff;
-    cp.rc2 = c.cutoff_sqr;
-    cp.thr = 36.8 * c.cutoff_sqr;
-    cp.inv_rc2 = c.one_per_cutoff_sqr;
-    // r < (1 + sqrt(36.8)) rc is necessary for the predicate; the margin
-    // covers every rounding of the exact test.
-    const double rmax = (1.0 + 6.06630035524124) * c.cutoff;
-    cp.prefilter = rmax * rmax * (1.0 + 1e-9) + 1e-300;
-    if (!(cp.prefilter < 1e300)) cp.prefilter = __builtin_inf();
-    return cp;
+    return jac::make_cut_params(c.cutoff, c.cutoff_sqr, c.one_per_cutoff_sqr);
 }
 
 int launch_scan(Context &c, int n, const int *cnt, int *out, long long *d_total)

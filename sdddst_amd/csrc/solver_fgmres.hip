@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "context.h"
+#include "precond_core.h"
 #include "solvers.h"
 
 namespace sdamd {
@@ -108,47 +109,17 @@ __global__ void k_jacobi_setup(int n, const int *__restrict__ diag, const double
     if (i < n) dinv[i] = 1.0 / Ax[diag[i]];
 }
 
-// Pair-block Jacobi.  Row i of A lives at positions Ap[i]..Ap[i+1] with
-// A_ij = AxRow[k] and A_ji = Ax[k] (column i of the CSC holds row i's
-// transpose partners).  The partner of i is the j maximising the coupling
-// |A_ij A_ji| / |A_ii A_jj|; one wavefront per row.
+// Pair-block Jacobi (row bodies in precond_core.h), one wavefront per row.
 __global__ __launch_bounds__(256) void k_pair_partner(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                       const int *__restrict__ diag, const double *__restrict__ Ax,
                                                       const double *__restrict__ AxRow, int *__restrict__ partner,
                                                       double *__restrict__ strength)
 {
-    const int lane = (int)(threadIdx.x & 63);
     const int i = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (i >= n) return;
-    const double aii = __builtin_fabs(AxRow[diag[i]]);
-    double best = -1.0;
-    int bj = -1;
-    for (int k = Ap[i] + lane; k < Ap[i + 1]; k += 64) {
-        const int j = Ai[k];
-        if (j == i) continue;
-        const double ajj = __builtin_fabs(AxRow[diag[j]]);
-        const double s = __builtin_fabs(AxRow[k] * Ax[k]) / (aii * ajj);
-        if (s > best || (s == best && j < bj)) {
-            best = s;
-            bj = j;
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_down(best, off, 64);
-        const int oj = __shfl_down(bj, off, 64);
-        if (ob > best || (ob == best && oj >= 0 && (bj < 0 || oj < bj))) {
-            best = ob;
-            bj = oj;
-        }
-    }
-    if (lane == 0) {
-        partner[i] = bj;
-        strength[i] = best;
-    }
+    pairblk::partner_row(i, Ap, Ai, diag, Ax, AxRow, partner, strength);
 }
 
-// Mutual partners with coupling above the threshold form a 2x2 block; the
-// row of the block inverse is stored as (binv_ii, binv_ij, j), else 1/A_ii.
 __global__ __launch_bounds__(256) void k_pair_blocks(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                      const int *__restrict__ diag, const double *__restrict__ Ax,
                                                      const double *__restrict__ AxRow, const int *__restrict__ partner,
@@ -157,26 +128,7 @@ __global__ __launch_bounds__(256) void k_pair_blocks(int n, const int *__restric
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    const double aii = AxRow[diag[i]];
-    const int j = partner[i];
-    if (j >= 0 && partner[j] == i && strength[i] > threshold) {
-        // A_ij and A_ji: position of j in row i
-        int lo = Ap[i], hi = Ap[i + 1];
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (Ai[mid] < j) lo = mid + 1;
-            else hi = mid;
-        }
-        const double aij = AxRow[lo], aji = Ax[lo], ajj = AxRow[diag[j]];
-        const double det = aii * ajj - aij * aji;
-        binv[2 * i] = ajj / det;
-        binv[2 * i + 1] = -aij / det;
-        bcol[i] = j;
-    } else {
-        binv[2 * i] = 1.0 / aii;
-        binv[2 * i + 1] = 0.0;
-        bcol[i] = -1;
-    }
+    pairblk::block_row(i, Ap, Ai, diag, Ax, AxRow, partner, strength, threshold, binv, bcol);
 }
 
 __global__ void k_pair_apply(int n, const double *__restrict__ binv, const int *__restrict__ bcol,

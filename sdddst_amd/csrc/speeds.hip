@@ -29,6 +29,7 @@
 
 #include "context.h"
 #include "field.h"
+#include "step_core.h"
 
 namespace sdamd {
 
@@ -128,11 +129,7 @@ __device__ __forceinline__ double pd_pair(double xi, double yi, double bI, doubl
 }
 
 // precision_handler.cpp:72-87 applied to the minimum candidate.
-__device__ __forceinline__ void tol_update(double &tol, double distanceSqr, double minPrecSqr)
-{
-    const double tmp = distanceSqr * 0.25 * 1e-2;
-    if (tmp < minPrecSqr && tmp < tol) tol = (tmp == 0.0) ? 1e-12 : tmp;
-}
+using stepf::tol_update;
 
 __global__ void k_speeds_finish(int n, int nb, int pc, const double *__restrict__ partial,
                                 const double *__restrict__ x, const double *__restrict__ y,

@@ -12,6 +12,7 @@
 
 #include "context.h"
 #include "field.h"
+#include "step_core.h"
 
 namespace sdamd {
 
@@ -23,7 +24,7 @@ __global__ void k_residual(int n, double h, const double *__restrict__ xn, const
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    g[i] = xn[i] - (1.0 + d[i]) * 0.5 * h * csp[i] - xo[i] - (1.0 - d[i]) * 0.5 * h * isp[i];
+    g[i] = stepf::residual(xn[i], xo[i], csp[i], isp[i], d[i], h);
 }
 
 __global__ void k_newton_update(int n, double *__restrict__ x, const double *__restrict__ delta)
@@ -34,24 +35,6 @@ __global__ void k_newton_update(int n, double *__restrict__ x, const double *__r
 }
 
 // precision_handler.cpp:103-119 + simulation.cpp:646-649
-__device__ double new_step_size(double maxRatio, double h, double hmax)
-{
-    double hn;
-    if (0.0 == maxRatio) {
-        hn = h * 2.0;
-    } else {
-        double tmp = 1. / sdlibm::sqrt(maxRatio);
-        tmp = sdlibm::pow_one_third(tmp);
-        if (tmp > 2.0) tmp = 2.0;
-        hn = 0.9 * h * tmp;
-    }
-    if (hmax >= 0 && hmax < hn) hn = hmax;
-    return hn;
-}
-
-// One workgroup: max_i err_i^2/tol_i over err_i = |x_big - x_small2| > 0,
-// first index of the maximum (strict '>' in updateError keeps the first),
-// then the tolerance reset to prec^2 (PrecisionHandler::reset).
 __global__ __launch_bounds__(1024) void k_error_control(int n, const double *__restrict__ xbig,
                                                         const double *__restrict__ xsmall2, double *__restrict__ tol,
                                                         double minPrecSqr, double h, double hmax,
@@ -92,7 +75,7 @@ __global__ __launch_bounds__(1024) void k_error_control(int n, const double *__r
         out->max_ratio = m;
         out->selected = (m > 0.0) ? sidx[0] : 0;
         out->accept = m < 1.0 ? 1 : 0;
-        out->h_next = new_step_size(m, h, hmax);
+        out->h_next = stepf::new_step_size(m, h, hmax);
     }
 }
 

@@ -229,9 +229,11 @@ int sdddst_kernel_stats(sdddst_ctx *ctx, const char *kernel, int *count, double 
  *   target_steps successful steps (or max_attempts ran); one attempt is a
  *   fixed launch sequence, replayed from a captured HIP graph when
  *   use_graph; attempts receives the number run.
- * autotune: times the speeds workgroup size (64/128/256 targets) and the
- *   solver workgroup size (128..1024 lanes per realization) on this
- *   ensemble, keeps the fastest, writes a report; set_tiles sets them.
+ * autotune: times the speeds kernels (symmetric 64x64 tiles, or one lane
+ *   per target in workgroups of 64/128/256) and the solver shapes
+ *   (256/512/1024 lanes per realization x 4/8/16/64 lanes per matrix row)
+ *   on this ensemble, keeps the fastest, writes a report; set_tiles sets
+ *   them (speeds_block 0: the symmetric tiles).
  * state: per realization time, step size, successful and failed steps.
  * positions: x of every realization (concatenated).
  * stats: attempts run, FGMRES iterations and solves, dense-LU fallbacks,
@@ -240,14 +242,15 @@ typedef struct sdddst_ensemble sdddst_ensemble;
 int sdddst_ensemble_create(int num_systems, const int *n, const double *x, const double *y, const double *b,
                            const sdddst_params *params, double sigma, double initial_step, sdddst_ensemble **out);
 int sdddst_ensemble_destroy(sdddst_ensemble *ens);
-int sdddst_ensemble_set_tiles(sdddst_ensemble *ens, int speeds_block, int solver_block);
+int sdddst_ensemble_set_tiles(sdddst_ensemble *ens, int speeds_block, int solver_block, int spmv_lanes);
 int sdddst_ensemble_autotune(sdddst_ensemble *ens, char *report, int report_len);
 int sdddst_ensemble_run(sdddst_ensemble *ens, int64_t target_steps, int64_t max_attempts, int use_graph,
                         int64_t *attempts);
 int sdddst_ensemble_state(sdddst_ensemble *ens, double *t, double *h, int64_t *steps, int64_t *failed);
 int sdddst_ensemble_positions(sdddst_ensemble *ens, double *x);
 int sdddst_ensemble_stats(sdddst_ensemble *ens, int64_t *attempts, int64_t *solver_iters, int64_t *solves,
-                          int64_t *lu_fallbacks, int64_t *regrows, int *speeds_block, int *solver_block);
+                          int64_t *lu_fallbacks, int64_t *regrows, int *speeds_block, int *solver_block,
+                          int *spmv_lanes);
 
 #ifdef __cplusplus
 }

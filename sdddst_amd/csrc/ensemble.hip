@@ -33,6 +33,7 @@
 #include "context.h"
 #include "field.h"
 #include "jac_core.h"
+#include "pair_core.h"
 #include "precond_core.h"
 #include "step_core.h"
 
@@ -47,8 +48,8 @@ enum { DISL = 0, BIG = 1, FS = 2, SS = 3, NSLOT = 4 };
 enum { VINIT = 0, VINIT2 = 1, VSPEED = 2, VSPEED2 = 3, NVB = 4 };
 
 struct SysDesc {
-    int off, n, words, pad;
-    long long moff, nzoff, cap, voff, zoff, loff;
+    int off, n, words, nb;          // nb: 64-wide tiles per side
+    long long moff, nzoff, cap, voff, zoff, loff, poff;
     jac::CutParams cp;
 };
 
@@ -138,6 +139,61 @@ __global__ void k_e_speeds(const int *__restrict__ blk, const SysDesc *__restric
     acc += bI * sigma;
     v[off + i] = acc;
     if (update_tol) tol[off + i] = t;
+}
+
+__global__ void k_e_fill_minr2(int nt, const int *__restrict__ sys, const SysState *__restrict__ st,
+                               int only_accepted, unsigned long long *__restrict__ minr2)
+{
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= nt) return;
+    const SysState &q = st[sys[g]];
+    if (!live(q) || (only_accepted && !q.accept)) return;
+    minr2[g] = pairtile::kInfBits;
+}
+
+// The symmetric tiles of every realization (tile bodies shared with the
+// single-system kernel); tl holds (realization, tile id) per workgroup.
+template <int Mode>
+__global__ __launch_bounds__(64) void k_e_pair_tiles(const int *__restrict__ tl, const SysDesc *__restrict__ D,
+                                                     const SysState *__restrict__ st, int only_accepted,
+                                                     const double *__restrict__ x, const double *__restrict__ y,
+                                                     const double *__restrict__ b, double *__restrict__ partial,
+                                                     unsigned long long *__restrict__ minr2)
+{
+    const int s = tl[2 * blockIdx.x];
+    const SysState &q = st[s];
+    if (!live(q) || (only_accepted && !q.accept)) return;
+    int bi, bj;
+    pairtile::tile_coords(tl[2 * blockIdx.x + 1], bi, bj);
+    const SysDesc &d = D[s];
+    pairtile::tile<Mode>(bi, bj, d.n, x + d.off, y + d.off, b + d.off, partial + d.poff, minr2 + d.off);
+}
+
+// Sum of a target's tile partials in tile order, tolerance update, stress.
+__global__ void k_e_speeds_finish(int nt, const int *__restrict__ sys, const SysDesc *__restrict__ D,
+                                  const SysState *__restrict__ st, int only_accepted,
+                                  const double *__restrict__ partial, const unsigned long long *__restrict__ minr2,
+                                  const double *__restrict__ b, double sigma, double *__restrict__ tol,
+                                  double minPrecSqr, int update_tol, double *__restrict__ v)
+{
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= nt) return;
+    const int s = sys[g];
+    const SysState &q = st[s];
+    if (!live(q) || (only_accepted && !q.accept)) return;
+    const SysDesc &d = D[s];
+    const int i = g - d.off;
+    const double *ps = partial + d.poff;
+    double acc = 0.0;
+    for (int t = 0; t < d.nb; ++t) acc += ps[(size_t)t * d.n + i];
+    if (update_tol) {
+        double t = tol[g];
+        const unsigned long long bits = minr2[g];
+        if (bits != pairtile::kInfBits) stepf::tol_update(t, __longlong_as_double((long long)bits), minPrecSqr);
+        tol[g] = t;
+    }
+    acc += b[g] * sigma;
+    v[g] = acc;
 }
 
 // -- Jacobian (column bodies shared with the single-system path) -------------
@@ -355,19 +411,23 @@ __device__ __forceinline__ double block_sum(double v, double *red, double *bcast
     return *bcast;
 }
 
-// out = A v on the CSR view (rows by wavefronts)
-template <int BS>
+// out = A v on the CSR view: groups of L lanes per row, 64 / L rows per
+// wavefront in flight (L is a tuning parameter: short rows want small L).
+template <int BS, int L>
 __device__ __forceinline__ void block_spmv(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                            const double *__restrict__ Av, const double *__restrict__ v,
                                            double *__restrict__ out)
 {
-    constexpr int NW = BS / 64;
+    constexpr int NW = BS / 64, G = 64 / L;
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    for (int r = w; r < n; r += NW) {
+    const int sub = lane / L, sl = lane % L;
+    for (int r0 = w * G; r0 < n; r0 += NW * G) {
+        const int r = r0 + sub;
         double s = 0.0;
-        for (int k = Ap[r] + lane; k < Ap[r + 1]; k += 64) s += Av[k] * v[Ai[k]];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (lane == 0) out[r] = s;
+        if (r < n)
+            for (int k = Ap[r] + sl; k < Ap[r + 1]; k += L) s += Av[k] * v[Ai[k]];
+        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (sl == 0 && r < n) out[r] = s;
     }
     __syncthreads();
 }
@@ -452,7 +512,7 @@ __device__ bool block_dense_solve(int n, double *__restrict__ L, double *__restr
     return true;
 }
 
-template <int BS>
+template <int BS, int L>
 __global__ __launch_bounds__(BS) void k_e_fgmres(const SysDesc *__restrict__ D, SysState *__restrict__ st,
                                                  const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                  const double *__restrict__ Ax, const double *__restrict__ AxRow,
@@ -500,7 +560,7 @@ __global__ __launch_bounds__(BS) void k_e_fgmres(const SysDesc *__restrict__ D, 
             for (int i = tid; i < n; i += BS) V[i] = g[i];
             __syncthreads();
         } else {
-            block_spmv<BS>(n, ap, ai, ar, x, V);
+            block_spmv<BS, L>(n, ap, ai, ar, x, V);
             for (int i = tid; i < n; i += BS) V[i] = g[i] - V[i];
             __syncthreads();
         }
@@ -525,7 +585,7 @@ __global__ __launch_bounds__(BS) void k_e_fgmres(const SysDesc *__restrict__ D, 
                 zk[i] = r;
             }
             __syncthreads();
-            block_spmv<BS>(n, ap, ai, ar, zk, wv);
+            block_spmv<BS, L>(n, ap, ai, ar, zk, wv);
             // classical Gram-Schmidt, twice
             if (tid <= k) H[tid * M + k] = 0.0;
             for (int pass = 0; pass < 2; ++pass) {
@@ -597,7 +657,7 @@ __global__ __launch_bounds__(BS) void k_e_fgmres(const SysDesc *__restrict__ D, 
         __syncthreads();
     }
     // true residual
-    block_spmv<BS>(n, ap, ai, ar, x, V);
+    block_spmv<BS, L>(n, ap, ai, ar, x, V);
     acc = 0;
     for (int i = tid; i < n; i += BS) {
         const double r = g[i] - V[i];
@@ -614,14 +674,14 @@ __global__ __launch_bounds__(BS) void k_e_fgmres(const SysDesc *__restrict__ D, 
         if (tid == 0) st[s].solve_fail += 1;
         return;
     }
-    double *L = dense + d.loff;
-    for (long long t = tid; t < (long long)n * n; t += BS) L[t] = 0.0;
+    double *LU = dense + d.loff;
+    for (long long t = tid; t < (long long)n * n; t += BS) LU[t] = 0.0;
     __syncthreads();
     for (int c = 0; c < n; ++c)
-        for (int k = ap[c] + tid; k < ap[c + 1]; k += BS) L[ai[k] + (size_t)c * n] = Ax[d.nzoff + k];
+        for (int k = ap[c] + tid; k < ap[c + 1]; k += BS) LU[ai[k] + (size_t)c * n] = Ax[d.nzoff + k];
     for (int i = tid; i < n; i += BS) x[i] = g[i];
     __syncthreads();
-    const bool ok = block_dense_solve<BS>(n, L, x, red, redi, &bcast, &bcasti);
+    const bool ok = block_dense_solve<BS>(n, LU, x, red, redi, &bcast, &bcasti);
     if (tid == 0) {
         st[s].lu_fallbacks += 1;
         if (!ok) st[s].solve_fail += 1;
@@ -732,10 +792,14 @@ struct Ensemble {
     double *strength = nullptr, *binv = nullptr;
     double *V = nullptr, *Z = nullptr, *dense = nullptr;
     long long nzTotal = 0, vTotal = 0, zTotal = 0, lTotal = 0, mTotal = 0;
-    // launch configuration (autotuned)
-    int speedsBlock = 128, solverBlock = 256;
-    int *blk = nullptr;
-    int nblk = 0, blkFor = 0;
+    double *partial = nullptr;
+    unsigned long long *minr2 = nullptr;
+    long long pTotal = 0;
+    // launch configuration (autotuned): speedsBlock 0 = symmetric tiles,
+    // else one lane per target in workgroups of that size
+    int speedsBlock = 0, solverBlock = 256, spmvLanes = 8;
+    int *blk = nullptr, *tl = nullptr;
+    int nblk = 0, blkFor = 0, ntiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long long attempts = 0, regrows = 0;
@@ -766,9 +830,11 @@ long long default_cap(int n, double m)
 
 int ens_layout(Ensemble &e)
 {
-    long long nz = 0, vo = 0, zo = 0, lo = 0, mo = 0;
+    long long nz = 0, vo = 0, zo = 0, lo = 0, mo = 0, po = 0;
     for (int s = 0; s < e.S; ++s) {
         SysDesc &d = e.hd[s];
+        d.poff = po;
+        po += (long long)d.nb * d.n;
         d.moff = mo;
         mo += (long long)d.n * d.words;
         d.nzoff = nz;
@@ -806,7 +872,30 @@ int ens_layout(Ensemble &e)
         if ((rc = ealloc(&e.dense, lo))) return rc;
         e.lTotal = lo;
     }
+    if (po != e.pTotal) {
+        if ((rc = ealloc(&e.partial, po))) return rc;
+        e.pTotal = po;
+    }
     SD_HIP(hipMemcpyAsync(e.D, e.hd.data(), sizeof(SysDesc) * e.S, hipMemcpyHostToDevice, e.stream));
+    return SDDDST_OK;
+}
+
+int build_tiles(Ensemble &e)
+{
+    std::vector<int> h;
+    for (int s = 0; s < e.S; ++s) {
+        const long long nt = (long long)e.hd[s].nb * (e.hd[s].nb + 1) / 2;
+        for (long long t = 0; t < nt; ++t) {
+            h.push_back(s);
+            h.push_back((int)t);
+        }
+    }
+    SD_CHECK(h.size() / 2 < (1u << 31), SDDDST_ERR_ARG, "too many tiles");
+    int rc = ealloc(&e.tl, (long long)h.size());
+    if (rc) return rc;
+    SD_HIP(hipMemcpyAsync(e.tl, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, e.stream));
+    SD_HIP(hipStreamSynchronize(e.stream));
+    e.ntiles = (int)(h.size() / 2);
     return SDDDST_OK;
 }
 
@@ -829,8 +918,22 @@ int build_blocks(Ensemble &e, int T)
 
 int launch_speeds(Ensemble &e, int slot, int buf, int update_tol, int only_accepted)
 {
-    const size_t shm = sizeof(double) * 3 * e.speedsBlock;
     const double mp2 = e.p.precision * e.p.precision;
+    if (e.speedsBlock == 0) {
+        hipLaunchKernelGGL(k_e_fill_minr2, dim3(nb(e.NT)), dim3(256), 0, e.stream, e.NT, e.sys, e.st, only_accepted,
+                           e.minr2);
+        if (e.p.field_mode == SDDDST_FIELD_FAST)
+            hipLaunchKernelGGL(k_e_pair_tiles<SDDDST_FIELD_FAST>, dim3(e.ntiles), dim3(64), 0, e.stream, e.tl, e.D,
+                               e.st, only_accepted, e.x[slot], e.y, e.b, e.partial, e.minr2);
+        else
+            hipLaunchKernelGGL(k_e_pair_tiles<SDDDST_FIELD_EXACT>, dim3(e.ntiles), dim3(64), 0, e.stream, e.tl, e.D,
+                               e.st, only_accepted, e.x[slot], e.y, e.b, e.partial, e.minr2);
+        hipLaunchKernelGGL(k_e_speeds_finish, dim3(nb(e.NT)), dim3(256), 0, e.stream, e.NT, e.sys, e.D, e.st,
+                           only_accepted, e.partial, e.minr2, e.b, e.sigma, e.tol, mp2, update_tol, e.v[buf]);
+        SD_HIP(hipGetLastError());
+        return SDDDST_OK;
+    }
+    const size_t shm = sizeof(double) * 3 * e.speedsBlock;
     if (e.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL(k_e_speeds<SDDDST_FIELD_FAST>, dim3(e.nblk), dim3(e.speedsBlock), shm, e.stream, e.blk, e.D,
                            e.st, only_accepted, e.x[slot], e.y, e.b, e.sigma, e.tol, mp2, update_tol, e.v[buf]);
@@ -868,26 +971,31 @@ int launch_jacobian(Ensemble &e, int slot, double hscale, bool build)
     return SDDDST_OK;
 }
 
+template <int BS, int L>
+void solve_variant(Ensemble &e, double tol)
+{
+    hipLaunchKernelGGL((k_e_fgmres<BS, L>), dim3(e.S), dim3(BS), 0, e.stream, e.D, e.st, e.Ap, e.Ai, e.Ax, e.AxRow,
+                       e.binv, e.bcol, e.g, e.delta, e.V, e.Z, e.dense, tol);
+}
+
+template <int BS>
+void solve_lanes(Ensemble &e, double tol)
+{
+    switch (e.spmvLanes) {
+    case 4: solve_variant<BS, 4>(e, tol); break;
+    case 16: solve_variant<BS, 16>(e, tol); break;
+    case 64: solve_variant<BS, 64>(e, tol); break;
+    default: solve_variant<BS, 8>(e, tol); break;
+    }
+}
+
 int launch_solve(Ensemble &e)
 {
     const double tol = 1e-10;
     switch (e.solverBlock) {
-    case 128:
-        hipLaunchKernelGGL(k_e_fgmres<128>, dim3(e.S), dim3(128), 0, e.stream, e.D, e.st, e.Ap, e.Ai, e.Ax, e.AxRow,
-                           e.binv, e.bcol, e.g, e.delta, e.V, e.Z, e.dense, tol);
-        break;
-    case 512:
-        hipLaunchKernelGGL(k_e_fgmres<512>, dim3(e.S), dim3(512), 0, e.stream, e.D, e.st, e.Ap, e.Ai, e.Ax, e.AxRow,
-                           e.binv, e.bcol, e.g, e.delta, e.V, e.Z, e.dense, tol);
-        break;
-    case 1024:
-        hipLaunchKernelGGL(k_e_fgmres<1024>, dim3(e.S), dim3(1024), 0, e.stream, e.D, e.st, e.Ap, e.Ai, e.Ax, e.AxRow,
-                           e.binv, e.bcol, e.g, e.delta, e.V, e.Z, e.dense, tol);
-        break;
-    default:
-        hipLaunchKernelGGL(k_e_fgmres<256>, dim3(e.S), dim3(256), 0, e.stream, e.D, e.st, e.Ap, e.Ai, e.Ax, e.AxRow,
-                           e.binv, e.bcol, e.g, e.delta, e.V, e.Z, e.dense, tol);
-        break;
+    case 512: solve_lanes<512>(e, tol); break;
+    case 1024: solve_lanes<1024>(e, tol); break;
+    default: solve_lanes<256>(e, tol); break;
     }
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
@@ -1008,7 +1116,7 @@ void ensemble_free(Ensemble *e)
     void *ptrs[] = {e->D,    e->st,   e->sys,     e->y,      e->b,     e->tol,  e->g,       e->delta,
                     e->dvec, e->cnt,  e->Ap,      e->diag,   e->flags, e->mask, e->Ai,      e->mirror,
                     e->Ax,   e->AxRow, e->tau,    e->mult,   e->core,  e->partner, e->bcol, e->strength,
-                    e->binv, e->V,    e->Z,       e->dense,  e->blk};
+                    e->binv, e->V,    e->Z,       e->dense,  e->blk,  e->tl,   e->partial, e->minr2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : e->x)
@@ -1044,6 +1152,7 @@ int ensemble_create(int S, const int *n, const double *x, const double *y, const
         d.off = (int)nt;
         d.n = n[s];
         d.words = (n[s] + 63) / 64;
+        d.nb = d.words;
         double m = p->cutoff_multiplier;
         if (m >= 1. / 12. * sqrt(2.0 * double(n[s]))) m = 1e20;
         const double rc = 1. / sqrt((double)n[s]) * m;
@@ -1073,7 +1182,7 @@ int ensemble_create(int S, const int *n, const double *x, const double *y, const
         (rc = ealloc(&e->delta, NT)) || (rc = ealloc(&e->dvec, NT)) || (rc = ealloc(&e->cnt, NT)) ||
         (rc = ealloc(&e->Ap, (long long)NT + S)) || (rc = ealloc(&e->diag, NT)) || (rc = ealloc(&e->flags, S)) ||
         (rc = ealloc(&e->partner, NT)) || (rc = ealloc(&e->bcol, NT)) || (rc = ealloc(&e->strength, NT)) ||
-        (rc = ealloc(&e->binv, 2LL * NT)))
+        (rc = ealloc(&e->binv, 2LL * NT)) || (rc = ealloc(&e->minr2, NT)))
         return fail(rc);
     for (int k = 0; k < ens::NSLOT; ++k)
         if ((rc = ealloc(&e->x[k], NT))) return fail(rc);
@@ -1100,7 +1209,7 @@ int ensemble_create(int S, const int *n, const double *x, const double *y, const
         (rc = up(e->y, y, sizeof(double) * NT)) || (rc = up(e->b, b, sizeof(double) * NT)) ||
         (rc = up(e->tol, tol0.data(), sizeof(double) * NT)) || (rc = up(e->st, h.data(), sizeof(SysState) * S)))
         return fail(rc);
-    if ((rc = build_blocks(*e, e->speedsBlock))) return fail(rc);
+    if ((rc = build_tiles(*e))) return fail(rc);
     // the first step's speeds (simulation.cpp:450-456): tolerance updated
     if ((rc = launch_speeds(*e, ens::DISL, ens::VINIT, 1, 0))) return fail(rc);
     SD_HIP(hipStreamSynchronize(e->stream));
@@ -1108,18 +1217,21 @@ int ensemble_create(int S, const int *n, const double *x, const double *y, const
     return SDDDST_OK;
 }
 
-int ensemble_set_tiles(Ensemble &e, int speeds_block, int solver_block)
+int ensemble_set_tiles(Ensemble &e, int speeds_block, int solver_block, int spmv_lanes)
 {
-    SD_CHECK(speeds_block == 64 || speeds_block == 128 || speeds_block == 256, SDDDST_ERR_ARG,
-             "speeds block must be 64, 128 or 256");
-    SD_CHECK(solver_block == 128 || solver_block == 256 || solver_block == 512 || solver_block == 1024, SDDDST_ERR_ARG,
-             "solver block must be 128, 256, 512 or 1024");
-    if (speeds_block != e.blkFor) {
+    SD_CHECK(speeds_block == 0 || speeds_block == 64 || speeds_block == 128 || speeds_block == 256, SDDDST_ERR_ARG,
+             "speeds block must be 0 (symmetric tiles), 64, 128 or 256");
+    SD_CHECK(solver_block == 256 || solver_block == 512 || solver_block == 1024, SDDDST_ERR_ARG,
+             "solver block must be 256, 512 or 1024");
+    SD_CHECK(spmv_lanes == 4 || spmv_lanes == 8 || spmv_lanes == 16 || spmv_lanes == 64, SDDDST_ERR_ARG,
+             "lanes per row must be 4, 8, 16 or 64");
+    if (speeds_block > 0 && speeds_block != e.blkFor) {
         int rc = build_blocks(e, speeds_block);
         if (rc) return rc;
     }
     e.speedsBlock = speeds_block;
     e.solverBlock = solver_block;
+    e.spmvLanes = spmv_lanes;
     drop_graph(e);
     return SDDDST_OK;
 }
@@ -1173,8 +1285,8 @@ int ensemble_autotune(Ensemble &e, std::string &report)
     int bestT = e.speedsBlock;
     char line[160];
     int rc;
-    for (int T : {64, 128, 256}) {
-        if ((rc = build_blocks(e, T))) return rc;
+    for (int T : {0, 64, 128, 256}) {
+        if (T > 0 && (rc = build_blocks(e, T))) return rc;
         e.speedsBlock = T;
         float ms = 0;
         for (int r = 0; r < 3; ++r) {
@@ -1187,14 +1299,15 @@ int ensemble_autotune(Ensemble &e, std::string &report)
             if (r > 0) ms += t;
         }
         ms /= 2;
-        snprintf(line, sizeof line, "speeds block %4d: %.3f ms\n", T, ms);
+        if (T == 0) snprintf(line, sizeof line, "speeds symmetric 64x64 tiles: %.3f ms\n", ms);
+        else snprintf(line, sizeof line, "speeds one lane per target, block %4d: %.3f ms\n", T, ms);
         report += line;
         if (ms < best) {
             best = ms;
             bestT = T;
         }
     }
-    if ((rc = build_blocks(e, bestT))) return rc;
+    if (bestT > 0 && (rc = build_blocks(e, bestT))) return rc;
     e.speedsBlock = bestT;
     // solver: a Newton system at the current positions (stage I's first
     // iteration), regrowing the CSC capacity if a count exceeds it
@@ -1212,9 +1325,11 @@ int ensemble_autotune(Ensemble &e, std::string &report)
         hipLaunchKernelGGL(k_e_activate, dim3(nb(e.S)), dim3(256), 0, e.stream, e.S, e.st, e.target);
     }
     best = 1e30f;
-    int bestB = e.solverBlock;
-    for (int B : {128, 256, 512, 1024}) {
+    int bestB = e.solverBlock, bestL = e.spmvLanes;
+    for (int B : {256, 512, 1024})
+    for (int L : {4, 8, 16, 64}) {
         e.solverBlock = B;
+        e.spmvLanes = L;
         float ms = 0;
         for (int r = 0; r < 3; ++r) {
             SD_HIP(hipEventRecord(a, e.stream));
@@ -1226,15 +1341,18 @@ int ensemble_autotune(Ensemble &e, std::string &report)
             if (r > 0) ms += t;
         }
         ms /= 2;
-        snprintf(line, sizeof line, "solver block %4d: %.3f ms\n", B, ms);
+        snprintf(line, sizeof line, "solver block %4d, %2d lanes per row: %.3f ms\n", B, L, ms);
         report += line;
         if (ms < best) {
             best = ms;
             bestB = B;
+            bestL = L;
         }
     }
     e.solverBlock = bestB;
-    snprintf(line, sizeof line, "chosen: speeds block %d, solver block %d\n", bestT, bestB);
+    e.spmvLanes = bestL;
+    snprintf(line, sizeof line, "chosen: speeds %s %d, solver block %d, %d lanes per row\n",
+             bestT == 0 ? "tiles" : "block", bestT, bestB, bestL);
     report += line;
     // the solves above only touched scratch; restore the counters
     if ((rc = read_state(e))) return rc;
@@ -1274,7 +1392,7 @@ int ensemble_positions(Ensemble &e, double *x)
 }
 
 int ensemble_stats(Ensemble &e, int64_t *attempts, int64_t *iters, int64_t *solves, int64_t *lu, int64_t *regrows,
-                   int *speeds_block, int *solver_block)
+                   int *speeds_block, int *solver_block, int *spmv_lanes)
 {
     int rc = read_state(e);
     if (rc) return rc;
@@ -1291,6 +1409,7 @@ int ensemble_stats(Ensemble &e, int64_t *attempts, int64_t *iters, int64_t *solv
     if (regrows) *regrows = e.regrows;
     if (speeds_block) *speeds_block = e.speedsBlock;
     if (solver_block) *solver_block = e.solverBlock;
+    if (spmv_lanes) *spmv_lanes = e.spmvLanes;
     return SDDDST_OK;
 }
 
@@ -1324,10 +1443,10 @@ int sdddst_ensemble_destroy(sdddst_ensemble *ens)
     return SDDDST_OK;
 }
 
-int sdddst_ensemble_set_tiles(sdddst_ensemble *ens, int speeds_block, int solver_block)
+int sdddst_ensemble_set_tiles(sdddst_ensemble *ens, int speeds_block, int solver_block, int spmv_lanes)
 {
     SD_CHECK(ens && ens->e, SDDDST_ERR_ARG, "ensemble is NULL");
-    return sdamd::ensemble_set_tiles(*ens->e, speeds_block, solver_block);
+    return sdamd::ensemble_set_tiles(*ens->e, speeds_block, solver_block, spmv_lanes);
 }
 
 int sdddst_ensemble_autotune(sdddst_ensemble *ens, char *report, int report_len)
@@ -1366,11 +1485,12 @@ int sdddst_ensemble_positions(sdddst_ensemble *ens, double *x)
 }
 
 int sdddst_ensemble_stats(sdddst_ensemble *ens, int64_t *attempts, int64_t *solver_iters, int64_t *solves,
-                          int64_t *lu_fallbacks, int64_t *regrows, int *speeds_block, int *solver_block)
+                          int64_t *lu_fallbacks, int64_t *regrows, int *speeds_block, int *solver_block,
+                          int *spmv_lanes)
 {
     SD_CHECK(ens && ens->e, SDDDST_ERR_ARG, "ensemble is NULL");
     return sdamd::ensemble_stats(*ens->e, attempts, solver_iters, solves, lu_fallbacks, regrows, speeds_block,
-                                 solver_block);
+                                 solver_block, spmv_lanes);
 }
 
 }  // extern "C"

@@ -23,12 +23,12 @@ _SIGS = {
     "sdddst_ensemble_create": [ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.POINTER(Params), ctypes.c_double,
                                ctypes.c_double, ctypes.POINTER(_vp)],
     "sdddst_ensemble_destroy": [_vp],
-    "sdddst_ensemble_set_tiles": [_vp, ctypes.c_int, ctypes.c_int],
+    "sdddst_ensemble_set_tiles": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "sdddst_ensemble_autotune": [_vp, ctypes.c_char_p, ctypes.c_int],
     "sdddst_ensemble_run": [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _i64p],
     "sdddst_ensemble_state": [_vp, _dp, _dp, _i64p, _i64p],
     "sdddst_ensemble_positions": [_vp, _dp],
-    "sdddst_ensemble_stats": [_vp, _i64p, _i64p, _i64p, _i64p, _i64p, _ip, _ip],
+    "sdddst_ensemble_stats": [_vp, _i64p, _i64p, _i64p, _i64p, _i64p, _ip, _ip, _ip],
 }
 
 
@@ -50,6 +50,7 @@ class EnsembleStats:
     regrows: int
     speeds_block: int
     solver_block: int
+    spmv_lanes: int
 
 
 class Ensemble:
@@ -93,8 +94,9 @@ class Ensemble:
     def __exit__(self, *exc):
         self.close()
 
-    def set_tiles(self, speeds_block: int, solver_block: int):
-        self._check(self.lib.sdddst_ensemble_set_tiles(self.h, speeds_block, solver_block))
+    def set_tiles(self, speeds_block: int, solver_block: int, spmv_lanes: int = 8):
+        """speeds_block 0: symmetric tiles; else one lane per target."""
+        self._check(self.lib.sdddst_ensemble_set_tiles(self.h, speeds_block, solver_block, spmv_lanes))
 
     def autotune(self) -> str:
         buf = ctypes.create_string_buffer(4096)
@@ -125,7 +127,7 @@ class Ensemble:
 
     def stats(self) -> EnsembleStats:
         v = [ctypes.c_int64() for _ in range(5)]
-        sb, vb = ctypes.c_int(), ctypes.c_int()
+        sb, vb, lb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.sdddst_ensemble_stats(self.h, *(ctypes.byref(q) for q in v), ctypes.byref(sb),
-                                                   ctypes.byref(vb)))
-        return EnsembleStats(*(q.value for q in v), sb.value, vb.value)
+                                                   ctypes.byref(vb), ctypes.byref(lb)))
+        return EnsembleStats(*(q.value for q in v), sb.value, vb.value, lb.value)

@@ -87,4 +87,5 @@ def test_autotune_and_capacity_regrowth():
         assert s[k] == sr and f[k] == fr
         assert abs(t[k] - tr) <= 1e-9 * abs(tr)
         assert np.max(np.abs(xg[k] - xr)) < 1e-9
-    assert st.speeds_block in (64, 128, 256) and st.solver_block in (128, 256, 512, 1024)
+    assert st.speeds_block in (0, 64, 128, 256) and st.solver_block in (256, 512, 1024)
+    assert st.spmv_lanes in (4, 8, 16, 64)

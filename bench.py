@@ -9,9 +9,15 @@ external stress 0.02, cutoff multiplier 2, precision 1e-6, initial step
 1e-6.  A "step" is one successful implicit step of sdddstCore::Simulation
 (three stages, Jacobian assemblies, Newton solves, the error controller and
 the log); failed attempts inside the timed region count against the rate.
+The headline solves with FGMRES and the learned preconditioner; the same
+run with the pair-block preconditioner alone (the method without the
+network), a few steps of the direct-solver path and the batched ensemble of
+256 realizations (N 64..2048, 500 steps each) are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver fgmres]
-Prints one JSON line.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precond learned]
+Prints one JSON line.  `python bench.py --copy-calibration` runs only a plain
+device copy of known size (the reference kernel for scaling the HBM byte
+counters of a profiler run).
 """
 from __future__ import annotations
 
@@ -41,7 +47,13 @@ def parse():
     p.add_argument("--n", type=int, default=16384)
     p.add_argument("--solver", default="fgmres", choices=["fgmres", "dense-lu", "auto", "umfpack"])
     p.add_argument("--field-mode", default="fast", choices=["fast", "exact"])
-    p.add_argument("--precond", default="pair", choices=["pair", "jacobi", "learned"])
+    p.add_argument("--precond", default="learned", choices=["pair", "jacobi", "learned"])
+    p.add_argument("--compare-steps", type=int, default=200,
+                   help="steps of the same workload with the pair-block preconditioner (0: skip)")
+    p.add_argument("--ensemble-steps", type=int, default=500,
+                   help="steps per realization of the 256-realization ensemble (0: skip)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run only a 512 MiB device-to-device copy kernel and exit")
     p.add_argument("--plain-steps", type=int, default=2,
                    help="also time this many steps of the direct-solver path (0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -49,12 +61,64 @@ def parse():
     return p.parse_args()
 
 
-def make_sim(args, solver, x, y, b):
+def make_sim(args, solver, x, y, b, precond=None):
     from sdddst_amd.simulation import Simulation
     return Simulation(x, y, b, cutoff_multiplier=2.0, precision=1e-6, initial_step=1e-6, stress="constant",
                       stress_value=0.02, calculate_strain=True, solver=solver, field_mode=args.field_mode,
-                      preconditioner=args.precond,
+                      preconditioner=precond or args.precond,
                       solver_tol=1e-10, solver_max_iter=400, solver_restart=40)
+
+
+def copy_calibration():
+    """A plain copy of 512 MiB (64 Mi float64) through torch: 512 MiB read,
+    512 MiB written; the profiler's FETCH_SIZE / WRITE_SIZE of this dispatch
+    give the scale of those counters."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": c.numel() * 8, "dispatches": 3}))
+
+
+def traffic_record():
+    """HBM bytes per launch of the pair kernel from the committed profiler
+    summary (two counter runs, scaled by the copy calibration)."""
+    path = os.path.join(HERE, "profiles", "pair_kernel_n16384_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+        return rec.get("hbm_bytes_per_launch"), rec
+    except (OSError, ValueError):
+        return None, None
+
+
+def ensemble_run(steps):
+    """The 256-realization batched ensemble (N log-uniform in [64, 2048],
+    cutoff multiplier 1, precision 1e-6, zero stress), autotuned, one
+    graph-captured attempt per replay."""
+    from sdddst_amd import configs
+    from sdddst_amd.ensemble import Ensemble
+    sizes, x, y, b = configs.ensemble_workload(256, 3)
+    with Ensemble(sizes, x, y, b, cutoff_multiplier=1.0, precision=1e-6, field_mode="fast") as e:
+        report = e.autotune()
+        e.run(1)
+        t0 = time.perf_counter()
+        attempts = e.run(steps + 1)
+        wall = time.perf_counter() - t0
+        t, h, s, f = e.state()
+        st = e.stats()
+    done = int(s.sum()) - len(sizes)
+    return {"workload": "256 realizations, N log-uniform in [64, 2048] (seed 3), m=1, precision 1e-6, zero stress",
+            "steps_per_realization": steps, "sum_n": int(sum(sizes)), "wall_s": wall, "attempts": attempts,
+            "ms_per_attempt": wall / attempts * 1e3, "realization_steps_per_s": done / wall,
+            "failed_attempts": int(f.sum()), "solver_iterations_per_solve": st.solver_iterations / max(1, st.solves),
+            "lu_fallbacks": st.lu_fallbacks,
+            "autotuned": {"speeds": "tiles" if st.speeds_block == 0 else f"per-target block {st.speeds_block}",
+                          "solver_block": st.solver_block, "spmv_lanes_per_row": st.spmv_lanes},
+            "autotune_report": report.strip().split("\n")}
 
 
 def log(msg):
@@ -113,6 +177,9 @@ def cpu_baseline(x, y, b, seconds):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(one process drives one MI355X; see the sharding design)", file=sys.stderr)
@@ -148,6 +215,7 @@ def main():
     sim.close()
 
     steps_per_s = ok / wall
+    traffic, traffic_rec = traffic_record()
     pairs = n * (n - 1) / 2
     avg_pair_ms = pair_ms / max(pair_cnt, 1)
     achieved_tflops = FLOPS_PER_PAIR * pairs / (avg_pair_ms * 1e-3) / 1e12
@@ -170,7 +238,7 @@ def main():
                    "failed_steps_in_timed_region": failed},
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": achieved_tflops,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
+                     "traffic": traffic, "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
                      "avg_launch_ms": avg_pair_ms, "launches": pair_cnt},
         "pairs_per_s": pairs / (avg_pair_ms * 1e-3),
         "speeds_eval_ms": sp_ms / max(sp_cnt, 1),
@@ -178,7 +246,22 @@ def main():
         "final_step_size": st.step_size,
         "solver_iterations_per_solve": iters / solves if solves else None,
         "solver_fallbacks_total": st.solver_fallbacks,
+        "solver_precond_fallbacks_total": st.solver_precond_fallbacks,
     }
+    if traffic_rec:
+        line["roofline"]["traffic_source"] = traffic_rec.get("source")
+    if args.compare_steps > 0 and args.solver == "fgmres" and args.precond != "pair":
+        log("pair-block preconditioner (the method without the network)")
+        sim3 = make_sim(args, "fgmres", x, y, b, precond="pair")
+        s3 = sim3.state()
+        w3, ok3, f3 = timed_steps(sim3, args.compare_steps, "pair timed")
+        st3 = sim3.state()
+        sv = st3.solver_total_solves - s3.solver_total_solves
+        line["pair_block_preconditioner"] = {
+            "steps_per_s": ok3 / w3, "steps": ok3, "failed": f3,
+            "solver_iterations_per_solve": (st3.solver_total_iters - s3.solver_total_iters) / sv if sv else None,
+            "headline_speedup": steps_per_s / (ok3 / w3)}
+        sim3.close()
     if args.plain_steps > 0 and args.solver != "dense-lu":
         log("direct-solver path")
         sim2 = make_sim(args, "dense-lu", x, y, b)
@@ -188,6 +271,9 @@ def main():
                                        "failed": f2,
                                        "speedup_of_headline": steps_per_s / (ok2 / w2)}
         sim2.close()
+    if args.ensemble_steps > 0:
+        log("batched ensemble")
+        line["ensemble_256"] = ensemble_run(args.ensemble_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(x, y, b, args.cpu_seconds)

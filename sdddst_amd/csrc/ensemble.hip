@@ -317,9 +317,10 @@ __global__ __launch_bounds__(256) void k_e_jac_raw(int nt, const int *__restrict
                     tau + d.nzoff, mult + d.nzoff, mirror + d.nzoff, 0.0, Ax + d.nzoff, flags + cv.s);
 }
 
-__global__ void k_e_jac_diag(int nt, const int *__restrict__ sys, const SysDesc *__restrict__ D,
-                             const SysState *__restrict__ st, const int *__restrict__ Ap,
-                             const int *__restrict__ diag, double *__restrict__ Ax, double *__restrict__ dvec)
+__global__ __launch_bounds__(256) void k_e_jac_diag(int nt, const int *__restrict__ sys,
+                                                    const SysDesc *__restrict__ D, const SysState *__restrict__ st,
+                                                    const int *__restrict__ Ap, const int *__restrict__ diag,
+                                                    double *__restrict__ Ax, double *__restrict__ dvec)
 {
     ColView cv;
     const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);

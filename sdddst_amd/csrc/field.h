@@ -11,9 +11,11 @@
 // xy_fast() is the production form of xy(): the image at the
 // minimum-image offset (the only one that can suffer cancellation) is
 // evaluated exactly as the reference does; the nine far images take their
-// cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors.  Far
-// image terms then differ from the reference by a few ulps of terms that are
-// at least e^-pi smaller than the near term.
+// cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors, and
+// their nine denominators (cosh - cos)^2 are inverted together (prefix
+// products, one division, back substitution: the products stay within
+// [1e2, 1e160]).  Far-image terms then differ from the reference by a few
+// ulps of terms that are at least e^-pi smaller than the near term.
 #pragma once
 
 #include "libm_compat.h"
@@ -157,19 +159,39 @@ SDLIBM_FN double xy_fast(double dx, double dy)
     const bool neg = dx < 0;
     const double s = neg ? 1.0 : -1.0;
     const double sdx = neg ? -dx : dx;
-    // cosh(2 pi (dx + j)) for j = +-1..+-5.
-    double g = near;
+    // Far images: offsets -1, +1, -2, +2, -3, +3, then 5s, -4s, +4s (the
+    // last three weighted as in AnalyticField.cpp:148-159).
+    constexpr int kFar = 2 * (kImages - 1) + 3;
+    double num[kFar], den[kFar];
     for (int k = 1; k < kImages; ++k) {
         const double cm = 0.5 * (ep * sdlibm_exp_m2pik[k] + em * sdlibm_exp_2pik[k]);
         const double cp = 0.5 * (ep * sdlibm_exp_2pik[k] + em * sdlibm_exp_m2pik[k]);
-        g = g + f_given_cosh(dx - (double)k, cm, cy) + f_given_cosh(dx + (double)k, cp, cy);
+        num[2 * k - 2] = (dx - (double)k) * (cm * cy - 1.0);
+        den[2 * k - 2] = (cm - cy) * (cm - cy);
+        num[2 * k - 1] = (dx + (double)k) * (cp * cy - 1.0);
+        den[2 * k - 1] = (cp - cy) * (cp - cy);
     }
-    // Images at dx + 5s, dx - 4s, dx + 4s.
     const double c5 = neg ? 0.5 * (ep * sdlibm_exp_2pik[5] + em * sdlibm_exp_m2pik[5]) : 0.5 * (ep * sdlibm_exp_m2pik[5] + em * sdlibm_exp_2pik[5]);
     const double c4m = neg ? 0.5 * (ep * sdlibm_exp_m2pik[4] + em * sdlibm_exp_2pik[4]) : 0.5 * (ep * sdlibm_exp_2pik[4] + em * sdlibm_exp_m2pik[4]);
     const double c4p = neg ? 0.5 * (ep * sdlibm_exp_2pik[4] + em * sdlibm_exp_m2pik[4]) : 0.5 * (ep * sdlibm_exp_m2pik[4] + em * sdlibm_exp_2pik[4]);
-    return sdx * f_given_cosh(dx + 5.0 * s, c5, cy) + (1.0 + s * dx) * f_given_cosh(dx - 4.0 * s, c4m, cy) +
-           f_given_cosh(dx + 4.0 * s, c4p, cy) + g;
+    num[kFar - 3] = sdx * ((dx + 5.0 * s) * (c5 * cy - 1.0));
+    den[kFar - 3] = (c5 - cy) * (c5 - cy);
+    num[kFar - 2] = (1.0 + s * dx) * ((dx - 4.0 * s) * (c4m * cy - 1.0));
+    den[kFar - 2] = (c4m - cy) * (c4m - cy);
+    num[kFar - 1] = (dx + 4.0 * s) * (c4p * cy - 1.0);
+    den[kFar - 1] = (c4p - cy) * (c4p - cy);
+    // one division for the nine reciprocals
+    double pre[kFar];
+    pre[0] = den[0];
+    for (int m = 1; m < kFar; ++m) pre[m] = pre[m - 1] * den[m];
+    double inv = 1.0 / pre[kFar - 1];
+    double far = 0.0;
+    for (int m = kFar - 1; m > 0; --m) {
+        far += num[m] * (inv * pre[m - 1]);
+        inv *= den[m];
+    }
+    far += num[0] * inv;
+    return near + far * (2.0 * kPiSq);
 }
 
 // simulation.cpp:288-322: x-derivative of the point-defect force on a

@@ -172,14 +172,23 @@ __device__ __forceinline__ void raw_column(int j, const double *__restrict__ b, 
     }
 }
 
-// simulation.cpp:349-374 (one lane): the column sum in storage order, the
-// diagonal s and d_j.
+// simulation.cpp:349-374 (one lane per column): the column sum in storage
+// order, the diagonal s and d_j.  Eight loads are issued before their adds
+// so the memory latency overlaps; the adds keep the sequential order.
 __device__ __forceinline__ void diag_column(int j, const int *__restrict__ Ap, const int *__restrict__ diag,
                                             double *__restrict__ Ax, double *__restrict__ dvec)
 {
     double subSum = 0;
     const int e = Ap[j + 1];
-    for (int k = Ap[j]; k < e; ++k) subSum += Ax[k];
+    int k = Ap[j];
+    for (; k + 8 <= e; k += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = Ax[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) subSum += a[u];
+    }
+    for (; k < e; ++k) subSum += Ax[k];
     subSum *= -1.;
     Ax[diag[j]] = subSum;
     double dv;

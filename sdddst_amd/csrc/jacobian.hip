@@ -150,8 +150,8 @@ __global__ __launch_bounds__(256) void k_jac_raw(int n, const double *__restrict
     jac::raw_column(j, b, h, Ap, Ai, diag, tau, mult, mirror, pdtmp ? pdtmp[j] : 0.0, Ax, flags);
 }
 
-__global__ void k_jac_diag(int n, const int *__restrict__ Ap, const int *__restrict__ diag, double *__restrict__ Ax,
-                           double *__restrict__ dvec)
+__global__ __launch_bounds__(256) void k_jac_diag(int n, const int *__restrict__ Ap, const int *__restrict__ diag,
+                                                  double *__restrict__ Ax, double *__restrict__ dvec)
 {
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j >= n) return;
@@ -315,8 +315,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
         if (rc) return rc;
         c.pattern_valid = false;   // positions of the kept entries moved
     }
-    const int tblocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_jac_diag, dim3(tblocks), dim3(256), 0, c.stream, n, c.Ap, c.diag, c.Ax, c.dvec);
+    hipLaunchKernelGGL(k_jac_diag, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.Ap, c.diag, c.Ax, c.dvec);
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec, c.Ax,
                        c.AxRow);

@@ -78,6 +78,47 @@ __global__ __launch_bounds__(256) void k_dots(int n, int nv, const double *__res
     if (threadIdx.x == 0) *ticket = 0u;
 }
 
+// The same reduction by one workgroup of 1024 lanes (systems up to 2^18
+// rows): strided loads, wave sums, then the 16 wave partials of each v added
+// in wave order; no atomics, one launch.
+__global__ __launch_bounds__(1024) void k_dots1(int n, int nv, const double *__restrict__ V,
+                                                const double *__restrict__ w, double *__restrict__ out,
+                                                int accumulate)
+{
+    __shared__ double wsum[16][kMaxV];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    // four vectors x eight elements of loads in flight before their products
+    for (int v0 = 0; v0 < nv; v0 += 4) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * 1024) {
+            double wv[8], p[4][8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + u * 1024;
+                wv[u] = i < n ? w[i] : 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) p[q][u] = (i < n && v0 + q < nv) ? V[(size_t)(v0 + q) * n + i] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc[q] += p[q][u] * wv[u];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double a = acc[q];
+            for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+            if (lane == 0 && v0 + q < nv) wsum[wave][v0 + q] = a;
+        }
+    }
+    __syncthreads();
+    for (int v = (int)threadIdx.x; v < nv; v += 1024) {
+        double sum = 0.0;
+        for (int k = 0; k < 16; ++k) sum += wsum[k][v];
+        out[v] = accumulate ? out[v] + sum : sum;
+    }
+}
+
 // w -= sum_v h[v] V[v]   (h from the latest projection pass in hp)
 __global__ void k_orth(int n, int nv, const double *__restrict__ V, const double *__restrict__ hp,
                        double *__restrict__ w)
@@ -170,6 +211,16 @@ __global__ void k_resid(int n, const double *__restrict__ g, double *__restrict_
 
 static inline int nb256(int n) { return (n + 255) / 256; }
 
+// out[v] (+)= V[v] . w for v < nv
+static void launch_dots(Context &c, Fgmres &f, int nv, const double *V, const double *w, double *out, int accumulate)
+{
+    if (f.n <= (1 << 18))
+        hipLaunchKernelGGL(k_dots1, dim3(1), dim3(1024), 0, c.stream, f.n, nv, V, w, out, accumulate);
+    else
+        hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, nv, V, w, f.red, f.ticket, out,
+                           accumulate);
+}
+
 int Fgmres::init(Context &c)
 {
     if (V && n == c.n && m == c.solver_restart) return SDDDST_OK;
@@ -258,12 +309,12 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
     double *hp = f.hcol;                // this pass's projections
     double *hsum = f.hcol + (f.m + 2);  // accumulated
     for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, n, nv, f.V, f.w, f.red, f.ticket, hp, 0);
+        launch_dots(c, f, nv, f.V, f.w, hp, 0);
         hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, hp, f.w);
         hipLaunchKernelGGL(k_add, dim3(1), dim3(64), 0, c.stream, nv, hp, hsum, pass);
         SD_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, n, 1, f.w, f.w, f.red, f.ticket, hsum + nv, 0);
+    launch_dots(c, f, 1, f.w, f.w, hsum + nv, 0);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(hhost, hsum, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
@@ -273,7 +324,7 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
 
 static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
-    hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, 1, v, v, f.red, f.ticket, f.hcol, 0);
+    launch_dots(c, f, 1, v, v, f.hcol, 0);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(f.H, f.hcol, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));

@@ -6,8 +6,10 @@
 //                  (simulation.cpp:398-405, 516, 643-649;
 //                  precision_handler.cpp:62-119)
 //   sums           the log reductions of simulation.cpp:460-461, 508-509,
-//                  551, 557 and utility.h:92-100, in the reference's
-//                  sequential order so the logged values match bit for bit.
+//                  551, 557 and utility.h:92-100: in the reference's
+//                  sequential order with the exact and ordered field (the
+//                  logged values match bit for bit), as a fixed-order tree
+//                  with the fast field.
 #include <hip/hip_runtime.h>
 
 #include "context.h"
@@ -82,8 +84,12 @@ __global__ __launch_bounds__(1024) void k_error_control(int n, const double *__r
 // Sequential sums in index order (std::accumulate semantics).  The
 // workgroup stages 2048 values at a time in LDS with coalesced loads; one
 // lane then runs the three dependent chains from LDS.
+
 constexpr int kSumChunk = 2048;
 
+// Sequential sums in index order (the bits of the reference's loops): the
+// workgroup stages 2048 values at a time in LDS, one lane adds them.  Used
+// with the exact and ordered field, whose speeds carry the reference's bits.
 __global__ __launch_bounds__(256) void k_speed_sums(int n, const double *__restrict__ v,
                                                     const double *__restrict__ b, DevScalars *__restrict__ out)
 {
@@ -114,7 +120,7 @@ __global__ __launch_bounds__(256) void k_speed_sums(int n, const double *__restr
     }
 }
 
-// utility.h:92-100, staged the same way.
+// utility.h:92-100: sum of b_i * (x_new - x_old) in index order.
 __global__ __launch_bounds__(256) void k_strain(int n, const double *__restrict__ xo, const double *__restrict__ xn,
                                                 const double *__restrict__ b, DevScalars *__restrict__ out)
 {
@@ -134,6 +140,60 @@ __global__ __launch_bounds__(256) void k_strain(int n, const double *__restrict_
         __syncthreads();
     }
     if (threadIdx.x == 0) out->strain = r;
+}
+
+// The same sums as a fixed-order tree (one workgroup: contiguous chunks per
+// lane, wave sums, wave partials in order) for the fast field, whose speeds
+// differ from the reference's in the last bits anyway.
+__device__ __forceinline__ double tree_finish(double acc, double *red, int t)
+{
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((t & 63) == 0) red[t >> 6] = acc;
+    __syncthreads();
+    double s = 0.0;
+    if (t == 0)
+        for (int k = 0; k < 16; ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(1024) void k_speed_sums_tree(int n, const double *__restrict__ v,
+                                                          const double *__restrict__ b,
+                                                          DevScalars *__restrict__ out)
+{
+    __shared__ double red[16];
+    const int t = (int)threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int s0 = t * per, s1 = min(n, s0 + per);
+    double sa = 0.0, sq = 0.0, sbv = 0.0;
+    for (int k = s0; k < s1; ++k) {
+        const double vi = v[k];
+        sa += __builtin_fabs(vi);
+        sq += vi * vi;
+        sbv += b[k] * vi;
+    }
+    sa = tree_finish(sa, red, t);
+    sq = tree_finish(sq, red, t);
+    sbv = tree_finish(sbv, red, t);
+    if (t == 0) {
+        out->sum_abs = sa;
+        out->sum_sq = sq;
+        out->sum_bv = sbv;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_strain_tree(int n, const double *__restrict__ xo,
+                                                      const double *__restrict__ xn, const double *__restrict__ b,
+                                                      DevScalars *__restrict__ out)
+{
+    __shared__ double red[16];
+    const int t = (int)threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int s0 = t * per, s1 = min(n, s0 + per);
+    double r = 0.0;
+    for (int k = s0; k < s1; ++k) r += b[k] * (xn[k] - xo[k]);
+    r = tree_finish(r, red, t);
+    if (t == 0) out->strain = r;
 }
 
 __global__ void k_fill(double *p, double v, int n)
@@ -172,14 +232,21 @@ int launch_error_control(Context &c, double h, double hmax)
 
 int launch_speed_sums(Context &c, int buf)
 {
-    hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(256), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
+    if (c.p.field_mode == SDDDST_FIELD_FAST)
+        hipLaunchKernelGGL(k_speed_sums_tree, dim3(1), dim3(1024), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
+    else
+        hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(256), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
 int launch_strain(Context &c, int old_slot, int new_slot)
 {
-    hipLaunchKernelGGL(k_strain, dim3(1), dim3(256), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
+    if (c.p.field_mode == SDDDST_FIELD_FAST)
+        hipLaunchKernelGGL(k_strain_tree, dim3(1), dim3(1024), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b,
+                           c.ds);
+    else
+        hipLaunchKernelGGL(k_strain, dim3(1), dim3(256), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }

@@ -74,6 +74,9 @@ struct Context {
     unsigned char *jcore = nullptr;             // 1 where the pair lies within the cutoff (untapered)
     double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
+    int *cell = nullptr, *csorted = nullptr;    // cell of each dislocation, dislocations by cell
+    int *ccount = nullptr, *cstart = nullptr, *ccursor = nullptr;
+    int cell_cap = 0;
     bool pattern_valid = false;
     int pattern_slot = -1;                      // slot whose positions built the pattern
     bool jac_valid = false;

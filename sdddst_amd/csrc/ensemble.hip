@@ -185,7 +185,15 @@ __global__ void k_e_speeds_finish(int nt, const int *__restrict__ sys, const Sys
     const int i = g - d.off;
     const double *ps = partial + d.poff;
     double acc = 0.0;
-    for (int t = 0; t < d.nb; ++t) acc += ps[(size_t)t * d.n + i];
+    int t = 0;
+    for (; t + 8 <= d.nb; t += 8) {
+        double p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = ps[(size_t)(t + u) * d.n + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += p[u];
+    }
+    for (; t < d.nb; ++t) acc += ps[(size_t)t * d.n + i];
     if (update_tol) {
         double t = tol[g];
         const unsigned long long bits = minr2[g];
@@ -284,6 +292,7 @@ __global__ __launch_bounds__(256) void k_e_jac_fill(int nt, const int *__restric
                      diag + g);
 }
 
+template <int Mode>
 __global__ __launch_bounds__(256) void k_e_jac_values(int nt, const int *__restrict__ sys,
                                                       const SysDesc *__restrict__ D, const SysState *__restrict__ st,
                                                       const double *__restrict__ x, const double *__restrict__ y,
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(256) void k_e_jac_values(int nt, const int *__restr
     const int g = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (!column_of(g, nt, sys, D, st, true, cv)) return;
     const SysDesc &d = D[cv.s];
-    jac::values_column(cv.j, x + d.off, y + d.off, d.cp, Ap + d.off + cv.s, Ai + d.nzoff, diag + d.off,
+    jac::values_column<Mode>(cv.j, x + d.off, y + d.off, d.cp, Ap + d.off + cv.s, Ai + d.nzoff, diag + d.off,
                        tau + d.nzoff, mult + d.nzoff, mirror + d.nzoff, core + d.nzoff);
 }
 
@@ -955,8 +964,12 @@ int launch_jacobian(Ensemble &e, int slot, double hscale, bool build)
         hipLaunchKernelGGL(k_e_scan, dim3(e.S), dim3(1024), 0, e.stream, e.D, e.st, e.cnt, e.Ap);
         hipLaunchKernelGGL(k_e_jac_fill, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D, e.st, e.mask, e.Ap, e.Ai,
                            e.diag);
-        hipLaunchKernelGGL(k_e_jac_values, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D, e.st, e.x[slot], e.y,
-                           e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror, e.core);
+        if (e.p.field_mode == SDDDST_FIELD_FAST)
+            hipLaunchKernelGGL(k_e_jac_values<SDDDST_FIELD_FAST>, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D,
+                               e.st, e.x[slot], e.y, e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror, e.core);
+        else
+            hipLaunchKernelGGL(k_e_jac_values<SDDDST_FIELD_EXACT>, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D,
+                               e.st, e.x[slot], e.y, e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror, e.core);
     }
     hipLaunchKernelGGL(k_e_jac_raw, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D, e.st, hscale, e.b, e.Ap, e.Ai,
                        e.diag, e.tau, e.mult, e.mirror, e.Ax, e.flags);

@@ -194,6 +194,64 @@ SDLIBM_FN double xy_fast(double dx, double dy)
     return near + far * (2.0 * kPiSq);
 }
 
+// Production form of xy_diff_x(): the near image is the reference's f_dx;
+// the nine far images (offsets +-1..+-3, 5s, -4s, +4s with the weights of
+// AnalyticField.cpp:162-181) take cosh and sinh from one shared exp(2 pi dx)
+// and fixed exp(+-2 pi k), and their denominators cosh - cos are inverted
+// together (products within [1e1, 1e70]).
+SDLIBM_FN double xy_diff_x_fast(double dx, double dy)
+{
+    const double cy = sdlibm::cos(kPi * 2.0 * dy);
+    const double near = f_dx(dx, cy, dy);
+    const double a0 = dx * 2.0 * kPi;
+    const double t = sdlibm::exp(sdlibm::fabsd(a0));
+    const double ti = 1.0 / t;
+    const double ep = a0 < 0 ? ti : t, em = a0 < 0 ? t : ti;
+    const bool neg = dx < 0;
+    const double s = neg ? 1.0 : -1.0;
+    constexpr int kFar = 9;
+    int off[kFar];
+    double wd[kFar], wf[kFar];
+    for (int k = 1; k < kImages; ++k) {
+        off[2 * k - 2] = -k;
+        off[2 * k - 1] = k;
+        wd[2 * k - 2] = wd[2 * k - 1] = 1.0;
+        wf[2 * k - 2] = wf[2 * k - 1] = 0.0;
+    }
+    off[6] = neg ? 5 : -5;          // a = dx + 5s: -s (f + dx f_dx)
+    wd[6] = -s * dx;
+    wf[6] = -s;
+    off[7] = neg ? -4 : 4;          // dx - 4s: (1 + s dx) f_dx + s f
+    wd[7] = 1.0 + s * dx;
+    wf[7] = s;
+    off[8] = neg ? 4 : -4;          // dx + 4s: f_dx
+    wd[8] = 1.0;
+    wf[8] = 0.0;
+    double x[kFar], ch[kFar], sh[kFar], d[kFar], pre[kFar];
+    for (int m = 0; m < kFar; ++m) {
+        const int k = off[m];
+        const int ak = k < 0 ? -k : k;
+        const double up = k > 0 ? sdlibm_exp_2pik[ak] : sdlibm_exp_m2pik[ak];
+        const double dn = k > 0 ? sdlibm_exp_m2pik[ak] : sdlibm_exp_2pik[ak];
+        x[m] = dx + (double)k;
+        ch[m] = 0.5 * (ep * up + em * dn);
+        sh[m] = 0.5 * (ep * up - em * dn);
+        d[m] = ch[m] - cy;
+        pre[m] = m ? pre[m - 1] * d[m] : d[m];
+    }
+    double inv = 1.0 / pre[kFar - 1];
+    double far = 0.0;
+    for (int m = kFar - 1; m >= 0; --m) {
+        const double r = m ? inv * pre[m - 1] : inv;
+        inv *= d[m];
+        const double P = ch[m] * cy - 1.0;
+        const double r2 = r * r;
+        const double fdx = P * r2 + x[m] * (sh[m] * cy * (2.0 * kPi) * r2 - P * r2 * r * (4.0 * kPi) * sh[m]);
+        far += wd[m] * fdx + wf[m] * (x[m] * P * r2);
+    }
+    return near + far * (2.0 * kPiSq);
+}
+
 // simulation.cpp:288-322: x-derivative of the point-defect force on a
 // dislocation at (dx, dy) from a defect (the Maple expression, grouped as
 // three terms that keep its evaluation order), without b and the taper.

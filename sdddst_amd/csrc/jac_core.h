@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/sdddst_amd.h"
 #include "field.h"
 
 namespace sdamd {
@@ -119,7 +120,9 @@ __device__ __forceinline__ void fill_column(int j, int words, const unsigned lon
 
 // tau_x and the taper of every strictly-lower entry of column j (row i > j),
 // the position of its mirror (row j in column i) and the within-cutoff flag
-// of both positions.
+// of both positions.  Mode FAST takes the production derivative
+// (xy_diff_x_fast), the other modes the reference's expression.
+template <int Mode>
 __device__ __forceinline__ void values_column(int j, const double *__restrict__ x, const double *__restrict__ y,
                                               const CutParams &cp, const int *__restrict__ Ap,
                                               const int *__restrict__ Ai, const int *__restrict__ diag,
@@ -140,7 +143,8 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
             const double s = sdlibm::sqrt(r2) - cp.rc;
             m = sdlibm::exp(-(s * s) * cp.inv_rc2);
         }
-        tau[k] = sdfield::xy_diff_x(dx, dy);
+        if constexpr (Mode == SDDDST_FIELD_FAST) tau[k] = sdfield::xy_diff_x_fast(dx, dy);
+        else tau[k] = sdfield::xy_diff_x(dx, dy);
         mult[k] = m;
         const int mk = find_row(Ai, Ap[i], diag[i], j);
         mirror[k] = mk;

@@ -57,6 +57,119 @@ __global__ __launch_bounds__(256) void k_jac_count(int n, int words, const doubl
     jac::count_column(j, n, words, x, y, cp, cnt + j, mask + (size_t)j * words);
 }
 
+// Cell binning for the count: the unit cell is cut into G x G periodic cells
+// at least the predicate's pre-filter radius wide (with a 1e-6 margin, so no
+// rounding of a cell index can separate a qualifying pair by two cells);
+// every pair that can pass the predicate lies in neighbouring cells.
+__device__ __forceinline__ int cell_of(double x, double y, int G)
+{
+    const double xw = sdfield::normalize(x), yw = sdfield::normalize(y);
+    int cx = (int)((xw + 0.5) * G), cy = (int)((yw + 0.5) * G);
+    cx = cx < 0 ? 0 : (cx >= G ? G - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= G ? G - 1 : cy);
+    return cy * G + cx;
+}
+
+__global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y, int G,
+                             int *__restrict__ cell, int *__restrict__ ccount)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int c = cell_of(x[i], y[i], G);
+    cell[i] = c;
+    atomicAdd(&ccount[c], 1);
+}
+
+// cstart = exclusive scan of ccount (G*G cells, one workgroup); the fill
+// cursor starts at cstart.
+__global__ __launch_bounds__(1024) void k_cell_scan(int cells, const int *__restrict__ ccount,
+                                                    int *__restrict__ cstart, int *__restrict__ cursor)
+{
+    __shared__ int part[1024];
+    const int t = (int)threadIdx.x;
+    const int per = (cells + 1023) / 1024;
+    const int s0 = t * per, s1 = min(cells, s0 + per);
+    int acc = 0;
+    for (int k = s0; k < s1; ++k) acc += ccount[k];
+    part[t] = acc;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - acc;
+    for (int k = s0; k < s1; ++k) {
+        cstart[k] = run;
+        cursor[k] = run;
+        run += ccount[k];
+    }
+    if (t == 1023) cstart[cells] = part[1023];
+}
+
+__global__ void k_cell_scatter(int n, const int *__restrict__ cell, int *__restrict__ cursor, int *__restrict__ sorted)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    sorted[atomicAdd(&cursor[cell[i]], 1)] = i;
+}
+
+// Count of column j from the 3 x 3 neighbouring cells: the same predicate as
+// count_column, bits gathered in an LDS copy of the column's bitmask (the
+// order of the candidates does not matter), then written out whole.
+constexpr int kCellMaxWords = 1024;
+__global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const double *__restrict__ x,
+                                                         const double *__restrict__ y, CutParams cp, int G,
+                                                         const int *__restrict__ cell,
+                                                         const int *__restrict__ cstart,
+                                                         const int *__restrict__ sorted, int *__restrict__ cnt,
+                                                         unsigned long long *__restrict__ mask)
+{
+    extern __shared__ unsigned long long lm[];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int j = (int)(blockIdx.x * 4 + wave);
+    unsigned long long *m = lm + (size_t)wave * words;
+    for (int w = lane; w < words; w += 64) m[w] = 0ull;
+    __syncthreads();
+    if (j < n) {
+        const double xj = x[j], yj = y[j];
+        const int c = cell[j];
+        const int cx = c % G, cy = c / G;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int yy = (cy + oy + G) % G;
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int q = yy * G + (cx + ox + G) % G;
+                for (int k = cstart[q] + lane; k < cstart[q + 1]; k += 64) {
+                    const int i = sorted[k];
+                    bool in;
+                    if (i == j) {
+                        in = true;
+                    } else {
+                        const double xi = x[i], yi = y[i];
+                        const double dx = sdfield::normalize(i > j ? xi - xj : xj - xi);
+                        const double dy = sdfield::normalize(i > j ? yi - yj : yj - yi);
+                        double r2;
+                        in = jac::pair_pred(dx, dy, cp, r2);
+                    }
+                    if (in) atomicOr(&m[i >> 6], 1ull << (i & 63));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (j >= n) return;
+    int c = 0;
+    unsigned long long *mj = mask + (size_t)j * words;
+    for (int w = lane; w < words; w += 64) {
+        const unsigned long long v = m[w];
+        mj[w] = v;
+        c += __popcll(v);
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if (lane == 0) cnt[j] = c;
+}
+
 // Exclusive scan of n counts into Ap[0..n] by one workgroup of 1024 lanes:
 // per-lane sums of contiguous chunks, a work-efficient LDS scan of the 1024
 // partials, then the per-lane chunk scans.
@@ -100,6 +213,7 @@ __global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsign
 
 // tau_x and the taper of every strictly-lower entry (row i > column j) and
 // the position of its mirror (row j in column i).
+template <int Mode>
 __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                     CutParams cp, const int *__restrict__ Ap,
                                                     const int *__restrict__ Ai, const int *__restrict__ diag,
@@ -108,7 +222,7 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    jac::values_column(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core);
+    jac::values_column<Mode>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core);
 }
 
 // simulation.cpp:269-324: the point-defect sum tmp of column j (defects in
@@ -262,8 +376,32 @@ static int build_pattern(Context &c, int slot)
             return SDDDST_ERR_NOMEM;
         }
     }
-    hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, words, x, c.y, cp, c.colcount,
-                       c.jmask);
+    // cells at least the pre-filter radius wide; 3 x 3 neighbourhoods need G >= 3
+    const double rpre = sqrt(cp.prefilter);
+    const int G = rpre > 0 && rpre < 1.0 / 3.0 ? (int)floor(1.0 / (rpre * (1.0 + 1e-6))) : 0;
+    if (G >= 4 && words <= kCellMaxWords) {
+        const int cells = G * G;
+        if (cells > c.cell_cap) {
+            for (int **q : {&c.ccount, &c.cstart, &c.ccursor}) {
+                if (*q) SD_HIP(hipFree(*q));
+                *q = nullptr;
+                SD_HIP(hipMalloc(q, sizeof(int) * (cells + 1)));
+            }
+            c.cell_cap = cells;
+        }
+        if (!c.cell) SD_HIP(hipMalloc(&c.cell, sizeof(int) * n));
+        if (!c.csorted) SD_HIP(hipMalloc(&c.csorted, sizeof(int) * n));
+        hipLaunchKernelGGL(k_fill_int, dim3((cells + 255) / 256), dim3(256), 0, c.stream, c.ccount, 0, cells);
+        hipLaunchKernelGGL(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, x, c.y, G, c.cell, c.ccount);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c.stream, cells, c.ccount, c.cstart, c.ccursor);
+        hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.cell, c.ccursor,
+                           c.csorted);
+        hipLaunchKernelGGL(k_jac_count_cells, dim3(wblocks), dim3(256), sizeof(unsigned long long) * 4 * words,
+                           c.stream, n, words, x, c.y, cp, G, c.cell, c.cstart, c.csorted, c.colcount, c.jmask);
+    } else {
+        hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, words, x, c.y, cp, c.colcount,
+                           c.jmask);
+    }
     SD_HIP(hipGetLastError());
     long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
     int rc = launch_scan(c, n, c.colcount, c.Ap, d_total);
@@ -277,8 +415,12 @@ static int build_pattern(Context &c, int slot)
     c.nnz = total;
     hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag);
     SD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_jac_values, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap, c.Ai, c.diag,
-                       c.jtau, c.jmult, c.jmirror, c.jcore);
+    if (c.p.field_mode == SDDDST_FIELD_FAST)
+        hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_FAST>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap,
+                           c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore);
+    else
+        hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_EXACT>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp,
+                           c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore);
     SD_HIP(hipGetLastError());
     if (c.pc > 0) {
         hipLaunchKernelGGL(k_jac_pd, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.pc, x, c.y, c.b, c.px,

@@ -80,14 +80,17 @@ __global__ __launch_bounds__(256) void k_dots(int n, int nv, const double *__res
 
 // The same reduction by one workgroup of 1024 lanes (systems up to 2^18
 // rows): strided loads, wave sums, then the 16 wave partials of each v added
-// in wave order; no atomics, one launch.
+// in wave order; no atomics, one launch.  out[v] = V[v] . w; with_self also
+// out[nv] = w . w; acc (nullable) receives (accumulate ? acc : 0) + out.
 __global__ __launch_bounds__(1024) void k_dots1(int n, int nv, const double *__restrict__ V,
                                                 const double *__restrict__ w, double *__restrict__ out,
-                                                int accumulate)
+                                                double *__restrict__ acc_out, int accumulate, int with_self)
 {
     __shared__ double wsum[16][kMaxV];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    // four vectors x eight elements of loads in flight before their products
+    // four vectors x eight elements of loads in flight before their products;
+    // w . w rides along with the first group
+    double wsq = 0.0;
     for (int v0 = 0; v0 < nv; v0 += 4) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * 1024) {
@@ -103,6 +106,9 @@ __global__ __launch_bounds__(1024) void k_dots1(int n, int nv, const double *__r
             for (int q = 0; q < 4; ++q)
 #pragma unroll
                 for (int u = 0; u < 8; ++u) acc[q] += p[q][u] * wv[u];
+            if (v0 == 0 && with_self)
+#pragma unroll
+                for (int u = 0; u < 8; ++u) wsq += wv[u] * wv[u];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -111,11 +117,17 @@ __global__ __launch_bounds__(1024) void k_dots1(int n, int nv, const double *__r
             if (lane == 0 && v0 + q < nv) wsum[wave][v0 + q] = a;
         }
     }
+    const int nvt = nv + (with_self ? 1 : 0);
+    if (with_self) {
+        for (int off = 32; off > 0; off >>= 1) wsq += __shfl_down(wsq, off, 64);
+        if (lane == 0) wsum[wave][nv] = wsq;
+    }
     __syncthreads();
-    for (int v = (int)threadIdx.x; v < nv; v += 1024) {
+    for (int v = (int)threadIdx.x; v < nvt; v += 1024) {
         double sum = 0.0;
         for (int k = 0; k < 16; ++k) sum += wsum[k][v];
-        out[v] = accumulate ? out[v] + sum : sum;
+        out[v] = sum;
+        if (acc_out && v < nv) acc_out[v] = accumulate ? acc_out[v] + sum : sum;
     }
 }
 
@@ -211,14 +223,20 @@ __global__ void k_resid(int n, const double *__restrict__ g, double *__restrict_
 
 static inline int nb256(int n) { return (n + 255) / 256; }
 
-// out[v] (+)= V[v] . w for v < nv
-static void launch_dots(Context &c, Fgmres &f, int nv, const double *V, const double *w, double *out, int accumulate)
+// out[v] = V[v] . w for v < nv (and out[nv] = w . w with with_self);
+// acc (nullable) += or = out[0..nv)
+static void launch_dots(Context &c, Fgmres &f, int nv, const double *V, const double *w, double *out,
+                        double *acc = nullptr, int accumulate = 0, int with_self = 0)
 {
-    if (f.n <= (1 << 18))
-        hipLaunchKernelGGL(k_dots1, dim3(1), dim3(1024), 0, c.stream, f.n, nv, V, w, out, accumulate);
-    else
-        hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, nv, V, w, f.red, f.ticket, out,
-                           accumulate);
+    if (f.n <= (1 << 18)) {
+        hipLaunchKernelGGL(k_dots1, dim3(1), dim3(1024), 0, c.stream, f.n, nv, V, w, out, acc, accumulate,
+                           with_self);
+        return;
+    }
+    hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, nv, V, w, f.red, f.ticket, out, 0);
+    if (acc) hipLaunchKernelGGL(k_add, dim3(1), dim3(64), 0, c.stream, nv, out, acc, accumulate);
+    if (with_self)
+        hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, 1, w, w, f.red, f.ticket, out + nv, 0);
 }
 
 int Fgmres::init(Context &c)
@@ -303,20 +321,35 @@ int Fgmres::apply(Context &c, const double *v, double *z)
 
 // CGS2: two passes of (h = V^T w, w -= V h) accumulated into hsum, then
 // hsum[nv] = ||w||; one copy of the column to the host.
+// The second pass also returns ||w||^2 of its input, so the new norm is
+// ||w - V h2||^2 = ||w||^2 - ||h2||^2 (V orthonormal); h2 is tiny after the
+// first pass, and when it is not (||h2||^2 > ||w||^2 / 16) the norm is
+// computed explicitly.
 static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
 {
     const int n = f.n;
-    double *hp = f.hcol;                // this pass's projections
+    double *hp = f.hcol;                // this pass's projections (+ w.w)
     double *hsum = f.hcol + (f.m + 2);  // accumulated
     for (int pass = 0; pass < 2; ++pass) {
-        launch_dots(c, f, nv, f.V, f.w, hp, 0);
+        launch_dots(c, f, nv, f.V, f.w, hp, hsum, pass, pass);
         hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, hp, f.w);
-        hipLaunchKernelGGL(k_add, dim3(1), dim3(64), 0, c.stream, nv, hp, hsum, pass);
         SD_HIP(hipGetLastError());
     }
-    launch_dots(c, f, 1, f.w, f.w, hsum + nv, 0);
+    // one copy of hcol: h2 and w.w at [0, nv], h at [m+2, m+2+nv)
+    double *h2 = hhost + (f.m + 2);
+    SD_HIP(hipMemcpyAsync(h2, hp, sizeof(double) * (f.m + 2 + nv), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipStreamSynchronize(c.stream));
+    for (int v = 0; v < nv; ++v) hhost[v] = h2[f.m + 2 + v];
+    double s2 = 0.0;
+    for (int v = 0; v < nv; ++v) s2 += h2[v] * h2[v];
+    const double ww = h2[nv];
+    if (s2 <= ww * 0.0625) {
+        hhost[nv] = sqrt(ww - s2);
+        return SDDDST_OK;
+    }
+    launch_dots(c, f, 1, f.w, f.w, hsum + nv);
     SD_HIP(hipGetLastError());
-    SD_HIP(hipMemcpyAsync(hhost, hsum, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, c.stream));
+    SD_HIP(hipMemcpyAsync(hhost + nv, hsum + nv, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     hhost[nv] = sqrt(hhost[nv]);
     return SDDDST_OK;
@@ -324,7 +357,7 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
 
 static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
-    launch_dots(c, f, 1, v, v, f.hcol, 0);
+    launch_dots(c, f, 1, v, v, f.hcol);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(f.H, f.hcol, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
@@ -351,7 +384,7 @@ int Fgmres::solve(Context &c)
         c.last_rel_res = 0.0;
         return SDDDST_OK;
     }
-    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), hh(m + 2);
+    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), hh(3 * (m + 2) + 1);
     int total_it = 0;
     double rel = 1.0;
     const double *r0 = c.g;

@@ -80,7 +80,15 @@ __global__ void k_speeds_finish(int n, int nb, int pc, const double *__restrict_
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     double acc = 0.0;
-    for (int s = 0; s < nb; ++s) acc += partial[(size_t)s * n + i];
+    int s = 0;
+    for (; s + 8 <= nb; s += 8) {
+        double p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = partial[(size_t)(s + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += p[u];
+    }
+    for (; s < nb; ++s) acc += partial[(size_t)s * n + i];
     double t = update_tol ? tol[i] : 0.0;
     if (update_tol) {
         const unsigned long long bits = minr2[i];

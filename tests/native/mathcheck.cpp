@@ -38,13 +38,14 @@ void mc_batch1(int fn, long n, const double *x, double *out)
     }
 }
 
-// fn = 0 xy, 1 xy_fast, 2 xy_diff_x
+// fn = 0 xy, 1 xy_fast, 2 xy_diff_x, 3 xy_diff_x_fast
 void mc_batch2(int fn, long n, const double *dx, const double *dy, double *out)
 {
     for (long i = 0; i < n; ++i) {
         if (fn == 0) out[i] = sdfield::xy(dx[i], dy[i]);
         else if (fn == 1) out[i] = sdfield::xy_fast(dx[i], dy[i]);
-        else out[i] = sdfield::xy_diff_x(dx[i], dy[i]);
+        else if (fn == 2) out[i] = sdfield::xy_diff_x(dx[i], dy[i]);
+        else out[i] = sdfield::xy_diff_x_fast(dx[i], dy[i]);
     }
 }
 }

@@ -77,6 +77,10 @@ def test_field_fast_and_derivative():
     d = _mc2(2, dx, dy)
     assert np.mean(d.view(np.int64) == xyd.view(np.int64)) > 0.998
     assert np.max(np.abs(d - xyd) / np.maximum(np.abs(xyd), 1.0)) < 1e-13
+    # production derivative: far images from one shared exp and batched
+    # reciprocals; error relative to |tau_x| + 1, as for the fast field
+    df = _mc2(3, dx, dy)
+    assert np.max(np.abs(df - xyd) / (np.abs(xyd) + 1.0)) < 1e-14
 
 
 def test_integer_powers_correctly_rounded():

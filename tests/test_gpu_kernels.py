@@ -147,11 +147,11 @@ def test_speeds_sum_rule_and_determinism_n16384():
 
 # -- K2 Jacobian -----------------------------------------------------------
 
-@pytest.mark.parametrize("n,m,seed", [(256, 1.0, 1), (300, 1.0, 4), (1024, 2.0, 7), (64, 1.0, 3)])
+@pytest.mark.parametrize("n,m,seed", [(256, 1.0, 1), (300, 1.0, 4), (1024, 2.0, 7), (64, 1.0, 3), (4096, 1.0, 9), (4096, 2.0, 10)])
 def test_jacobian_pattern_bit_exact(n, m, seed):
     x, y, b = configs.dislocations(n, seed)
     h = 1e-4
-    with _ctx(n, cutoff_multiplier=m) as c:
+    with _ctx(n, cutoff_multiplier=m, field_mode=S.FIELD_EXACT) as c:
         c.upload_state(0, x, y, b)
         Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)
     rAp, rAi, rAx, rdv, ridx, raw = O.jacobian(x, y, b, m, h)
@@ -161,6 +161,20 @@ def test_jacobian_pattern_bit_exact(n, m, seed):
     rel = np.abs(Ax - rAx) / np.maximum(np.abs(rAx), 1e-300)
     assert np.mean(Ax.view(np.int64) == rAx.view(np.int64)) > 0.995
     assert rel.max() < 1e-12
+    assert np.max(np.abs(dv - rdv)) < 1e-13
+
+
+@pytest.mark.parametrize("n,m,seed", [(300, 1.0, 4), (1024, 2.0, 7)])
+def test_jacobian_fast_field(n, m, seed):
+    # production derivative: same pattern, values within 1e-12 of max(|A|, h)
+    x, y, b = configs.dislocations(n, seed)
+    h = 1e-4
+    with _ctx(n, cutoff_multiplier=m, field_mode=S.FIELD_FAST) as c:
+        c.upload_state(0, x, y, b)
+        Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)
+    rAp, rAi, rAx, rdv, ridx, raw = O.jacobian(x, y, b, m, h)
+    assert np.array_equal(Ap, rAp) and np.array_equal(Ai, rAi) and np.array_equal(dg, ridx)
+    assert np.max(np.abs(Ax - rAx) / np.maximum(np.abs(rAx), h)) < 1e-12
     assert np.max(np.abs(dv - rdv)) < 1e-13
 
 
@@ -194,7 +208,7 @@ def test_speeds_with_point_defects(mode):
 def test_jacobian_with_point_defects():
     n, pc, h = 400, 64, 1e-4
     x, y, b, px, py, A, K = _pd_setup(n, pc, 51)
-    with S.Context(n, pc=pc, cutoff_multiplier=2.0, A=A, KASQR=K) as c:
+    with S.Context(n, pc=pc, cutoff_multiplier=2.0, A=A, KASQR=K, field_mode=S.FIELD_EXACT) as c:
         c.upload_state(0, x, y, b)
         c.upload_points(px, py)
         Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)

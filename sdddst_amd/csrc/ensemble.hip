@@ -158,7 +158,7 @@ __global__ __launch_bounds__(64) void k_e_pair_tiles(const int *__restrict__ tl,
                                                      const SysState *__restrict__ st, int only_accepted,
                                                      const double *__restrict__ x, const double *__restrict__ y,
                                                      const double *__restrict__ b, double *__restrict__ partial,
-                                                     unsigned long long *__restrict__ minr2)
+                                                     unsigned long long *__restrict__ minr2, double r2_keep)
 {
     const int s = tl[2 * blockIdx.x];
     const SysState &q = st[s];
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(64) void k_e_pair_tiles(const int *__restrict__ tl,
     int bi, bj;
     pairtile::tile_coords(tl[2 * blockIdx.x + 1], bi, bj);
     const SysDesc &d = D[s];
-    pairtile::tile<Mode>(bi, bj, d.n, x + d.off, y + d.off, b + d.off, partial + d.poff, minr2 + d.off);
+    pairtile::tile<Mode>(bi, bj, d.n, x + d.off, y + d.off, b + d.off, partial + d.poff, minr2 + d.off, r2_keep);
 }
 
 // Sum of a target's tile partials in tile order, tolerance update, stress.
@@ -934,10 +934,12 @@ int launch_speeds(Ensemble &e, int slot, int buf, int update_tol, int only_accep
                            e.minr2);
         if (e.p.field_mode == SDDDST_FIELD_FAST)
             hipLaunchKernelGGL(k_e_pair_tiles<SDDDST_FIELD_FAST>, dim3(e.ntiles), dim3(64), 0, e.stream, e.tl, e.D,
-                               e.st, only_accepted, e.x[slot], e.y, e.b, e.partial, e.minr2);
+                               e.st, only_accepted, e.x[slot], e.y, e.b, e.partial, e.minr2,
+                               pairtile::tolerance_r2(mp2));
         else
             hipLaunchKernelGGL(k_e_pair_tiles<SDDDST_FIELD_EXACT>, dim3(e.ntiles), dim3(64), 0, e.stream, e.tl, e.D,
-                               e.st, only_accepted, e.x[slot], e.y, e.b, e.partial, e.minr2);
+                               e.st, only_accepted, e.x[slot], e.y, e.b, e.partial, e.minr2,
+                               pairtile::tolerance_r2(mp2));
         hipLaunchKernelGGL(k_e_speeds_finish, dim3(nb(e.NT)), dim3(256), 0, e.stream, e.NT, e.sys, e.D, e.st,
                            only_accepted, e.partial, e.minr2, e.b, e.sigma, e.tol, mp2, update_tol, e.v[buf]);
         SD_HIP(hipGetLastError());

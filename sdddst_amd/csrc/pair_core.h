@@ -11,7 +11,11 @@
 // column sums land in partial[bj][i] and partial[bi][j]: every target
 // receives exactly one partial per tile column, summed later in tile order,
 // so the result does not depend on scheduling.  minr2 takes the minimum r^2
-// per target (the tolerance update is monotone in r^2).
+// per target over the pairs with r^2 < r2_keep: the tolerance update
+// (precision_handler.cpp:72-87) acts only when r^2 * 0.25 * 1e-2 is below
+// prec^2, so with r2_keep a little above 400 prec^2 the kept minimum gives
+// the same update as the minimum over all pairs, and these rare pairs are
+// recorded with an atomic min instead of a rotated accumulator.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,7 +56,8 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
 template <int Mode>
 __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
-                                     double *__restrict__ partial, unsigned long long *__restrict__ minr2)
+                                     double *__restrict__ partial, unsigned long long *__restrict__ minr2,
+                                     double r2_keep)
 {
     __shared__ double sx[kTile], sy[kTile], sb[kTile];
     const int l = (int)threadIdx.x;
@@ -67,7 +72,6 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
     __syncthreads();
     const bool diag = (bi == bj);
     double row = 0.0, col = 0.0;
-    double mrow = __builtin_inf(), mcol = __builtin_inf();
     for (int s = 0; s < kTile; ++s) {
         const int m = (l + s) & 63;
         const int j = bj * kTile + m;
@@ -79,11 +83,13 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
         if (valid) {
             row += t;
             col -= t;
-            mrow = fmin(mrow, r2);
-            mcol = fmin(mcol, r2);
+            if (r2 < r2_keep) {
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
+                atomicMin(&minr2[i], bits);
+                atomicMin(&minr2[j], bits);
+            }
         }
         col = rotate_down1(col, l);
-        mcol = rotate_down1(mcol, l);
     }
     if (diag) {
         if (i < n) partial[(size_t)bi * n + i] = row + col;
@@ -91,10 +97,11 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
         if (i < n) partial[(size_t)bj * n + i] = row;
         if (jl < n) partial[(size_t)bi * n + jl] = col;
     }
-    if (i < n && mrow < __builtin_inf()) atomicMin(&minr2[i], (unsigned long long)__double_as_longlong(mrow));
-    if (!diag && jl < n && mcol < __builtin_inf())
-        atomicMin(&minr2[jl], (unsigned long long)__double_as_longlong(mcol));
 }
+
+// r^2 below which a pair can change a tolerance: 400 prec^2 with a margin
+// for the roundings of r^2 * 0.25 * 1e-2.
+__host__ __device__ inline double tolerance_r2(double minPrecSqr) { return minPrecSqr * 400.0 * (1.0 + 1e-6); }
 
 }  // namespace pairtile
 }  // namespace sdamd

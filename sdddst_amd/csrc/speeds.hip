@@ -42,11 +42,11 @@ using pairtile::kTile;
 template <int Mode>
 __global__ __launch_bounds__(64) void k_pair_tiles(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                    const double *__restrict__ b, double *__restrict__ partial,
-                                                   unsigned long long *__restrict__ minr2)
+                                                   unsigned long long *__restrict__ minr2, double r2_keep)
 {
     int bi, bj;
     pairtile::tile_coords((int)blockIdx.x, bi, bj);
-    pairtile::tile<Mode>(bi, bj, n, x, y, b, partial, minr2);
+    pairtile::tile<Mode>(bi, bj, n, x, y, b, partial, minr2, r2_keep);
 }
 
 // simulation.cpp:111-128: point-defect force on dislocation i and its
@@ -175,10 +175,10 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
-                           c.x[slot], c.y, c.b, c.partial, c.minr2);
+                           c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
     else
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
-                           c.x[slot], c.y, c.b, c.partial, c.minr2);
+                           c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
     SD_HIP(hipGetLastError());
     if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.partial, c.x[slot],

@@ -1,0 +1,75 @@
+"""The sdddst executable on the device against the oracle's stepping
+driver: result configuration and log of a zero-stress run with the
+reference's defaults, and a fixed-rate run with point defects (A and KASQR
+scaled by the command line, project_parser.cpp:208-212)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from sdddst_amd import configs
+from sdddst_amd.simulation import read_log
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "sdddst_amd", "sdddst")
+
+
+def _cli(*args):
+    r = subprocess.run([EXE, "-c", *args], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_cli_zero_stress_defaults_match_oracle(tmp_path):
+    n = 128
+    x, y, b = configs.dislocations(n, 21)
+    d, out, log = tmp_path / "in.dconf", tmp_path / "out.dconf", tmp_path / "run.log"
+    configs.write_dconf(str(d), x, y, b)
+    _cli("--dislocation-configuration", str(d), "--result-dislocation-configuration", str(out),
+         "--logfile-path", str(log), "--step-count-limit", "25", "--calculate-strain", "--field-mode", "exact",
+         "--linear-solver", "dense-lu")
+    # reference defaults: cutoff multiplier 1e20, precision 1e-5, h0 1e-6
+    x0, y0, b0 = configs.read_dconf(str(d))
+    o = O.OracleSim(x0, y0, b0, cutoff_multiplier=1e20, prec=1e-5, h0=1e-6, stress="zero")
+    while o.succesful_steps < 25:
+        o.step()
+    xr, yr, br = configs.read_dconf(str(out))
+    assert np.array_equal(yr, y0) and np.array_equal(br, b0)
+    assert np.max(np.abs(xr - o.x())) < 1e-9
+    lg, lo = read_log(str(log)), o.log()
+    assert lg.shape == lo.shape
+    cols = [0, 1, 2, 4, 5, 6, 7, 9, 10, 11]
+    a, r = lg[:, cols], lo[:, cols]
+    assert (np.isnan(a) == np.isnan(r)).all()
+    fin = ~np.isnan(r)
+    assert np.max(np.abs(a[fin] - r[fin]) / np.maximum(np.abs(r[fin]), 1e-300)) < 1e-6
+    o.close()
+
+
+def test_cli_point_defects_fixed_rate_match_oracle(tmp_path):
+    n, pc = 96, 24
+    x, y, b = configs.dislocations(n, 22)
+    px, py = configs.point_defects(n, pc, 22)
+    d, p, out = tmp_path / "in.dconf", tmp_path / "pd.dconf", tmp_path / "out.dconf"
+    configs.write_dconf(str(d), x, y, b)
+    with open(p, "w") as f:
+        for u, v in zip(px, py):
+            f.write("%.16e %.16e\n" % (u, v))
+    _cli("--dislocation-configuration", str(d), "--point-defect-configuration", str(p),
+         "--result-dislocation-configuration", str(out), "--step-count-limit", "8", "--cutoff-multiplier", "2",
+         "--position-precision", "1e-6", "--fixed-rate-external-stress", "50", "--field-mode", "exact",
+         "--linear-solver", "dense-lu")
+    x0, y0, b0 = configs.read_dconf(str(d))
+    p0 = np.loadtxt(str(p), ndmin=2)
+    A = (1e-4 * 16.0) * (1.0 / np.sqrt(n))        # A *= 1./sqrt(dc)
+    K = 1.65 * 1.65 * 1e6 / 256.0 * n
+    o = O.OracleSim(x0, y0, b0, cutoff_multiplier=2.0, prec=1e-6, h0=1e-6, stress="fixed-rate", p1=50.0,
+                    px=p0[:, 0].copy(), py=p0[:, 1].copy(), A=A, K=K)
+    while o.succesful_steps < 8:
+        o.step()
+    xr, _, _ = configs.read_dconf(str(out))
+    assert np.max(np.abs(xr - o.x())) < 1e-9
+    o.close()

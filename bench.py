@@ -52,6 +52,11 @@ def parse():
                    help="steps of the same workload with the pair-block preconditioner (0: skip)")
     p.add_argument("--ensemble-steps", type=int, default=500,
                    help="steps per realization of the 256-realization ensemble (0: skip)")
+    p.add_argument("--workload", default="n16384-m2-const0.02", choices=["n16384-m2-const0.02",
+                                                                          "n65536-pd2048-m2-ramp"],
+                   help="the headline workload, or the large combined run (N=65536 + 2048 point defects, "
+                        "stress ramped at 4e6 per unit time so that it reaches about 0.1 at step 100, "
+                        "learned preconditioner, 100 steps from t = 0, no warm-up)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run only a 512 MiB device-to-device copy kernel and exit")
     p.add_argument("--plain-steps", type=int, default=2,
@@ -175,10 +180,62 @@ def cpu_baseline(x, y, b, seconds):
             "pairs_per_s": pairs_per_s, "cpu": platform.processor() or platform.machine()}
 
 
+def large_run(args):
+    """N = 65536 dislocations (seed 5) + 2048 point defects (seed 5) with
+    A and KASQR scaled as the command line scales them, m = 2, precision
+    1e-6, fixed-rate stress 4e6, learned preconditioner; successful steps
+    per second of a run from t = 0, and the pair kernel's rate."""
+    from sdddst_amd import configs
+    from sdddst_amd._capi import kernel_stats, timing
+    from sdddst_amd.simulation import Simulation
+    n, pc, rate = 65536, 2048, 4.0e6
+    steps = args.steps if args.steps != 200 else 100
+    x, y, b = configs.dislocations(n, 5)
+    px, py = configs.point_defects(n, pc, 5)
+    A = (1e-4 * 16.0) * (1.0 / np.sqrt(n))
+    K = 1.65 * 1.65 * 1e6 / 256.0 * n
+    log(f"workload n={n} point defects={pc} fixed-rate {rate:g}")
+    sim = Simulation(x, y, b, cutoff_multiplier=2.0, precision=1e-6, initial_step=1e-6, stress="fixed-rate",
+                     stress_value=rate, solver="fgmres", preconditioner=args.precond, px=px, py=py, A=A, KASQR=K,
+                     solver_tol=1e-10, solver_max_iter=300, solver_restart=40)
+    ctx = sim.device_context
+    timing(ctx, True)
+    wall, ok, failed = timed_steps(sim, steps, "timed")
+    pair_cnt, pair_ms = kernel_stats(ctx, "pair")
+    timing(ctx, False)
+    st = sim.state()
+    sim.close()
+    pairs = n * (n - 1) / 2
+    avg = pair_ms / max(pair_cnt, 1)
+    tflops = FLOPS_PER_PAIR * pairs / (avg * 1e-3) / 1e12
+    line = {"metric": "successful implicit steps/s at N=65536 + 2048 point defects, 1e-6 precision",
+            "value": ok / wall, "unit": "steps/s", "n_gpus": 1, "steps": ok, "warmup": 0,
+            "ms_per_step": wall / ok * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (SplitMix64 seed 5 positions and point defects)",
+            "config": {"workload": "n65536-pd2048-m2-ramp", "n": n, "point_defects": pc, "cutoff_multiplier": 2.0,
+                       "precision": 1e-6, "stress": f"fixed-rate {rate:g}", "initial_step": 1e-6,
+                       "preconditioner": args.precond, "failed_steps_in_timed_region": failed,
+                       "final_time": st.sim_time, "final_stress": rate * st.sim_time},
+            "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": tflops,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
+                         "traffic": None, "pairs_per_launch": pairs, "avg_launch_ms": avg, "launches": pair_cnt},
+            "pairs_per_s": pairs / (avg * 1e-3),
+            "solver_iterations_per_solve": st.solver_total_iters / max(1, st.solver_total_solves),
+            "solver_fallbacks_total": st.solver_fallbacks,
+            "solver_precond_fallbacks_total": st.solver_precond_fallbacks}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "n65536-pd2048-m2-ramp":
+        if args.gpus != 1:
+            print("bench.py: this build measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        large_run(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

@@ -3,7 +3,9 @@
 // for the host UMFPACK factorisation (simulation.cpp:386-396) when no
 // libumfpack is installed, with the same call pattern: one factorisation per
 // integrate, ic solves against it.  Dense storage is N^2 doubles
-// (2 GiB at N = 16384, 32 GiB at N = 65536: within one MI355X's HBM).
+// (2 GiB at N = 16384, 32 GiB at N = 65536: within one MI355X's HBM).  The
+// 64-bit (ILP64) rocSOLVER interfaces are used: above N = 46340 the matrix
+// has more than 2^31 elements, beyond the 32-bit interface's index range.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -44,8 +46,8 @@ int DenseLu::init(Context &c)
         set_error("dense LU: cannot allocate %zu bytes", bytes);
         return SDDDST_ERR_NOMEM;
     }
-    SD_HIP(hipMalloc(&ipiv, sizeof(int) * c.n));
-    SD_HIP(hipMalloc(&info, sizeof(int)));
+    SD_HIP(hipMalloc(&ipiv, sizeof(int64_t) * c.n));
+    SD_HIP(hipMalloc(&info, sizeof(int64_t)));
     if (rocblas_create_handle(&handle) != rocblas_status_success) {
         set_error("rocblas_create_handle failed");
         return SDDDST_ERR_SOLVER;
@@ -76,23 +78,24 @@ int DenseLu::factor(Context &c)
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_scatter_dense, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.Ax, A);
     SD_HIP(hipGetLastError());
-    if (rocsolver_dgetrf(handle, n, n, A, n, ipiv, info) != rocblas_status_success) {
-        set_error("rocsolver_dgetrf failed");
+    if (rocsolver_dgetrf_64(handle, n, n, A, n, ipiv, info) != rocblas_status_success) {
+        set_error("rocsolver_dgetrf_64 failed");
         return SDDDST_ERR_SOLVER;
     }
-    int hinfo = 0;
-    SD_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    int64_t hinfo = 0;
+    SD_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
-    SD_CHECK(hinfo == 0, SDDDST_ERR_SOLVER, "dense LU: matrix is singular (U(%d,%d) = 0)", hinfo, hinfo);
+    SD_CHECK(hinfo == 0, SDDDST_ERR_SOLVER, "dense LU: matrix is singular (U(%lld,%lld) = 0)", (long long)hinfo,
+             (long long)hinfo);
     return SDDDST_OK;
 }
 
 int DenseLu::solve(Context &c)
 {
     SD_HIP(hipMemcpyAsync(c.delta, c.g, sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.stream));
-    if (rocsolver_dgetrs(handle, rocblas_operation_none, c.n, 1, A, c.n, ipiv, c.delta, c.n) !=
+    if (rocsolver_dgetrs_64(handle, rocblas_operation_none, c.n, 1, A, c.n, ipiv, c.delta, c.n) !=
         rocblas_status_success) {
-        set_error("rocsolver_dgetrs failed");
+        set_error("rocsolver_dgetrs_64 failed");
         return SDDDST_ERR_SOLVER;
     }
     return SDDDST_OK;

@@ -9,8 +9,8 @@ namespace sdamd {
 
 struct DenseLu {
     double *A = nullptr;
-    int *ipiv = nullptr;
-    int *info = nullptr;
+    int64_t *ipiv = nullptr;     // 64-bit interface: N^2 exceeds 2^31 above N = 46340
+    int64_t *info = nullptr;
     rocblas_handle handle = nullptr;
     int init(Context &c);
     int factor(Context &c);

@@ -53,6 +53,7 @@ def parse():
     p.add_argument("--ensemble-steps", type=int, default=500,
                    help="steps per realization of the 256-realization ensemble (0: skip)")
     p.add_argument("--workload", default="n16384-m2-const0.02", choices=["n16384-m2-const0.02",
+                                                                          "n16384-m4-zero-learned",
                                                                           "n65536-pd2048-m2-ramp"],
                    help="the headline workload, or the large combined run (N=65536 + 2048 point defects, "
                         "stress ramped at 4e6 per unit time so that it reaches about 0.1 at step 100, "
@@ -226,10 +227,61 @@ def large_run(args):
     print(json.dumps(line))
 
 
+def m4_run(args):
+    """N = 16384 (seed 4), cutoff multiplier 4 (about 50 within-cutoff
+    neighbours per row), zero stress, precision 1e-6: the learned
+    preconditioner against the pair blocks alone and against the direct
+    solver, each from t = 0."""
+    from sdddst_amd import configs
+    n = 16384
+    x, y, b = configs.dislocations(n, 4)
+    steps = args.steps
+    out = {}
+    for label, solver, precond, k in (("learned", "fgmres", "learned", steps), ("pair", "fgmres", "pair", steps),
+                                      ("direct", "dense-lu", "pair", max(1, args.plain_steps))):
+        from sdddst_amd.simulation import Simulation
+        sim = Simulation(x, y, b, cutoff_multiplier=4.0, precision=1e-6, initial_step=1e-6, stress="zero",
+                         solver=solver, preconditioner=precond, solver_tol=1e-10, solver_max_iter=300,
+                         solver_restart=40)
+        if args.warmup > 0 and label != "direct":
+            timed_steps(sim, 1, f"{label} warmup")
+            sim.close()
+            sim = Simulation(x, y, b, cutoff_multiplier=4.0, precision=1e-6, initial_step=1e-6, stress="zero",
+                             solver=solver, preconditioner=precond, solver_tol=1e-10, solver_max_iter=300,
+                             solver_restart=40)
+        s0 = sim.state()
+        w, ok, failed = timed_steps(sim, k, label)
+        st = sim.state()
+        sv = st.solver_total_solves - s0.solver_total_solves
+        out[label] = {"steps_per_s": ok / w, "steps": ok, "failed": failed,
+                      "solver_iterations_per_solve": (st.solver_total_iters - s0.solver_total_iters) / sv if sv
+                      else None, "precond_fallbacks": st.solver_precond_fallbacks,
+                      "direct_fallbacks": st.solver_fallbacks}
+        sim.close()
+        log(f"{label}: {ok / w:.2f} steps/s")
+    line = {"metric": "successful implicit steps/s at N=16384, cutoff multiplier 4, 1e-6 precision",
+            "value": out["learned"]["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": out["learned"]["steps"],
+            "warmup": args.warmup, "ms_per_step": 1e3 / out["learned"]["steps_per_s"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (SplitMix64 seed 4 positions, balanced +/-1 Burgers vectors)",
+            "config": {"workload": "n16384-m4-zero-learned", "n": n, "cutoff_multiplier": 4.0, "precision": 1e-6,
+                       "stress": 0.0, "initial_step": 1e-6},
+            "learned": out["learned"], "pair_block_preconditioner": out["pair"], "plain_direct_solver": out["direct"],
+            "learned_over_pair": out["learned"]["steps_per_s"] / out["pair"]["steps_per_s"],
+            "learned_over_direct": out["learned"]["steps_per_s"] / out["direct"]["steps_per_s"]}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "n16384-m4-zero-learned":
+        if args.gpus != 1:
+            print("bench.py: this build measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        m4_run(args)
         return
     if args.workload == "n65536-pd2048-m2-ramp":
         if args.gpus != 1:

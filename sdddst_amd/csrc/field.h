@@ -140,46 +140,59 @@ SDLIBM_FN double xy_fast(double dx, double dy)
     const double cy = sdlibm::cos(kPi * 2.0 * dy);
     const double a0 = dx * 2.0 * kPi;
     const double ax = sdlibm::fabsd(a0);
-    // Near image: the reference's cosh, evaluated once.
-    double c0, t;
+    // Near image: the reference's cosh, evaluated once.  On the exp branch
+    // 0.5 / t is taken as 0.5 * (1 / t): scaling by 2^-1 commutes with
+    // rounding while 0.5 / t stays normal (t < 2^1021, |a0| < 707; a wrapped
+    // dx gives t <= e^pi), so the value is the same and one division serves
+    // both uses.
+    double c0, t, ti;
     if (sdlibm::hiword(ax) < 0x3fd62e43u) {
         const double m = sdlibm::expm1(ax);
         const double w = 1.0 + m;
         c0 = 1.0 + (m * m) / (w + w);
         t = w;
+        ti = 1.0 / t;
     } else {
         t = sdlibm::exp(ax);
-        c0 = 0.5 * t + 0.5 / t;
+        ti = 1.0 / t;
+        c0 = 0.5 * t + 0.5 * ti;
     }
     const double near = (dx * dx + dy * dy > 1e-6) ? f_given_cosh(dx, c0, cy) : f_taylor(dx, dy);
-    // e^{+a0}, e^{-a0}
-    const double ti = 1.0 / t;
-    const double ep = a0 < 0 ? ti : t;
-    const double em = a0 < 0 ? t : ti;
-    const bool neg = dx < 0;
-    const double s = neg ? 1.0 : -1.0;
-    const double sdx = neg ? -dx : dx;
-    // Far images: offsets -1, +1, -2, +2, -3, +3, then 5s, -4s, +4s (the
-    // last three weighted as in AnalyticField.cpp:148-159).
+    // Far images in terms of ad = |dx| and the sign sg of dx (every image
+    // of the reference's sum is odd in dx, so the sum is sg * F(ad)):
+    //   dx -+ k (k = 1..3) -> sg * ((ad + k) fA_k + (ad - k) fB_k)
+    //   -s dx f(dx + 5s)  -> sg * ad (ad - 5) fB_5        (s = -sg)
+    //   (1 + s dx) f(dx - 4s) + f(dx + 4s) -> sg * ((1 - ad)(ad + 4) fA_4 + (ad - 4) fB_4)
+    // with fX = (X cy - 1) / (X - cy)^2, A_k = cosh(2 pi (ad + k)) =
+    // (t e^{2 pi k} + e^{-2 pi k} / t) / 2 and B_k = cosh(2 pi (k - ad)) =
+    // (e^{2 pi k} / t + t e^{-2 pi k}) / 2.
+    const double ad = sdlibm::fabsd(dx);
     constexpr int kFar = 2 * (kImages - 1) + 3;
     double num[kFar], den[kFar];
-    for (int k = 1; k < kImages; ++k) {
-        const double cm = 0.5 * (ep * sdlibm_exp_m2pik[k] + em * sdlibm_exp_2pik[k]);
-        const double cp = 0.5 * (ep * sdlibm_exp_2pik[k] + em * sdlibm_exp_m2pik[k]);
-        num[2 * k - 2] = (dx - (double)k) * (cm * cy - 1.0);
-        den[2 * k - 2] = (cm - cy) * (cm - cy);
-        num[2 * k - 1] = (dx + (double)k) * (cp * cy - 1.0);
-        den[2 * k - 1] = (cp - cy) * (cp - cy);
+    for (int k = 1; k <= kImages; ++k) {
+        const double hp = 0.5 * sdlibm_exp_2pik[k], hm = 0.5 * sdlibm_exp_m2pik[k];
+        const double A = __builtin_fma(t, hp, ti * hm);
+        const double B = __builtin_fma(ti, hp, t * hm);
+        const double gA = __builtin_fma(A, cy, -1.0), gB = __builtin_fma(B, cy, -1.0);
+        const double dA = A - cy, dB = B - cy;
+        const double kd = (double)k;
+        if (k < kImages) {
+            num[2 * k - 2] = (ad + kd) * gA;
+            num[2 * k - 1] = (ad - kd) * gB;
+        } else {
+            num[kFar - 2] = ((1.0 - ad) * (ad + kd)) * gA;
+            num[kFar - 1] = (ad - kd) * gB;
+        }
+        den[k < kImages ? 2 * k - 2 : kFar - 2] = dA * dA;
+        den[k < kImages ? 2 * k - 1 : kFar - 1] = dB * dB;
     }
-    const double c5 = neg ? 0.5 * (ep * sdlibm_exp_2pik[5] + em * sdlibm_exp_m2pik[5]) : 0.5 * (ep * sdlibm_exp_m2pik[5] + em * sdlibm_exp_2pik[5]);
-    const double c4m = neg ? 0.5 * (ep * sdlibm_exp_m2pik[4] + em * sdlibm_exp_2pik[4]) : 0.5 * (ep * sdlibm_exp_2pik[4] + em * sdlibm_exp_m2pik[4]);
-    const double c4p = neg ? 0.5 * (ep * sdlibm_exp_2pik[4] + em * sdlibm_exp_m2pik[4]) : 0.5 * (ep * sdlibm_exp_m2pik[4] + em * sdlibm_exp_2pik[4]);
-    num[kFar - 3] = sdx * ((dx + 5.0 * s) * (c5 * cy - 1.0));
-    den[kFar - 3] = (c5 - cy) * (c5 - cy);
-    num[kFar - 2] = (1.0 + s * dx) * ((dx - 4.0 * s) * (c4m * cy - 1.0));
-    den[kFar - 2] = (c4m - cy) * (c4m - cy);
-    num[kFar - 1] = (dx + 4.0 * s) * (c4p * cy - 1.0);
-    den[kFar - 1] = (c4p - cy) * (c4p - cy);
+    {
+        const double hp = 0.5 * sdlibm_exp_2pik[kImages + 1], hm = 0.5 * sdlibm_exp_m2pik[kImages + 1];
+        const double B = __builtin_fma(ti, hp, t * hm);
+        const double dB = B - cy;
+        num[kFar - 3] = (ad * (ad - (double)(kImages + 1))) * __builtin_fma(B, cy, -1.0);
+        den[kFar - 3] = dB * dB;
+    }
     // one division for the nine reciprocals
     double pre[kFar];
     pre[0] = den[0];
@@ -187,10 +200,11 @@ SDLIBM_FN double xy_fast(double dx, double dy)
     double inv = 1.0 / pre[kFar - 1];
     double far = 0.0;
     for (int m = kFar - 1; m > 0; --m) {
-        far += num[m] * (inv * pre[m - 1]);
+        far = __builtin_fma(num[m], inv * pre[m - 1], far);
         inv *= den[m];
     }
-    far += num[0] * inv;
+    far = __builtin_fma(num[0], inv, far);
+    far = dx < 0 ? -far : far;
     return near + far * (2.0 * kPiSq);
 }
 

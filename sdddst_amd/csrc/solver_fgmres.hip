@@ -25,7 +25,7 @@ namespace sdamd {
 
 namespace {
 
-constexpr int kRed = 64;    // workgroups of the reductions
+constexpr int kRedMax = 256;  // most workgroups of one projection launch
 constexpr int kMaxV = 64;   // most vectors one dots launch handles (restart + 1)
 
 // y = A x, one wavefront per row.
@@ -42,93 +42,78 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap,
     if (lane == 0) y[r] = s;
 }
 
-// out[v] (+)= sum_i V[v][i] * w[i] for v < nv in one launch: every block
-// reduces its rows per v (wave shuffles, then LDS across the 4 waves) into
-// partial[block][v]; the last block to finish (atomic ticket) adds the
-// partials in block order, so the result does not depend on scheduling.
-__global__ __launch_bounds__(256) void k_dots(int n, int nv, const double *__restrict__ V,
-                                              const double *__restrict__ w, double *__restrict__ partial,
-                                              unsigned *__restrict__ ticket, double *__restrict__ out, int accumulate)
+__device__ __forceinline__ double wave_sum(double s)
 {
-    __shared__ double wsum[4][kMaxV];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+// One classical Gram-Schmidt pass of the Arnoldi step in one launch.  With
+// hsub (nullable) w first becomes w - sum_v hsub[v] V[v] (stored back, the
+// subtraction of k_orth); then out[v] = V[v] . w for v < nv and, with
+// with_self, out[nv] = w . w; acc (nullable) receives
+// (accumulate ? acc : 0) + out[0..nv).  Each workgroup reduces its rows (wave
+// butterflies, the 4 waves in order) into partial[block][0..nv]; the last
+// workgroup to arrive (atomic ticket) adds the partials in a fixed tree over
+// the block index, so the sums do not depend on scheduling.
+__global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__restrict__ V, double *__restrict__ w,
+                                              const double *__restrict__ hsub, double *__restrict__ partial,
+                                              int pstride, unsigned *__restrict__ ticket, double *__restrict__ out,
+                                              double *__restrict__ acc_out, int accumulate, int with_self)
+{
+    __shared__ double wsum[4][kMaxV + 1];
     __shared__ bool last;
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    const int stride = (int)(gridDim.x * blockDim.x);
-    const int i0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    for (int v = 0; v < nv; ++v) {
-        double acc = 0.0;
-        for (int i = i0; i < n; i += stride) acc += V[(size_t)v * n + i] * w[i];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) wsum[wave][v] = acc;
+    const int stride = (int)(gridDim.x * 256);
+    const int i0 = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (hsub) {
+        for (int i = i0; i < n; i += stride) {
+            double a = w[i];
+            for (int v = 0; v < nv; ++v) a -= hsub[v] * V[(size_t)v * n + i];
+            w[i] = a;
+        }
+    }
+    const int nvt = nv + (with_self ? 1 : 0);
+    // four projections per sweep: their loads are in flight together
+    for (int v0 = 0; v0 < nvt; v0 += 4) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = i0; i < n; i += stride) {
+            const double wi = w[i];
+            double p[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int v = v0 + q;
+                p[q] = v < nv ? V[(size_t)v * n + i] : (v == nv ? wi : 0.0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += p[q] * wi;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double t = wave_sum(acc[q]);
+            if (lane == 0 && v0 + q < nvt) wsum[wave][v0 + q] = t;
+        }
     }
     __syncthreads();
-    for (int v = (int)threadIdx.x; v < nv; v += 256)
-        partial[(size_t)blockIdx.x * nv + v] = ((wsum[0][v] + wsum[1][v]) + wsum[2][v]) + wsum[3][v];
+    for (int v = (int)threadIdx.x; v < nvt; v += 256)
+        partial[(size_t)blockIdx.x * pstride + v] = ((wsum[0][v] + wsum[1][v]) + wsum[2][v]) + wsum[3][v];
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     __syncthreads();
     if (!last) return;
     __threadfence();
-    for (int v = (int)threadIdx.x; v < nv; v += 256) {
-        double sum = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) sum += __builtin_nontemporal_load(&partial[(size_t)b * nv + v]);
-        out[v] = accumulate ? out[v] + sum : sum;
+    for (int v = wave; v < nvt; v += 4) {
+        double t = 0.0;
+        for (int b = lane; b < (int)gridDim.x; b += 64)
+            t += __builtin_nontemporal_load(&partial[(size_t)b * pstride + v]);
+        t = wave_sum(t);
+        if (lane == 0) {
+            out[v] = t;
+            if (acc_out && v < nv) acc_out[v] = accumulate ? acc_out[v] + t : t;
+        }
     }
     if (threadIdx.x == 0) *ticket = 0u;
-}
-
-// The same reduction by one workgroup of 1024 lanes (systems up to 2^18
-// rows): strided loads, wave sums, then the 16 wave partials of each v added
-// in wave order; no atomics, one launch.  out[v] = V[v] . w; with_self also
-// out[nv] = w . w; acc (nullable) receives (accumulate ? acc : 0) + out.
-__global__ __launch_bounds__(1024) void k_dots1(int n, int nv, const double *__restrict__ V,
-                                                const double *__restrict__ w, double *__restrict__ out,
-                                                double *__restrict__ acc_out, int accumulate, int with_self)
-{
-    __shared__ double wsum[16][kMaxV];
-    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    // four vectors x eight elements of loads in flight before their products;
-    // w . w rides along with the first group
-    double wsq = 0.0;
-    for (int v0 = 0; v0 < nv; v0 += 4) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * 1024) {
-            double wv[8], p[4][8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int i = i0 + u * 1024;
-                wv[u] = i < n ? w[i] : 0.0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) p[q][u] = (i < n && v0 + q < nv) ? V[(size_t)(v0 + q) * n + i] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int u = 0; u < 8; ++u) acc[q] += p[q][u] * wv[u];
-            if (v0 == 0 && with_self)
-#pragma unroll
-                for (int u = 0; u < 8; ++u) wsq += wv[u] * wv[u];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double a = acc[q];
-            for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
-            if (lane == 0 && v0 + q < nv) wsum[wave][v0 + q] = a;
-        }
-    }
-    const int nvt = nv + (with_self ? 1 : 0);
-    if (with_self) {
-        for (int off = 32; off > 0; off >>= 1) wsq += __shfl_down(wsq, off, 64);
-        if (lane == 0) wsum[wave][nv] = wsq;
-    }
-    __syncthreads();
-    for (int v = (int)threadIdx.x; v < nvt; v += 1024) {
-        double sum = 0.0;
-        for (int k = 0; k < 16; ++k) sum += wsum[k][v];
-        out[v] = sum;
-        if (acc_out && v < nv) acc_out[v] = accumulate ? acc_out[v] + sum : sum;
-    }
 }
 
 // w -= sum_v h[v] V[v]   (h from the latest projection pass in hp)
@@ -140,6 +125,18 @@ __global__ void k_orth(int n, int nv, const double *__restrict__ V, const double
     double acc = w[i];
     for (int v = 0; v < nv; ++v) acc -= hp[v] * V[(size_t)v * n + i];
     w[i] = acc;
+}
+
+// y = a * (w - sum_v h[v] V[v]): the second Gram-Schmidt subtraction and the
+// normalisation of the next basis vector in one pass (w is left unchanged).
+__global__ void k_orth_scale(int n, int nv, const double *__restrict__ V, const double *__restrict__ hp,
+                             const double *__restrict__ w, double a, double *__restrict__ y)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double acc = w[i];
+    for (int v = 0; v < nv; ++v) acc -= hp[v] * V[(size_t)v * n + i];
+    y[i] = a * acc;
 }
 
 __global__ void k_scale_copy(int n, double a, const double *__restrict__ x, double *__restrict__ y)
@@ -195,12 +192,6 @@ __global__ void k_pair_apply(int n, const double *__restrict__ binv, const int *
     z[i] = r;
 }
 
-// hsum[v] = (accumulate ? hsum[v] : 0) + hp[v]
-__global__ void k_add(int nv, const double *__restrict__ hp, double *__restrict__ hsum, int accumulate)
-{
-    for (int v = (int)threadIdx.x; v < nv; v += 64) hsum[v] = accumulate ? hsum[v] + hp[v] : hp[v];
-}
-
 // x += sum_j y[j] Z[j]
 __global__ void k_combine(int n, int k, const double *__restrict__ Z, const double *__restrict__ yv,
                           double *__restrict__ x)
@@ -223,20 +214,19 @@ __global__ void k_resid(int n, const double *__restrict__ g, double *__restrict_
 
 static inline int nb256(int n) { return (n + 255) / 256; }
 
-// out[v] = V[v] . w for v < nv (and out[nv] = w . w with with_self);
-// acc (nullable) += or = out[0..nv)
-static void launch_dots(Context &c, Fgmres &f, int nv, const double *V, const double *w, double *out,
-                        double *acc = nullptr, int accumulate = 0, int with_self = 0)
+static inline int proj_blocks(int n)
 {
-    if (f.n <= (1 << 18)) {
-        hipLaunchKernelGGL(k_dots1, dim3(1), dim3(1024), 0, c.stream, f.n, nv, V, w, out, acc, accumulate,
-                           with_self);
-        return;
-    }
-    hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, nv, V, w, f.red, f.ticket, out, 0);
-    if (acc) hipLaunchKernelGGL(k_add, dim3(1), dim3(64), 0, c.stream, nv, out, acc, accumulate);
-    if (with_self)
-        hipLaunchKernelGGL(k_dots, dim3(kRed), dim3(256), 0, c.stream, f.n, 1, w, w, f.red, f.ticket, out + nv, 0);
+    const int b = nb256(n);
+    return b < kRedMax ? (b > 0 ? b : 1) : kRedMax;
+}
+
+// One projection pass (k_proj): with hsub, w -= V hsub first; out[v] = V[v].w
+// for v < nv (and out[nv] = w.w with with_self); acc (nullable) += or = out.
+static void launch_proj(Context &c, Fgmres &f, int nv, const double *V, double *w, const double *hsub,
+                        double *out, double *acc = nullptr, int accumulate = 0, int with_self = 0)
+{
+    hipLaunchKernelGGL(k_proj, dim3(proj_blocks(f.n)), dim3(256), 0, c.stream, f.n, nv, V, w, hsub, f.red, f.m + 2,
+                       f.ticket, out, acc, accumulate, with_self);
 }
 
 int Fgmres::init(Context &c)
@@ -254,10 +244,10 @@ int Fgmres::init(Context &c)
     SD_HIP(hipMalloc(&partner, sizeof(int) * n));
     SD_HIP(hipMalloc(&bcol, sizeof(int) * n));
     SD_CHECK(m + 1 <= kMaxV, SDDDST_ERR_ARG, "FGMRES restart %d exceeds %d", m, kMaxV - 1);
-    SD_HIP(hipMalloc(&red, sizeof(double) * (size_t)kRed * (m + 2)));
+    SD_HIP(hipMalloc(&red, sizeof(double) * (size_t)kRedMax * (m + 2)));
     SD_HIP(hipMalloc(&ticket, sizeof(unsigned)));
     SD_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), c.stream));
-    SD_HIP(hipMalloc(&hcol, sizeof(double) * 2 * (m + 2)));
+    SD_HIP(hipMalloc(&hcol, sizeof(double) * 3 * (m + 2)));
     SD_HIP(hipHostMalloc(&H, sizeof(double) * 2 * (m + 2), hipHostMallocDefault));
     return SDDDST_OK;
 }
@@ -325,39 +315,54 @@ int Fgmres::apply(Context &c, const double *v, double *z)
 // ||w - V h2||^2 = ||w||^2 - ||h2||^2 (V orthonormal); h2 is tiny after the
 // first pass, and when it is not (||h2||^2 > ||w||^2 / 16) the norm is
 // computed explicitly.
-static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost)
+// Arnoldi step for the new vector w against V[0..nv): CGS2 (two projection
+// passes, the first subtraction fused into the second pass).  On return
+// hhost[0..nv) = h = h1 + h2 and hhost[nv] = ||w - V h||, from
+// ||w'||^2 - ||h2||^2 (w' = w - V h1, orthogonal to V up to h2) unless the
+// second pass removed more than a quarter of ||w'||, when the norm is taken
+// explicitly.  next (nullable) receives (w - V h) / hhost[nv] when that is
+// positive; w then holds w' (or w - V h after the explicit norm).
+static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *next)
 {
     const int n = f.n;
-    double *hp = f.hcol;                // this pass's projections (+ w.w)
-    double *hsum = f.hcol + (f.m + 2);  // accumulated
-    for (int pass = 0; pass < 2; ++pass) {
-        launch_dots(c, f, nv, f.V, f.w, hp, hsum, pass, pass);
-        hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, hp, f.w);
-        SD_HIP(hipGetLastError());
-    }
-    // one copy of hcol: h2 and w.w at [0, nv], h at [m+2, m+2+nv)
-    double *h2 = hhost + (f.m + 2);
-    SD_HIP(hipMemcpyAsync(h2, hp, sizeof(double) * (f.m + 2 + nv), hipMemcpyDeviceToHost, c.stream));
+    double *h1 = f.hcol;                      // first pass
+    double *h2 = f.hcol + (f.m + 2);          // second pass (+ w'.w')
+    double *hsum = f.hcol + 2 * (f.m + 2);    // h1 + h2
+    launch_proj(c, f, nv, f.V, f.w, nullptr, h1, hsum, 0, 0);
+    launch_proj(c, f, nv, f.V, f.w, h1, h2, hsum, 1, 1);
+    SD_HIP(hipGetLastError());
+    // one copy: h2 and w'.w' at [0, nv], h at [m+2, m+2+nv)
+    double *hc = hhost + (f.m + 2);
+    SD_HIP(hipMemcpyAsync(hc, h2, sizeof(double) * (f.m + 2 + nv), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
-    for (int v = 0; v < nv; ++v) hhost[v] = h2[f.m + 2 + v];
+    for (int v = 0; v < nv; ++v) hhost[v] = hc[f.m + 2 + v];
     double s2 = 0.0;
-    for (int v = 0; v < nv; ++v) s2 += h2[v] * h2[v];
-    const double ww = h2[nv];
+    for (int v = 0; v < nv; ++v) s2 += hc[v] * hc[v];
+    const double ww = hc[nv];
     if (s2 <= ww * 0.0625) {
-        hhost[nv] = sqrt(ww - s2);
+        const double hn = sqrt(ww - s2);
+        hhost[nv] = hn;
+        if (next && hn > 0)
+            hipLaunchKernelGGL(k_orth_scale, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, h2, f.w, 1.0 / hn,
+                               next);
+        SD_HIP(hipGetLastError());
         return SDDDST_OK;
     }
-    launch_dots(c, f, 1, f.w, f.w, hsum + nv);
+    hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, h2, f.w);
+    launch_proj(c, f, 0, f.V, f.w, nullptr, hsum + nv, nullptr, 0, 1);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(hhost + nv, hsum + nv, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     hhost[nv] = sqrt(hhost[nv]);
+    if (next && hhost[nv] > 0)
+        hipLaunchKernelGGL(k_scale_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, 1.0 / hhost[nv], f.w, next);
+    SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
 static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
-    launch_dots(c, f, 1, v, v, f.hcol);
+    launch_proj(c, f, 0, f.V, const_cast<double *>(v), nullptr, f.hcol, nullptr, 0, 1);
     SD_HIP(hipGetLastError());
     SD_HIP(hipMemcpyAsync(f.H, f.hcol, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
@@ -413,12 +418,9 @@ int Fgmres::solve(Context &c)
             hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow,
                                Z + (size_t)k * N, w);
             SD_HIP(hipGetLastError());
-            rc = arnoldi_orth(c, *this, k + 1, hh.data());
+            rc = arnoldi_orth(c, *this, k + 1, hh.data(), V + (size_t)(k + 1) * N);
             if (rc) return rc;
             const double hnext = hh[k + 1];
-            if (hnext > 0)
-                hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / hnext, w,
-                                   V + (size_t)(k + 1) * N);
             // apply previous rotations to the new column
             for (int i = 0; i < k; ++i) {
                 const double t = cs[i] * hh[i] + sn[i] * hh[i + 1];

@@ -175,37 +175,55 @@ __global__ void k_core_mirror(int64_t ne, const int *__restrict__ Cp, const int 
     Cmir[e] = lo;
 }
 
-// Node and edge features and the FP64 scales of C (one lane per row).
-__global__ void k_features(int n, const int *__restrict__ Cp, const int *__restrict__ Ci,
-                           const int *__restrict__ Ck, const int *__restrict__ diag, const double *__restrict__ Ax,
-                           const double *__restrict__ AxRow, const double *__restrict__ binv,
-                           const int *__restrict__ bcol, float *__restrict__ U, float *__restrict__ E,
-                           double *__restrict__ se, double *__restrict__ sd)
+// Sum over the G lanes of a lane group (G a power of two dividing 64).
+template <int G>
+__device__ __forceinline__ double group_sum(double s)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    const double bdi = binv[2 * i];
-    double sr = 0, sc = 0, t = 0;
-    for (int e = Cp[i]; e < Cp[i + 1]; ++e) {
-        const int k = Ck[e], j = Ci[e];
-        const double aij = AxRow[k], aji = Ax[k], bdj = binv[2 * j];
-        const double xe = bdi * aij, ye = aij * bdj, we = bdj * aji;
-        se[e] = -bdi * aij * bdj / (1.0 + fabs(xe) + fabs(we));
-        t += xe * we / (1.0 + fabs(xe * we));
-        sr += fabs(xe);
-        sc += fabs(aji * bdi);
-        const double xw = fabs(xe * we);
-        E[4 * (size_t)e + 0] = (float)(xe / (1.0 + fabs(xe)));
-        E[4 * (size_t)e + 1] = (float)(ye / (1.0 + fabs(ye)));
-        E[4 * (size_t)e + 2] = (float)(we / (1.0 + fabs(we)));
-        E[4 * (size_t)e + 3] = (float)(xw / (1.0 + xw));
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+    return s;
+}
+
+// Node and edge features and the FP64 scales of C.  A group of G lanes
+// shares a row: lane g takes its edges g, g + G, ... (coalesced loads and
+// 16-byte feature stores), the row sums close with a butterfly in the group.
+template <int G>
+__global__ __launch_bounds__(256) void k_features(int n, const int *__restrict__ Cp, const int *__restrict__ Ci,
+                                                  const int *__restrict__ Ck, const int *__restrict__ diag,
+                                                  const double *__restrict__ Ax, const double *__restrict__ AxRow,
+                                                  const double *__restrict__ binv, const int *__restrict__ bcol,
+                                                  float *__restrict__ U, float *__restrict__ E,
+                                                  double *__restrict__ se, double *__restrict__ sd)
+{
+    const int g = (int)(threadIdx.x & (G - 1));
+    const int i = (int)((blockIdx.x * 256 + threadIdx.x) / G);
+    const bool ok = i < n;
+    double sr = 0, sc = 0, t = 0, bdi = 0;
+    if (ok) {
+        bdi = binv[2 * i];
+        const int end = Cp[i + 1];
+        for (int e = Cp[i] + g; e < end; e += G) {
+            const int k = Ck[e], j = Ci[e];
+            const double aij = AxRow[k], aji = Ax[k], bdj = binv[2 * j];
+            const double xe = bdi * aij, ye = aij * bdj, we = bdj * aji;
+            se[e] = -bdi * aij * bdj / (1.0 + fabs(xe) + fabs(we));
+            t += xe * we / (1.0 + fabs(xe * we));
+            sr += fabs(xe);
+            sc += fabs(aji * bdi);
+            const double xw = fabs(xe * we);
+            *reinterpret_cast<float4 *>(E + 4 * (size_t)e) =
+                make_float4((float)(xe / (1.0 + fabs(xe))), (float)(ye / (1.0 + fabs(ye))),
+                            (float)(we / (1.0 + fabs(we))), (float)(xw / (1.0 + xw)));
+        }
     }
+    sr = group_sum<G>(sr);
+    sc = group_sum<G>(sc);
+    t = group_sum<G>(t);
+    if (!ok || g != 0) return;
     sd[i] = bdi * t;
     const double rho = fabs(fabs(AxRow[diag[i]]) * bdi);
-    U[4 * i + 0] = (float)(sr / (1.0 + sr));
-    U[4 * i + 1] = (float)(sc / (1.0 + sc));
-    U[4 * i + 2] = bcol[i] >= 0 ? 1.0f : 0.0f;
-    U[4 * i + 3] = (float)(rho / (1.0 + rho));
+    *reinterpret_cast<float4 *>(U + 4 * (size_t)i) =
+        make_float4((float)(sr / (1.0 + sr)), (float)(sc / (1.0 + sc)), bcol[i] >= 0 ? 1.0f : 0.0f,
+                    (float)(rho / (1.0 + rho)));
 }
 
 // -- forward ----------------------------------------------------------------
@@ -334,21 +352,32 @@ __global__ void k_gnn_nodes(int n, const float *__restrict__ th, const float *__
     Cd[i] = (double)g * sd[i];
 }
 
-// z_p = M v_p for probe p = blockIdx.y (one lane per row).
-__global__ void k_spai_apply(int n, const double *__restrict__ binv, const int *__restrict__ bcol,
-                             const int *__restrict__ Cp, const int *__restrict__ Ci, const double *__restrict__ Ce,
-                             const double *__restrict__ Cd, const double *__restrict__ v, double *__restrict__ z)
+// z_p = M v_p for probe p = blockIdx.y; G lanes per row share its core
+// entries, as in k_features.
+template <int G>
+__global__ __launch_bounds__(256) void k_spai_apply(int n, const double *__restrict__ binv,
+                                                    const int *__restrict__ bcol, const int *__restrict__ Cp,
+                                                    const int *__restrict__ Ci, const double *__restrict__ Ce,
+                                                    const double *__restrict__ Cd, const double *__restrict__ v,
+                                                    double *__restrict__ z)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
+    const int g = (int)(threadIdx.x & (G - 1));
+    const int i = (int)((blockIdx.x * 256 + threadIdx.x) / G);
+    const bool ok = i < n;
     const size_t off = (size_t)blockIdx.y * n;
     const double *vp = v + off;
+    double s = 0.0;
+    if (ok) {
+        const int end = Cp[i + 1];
+        for (int e = Cp[i] + g; e < end; e += G) s += Ce[e] * vp[Ci[e]];
+    }
+    s = group_sum<G>(s);
+    if (!ok || g != 0) return;
     const int j = bcol[i];
     double r = binv[2 * i] * vp[i];
     if (j >= 0) r += binv[2 * i + 1] * vp[j];
     r += Cd[i] * vp[i];
-    for (int e = Cp[i]; e < Cp[i + 1]; ++e) r += Ce[e] * vp[Ci[e]];
-    z[off + i] = r;
+    z[off + i] = r + s;
 }
 
 // -- training ---------------------------------------------------------------
@@ -628,6 +657,39 @@ __global__ void k_adam(int np, const double *__restrict__ g, float *__restrict__
 
 static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// Lanes per row of the row-group kernels: about the mean core degree, so a
+// row takes one or two passes.
+static int row_group(const Spai &s)
+{
+    const double avg = s.n > 0 ? (double)s.ne / s.n : 0.0;
+    return avg <= 3 ? 4 : avg <= 7 ? 8 : avg <= 15 ? 16 : 32;
+}
+
+template <int G>
+static void launch_features(Context &c, const Spai &s, const Fgmres &f)
+{
+    hipLaunchKernelGGL(k_features<G>, dim3(nblk((int64_t)c.n * G)), dim3(256), 0, c.stream, c.n, s.Cp, s.Ci, s.Ck,
+                       c.diag, c.Ax, c.AxRow, f.binv, f.bcol, s.U, s.E, s.se, s.sd);
+}
+
+template <int G>
+static void launch_apply_g(Context &c, const Spai &s, const Fgmres &f, int k, const double *v, double *z)
+{
+    hipLaunchKernelGGL(k_spai_apply<G>, dim3(nblk((int64_t)c.n * G), k), dim3(256), 0, c.stream, c.n, f.binv,
+                       f.bcol, s.Cp, s.Ci, s.Ce, s.Cd, v, z);
+}
+
+// z_p = M v_p for k column vectors of length n.
+static void launch_apply(Context &c, const Spai &s, const Fgmres &f, int k, const double *v, double *z)
+{
+    switch (row_group(s)) {
+    case 4: launch_apply_g<4>(c, s, f, k, v, z); break;
+    case 8: launch_apply_g<8>(c, s, f, k, v, z); break;
+    case 16: launch_apply_g<16>(c, s, f, k, v, z); break;
+    default: launch_apply_g<32>(c, s, f, k, v, z); break;
+    }
+}
+
 static int spai_alloc_nodes(Context &c, Spai &s)
 {
     if (s.n == c.n && s.act) return SDDDST_OK;
@@ -710,8 +772,12 @@ static int spai_forward(Context &c, const Fgmres &f)
     if (ne > 0)
         hipLaunchKernelGGL(k_core_mirror, dim3(nblk(ne)), dim3(256), 0, c.stream, (int64_t)ne, s.Cp, s.Ci, s.Cr,
                            s.Cmir);
-    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, c.stream, n, s.Cp, s.Ci, s.Ck, c.diag, c.Ax, c.AxRow,
-                       f.binv, f.bcol, s.U, s.E, s.se, s.sd);
+    switch (row_group(s)) {
+    case 4: launch_features<4>(c, s, f); break;
+    case 8: launch_features<8>(c, s, f); break;
+    case 16: launch_features<16>(c, s, f); break;
+    default: launch_features<32>(c, s, f); break;
+    }
     float *a0 = s.act_(0), *h0 = s.act_(1), *z1 = s.act_(2), *S1 = s.act_(3), *ph1 = s.act_(4), *h1 = s.act_(5);
     float *z2 = s.act_(6), *S2 = s.act_(7), *ph2 = s.act_(8), *h2 = s.act_(9);
     hipLaunchKernelGGL(k_gnn_in, dim3(nblk(n, 8)), dim3(256), 0, c.stream, n, s.theta, s.U, a0, h0, z1);
@@ -738,8 +804,7 @@ int spai_factor(Context &c, const Fgmres &f)
 int spai_apply(Context &c, const Fgmres &f, const double *v, double *z)
 {
     Spai &s = *c.spai;
-    hipLaunchKernelGGL(k_spai_apply, dim3(nblk(c.n), 1), dim3(256), 0, c.stream, c.n, f.binv, f.bcol, s.Cp, s.Ci,
-                       s.Ce, s.Cd, v, z);
+    launch_apply(c, s, f, 1, v, z);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
@@ -774,8 +839,7 @@ int spai_train_step(Context &c, Fgmres &f, int k, unsigned long long seed, doubl
     const int64_t ne = s.ne;
     // loss and dL/dR
     hipLaunchKernelGGL(k_probes, dim3(nblk((int64_t)k * n)), dim3(256), 0, c.stream, k, n, seed, s.W);
-    hipLaunchKernelGGL(k_spai_apply, dim3(nblk(n), k), dim3(256), 0, c.stream, n, f.binv, f.bcol, s.Cp, s.Ci, s.Ce,
-                       s.Cd, s.W, s.Y);
+    launch_apply(c, s, f, k, s.W, s.Y);
     hipLaunchKernelGGL(k_spmv_batch, dim3(nblk(n, 4), k), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.AxRow, s.Y, s.R);
     hipLaunchKernelGGL(k_loss_rows, dim3(nblk(n)), dim3(256), 0, c.stream, k, n, eps, s.W, s.R, s.lrow);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, c.stream, n, s.lrow, 1.0 / n, s.loss);

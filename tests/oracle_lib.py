@@ -127,8 +127,8 @@ def mathcheck():
     global _mc
     if _mc is not None:
         return _mc
-    if not os.path.exists(MATHCHECK_SO):
-        _build(os.path.join(ROOT, "tests", "native"), "libmathcheck.so")
+    # make rebuilds it when the device headers it compiles have changed
+    _build(os.path.join(ROOT, "tests", "native"), "libmathcheck.so")
     lib = ctypes.CDLL(MATHCHECK_SO)
     lib.mc_batch1.argtypes = [ctypes.c_int, L, DP, DP]
     lib.mc_batch2.argtypes = [ctypes.c_int, L, DP, DP, DP]

@@ -29,13 +29,13 @@ constexpr int kImages = 4;                // ANALYTIC_FIELD_N (include/constants
 
 // utility.h:29-40.  Each +-1 step of the reference loop is exact, so the
 // loop equals d - k for the integer k that lands in [-0.5, 0.5); a zero
-// (either sign) passes through the loop unchanged.
+// (either sign) passes through the loop unchanged.  d - rint(d) is exact and
+// lies in [-0.5, 0.5] (ties round to even), so only +0.5 needs the step.
 SDLIBM_FN double normalize(double d)
 {
     if (d == 0.0) return d;
     double r = d - __builtin_rint(d);
     if (r >= 0.5) r -= 1.0;
-    if (r < -0.5) r += 1.0;
     return r;
 }
 

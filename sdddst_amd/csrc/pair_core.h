@@ -99,6 +99,67 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
     }
 }
 
+// Strict upper triangle: tile id -> (bi, bj), 0 <= bi < bj < nb, column by
+// column.
+__device__ __forceinline__ void strict_coords(int t, int &bi, int &bj)
+{
+    int c = (int)((1.0 + sqrt(8.0 * (double)t + 1.0)) * 0.5);
+    while ((long)c * (c - 1) / 2 > t) --c;
+    while ((long)(c + 1) * c / 2 <= t) ++c;
+    bj = c;
+    bi = (int)(t - (long)c * (c - 1) / 2);
+}
+
+// A diagonal tile in half the sweeps.  Lane l meets m = (l + s) mod 64 for
+// s = 1..32: every unordered pair of the tile once (s = 32 pairs l with
+// l + 32, kept for l < 32).  The pair term is evaluated lower index first,
+// as in the off-diagonal tiles: when m < l the difference is taken as
+// x_m - x_l and the term enters the row with a minus sign and the column
+// with a plus sign.  Called by one whole 64-lane workgroup.
+template <int Mode>
+__device__ __forceinline__ void diag_tile(int bk, int n, const double *__restrict__ x, const double *__restrict__ y,
+                                          const double *__restrict__ b, double *__restrict__ partial,
+                                          unsigned long long *__restrict__ minr2, double r2_keep)
+{
+    __shared__ double sx[kTile], sy[kTile], sb[kTile];
+    const int l = (int)threadIdx.x;
+    const int i = bk * kTile + l;
+    const double xi = i < n ? x[i] : 0.0;
+    const double yi = i < n ? y[i] : 0.0;
+    const double bI = i < n ? b[i] : 0.0;
+    sx[l] = xi;
+    sy[l] = yi;
+    sb[l] = bI;
+    __syncthreads();
+    double row = 0.0, col = 0.0;
+    for (int s = 1; s <= kTile / 2; ++s) {
+        const int m = (l + s) & 63;
+        const int j = bk * kTile + m;
+        const bool lower = m > l;
+        const bool valid = (i < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
+        const double ex = lower ? xi - sx[m] : sx[m] - xi;
+        const double ey = lower ? yi - sy[m] : sy[m] - yi;
+        const double dx = sdfield::normalize(ex);
+        const double dy = sdfield::normalize(ey);
+        const double t = bI * sb[m] * field_tau<Mode>(dx, dy);
+        const double r2 = dx * dx + dy * dy;
+        if (valid) {
+            const double tl = lower ? t : -t;
+            row += tl;
+            col -= tl;
+            if (r2 < r2_keep) {
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
+                atomicMin(&minr2[i], bits);
+                atomicMin(&minr2[j], bits);
+            }
+        }
+        col = rotate_down1(col, l);
+    }
+    // after sweeps 1..32 the column sum of source m sits in lane (m - 33) mod 64
+    const double cs = __shfl(col, (l + 31) & 63, 64);
+    if (i < n) partial[(size_t)bk * n + i] = row + cs;
+}
+
 // r^2 below which a pair can change a tolerance: 400 prec^2 with a margin
 // for the roundings of r^2 * 0.25 * 1e-2.
 __host__ __device__ inline double tolerance_r2(double minPrecSqr) { return minPrecSqr * 400.0 * (1.0 + 1e-6); }

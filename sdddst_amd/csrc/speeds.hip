@@ -39,14 +39,28 @@ namespace {
 using pairtile::kInfBits;
 using pairtile::kTile;
 
+// Work units of equal cost: the nb (nb - 1) / 2 off-diagonal tiles, then
+// the diagonal tiles two to a unit (each in half the sweeps), so that the
+// units divide evenly over the resident waves.
 template <int Mode>
-__global__ __launch_bounds__(64) void k_pair_tiles(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                   const double *__restrict__ b, double *__restrict__ partial,
-                                                   unsigned long long *__restrict__ minr2, double r2_keep)
+__global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, const double *__restrict__ x,
+                                                   const double *__restrict__ y, const double *__restrict__ b,
+                                                   double *__restrict__ partial, unsigned long long *__restrict__ minr2,
+                                                   double r2_keep)
 {
-    int bi, bj;
-    pairtile::tile_coords((int)blockIdx.x, bi, bj);
-    pairtile::tile<Mode>(bi, bj, n, x, y, b, partial, minr2, r2_keep);
+    const int u = (int)blockIdx.x;
+    if (u < nfull) {
+        int bi, bj;
+        pairtile::strict_coords(u, bi, bj);
+        pairtile::tile<Mode>(bi, bj, n, x, y, b, partial, minr2, r2_keep);
+        return;
+    }
+    const int bk = 2 * (u - nfull);
+    pairtile::diag_tile<Mode>(bk, n, x, y, b, partial, minr2, r2_keep);
+    if (bk + 1 < nb) {
+        __syncthreads();
+        pairtile::diag_tile<Mode>(bk + 1, n, x, y, b, partial, minr2, r2_keep);
+    }
 }
 
 // simulation.cpp:111-128: point-defect force on dislocation i and its
@@ -170,15 +184,16 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(threads), 0, c.stream, c.minr2, kInfBits, n);
     SD_HIP(hipGetLastError());
     const int nb = c.nb;
-    const long tiles = (long)nb * (nb + 1) / 2;
-    SD_CHECK(tiles < (1L << 31), SDDDST_ERR_ARG, "too many tiles for N=%d", n);
+    const long nfull = (long)nb * (nb - 1) / 2;
+    const long units = nfull + (nb + 1) / 2;
+    SD_CHECK(units < (1L << 31), SDDDST_ERR_ARG, "too many tiles for N=%d", n);
     if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
-        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
-                           c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
+        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
+                           (int)nfull, c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
     else
-        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)tiles), dim3(kTile), 0, c.stream, n,
-                           c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
+        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
+                           (int)nfull, c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
     SD_HIP(hipGetLastError());
     if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.partial, c.x[slot],

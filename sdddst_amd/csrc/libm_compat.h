@@ -285,23 +285,49 @@ SDLIBM_FN double do_sin(double x, double dx)
 }  // namespace detail
 
 // cos(x) for |x| <= 4 (the field evaluates it at 2*pi*d with |d| <= 0.5).
+// glibc's three argument ranges -- do_cos(x, 0) below 0.855469,
+// do_sin(pi/2 - |x|) below 2.426265 (with its Taylor form below 0.126), and
+// -do_cos(|x| - pi) -- are evaluated without divergent branches: one reduced
+// argument, one table lookup and the do_cos / do_sin combinations selected
+// per lane, each with the operation order of its branch form, so the value
+// is the same.  (Random arguments put all three ranges in most wavefronts.)
 SDLIBM_FN double cos(double x)
 {
     using namespace detail;
     const double ax = fabsd(x);
-    const uint32_t k = hiword(ax);
-    if (k < 0x3e400000u) return 1.0;                   // |x| < 2^-27
-    if (k < 0x3feb6000u) return do_cos(x, 0.0);        // |x| < 0.855469
-    if (k < 0x400368fdu) {                             // |x| < 2.426265
-        const double y = hp0 - ax;
-        const double a = y + hp1;
-        const double da = (y - a) + hp1;
-        return do_sin(a, da);
-    }
-    const double t = ax - pi_hi;                       // exact for |x| in [pi/2, 2pi]
-    const double a = t - pi_lo;
-    const double da = (t - a) - pi_lo;
-    return -do_cos(a, da);
+    const uint32_t hk = hiword(ax);
+    const bool r1 = hk < 0x3feb6000u;                  // |x| < 0.855469
+    const bool r2 = !r1 && hk < 0x400368fdu;           // |x| < 2.426265
+    const double y2 = hp0 - ax;
+    const double a2 = y2 + hp1;
+    const double da2 = (y2 - a2) + hp1;
+    const double t3 = ax - pi_hi;                      // exact for |x| in [pi/2, 2pi]
+    const double a3 = t3 - pi_lo;
+    const double da3 = (t3 - a3) - pi_lo;
+    const double a = r1 ? x : (r2 ? a2 : a3);
+    const double da0 = r1 ? 0.0 : (r2 ? da2 : da3);
+    const double da = (r2 ? a <= 0 : a < 0) ? -da0 : da0;
+    const double u = big + fabsd(a);
+    const double xr = fabsd(a) - (u - big);
+    const int k = tab_index(u);
+    const double y = r2 ? xr : xr + da;                // do_sin keeps dx apart
+    const double xx = y * y;
+    const double P = fmad(xx, sn5, sn3);
+    const double s = r2 ? y + fmad(y * xx, P, da) : fmad(y * xx, P, y);
+    const double q = xx * fmad(xx, fmad(xx, cs6, cs4), cs2);
+    const double c = r2 ? fmad(y, da, q) : q;
+    const double sn = sdlibm_sincos_tab[k], ssn = sdlibm_sincos_tab[k + 1];
+    const double cs = sdlibm_sincos_tab[k + 2], ccs = sdlibm_sincos_tab[k + 3];
+    const double A = r2 ? sn : cs, Alo = r2 ? ssn : ccs;
+    const double B = r2 ? cs : -sn, Blo = r2 ? ccs : -ssn;
+    const double res = A + fmad(B, s, fmad(-A, c, fmad(s, Blo, Alo)));
+    // do_sin's Taylor form for |a| < 0.126 (dx as passed in)
+    const double tx = a2 * a2;
+    const double tP = fmad(fmad(fmad(fmad(ts5, tx, ts4), tx, ts3), tx, ts2), tx, ts1);
+    const double tay = a2 + fmad(fmad(tP, a2, -0.5 * da2), tx, da2);
+    const double v2 = fabsd(a2) < 0.126 ? tay : copysignd(res, a2);
+    const double v = r1 ? res : (r2 ? v2 : -res);
+    return hk < 0x3e400000u ? 1.0 : v;                 // |x| < 2^-27
 }
 
 // sin(x) for |x| <= 4.

@@ -62,6 +62,10 @@ struct Context {
     unsigned long long *minr2 = nullptr; // per-call minimum r^2 bits
     double *partial = nullptr;          // [nb][n] pair partial sums
     int nb = 0;                         // number of 64-wide blocks
+    // pair-tile order: perm[p] = dislocation at tile position p (by x bin)
+    int *perm = nullptr, *xkey = nullptr, *xkey_sorted = nullptr, *xidx = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
     // Jacobian (CSC)
@@ -123,6 +127,7 @@ void compute_cutoff(Context &c);
 
 // kernel launchers (return SDDDST status)
 int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol);
+int pair_order_alloc(Context &c);   // buffers of the x ordering of the pair tiles
 int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern);
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
 int launch_newton_update(Context &c, int slot);

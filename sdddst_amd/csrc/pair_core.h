@@ -16,6 +16,14 @@
 // prec^2, so with r2_keep a little above 400 prec^2 the kept minimum gives
 // the same update as the minimum over all pairs, and these rare pairs are
 // recorded with an atomic min instead of a rotated accumulator.
+//
+// With Perm, tile positions are sorted positions: perm[p] is the dislocation
+// at position p (the single-system kernel orders them by x, so that most
+// tiles hold only pairs with |dx| above the small-argument range of cosh and
+// its wavefronts skip that code).  Every pair term is still evaluated lower
+// original index first -- the difference is negated when the source has the
+// lower index, and the term then enters the row with a minus sign and the
+// column with a plus sign -- and the sums land at original indices.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,50 +60,69 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
     bi = t - c * (c + 1) / 2;
 }
 
+// One pair term for target (xi, yi, bI) of original index io and source
+// slot m, lower original index first; returns the row contribution.
+template <int Mode, bool Perm>
+__device__ __forceinline__ double pair_term(double xi, double yi, double bI, int io, int jo, double sxm, double sym,
+                                            double sbm, bool lower_by_lane, double &r2)
+{
+    const bool lower = Perm ? io < jo : lower_by_lane;
+    const double d = xi - sxm, e = yi - sym;
+    const double dx = sdfield::normalize(lower ? d : -d);
+    const double dy = sdfield::normalize(lower ? e : -e);
+    const double t = bI * sbm * field_tau<Mode>(dx, dy);
+    r2 = dx * dx + dy * dy;
+    return lower ? t : -t;
+}
+
 // Called by one whole 64-lane workgroup.
-template <int Mode>
+template <int Mode, bool Perm = false>
 __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
-                                     double *__restrict__ partial, unsigned long long *__restrict__ minr2,
-                                     double r2_keep)
+                                     const int *__restrict__ perm, double *__restrict__ partial,
+                                     unsigned long long *__restrict__ minr2, double r2_keep)
 {
     __shared__ double sx[kTile], sy[kTile], sb[kTile];
+    __shared__ int sid[kTile];
     const int l = (int)threadIdx.x;
-    const int i = bi * kTile + l;
-    const int jl = bj * kTile + l;
-    sx[l] = jl < n ? x[jl] : 0.0;
-    sy[l] = jl < n ? y[jl] : 0.0;
-    sb[l] = jl < n ? b[jl] : 0.0;
-    const double xi = i < n ? x[i] : 0.0;
-    const double yi = i < n ? y[i] : 0.0;
-    const double bI = i < n ? b[i] : 0.0;
+    const int p = bi * kTile + l;              // target position
+    const int q = bj * kTile + l;              // source position held in slot l
+    const int io = Perm ? (p < n ? perm[p] : 0) : p;
+    const int jo = Perm ? (q < n ? perm[q] : 0) : q;
+    sx[l] = q < n ? x[jo] : 0.0;
+    sy[l] = q < n ? y[jo] : 0.0;
+    sb[l] = q < n ? b[jo] : 0.0;
+    sid[l] = jo;
+    const double xi = p < n ? x[io] : 0.0;
+    const double yi = p < n ? y[io] : 0.0;
+    const double bI = p < n ? b[io] : 0.0;
     __syncthreads();
     const bool diag = (bi == bj);
     double row = 0.0, col = 0.0;
     for (int s = 0; s < kTile; ++s) {
         const int m = (l + s) & 63;
         const int j = bj * kTile + m;
-        const bool valid = (i < n) && (j < n) && (!diag || l < m);
-        const double dx = sdfield::normalize(xi - sx[m]);
-        const double dy = sdfield::normalize(yi - sy[m]);
-        const double t = bI * sb[m] * field_tau<Mode>(dx, dy);
-        const double r2 = dx * dx + dy * dy;
+        const bool valid = (p < n) && (j < n) && (!diag || l < m);
+        const int jm = Perm ? sid[m] : j;
+        double r2;
+        // without Perm the target has the lower index in every tile it meets
+        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2);
         if (valid) {
             row += t;
             col -= t;
             if (r2 < r2_keep) {
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
-                atomicMin(&minr2[i], bits);
-                atomicMin(&minr2[j], bits);
+                atomicMin(&minr2[io], bits);
+                atomicMin(&minr2[jm], bits);
             }
         }
         col = rotate_down1(col, l);
     }
     if (diag) {
-        if (i < n) partial[(size_t)bi * n + i] = row + col;
+        if (p < n) partial[(size_t)bi * n + io] = row + col;
     } else {
-        if (i < n) partial[(size_t)bj * n + i] = row;
-        if (jl < n) partial[(size_t)bi * n + jl] = col;
+        if (p < n) partial[(size_t)bj * n + io] = row;
+        if (q < n) partial[(size_t)bi * n + jo] = col;
     }
 }
 
@@ -113,51 +140,50 @@ __device__ __forceinline__ void strict_coords(int t, int &bi, int &bj)
 // A diagonal tile in half the sweeps.  Lane l meets m = (l + s) mod 64 for
 // s = 1..32: every unordered pair of the tile once (s = 32 pairs l with
 // l + 32, kept for l < 32).  The pair term is evaluated lower index first,
-// as in the off-diagonal tiles: when m < l the difference is taken as
-// x_m - x_l and the term enters the row with a minus sign and the column
-// with a plus sign.  Called by one whole 64-lane workgroup.
-template <int Mode>
+// as in the off-diagonal tiles: when the source has the lower index the
+// difference is negated and the term enters the row with a minus sign and
+// the column with a plus sign.  Called by one whole 64-lane workgroup.
+template <int Mode, bool Perm = false>
 __device__ __forceinline__ void diag_tile(int bk, int n, const double *__restrict__ x, const double *__restrict__ y,
-                                          const double *__restrict__ b, double *__restrict__ partial,
-                                          unsigned long long *__restrict__ minr2, double r2_keep)
+                                          const double *__restrict__ b, const int *__restrict__ perm,
+                                          double *__restrict__ partial, unsigned long long *__restrict__ minr2,
+                                          double r2_keep)
 {
     __shared__ double sx[kTile], sy[kTile], sb[kTile];
+    __shared__ int sid[kTile];
     const int l = (int)threadIdx.x;
-    const int i = bk * kTile + l;
-    const double xi = i < n ? x[i] : 0.0;
-    const double yi = i < n ? y[i] : 0.0;
-    const double bI = i < n ? b[i] : 0.0;
+    const int p = bk * kTile + l;
+    const int io = Perm ? (p < n ? perm[p] : 0) : p;
+    const double xi = p < n ? x[io] : 0.0;
+    const double yi = p < n ? y[io] : 0.0;
+    const double bI = p < n ? b[io] : 0.0;
     sx[l] = xi;
     sy[l] = yi;
     sb[l] = bI;
+    sid[l] = io;
     __syncthreads();
     double row = 0.0, col = 0.0;
     for (int s = 1; s <= kTile / 2; ++s) {
         const int m = (l + s) & 63;
         const int j = bk * kTile + m;
-        const bool lower = m > l;
-        const bool valid = (i < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
-        const double ex = lower ? xi - sx[m] : sx[m] - xi;
-        const double ey = lower ? yi - sy[m] : sy[m] - yi;
-        const double dx = sdfield::normalize(ex);
-        const double dy = sdfield::normalize(ey);
-        const double t = bI * sb[m] * field_tau<Mode>(dx, dy);
-        const double r2 = dx * dx + dy * dy;
+        const bool valid = (p < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
+        const int jm = Perm ? sid[m] : j;
+        double r2;
+        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], m > l, r2);
         if (valid) {
-            const double tl = lower ? t : -t;
-            row += tl;
-            col -= tl;
+            row += t;
+            col -= t;
             if (r2 < r2_keep) {
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
-                atomicMin(&minr2[i], bits);
-                atomicMin(&minr2[j], bits);
+                atomicMin(&minr2[io], bits);
+                atomicMin(&minr2[jm], bits);
             }
         }
         col = rotate_down1(col, l);
     }
     // after sweeps 1..32 the column sum of source m sits in lane (m - 33) mod 64
     const double cs = __shfl(col, (l + 31) & 63, 64);
-    if (i < n) partial[(size_t)bk * n + i] = row + cs;
+    if (p < n) partial[(size_t)bk * n + io] = row + cs;
 }
 
 // r^2 below which a pair can change a tolerance: 400 prec^2 with a margin

@@ -2,21 +2,26 @@
 // simulation.cpp:86-131) and the tolerance update folded into it
 // (PrecisionHandler::updateTolerance, precision_handler.cpp:72-87).
 //
-// Layout: x, y, b are SoA FP64 arrays of N.  The N(N-1)/2 unordered pairs
-// are cut into 64x64 tiles (bi <= bj) of the upper triangle; one wavefront
-// owns one tile.  Lane l holds target i = 64*bi + l in registers and walks
-// the 64 sources of block bj in a rotated order (lane l meets source
-// (l + s) mod 64 at step s), so every lane computes a distinct pair each step
-// and each pair term t = b_i b_j tau(x_i - x_j, y_i - y_j) is evaluated once:
-//   - the row sum for i (sum over j of +t) stays in a register;
-//   - the column sum for j (sum over i of -t) travels with the source: the
-//     accumulator rotates one lane per step (ds_bpermute), and after 64 steps
-//     lane l holds the column sum of source 64*bj + l.
+// Layout: x, y, b are SoA FP64 arrays of N.  The dislocations are first
+// ordered by x bin (perm, a stable radix sort of the bin of the wrapped x):
+// tile positions are positions in that order, so most tiles hold only pairs
+// far apart in x.  The N(N-1)/2 unordered pairs are cut into 64x64 tiles
+// (bi <= bj) of the upper triangle; one wavefront owns one tile.  Lane l
+// holds target p = 64*bi + l in registers and walks the 64 sources of block
+// bj in a rotated order (lane l meets source (l + s) mod 64 at step s), so
+// every lane computes a distinct pair each step and each pair term
+// t = b_i b_j tau(x_i - x_j, y_i - y_j) (i < j in the original numbering) is
+// evaluated once:
+//   - the row sum of the target stays in a register;
+//   - the column sum of the source travels with it: the accumulator rotates
+//     one lane per step (ds_bpermute), and after 64 steps lane l holds the
+//     column sum of source 64*bj + l.
 // Each tile writes its row sums into slot bj and its column sums into slot bi
-// of a [nb][N] partial buffer (the diagonal tile writes row + column into
-// slot bi), so every (slot, target) is written exactly once; the final
-// kernel adds the nb slots of a target in slot order.  The result is
-// deterministic (no atomics on sums) and independent of scheduling.
+// of a [nb][N] partial buffer at the original indices (a diagonal tile writes
+// row + column into slot bi), so every (slot, dislocation) is written exactly
+// once; the final kernel adds the nb slots of a dislocation in slot order.
+// The result is deterministic (no atomics on sums, an order that depends on
+// the positions only) and independent of scheduling.
 //
 // The minimum r^2 per target (for the tolerance) is exact under any order
 // and is combined with 64-bit atomic min on the bit pattern (r^2 >= 0).
@@ -26,6 +31,7 @@
 // +t(i, j) for j > i ascending.  It evaluates every pair twice and serves
 // validation: with exact pair terms it reproduces the reference's speeds.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include "context.h"
 #include "field.h"
@@ -42,25 +48,41 @@ using pairtile::kTile;
 // Work units of equal cost: the nb (nb - 1) / 2 off-diagonal tiles, then
 // the diagonal tiles two to a unit (each in half the sweeps), so that the
 // units divide evenly over the resident waves.
+// Tile positions follow perm (dislocations ordered by x bin).
 template <int Mode>
 __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, const double *__restrict__ x,
                                                    const double *__restrict__ y, const double *__restrict__ b,
-                                                   double *__restrict__ partial, unsigned long long *__restrict__ minr2,
-                                                   double r2_keep)
+                                                   const int *__restrict__ perm, double *__restrict__ partial,
+                                                   unsigned long long *__restrict__ minr2, double r2_keep)
 {
     const int u = (int)blockIdx.x;
     if (u < nfull) {
         int bi, bj;
         pairtile::strict_coords(u, bi, bj);
-        pairtile::tile<Mode>(bi, bj, n, x, y, b, partial, minr2, r2_keep);
+        pairtile::tile<Mode, true>(bi, bj, n, x, y, b, perm, partial, minr2, r2_keep);
         return;
     }
     const int bk = 2 * (u - nfull);
-    pairtile::diag_tile<Mode>(bk, n, x, y, b, partial, minr2, r2_keep);
+    pairtile::diag_tile<Mode, true>(bk, n, x, y, b, perm, partial, minr2, r2_keep);
     if (bk + 1 < nb) {
         __syncthreads();
-        pairtile::diag_tile<Mode>(bk + 1, n, x, y, b, partial, minr2, r2_keep);
+        pairtile::diag_tile<Mode, true>(bk + 1, n, x, y, b, perm, partial, minr2, r2_keep);
     }
+}
+
+// Sort keys of the pair-tile order: the bin of the wrapped x among nb equal
+// bins, and the index (a stable radix sort keeps equal bins in index order,
+// so the order is a function of the positions alone).
+__global__ void k_xbin(int n, int nbins, const double *__restrict__ x, int *__restrict__ key, int *__restrict__ idx)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const double xi = x[i];
+    const double w = xi - floor(xi + 0.5);
+    int k = (int)((w + 0.5) * (double)nbins);
+    k = k < 0 ? 0 : (k >= nbins ? nbins - 1 : k);
+    key[i] = k;
+    idx[i] = i;
 }
 
 // simulation.cpp:111-128: point-defect force on dislocation i and its
@@ -165,6 +187,38 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long val, int n)
 
 }  // namespace
 
+static int key_bits(int nbins)
+{
+    int bits = 1;
+    while ((1 << bits) < nbins) ++bits;
+    return bits;
+}
+
+int pair_order_alloc(Context &c)
+{
+    const int n = c.n;
+    for (int **p : {&c.perm, &c.xkey, &c.xkey_sorted, &c.xidx})
+        SD_HIP(hipMalloc((void **)p, sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    size_t bytes = 0;
+    SD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,
+                                              key_bits(c.nb), c.stream));
+    c.sort_tmp_bytes = bytes > 0 ? bytes : 1;
+    SD_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes));
+    return SDDDST_OK;
+}
+
+// perm = the dislocations ordered by x bin (index order within a bin).
+static int pair_order(Context &c, const double *x)
+{
+    const int n = c.n;
+    hipLaunchKernelGGL(k_xbin, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.nb, x, c.xkey, c.xidx);
+    SD_HIP(hipGetLastError());
+    size_t bytes = c.sort_tmp_bytes;
+    SD_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,
+                                              key_bits(c.nb), c.stream));
+    return SDDDST_OK;
+}
+
 int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol)
 {
     const int n = c.n;
@@ -187,13 +241,16 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     const long nfull = (long)nb * (nb - 1) / 2;
     const long units = nfull + (nb + 1) / 2;
     SD_CHECK(units < (1L << 31), SDDDST_ERR_ARG, "too many tiles for N=%d", n);
+    if ((rc = pair_order(c, c.x[slot]))) return rc;
     if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
-                           (int)nfull, c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
+                           (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
+                           pairtile::tolerance_r2(minPrecSqr));
     else
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
-                           (int)nfull, c.x[slot], c.y, c.b, c.partial, c.minr2, pairtile::tolerance_r2(minPrecSqr));
+                           (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
+                           pairtile::tolerance_r2(minPrecSqr));
     SD_HIP(hipGetLastError());
     if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.partial, c.x[slot],

@@ -23,7 +23,8 @@
 // its wavefronts skip that code).  Every pair term is still evaluated lower
 // original index first -- the difference is negated when the source has the
 // lower index, and the term then enters the row with a minus sign and the
-// column with a plus sign -- and the sums land at original indices.
+// column with a plus sign.  The partial sums stay at sorted positions
+// (coalesced stores); the finishing pass maps them back through perm.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -119,10 +120,10 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
         col = rotate_down1(col, l);
     }
     if (diag) {
-        if (p < n) partial[(size_t)bi * n + io] = row + col;
+        if (p < n) partial[(size_t)bi * n + p] = row + col;
     } else {
-        if (p < n) partial[(size_t)bj * n + io] = row;
-        if (q < n) partial[(size_t)bi * n + jo] = col;
+        if (p < n) partial[(size_t)bj * n + p] = row;
+        if (q < n) partial[(size_t)bi * n + q] = col;
     }
 }
 
@@ -183,7 +184,7 @@ __device__ __forceinline__ void diag_tile(int bk, int n, const double *__restric
     }
     // after sweeps 1..32 the column sum of source m sits in lane (m - 33) mod 64
     const double cs = __shfl(col, (l + 31) & 63, 64);
-    if (p < n) partial[(size_t)bk * n + io] = row + cs;
+    if (p < n) partial[(size_t)bk * n + p] = row + cs;
 }
 
 // r^2 below which a pair can change a tolerance: 400 prec^2 with a margin

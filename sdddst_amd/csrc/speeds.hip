@@ -17,9 +17,10 @@
 //     one lane per step (ds_bpermute), and after 64 steps lane l holds the
 //     column sum of source 64*bj + l.
 // Each tile writes its row sums into slot bj and its column sums into slot bi
-// of a [nb][N] partial buffer at the original indices (a diagonal tile writes
-// row + column into slot bi), so every (slot, dislocation) is written exactly
-// once; the final kernel adds the nb slots of a dislocation in slot order.
+// of a [nb][N] partial buffer at the sorted positions (a diagonal tile writes
+// row + column into slot bi), so every (slot, position) is written exactly
+// once; the final kernel adds the nb slots of a position in slot order and
+// stores the speed of dislocation perm[position].
 // The result is deterministic (no atomics on sums, an order that depends on
 // the positions only) and independent of scheduling.
 //
@@ -106,25 +107,28 @@ __device__ __forceinline__ double pd_pair(double xi, double yi, double bI, doubl
 // precision_handler.cpp:72-87 applied to the minimum candidate.
 using stepf::tol_update;
 
-__global__ void k_speeds_finish(int n, int nb, int pc, const double *__restrict__ partial,
-                                const double *__restrict__ x, const double *__restrict__ y,
-                                const double *__restrict__ b, const double *__restrict__ px,
-                                const double *__restrict__ py, double A, double K, double sigma,
-                                const unsigned long long *__restrict__ minr2, double *__restrict__ tol,
-                                double minPrecSqr, int update_tol, double *__restrict__ v)
+__global__ void k_speeds_finish(int n, int nb, int pc, const int *__restrict__ perm,
+                                const double *__restrict__ partial, const double *__restrict__ x,
+                                const double *__restrict__ y, const double *__restrict__ b,
+                                const double *__restrict__ px, const double *__restrict__ py, double A, double K,
+                                double sigma, const unsigned long long *__restrict__ minr2,
+                                double *__restrict__ tol, double minPrecSqr, int update_tol,
+                                double *__restrict__ v)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
+    // partial holds sorted positions; perm maps position q to dislocation i
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= n) return;
+    const int i = perm[q];
     double acc = 0.0;
     int s = 0;
     for (; s + 8 <= nb; s += 8) {
         double p[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) p[u] = partial[(size_t)(s + u) * n + i];
+        for (int u = 0; u < 8; ++u) p[u] = partial[(size_t)(s + u) * n + q];
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc += p[u];
     }
-    for (; s < nb; ++s) acc += partial[(size_t)s * n + i];
+    for (; s < nb; ++s) acc += partial[(size_t)s * n + q];
     double t = update_tol ? tol[i] : 0.0;
     if (update_tol) {
         const unsigned long long bits = minr2[i];
@@ -253,7 +257,8 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
                            pairtile::tolerance_r2(minPrecSqr));
     SD_HIP(hipGetLastError());
     if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
-    hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.partial, c.x[slot],
+    hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.perm, c.partial,
+                       c.x[slot],
                        c.y, c.b, c.px, c.py, c.p.A, c.p.KASQR, sigma, c.minr2, c.tol, minPrecSqr, update_tol,
                        c.v[out_buf]);
     SD_HIP(hipGetLastError());

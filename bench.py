@@ -101,6 +101,40 @@ def traffic_record():
         return None, None
 
 
+def valu_record():
+    """Executed instructions per pair of the pair kernel from the committed
+    counter profile (SQ_INSTS_VALU and the FP64 ADD/MUL/FMA/TRANS counts of
+    one rocprofv3 run over the benchmark workload)."""
+    path = os.path.join(HERE, "profiles", "pair_kernel_n16384_valu.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def execution_fields(pairs, avg_ms):
+    """Hardware use of the pair kernel beside the cost-model roofline: the
+    fast field needs fewer FP64 operations per pair than the reference's
+    expression tree that the 745-flop cost model counts, so frac can pass 1;
+    these fields give the FP64 flops the kernel executes (FMA = 2) against
+    the FP64 peak, and its VALU instructions against the VALU issue rate
+    (one 64-lane instruction per CU per clock = FP64 peak / 2 lane-ops/s)."""
+    rec = valu_record()
+    out = {"cost_model": "745 FP64 flops per pair: 165 add/sub/mul + 10 div x 15 + 10 cosh x 40 + 1 cos x 30"}
+    if not rec:
+        return out
+    t = avg_ms * 1e-3
+    flops = rec["fp64_flops_per_pair"]
+    insts = rec["valu_insts_per_pair"]
+    out.update({"executed_fp64_flops_per_pair": flops,
+                "fp64_pipe_frac": flops * pairs / t / (FP64_PEAK_TFLOPS * 1e12),
+                "valu_insts_per_pair": insts,
+                "valu_issue_frac": insts * pairs / t / (FP64_PEAK_TFLOPS * 1e12 / 2),
+                "execution_source": rec.get("source")})
+    return out
+
+
 def ensemble_run(steps):
     """The 256-realization batched ensemble (N log-uniform in [64, 2048],
     cutoff multiplier 1, precision 1e-6, zero stress), autotuned, one
@@ -359,6 +393,7 @@ def main():
     }
     if traffic_rec:
         line["roofline"]["traffic_source"] = traffic_rec.get("source")
+    line["roofline"].update(execution_fields(pairs, avg_pair_ms))
     if args.compare_steps > 0 and args.solver == "fgmres" and args.precond != "pair":
         log("pair-block preconditioner (the method without the network)")
         sim3 = make_sim(args, "fgmres", x, y, b, precond="pair")

@@ -12,9 +12,9 @@
 // minimum-image offset (the only one that can suffer cancellation) is
 // evaluated exactly as the reference does; the nine far images take their
 // cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors, and
-// their nine denominators (cosh - cos)^2 are inverted together (prefix
-// products, one division, back substitution: the products stay within
-// [1e2, 1e160]).  Far-image terms then differ from the reference by a few
+// their nine fractions are summed as one (numerator and product of the
+// denominators (cosh - cos)^2, which stays within [1e2, 1e160]; one
+// division).  Far-image terms then differ from the reference by a few
 // ulps of terms that are at least e^-pi smaller than the near term.
 #pragma once
 
@@ -193,17 +193,14 @@ SDLIBM_FN double xy_fast(double dx, double dy)
         num[kFar - 3] = (ad * (ad - (double)(kImages + 1))) * __builtin_fma(B, cy, -1.0);
         den[kFar - 3] = dB * dB;
     }
-    // one division for the nine reciprocals
-    double pre[kFar];
-    pre[0] = den[0];
-    for (int m = 1; m < kFar; ++m) pre[m] = pre[m - 1] * den[m];
-    double inv = 1.0 / pre[kFar - 1];
-    double far = 0.0;
-    for (int m = kFar - 1; m > 0; --m) {
-        far = __builtin_fma(num[m], inv * pre[m - 1], far);
-        inv *= den[m];
+    // the nine fractions as one: P / Q with Q the product of the
+    // denominators (within [1e2, 1e160]), one division
+    double P = num[0], Q = den[0];
+    for (int m = 1; m < kFar; ++m) {
+        P = __builtin_fma(num[m], Q, P * den[m]);
+        Q *= den[m];
     }
-    far = __builtin_fma(num[0], inv, far);
+    double far = P / Q;
     far = dx < 0 ? -far : far;
     return near + far * (2.0 * kPiSq);
 }

@@ -316,11 +316,14 @@ SDLIBM_FN double cos(double x)
     const double s = r2 ? y + fmad(y * xx, P, da) : fmad(y * xx, P, y);
     const double q = xx * fmad(xx, fmad(xx, cs6, cs4), cs2);
     const double c = r2 ? fmad(y, da, q) : q;
-    const double sn = sdlibm_sincos_tab[k], ssn = sdlibm_sincos_tab[k + 1];
-    const double cs = sdlibm_sincos_tab[k + 2], ccs = sdlibm_sincos_tab[k + 3];
-    const double A = r2 ? sn : cs, Alo = r2 ? ssn : ccs;
-    const double B = r2 ? cs : -sn, Blo = r2 ? ccs : -ssn;
-    const double res = A + fmad(B, s, fmad(-A, c, fmad(s, Blo, Alo)));
+    // do_sin combines (sn, ssn) with (cs, ccs), do_cos (cs, ccs) with
+    // (-sn, -ssn): pick the table pairs by offset and carry do_cos's minus
+    // sign on s (B * s = (-B') * s = B' * (-s) exactly)
+    const int ka = k + (r2 ? 0 : 2), kb = k + (r2 ? 2 : 0);
+    const double A = sdlibm_sincos_tab[ka], Alo = sdlibm_sincos_tab[ka + 1];
+    const double B = sdlibm_sincos_tab[kb], Blo = sdlibm_sincos_tab[kb + 1];
+    const double sb = r2 ? s : -s;
+    const double res = A + fmad(B, sb, fmad(-A, c, fmad(sb, Blo, Alo)));
     // do_sin's Taylor form for |a| < 0.126 (dx as passed in)
     const double tx = a2 * a2;
     const double tP = fmad(fmad(fmad(fmad(ts5, tx, ts4), tx, ts3), tx, ts2), tx, ts1);

@@ -62,12 +62,18 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
 }
 
 // One pair term for target (xi, yi, bI) of original index io and source
-// slot m, lower original index first; returns the row contribution.
+// slot m, lower original index first; returns the row contribution.  The
+// fast field is odd bit for bit (tau(-d) = -tau(d): cos is even, the near
+// image's expression and the far images' sg(dx) F(|dx|) flip sign exactly),
+// and the wrap satisfies normalize(-d) = -normalize(d) except at |d| = 1/2
+// exactly, so for it the target-minus-source term already equals the
+// lower-first term with its sign; dx = +-0 differs only by the far images'
+// rounding residue (below 1e-17).
 template <int Mode, bool Perm>
 __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int io, int jo, double sxm, double sym,
                                             double sbm, bool lower_by_lane, double &r2)
 {
-    const bool lower = Perm ? io < jo : lower_by_lane;
+    const bool lower = (Perm && Mode != SDDDST_FIELD_FAST) ? io < jo : (Perm ? true : lower_by_lane);
     const double d = xi - sxm, e = yi - sym;
     const double dx = sdfield::normalize(lower ? d : -d);
     const double dy = sdfield::normalize(lower ? e : -e);
@@ -104,7 +110,9 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
         const int m = (l + s) & 63;
         const int j = bj * kTile + m;
         const bool valid = (p < n) && (j < n) && (!diag || l < m);
-        const int jm = Perm ? sid[m] : j;
+        // the source's original index: for the lower-first rule (exact field)
+        // and for the rare tolerance candidates
+        const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
         double r2;
         // without Perm the target has the lower index in every tile it meets
         const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2);
@@ -114,7 +122,7 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
             if (r2 < r2_keep) {
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
                 atomicMin(&minr2[io], bits);
-                atomicMin(&minr2[jm], bits);
+                atomicMin(&minr2[Perm ? sid[m] : j], bits);
             }
         }
         col = rotate_down1(col, l);
@@ -168,7 +176,7 @@ __device__ __forceinline__ void diag_tile(int bk, int n, const double *__restric
         const int m = (l + s) & 63;
         const int j = bk * kTile + m;
         const bool valid = (p < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
-        const int jm = Perm ? sid[m] : j;
+        const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
         double r2;
         const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], m > l, r2);
         if (valid) {
@@ -177,7 +185,7 @@ __device__ __forceinline__ void diag_tile(int bk, int n, const double *__restric
             if (r2 < r2_keep) {
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
                 atomicMin(&minr2[io], bits);
-                atomicMin(&minr2[jm], bits);
+                atomicMin(&minr2[Perm ? sid[m] : j], bits);
             }
         }
         col = rotate_down1(col, l);

@@ -39,6 +39,12 @@ SDLIBM_FN double normalize(double d)
     return r;
 }
 
+// normalize() without its two exact-boundary adjustments, for the fast
+// field: d - rint(d) lies in [-0.5, 0.5] and equals normalize(d) except at
+// d = k + 1/2 with k even (+0.5 instead of -0.5; rint ties to even) and for
+// d = -0 (+0).
+SDLIBM_FN double wrap_fast(double d) { return d - __builtin_rint(d); }
+
 // AnalyticField.cpp:33-38 (taylor branch of f, r^2 <= 1e-6).
 SDLIBM_FN double f_taylor(double dx, double dy)
 {

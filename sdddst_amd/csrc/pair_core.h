@@ -75,15 +75,24 @@ __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int
 {
     const bool lower = (Perm && Mode != SDDDST_FIELD_FAST) ? io < jo : (Perm ? true : lower_by_lane);
     const double d = xi - sxm, e = yi - sym;
-    const double dx = sdfield::normalize(lower ? d : -d);
-    const double dy = sdfield::normalize(lower ? e : -e);
+    double dx, dy;
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        // the wrap as d - rint(d): the reference loop's value except at some
+        // exact half-integers (+0.5 for -0.5) and the sign of a zero
+        dx = sdfield::wrap_fast(lower ? d : -d);
+        dy = sdfield::wrap_fast(lower ? e : -e);
+    } else {
+        dx = sdfield::normalize(lower ? d : -d);
+        dy = sdfield::normalize(lower ? e : -e);
+    }
     const double t = bI * sbm * field_tau<Mode>(dx, dy);
     r2 = dx * dx + dy * dy;
     return lower ? t : -t;
 }
 
-// Called by one whole 64-lane workgroup.
-template <int Mode, bool Perm = false>
+// Called by one whole 64-lane workgroup.  Full: both blocks lie inside n
+// and bi < bj, so every pair of the tile counts.
+template <int Mode, bool Perm = false, bool Full = false>
 __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
                                      const int *__restrict__ perm, double *__restrict__ partial,
@@ -109,7 +118,7 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
     for (int s = 0; s < kTile; ++s) {
         const int m = (l + s) & 63;
         const int j = bj * kTile + m;
-        const bool valid = (p < n) && (j < n) && (!diag || l < m);
+        const bool valid = Full || ((p < n) && (j < n) && (!diag || l < m));
         // the source's original index: for the lower-first rule (exact field)
         // and for the rare tolerance candidates
         const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;

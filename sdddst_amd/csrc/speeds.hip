@@ -60,7 +60,8 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
     if (u < nfull) {
         int bi, bj;
         pairtile::strict_coords(u, bi, bj);
-        pairtile::tile<Mode, true>(bi, bj, n, x, y, b, perm, partial, minr2, r2_keep);
+        if ((bj + 1) * kTile <= n) pairtile::tile<Mode, true, true>(bi, bj, n, x, y, b, perm, partial, minr2, r2_keep);
+        else pairtile::tile<Mode, true, false>(bi, bj, n, x, y, b, perm, partial, minr2, r2_keep);
         return;
     }
     const int bk = 2 * (u - nfull);

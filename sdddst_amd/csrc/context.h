@@ -85,8 +85,17 @@ struct Context {
     int pattern_slot = -1;                      // slot whose positions built the pattern
     bool jac_valid = false;
     bool jac_symmetric = true;
-    bool direct_once = false;                   // this Jacobian goes to the direct solver                  // pattern symmetric (no zero-value compaction)
+    bool direct_once = false;                   // this Jacobian goes to the direct solver
     double jac_h = 0;
+    // fused assembly: unscaled values (Araw, nnz), their column sums after
+    // pass 2 of the reference (sraw, n) at step size raw_h, and the pair
+    // partners of the scaled matrix written by the scaling pass
+    double *Araw = nullptr, *sraw = nullptr;
+    bool raw_valid = false;
+    double raw_h = 0;
+    int *partner = nullptr;
+    double *strength = nullptr;
+    bool partner_valid = false;
     int *d_flags = nullptr;             // device flags: [0] zero-valued upper entries found
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;

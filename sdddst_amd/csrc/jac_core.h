@@ -148,6 +148,7 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
         mult[k] = m;
         const int mk = find_row(Ai, Ap[i], diag[i], j);
         mirror[k] = mk;
+        mirror[mk] = k;    // the upper entry's mirror: this lower entry
         const unsigned char inside = r2 > cp.rc2 ? 0 : 1;
         core[k] = inside;
         core[mk] = inside;

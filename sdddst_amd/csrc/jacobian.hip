@@ -281,6 +281,152 @@ __global__ __launch_bounds__(256) void k_jac_scale(int n, const int *__restrict_
     jac::scale_column(j, Ap, Ai, diag, dvec, Ax, AxRow);
 }
 
+// Fused pass 1 (simulation.cpp:236-374 without the row scaling): one
+// wavefront per column j computes the column's unscaled values in storage
+// order -- upper entries (i, j), i < j, as the copy of the lower entry (j, i)
+// of column i, h * b_j * b_i * tau * taper at that entry (simulation.cpp:
+// 260-268); the diagonal -pd * h; lower entries h * b_i * b_j * tau * taper
+// (simulation.cpp:337-343) -- stores them coalesced, and lane 0 adds them in
+// storage order from LDS (simulation.cpp:351-362), giving s = -sum, the raw
+// diagonal, and d_j (363-373).  An exactly zero lower value is flagged (its
+// upper copy must not exist; the caller then takes the compacting path).
+constexpr int kSumChunk = 512;
+__global__ __launch_bounds__(256) void k_jac_rawsum(int n, const double *__restrict__ b, double h,
+                                                    const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                    const int *__restrict__ diag, const double *__restrict__ tau,
+                                                    const double *__restrict__ mult, const int *__restrict__ mirror,
+                                                    const double *__restrict__ pdtmp, double *__restrict__ Araw,
+                                                    double *__restrict__ sraw, double *__restrict__ dvec,
+                                                    int *__restrict__ flags)
+{
+    __shared__ double buf[4][kSumChunk];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int j = (int)(blockIdx.x * 4 + wave);
+    if (j >= n) return;
+    const int beg = Ap[j], end = Ap[j + 1], d = diag[j];
+    const double bj = b[j];
+    const double pd = pdtmp ? pdtmp[j] : 0.0;
+    double *lb = buf[wave];
+    double subSum = 0;
+    for (int c0 = beg; c0 < end; c0 += kSumChunk) {
+        const int c1 = min(end, c0 + kSumChunk);
+        for (int k = c0 + lane; k < c1; k += 64) {
+            double v;
+            if (k < d) {
+                const int kk = mirror[k];
+                v = h * bj * b[Ai[k]] * tau[kk] * mult[kk];
+            } else if (k == d) {
+                v = -pd * h;
+            } else {
+                v = h * b[Ai[k]] * bj * tau[k] * mult[k];
+                if (v == 0.0) atomicOr(flags, 1);
+            }
+            lb[k - c0] = v;
+            Araw[k] = v;
+        }
+        // the wavefront's LDS stores are visible to its own later loads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0)
+            for (int t = 0; t < c1 - c0; ++t) subSum += lb[t];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (lane != 0) return;
+    subSum *= -1.;
+    Araw[d] = subSum;
+    sraw[j] = subSum;
+    double dv;
+    if (subSum > 0) {
+        double t = 1. / subSum;
+        t += 1.;
+        t *= t;
+        dv = 1. / t;
+    } else {
+        dv = 0.;
+    }
+    dvec[j] = dv;
+}
+
+// d_j for the column sums s * factor (factor a power of two: the unscaled
+// values and their ordered sums scale exactly).
+__global__ void k_jac_dvec(int n, const double *__restrict__ sraw, double factor, double *__restrict__ dvec)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= n) return;
+    const double s = sraw[j] * factor;
+    double dv;
+    if (s > 0) {
+        double t = 1. / s;
+        t += 1.;
+        t *= t;
+        dv = 1. / t;
+    } else {
+        dv = 0.;
+    }
+    dvec[j] = dv;
+}
+
+// Fused pass 2 (simulation.cpp:376-383) for column j with the unscaled values
+// Araw * factor: Ax = raw * (1 + d_i) / 2 and AxRow = raw * (1 + d_j) / 2,
+// +1 on the diagonal; and the pair partner of row j from the same values
+// (the rule of partner_row in precond_core.h: the j' maximising
+// |A_jj' A_j'j| / (|A_jj| |A_j'j'|), ties to the smaller j', with |A_j'j'|
+// formed as the scaling forms it).
+__global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__restrict__ Ap,
+                                                           const int *__restrict__ Ai,
+                                                           const int *__restrict__ diag,
+                                                           const double *__restrict__ dvec,
+                                                           const double *__restrict__ Araw,
+                                                           const double *__restrict__ sraw, double factor,
+                                                           double *__restrict__ Ax, double *__restrict__ AxRow,
+                                                           int *__restrict__ partner, double *__restrict__ strength)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    const double Dj = (1.0 + dvec[j]) * 0.5;
+    const int dpos = diag[j];
+    const double ajj = __builtin_fabs(sraw[j] * factor * Dj + 1.0);
+    double best = -1.0;
+    int bj = -1;
+    for (int k = Ap[j] + lane; k < Ap[j + 1]; k += 64) {
+        const double raw = Araw[k] * factor;
+        const int i = Ai[k];
+        const double Di = (1.0 + dvec[i]) * 0.5;
+        double a = raw * Di;
+        double r = raw * Dj;
+        if (k == dpos) {
+            a += 1.0;
+            r += 1.0;
+        }
+        Ax[k] = a;
+        AxRow[k] = r;
+        if (i != j) {
+            const double aii = __builtin_fabs(sraw[i] * factor * Di + 1.0);
+            const double st = __builtin_fabs(r * a) / (ajj * aii);
+            if (st > best || (st == best && i < bj)) {
+                best = st;
+                bj = i;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int oj = __shfl_down(bj, off, 64);
+        if (ob > best || (ob == best && oj >= 0 && (bj < 0 || oj < bj))) {
+            best = ob;
+            bj = oj;
+        }
+    }
+    if (lane == 0) {
+        partner[j] = bj;
+        strength[j] = best;
+    }
+}
+
 __global__ void k_fill_int(int *p, int v, int n)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -342,15 +488,16 @@ static int ensure_capacity(Context &c, int64_t need)
     if (need <= c.cap) return SDDDST_OK;
     int64_t ncap = need + need / 8 + 1024;
     void **bufs[] = {(void **)&c.Ai,   (void **)&c.Ax,      (void **)&c.AxRow, (void **)&c.jtau,
-                     (void **)&c.jmult, (void **)&c.jmirror, (void **)&c.jcore};
+                     (void **)&c.jmult, (void **)&c.jmirror, (void **)&c.jcore, (void **)&c.Araw};
     for (void **p : bufs) {
         if (*p) SD_HIP(hipFree(*p));
         *p = nullptr;
     }
     c.cap = 0;
-    const size_t sizes[] = {sizeof(int),    sizeof(double), sizeof(double),       sizeof(double),
-                            sizeof(double), sizeof(int),    sizeof(unsigned char)};
-    for (int k = 0; k < 7; ++k) {
+    c.raw_valid = false;
+    const size_t sizes[] = {sizeof(int),    sizeof(double), sizeof(double),        sizeof(double),
+                            sizeof(double), sizeof(int),    sizeof(unsigned char), sizeof(double)};
+    for (int k = 0; k < 8; ++k) {
         if (hipMalloc(bufs[k], sizes[k] * ncap) != hipSuccess) {
             *bufs[k] = nullptr;
             set_error("device allocation of %lld Jacobian entries failed", (long long)ncap);
@@ -433,7 +580,11 @@ static int build_pattern(Context &c, int slot)
 }
 
 // reuse_pattern: the positions are those of the last build (stage II of a
-// step); only the h-dependent passes run.
+// step); only the h-dependent passes run.  With h half the step size of the
+// last unscaled values (stage II after stage I) those values and their
+// column sums are reused times 0.5, which is exact (h enters every unscaled
+// value and every partial sum as a factor, and halving commutes with
+// rounding).
 int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
 {
     const int n = c.n;
@@ -443,25 +594,49 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     if (!reuse_pattern || !c.pattern_valid) {
         rc = build_pattern(c, slot);
         if (rc) return rc;
+        c.raw_valid = false;
     }
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
-    hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag, c.jtau,
-                       c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Ax, c.d_flags);
-    SD_HIP(hipGetLastError());
-    int flag = 0;
-    SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
-    c.jac_symmetric = flag == 0;
-    if (flag) {
-        rc = compact_zero_upper(c);
-        if (rc) return rc;
-        c.pattern_valid = false;   // positions of the kept entries moved
+    c.partner_valid = false;
+    double factor = 1.0;
+    if (reuse_pattern && c.raw_valid && c.jac_symmetric && h == 0.5 * c.raw_h) {
+        factor = 0.5;
+        hipLaunchKernelGGL(k_jac_dvec, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.sraw, factor, c.dvec);
+        SD_HIP(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
+        hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
+                           c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec,
+                           c.d_flags);
+        SD_HIP(hipGetLastError());
+        int flag = 0;
+        SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        SD_HIP(hipStreamSynchronize(c.stream));
+        c.jac_symmetric = flag == 0;
+        c.raw_valid = flag == 0;
+        c.raw_h = h;
+        if (flag) {
+            // an exactly zero value: its upper copy goes (simulation.cpp:263-267)
+            hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
+                               c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Ax, c.d_flags);
+            SD_HIP(hipGetLastError());
+            rc = compact_zero_upper(c);
+            if (rc) return rc;
+            c.pattern_valid = false;   // positions of the kept entries moved
+            hipLaunchKernelGGL(k_jac_diag, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.Ap, c.diag, c.Ax,
+                               c.dvec);
+            hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec,
+                               c.Ax, c.AxRow);
+            SD_HIP(hipGetLastError());
+            if (c.timing && (rc = timer_end(c, c.t_jac))) return rc;
+            c.jac_valid = true;
+            c.jac_h = h;
+            return SDDDST_OK;
+        }
     }
-    hipLaunchKernelGGL(k_jac_diag, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.Ap, c.diag, c.Ax, c.dvec);
+    hipLaunchKernelGGL(k_jac_scale_partner, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec,
+                       c.Araw, c.sraw, factor, c.Ax, c.AxRow, c.partner, c.strength);
     SD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec, c.Ax,
-                       c.AxRow);
-    SD_HIP(hipGetLastError());
+    c.partner_valid = true;
     if (c.timing && (rc = timer_end(c, c.t_jac))) return rc;
     c.jac_valid = true;
     c.jac_h = h;

@@ -240,8 +240,6 @@ int Fgmres::init(Context &c)
     SD_HIP(hipMalloc(&w, sizeof(double) * n));
     SD_HIP(hipMalloc(&dinv, sizeof(double) * n));
     SD_HIP(hipMalloc(&binv, sizeof(double) * 2 * n));
-    SD_HIP(hipMalloc(&strength, sizeof(double) * n));
-    SD_HIP(hipMalloc(&partner, sizeof(int) * n));
     SD_HIP(hipMalloc(&bcol, sizeof(int) * n));
     SD_CHECK(m + 1 <= kMaxV, SDDDST_ERR_ARG, "FGMRES restart %d exceeds %d", m, kMaxV - 1);
     SD_HIP(hipMalloc(&red, sizeof(double) * (size_t)kRedMax * (m + 2)));
@@ -262,10 +260,10 @@ void Fgmres::release()
     if (hcol) (void)hipFree(hcol);
     if (H) (void)hipHostFree(H);
     if (ticket) (void)hipFree(ticket);
-    for (void *p : {(void *)binv, (void *)strength, (void *)partner, (void *)bcol})
+    for (void *p : {(void *)binv, (void *)bcol})
         if (p) (void)hipFree(p);
-    binv = strength = nullptr;
-    partner = bcol = nullptr;
+    binv = nullptr;
+    bcol = nullptr;
     V = Z = w = dinv = red = hcol = H = nullptr;
     ticket = nullptr;
 }
@@ -274,10 +272,15 @@ int Fgmres::pair_factor(Context &c)
 {
     int rc = init(c);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pair_partner, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
-                       c.AxRow, partner, strength);
+    // the fused scaling pass of the assembly leaves the partners of this
+    // matrix in the context; otherwise (compacted pattern) find them here
+    if (!c.partner_valid) {
+        hipLaunchKernelGGL(k_pair_partner, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
+                           c.AxRow, c.partner, c.strength);
+        c.partner_valid = true;
+    }
     hipLaunchKernelGGL(k_pair_blocks, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax, c.AxRow,
-                       partner, strength, 1e-3, binv, bcol);
+                       c.partner, c.strength, 1e-3, binv, bcol);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }

@@ -31,8 +31,7 @@ struct Fgmres {
     double *H = nullptr;     // host (m+1) x m Hessenberg (pinned)
     unsigned *ticket = nullptr;  // last-block counter of the fused reductions
     double *binv = nullptr;      // pair-block inverse rows (2 per row)
-    double *strength = nullptr;
-    int *partner = nullptr, *bcol = nullptr;
+    int *bcol = nullptr;
     int init(Context &c);
     int factor(Context &c);
     int pair_factor(Context &c);                         // pair-block inverse of A

@@ -42,6 +42,29 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap,
     if (lane == 0) y[r] = s;
 }
 
+// The same product with x staged in LDS (systems up to kSpmvLdsMax rows):
+// one persistent 1024-lane workgroup per CU copies x once, then its
+// wavefronts take rows with stride 16 * grid; the gathers of x hit LDS
+// instead of the cache hierarchy.  Each row's sum is formed exactly as in
+// k_spmv.
+constexpr int kSpmvLdsMax = 20480;   // 160 KiB of FP64
+__global__ __launch_bounds__(1024) void k_spmv_lds(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                   const double *__restrict__ Av, const double *__restrict__ x,
+                                                   double *__restrict__ y)
+{
+    extern __shared__ double xs[];
+    for (int i = (int)threadIdx.x; i < n; i += 1024) xs[i] = x[i];
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63);
+    for (int r = (int)(blockIdx.x * 16 + (threadIdx.x >> 6)); r < n; r += (int)gridDim.x * 16) {
+        double s = 0.0;
+        const int e = Ap[r + 1];
+        for (int k = Ap[r] + lane; k < e; k += 64) s += Av[k] * xs[Ai[k]];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) y[r] = s;
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double s)
 {
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -213,6 +236,29 @@ __global__ void k_resid(int n, const double *__restrict__ g, double *__restrict_
 }  // namespace
 
 static inline int nb256(int n) { return (n + 255) / 256; }
+
+// y = A x on the CSR view (AxRow): LDS-staged x when it fits.
+static int launch_spmv(Context &c, const double *x, double *y)
+{
+    const int n = c.n;
+    static int cus = 0;
+    if (n <= kSpmvLdsMax) {
+        if (cus == 0) {
+            int dev = 0, cu = 0;
+            SD_HIP(hipGetDevice(&dev));
+            SD_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+            SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(sizeof(double) * kSpmvLdsMax)));
+            cus = cu > 0 ? cu : 1;
+        }
+        hipLaunchKernelGGL(k_spmv_lds, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n, c.Ap, c.Ai, c.AxRow, x,
+                           y);
+    } else {
+        hipLaunchKernelGGL(k_spmv, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.AxRow, x, y);
+    }
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
 
 static inline int proj_blocks(int n)
 {
@@ -399,7 +445,7 @@ int Fgmres::solve(Context &c)
     for (int cycle = 0; total_it < max_it; ++cycle) {
         // r = g - A delta
         if (cycle > 0) {
-            hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow, c.delta, w);
+            if ((rc = launch_spmv(c, c.delta, w))) return rc;
             hipLaunchKernelGGL(k_resid, dim3(nb256(N)), dim3(256), 0, c.stream, N, c.g, w);
             SD_HIP(hipGetLastError());
             r0 = w;
@@ -418,8 +464,7 @@ int Fgmres::solve(Context &c)
             // z_k = M^-1 v_k ; w = A z_k
             rc = apply(c, V + (size_t)k * N, Z + (size_t)k * N);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow,
-                               Z + (size_t)k * N, w);
+            if ((rc = launch_spmv(c, Z + (size_t)k * N, w))) return rc;
             SD_HIP(hipGetLastError());
             rc = arnoldi_orth(c, *this, k + 1, hh.data(), V + (size_t)(k + 1) * N);
             if (rc) return rc;
@@ -456,7 +501,7 @@ int Fgmres::solve(Context &c)
         SD_HIP(hipStreamSynchronize(c.stream));
     }
     // true residual
-    hipLaunchKernelGGL(k_spmv, dim3((N + 3) / 4), dim3(256), 0, c.stream, N, c.Ap, c.Ai, c.AxRow, c.delta, w);
+    if ((rc = launch_spmv(c, c.delta, w))) return rc;
     hipLaunchKernelGGL(k_resid, dim3(nb256(N)), dim3(256), 0, c.stream, N, c.g, w);
     SD_HIP(hipGetLastError());
     double rn;

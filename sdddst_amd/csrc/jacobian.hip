@@ -170,35 +170,48 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
     if (lane == 0) cnt[j] = c;
 }
 
-// Exclusive scan of n counts into Ap[0..n] by one workgroup of 1024 lanes:
-// per-lane sums of contiguous chunks, a work-efficient LDS scan of the 1024
-// partials, then the per-lane chunk scans.
+// Exclusive scan of n counts into Ap[0..n] (and the total) by one workgroup
+// of 1024 lanes: tiles of 8192 counts go through LDS with coalesced loads
+// and stores; each lane scans 8 consecutive counts, the 1024 lane sums are
+// scanned (Hillis-Steele) and a 64-bit carry joins the tiles.
 __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cnt, int *__restrict__ Ap,
                                                long long *__restrict__ total)
 {
+    constexpr int kPer = 8, kTileN = 1024 * kPer;
+    __shared__ int buf[kTileN];
     __shared__ long long part[1024];
     const int t = (int)threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int s0 = t * per, s1 = min(n, s0 + per);
-    long long acc = 0;
-    for (int k = s0; k < s1; ++k) acc += cnt[k];
-    part[t] = acc;
-    __syncthreads();
-    // inclusive Hillis-Steele scan of the 1024 partials
-    for (int off = 1; off < 1024; off <<= 1) {
-        const long long v = t >= off ? part[t - off] : 0;
+    long long carry = 0;
+    for (int base = 0; base < n; base += kTileN) {
+        const int m = min(kTileN, n - base);
+        for (int q = t; q < kTileN; q += 1024) buf[q] = q < m ? cnt[base + q] : 0;
         __syncthreads();
-        part[t] += v;
+        long long acc = 0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) acc += buf[t * kPer + u];
+        part[t] = acc;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const long long v = t >= off ? part[t - off] : 0;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        long long run = carry + part[t] - acc;   // exclusive prefix of this lane's counts
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int v = buf[t * kPer + u];
+            buf[t * kPer + u] = (int)run;
+            run += v;
+        }
+        __syncthreads();
+        for (int q = t; q < m; q += 1024) Ap[base + q] = buf[q];
+        carry += part[1023];
         __syncthreads();
     }
-    long long run = part[t] - acc;   // exclusive prefix of this lane's chunk
-    for (int k = s0; k < s1; ++k) {
-        Ap[k] = (int)run;
-        run += cnt[k];
-    }
-    if (t == 1023) {
-        *total = part[1023];
-        Ap[n] = (int)part[1023];
+    if (t == 0) {
+        *total = carry;
+        Ap[n] = (int)carry;
     }
 }
 

@@ -231,12 +231,13 @@ __global__ __launch_bounds__(256) void k_features(int n, const int *__restrict__
 // 32x32 matrices are staged transposed in LDS (lane = output channel reads
 // consecutive words).
 
+constexpr int kLdT = 33;   // padded row of a transposed 32x32 matrix in LDS (no bank conflicts)
 __device__ __forceinline__ void stage_t(float *dst, const float *__restrict__ src)
 {
-    // dst[c*32 + r] = src[r*32 + c]
+    // dst[c*kLdT + r] = src[r*32 + c]
     for (int t = (int)threadIdx.x; t < lay::H * lay::H; t += 256) {
         const int r = t >> 5, c = t & 31;
-        dst[c * 32 + r] = src[t];
+        dst[c * kLdT + r] = src[t];
     }
 }
 
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(256) void k_gnn_in(int n, const float *__restrict__
                                                 float *__restrict__ a0, float *__restrict__ h0,
                                                 float *__restrict__ z1)
 {
-    __shared__ float Pt[1024];
+    __shared__ float Pt[32 * kLdT];
     __shared__ float hs[8][32];
     stage_t(Pt, th + lay::layer(1) + lay::LP);
     const int nl = (int)(threadIdx.x >> 5), ch = (int)(threadIdx.x & 31);
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(256) void k_gnn_in(int n, const float *__restrict__
     __syncthreads();
     if (ok) {
         float z = 0.f;
-        for (int c = 0; c < 32; ++c) z += Pt[c * 32 + ch] * hs[nl][c];
+        for (int c = 0; c < 32; ++c) z += Pt[c * kLdT + ch] * hs[nl][c];
         z1[(size_t)i * 32 + ch] = z;
     }
 }
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(256) void k_gnn_mp(int n, int l, const float *__res
                                                 float *__restrict__ ph, float *__restrict__ hout,
                                                 float *__restrict__ znext)
 {
-    __shared__ float Ut[1024], Vt[1024], Pn[1024];
+    __shared__ float Ut[32 * kLdT], Vt[32 * kLdT], Pn[32 * kLdT];
     __shared__ float ss[8][32], hs[8][32];
     const float *L = th + lay::layer(l);
     stage_t(Ut, L + lay::LU);
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gnn_mp(int n, int l, const float *__res
     float h = 0.f;
     if (ok) {
         float p = L[lay::LD + ch];
-        for (int c = 0; c < 32; ++c) p += Ut[c * 32 + ch] * hs[nl][c] + Vt[c * 32 + ch] * ss[nl][c];
+        for (int c = 0; c < 32; ++c) p += Ut[c * kLdT + ch] * hs[nl][c] + Vt[c * kLdT + ch] * ss[nl][c];
         ph[(size_t)i * 32 + ch] = p;
         h = fmaxf(p, 0.f);
         hout[(size_t)i * 32 + ch] = h;
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(256) void k_gnn_mp(int n, int l, const float *__res
     __syncthreads();
     if (ok) {
         float zz = 0.f;
-        for (int c = 0; c < 32; ++c) zz += Pn[c * 32 + ch] * hs[nl][c];
+        for (int c = 0; c < 32; ++c) zz += Pn[c * kLdT + ch] * hs[nl][c];
         znext[(size_t)i * 32 + ch] = zz;
     }
 }

@@ -78,6 +78,7 @@ struct Context {
     unsigned char *jcore = nullptr;             // 1 where the pair lies within the cutoff (untapered)
     double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
+    int *jwstart = nullptr;                     // [n][ceil(n/64)] position of each mask word's first row
     int *cell = nullptr, *csorted = nullptr;    // cell of each dislocation, dislocations by cell
     int *ccount = nullptr, *cstart = nullptr, *ccursor = nullptr;
     int cell_cap = 0;

@@ -90,8 +90,12 @@ __device__ __forceinline__ void count_column(int j, int n, int words, const doub
 
 // Rows of column j in ascending order from its bitmask, starting at base;
 // records the diagonal position.
+// wstart_j (nullable) receives the position in Ai of the first row of each
+// mask word of the column, so that the position of row r is
+// wstart_j[r / 64] + popcount(word bits below r) (see row_position).
 __device__ __forceinline__ void fill_column(int j, int words, const unsigned long long *__restrict__ mj, int base,
-                                            int *__restrict__ Ai, int *__restrict__ diag_j)
+                                            int *__restrict__ Ai, int *__restrict__ diag_j,
+                                            int *__restrict__ wstart_j = nullptr)
 {
     const int lane = (int)(threadIdx.x & 63);
     for (int w0 = 0; w0 < words; w0 += 64) {
@@ -105,6 +109,7 @@ __device__ __forceinline__ void fill_column(int j, int words, const unsigned lon
             if (lane >= off) incl += t;
         }
         int pos = base + incl - pc;
+        if (wstart_j && w < words) wstart_j[w] = pos;
         unsigned long long bits = m;
         while (bits) {
             const int bit = __ffsll((long long)bits) - 1;
@@ -118,6 +123,16 @@ __device__ __forceinline__ void fill_column(int j, int words, const unsigned lon
     }
 }
 
+// Position of row r in column i from the column's mask words and word
+// starts (the same answer as find_row over the column's rows).
+__device__ __forceinline__ int row_position(int words, const unsigned long long *__restrict__ mask,
+                                            const int *__restrict__ wstart, int i, int r)
+{
+    const size_t w = (size_t)i * words + (r >> 6);
+    const unsigned long long below = (1ull << (r & 63)) - 1ull;
+    return wstart[w] + __popcll(mask[w] & below);
+}
+
 // tau_x and the taper of every strictly-lower entry of column j (row i > j),
 // the position of its mirror (row j in column i) and the within-cutoff flag
 // of both positions.  Mode FAST takes the production derivative
@@ -127,7 +142,9 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
                                               const CutParams &cp, const int *__restrict__ Ap,
                                               const int *__restrict__ Ai, const int *__restrict__ diag,
                                               double *__restrict__ tau, double *__restrict__ mult,
-                                              int *__restrict__ mirror, unsigned char *__restrict__ core)
+                                              int *__restrict__ mirror, unsigned char *__restrict__ core,
+                                              int words = 0, const unsigned long long *__restrict__ mask = nullptr,
+                                              const int *__restrict__ wstart = nullptr)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int d = diag[j];
@@ -135,6 +152,9 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
     const double xj = x[j], yj = y[j];
     for (int k = d + 1 + lane; k < end; k += 64) {
         const int i = Ai[k];
+        // mirror: row j in column i (two loads with the word starts, else a
+        // binary search of the column)
+        const int mk = wstart ? row_position(words, mask, wstart, i, j) : find_row(Ai, Ap[i], diag[i], j);
         const double dx = sdfield::normalize(x[i] - xj);
         const double dy = sdfield::normalize(y[i] - yj);
         const double r2 = dx * dx + dy * dy;
@@ -146,7 +166,6 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
         if constexpr (Mode == SDDDST_FIELD_FAST) tau[k] = sdfield::xy_diff_x_fast(dx, dy);
         else tau[k] = sdfield::xy_diff_x(dx, dy);
         mult[k] = m;
-        const int mk = find_row(Ai, Ap[i], diag[i], j);
         mirror[k] = mk;
         mirror[mk] = k;    // the upper entry's mirror: this lower entry
         const unsigned char inside = r2 > cp.rc2 ? 0 : 1;

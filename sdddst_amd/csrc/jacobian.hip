@@ -217,11 +217,11 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cn
 
 __global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsigned long long *__restrict__ mask,
                                                   const int *__restrict__ Ap, int *__restrict__ Ai,
-                                                  int *__restrict__ diag)
+                                                  int *__restrict__ diag, int *__restrict__ wstart)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    jac::fill_column(j, words, mask + (size_t)j * words, Ap[j], Ai, diag + j);
+    jac::fill_column(j, words, mask + (size_t)j * words, Ap[j], Ai, diag + j, wstart + (size_t)j * words);
 }
 
 // tau_x and the taper of every strictly-lower entry (row i > column j) and
@@ -231,11 +231,13 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
                                                     CutParams cp, const int *__restrict__ Ap,
                                                     const int *__restrict__ Ai, const int *__restrict__ diag,
                                                     double *__restrict__ tau, double *__restrict__ mult,
-                                                    int *__restrict__ mirror, unsigned char *__restrict__ core)
+                                                    int *__restrict__ mirror, unsigned char *__restrict__ core,
+                                                    int words, const unsigned long long *__restrict__ mask,
+                                                    const int *__restrict__ wstart)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    jac::values_column<Mode>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core);
+    jac::values_column<Mode>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart);
 }
 
 // simulation.cpp:269-324: the point-defect sum tmp of column j (defects in
@@ -529,12 +531,15 @@ static int build_pattern(Context &c, int slot)
     const double *x = c.x[slot];
     const int wblocks = (n + 3) / 4;
     const int words = (n + 63) / 64;
-    if (!c.jmask) {
-        if (hipMalloc(&c.jmask, sizeof(unsigned long long) * (size_t)n * words) != hipSuccess) {
-            c.jmask = nullptr;
-            set_error("device allocation of the %d x %d membership bitmask failed", n, words);
-            return SDDDST_ERR_NOMEM;
-        }
+    if (!c.jmask && hipMalloc(&c.jmask, sizeof(unsigned long long) * (size_t)n * words) != hipSuccess) {
+        c.jmask = nullptr;
+        set_error("device allocation of the %d x %d membership bitmask failed", n, words);
+        return SDDDST_ERR_NOMEM;
+    }
+    if (!c.jwstart && hipMalloc(&c.jwstart, sizeof(int) * (size_t)n * words) != hipSuccess) {
+        c.jwstart = nullptr;
+        set_error("device allocation of the %d x %d mask word starts failed", n, words);
+        return SDDDST_ERR_NOMEM;
     }
     // cells at least the pre-filter radius wide; 3 x 3 neighbourhoods need G >= 3
     const double rpre = sqrt(cp.prefilter);
@@ -573,14 +578,15 @@ static int build_pattern(Context &c, int slot)
     rc = ensure_capacity(c, total);
     if (rc) return rc;
     c.nnz = total;
-    hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag);
+    hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag,
+                       c.jwstart);
     SD_HIP(hipGetLastError());
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_FAST>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap,
-                           c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore);
+                           c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart);
     else
         hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_EXACT>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp,
-                           c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore);
+                           c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart);
     SD_HIP(hipGetLastError());
     if (c.pc > 0) {
         hipLaunchKernelGGL(k_jac_pd, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.pc, x, c.y, c.b, c.px,

@@ -120,12 +120,14 @@ __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__res
     __syncthreads();
     for (int v = (int)threadIdx.x; v < nvt; v += 256)
         partial[(size_t)blockIdx.x * pstride + v] = ((wsum[0][v] + wsum[1][v]) + wsum[2][v]) + wsum[3][v];
-    __threadfence();
+    // release the partials before the ticket, acquire them in the last
+    // workgroup (device scope)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     __syncthreads();
     if (!last) return;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     for (int v = wave; v < nvt; v += 4) {
         double t = 0.0;
         for (int b = lane; b < (int)gridDim.x; b += 64)

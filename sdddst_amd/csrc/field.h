@@ -11,10 +11,10 @@
 // xy_fast() is the production form of xy(): the image at the
 // minimum-image offset (the only one that can suffer cancellation) is
 // evaluated exactly as the reference does; the nine far images take their
-// cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors, and
-// their nine fractions are summed as one (numerator and product of the
-// denominators (cosh - cos)^2, which stays within [1e2, 1e160]; one
-// division).  Far-image terms then differ from the reference by a few
+// cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors.  The
+// four images within 2.5 of the cell are summed as one fraction (numerator
+// and product of the denominators (cosh - cos)^2; one division); the five
+// beyond it take the two-term expansion in 1/cosh, exact to 1e-19.  Far-image terms then differ from the reference by a few
 // ulps of terms that are at least e^-pi smaller than the near term.
 #pragma once
 
@@ -173,40 +173,38 @@ SDLIBM_FN double xy_fast(double dx, double dy)
     // (t e^{2 pi k} + e^{-2 pi k} / t) / 2 and B_k = cosh(2 pi (k - ad)) =
     // (e^{2 pi k} / t + t e^{-2 pi k}) / 2.
     const double ad = sdlibm::fabsd(dx);
-    constexpr int kFar = 2 * (kImages - 1) + 3;
-    double num[kFar], den[kFar];
-    for (int k = 1; k <= kImages; ++k) {
+    // |u| = ad + k and k - ad for k = 1, 2: the four fractions summed as one,
+    // P / Q with Q the product of their denominators, one division
+    double num[4], den[4];
+    for (int k = 1; k <= 2; ++k) {
         const double hp = 0.5 * sdlibm_exp_2pik[k], hm = 0.5 * sdlibm_exp_m2pik[k];
         const double A = __builtin_fma(t, hp, ti * hm);
         const double B = __builtin_fma(ti, hp, t * hm);
-        const double gA = __builtin_fma(A, cy, -1.0), gB = __builtin_fma(B, cy, -1.0);
         const double dA = A - cy, dB = B - cy;
-        const double kd = (double)k;
-        if (k < kImages) {
-            num[2 * k - 2] = (ad + kd) * gA;
-            num[2 * k - 1] = (ad - kd) * gB;
-        } else {
-            num[kFar - 2] = ((1.0 - ad) * (ad + kd)) * gA;
-            num[kFar - 1] = (ad - kd) * gB;
-        }
-        den[k < kImages ? 2 * k - 2 : kFar - 2] = dA * dA;
-        den[k < kImages ? 2 * k - 1 : kFar - 1] = dB * dB;
+        num[2 * k - 2] = (ad + (double)k) * __builtin_fma(A, cy, -1.0);
+        num[2 * k - 1] = (ad - (double)k) * __builtin_fma(B, cy, -1.0);
+        den[2 * k - 2] = dA * dA;
+        den[2 * k - 1] = dB * dB;
     }
-    {
-        const double hp = 0.5 * sdlibm_exp_2pik[kImages + 1], hm = 0.5 * sdlibm_exp_m2pik[kImages + 1];
-        const double B = __builtin_fma(ti, hp, t * hm);
-        const double dB = B - cy;
-        num[kFar - 3] = (ad * (ad - (double)(kImages + 1))) * __builtin_fma(B, cy, -1.0);
-        den[kFar - 3] = dB * dB;
-    }
-    // the nine fractions as one: P / Q with Q the product of the
-    // denominators (within [1e2, 1e160]), one division
     double P = num[0], Q = den[0];
-    for (int m = 1; m < kFar; ++m) {
+    for (int m = 1; m < 4; ++m) {
         P = __builtin_fma(num[m], Q, P * den[m]);
         Q *= den[m];
     }
-    double far = P / Q;
+    // |u| >= 2.5 (k = 3 both sides, the weighted 4th pair, the 5th): with
+    // a = 1 / cosh(2 pi u) <= 3e-7, (c cy - 1) / (c - cy)^2 =
+    // a cy + a^2 (2 cy^2 - 1) + O(a^3), and a = 2 e^{-2 pi |u|} to within
+    // e^{-4 pi |u|} <= 3e-14 relative: the omitted parts are below 1e-19.
+    const double c2 = __builtin_fma(2.0 * cy, cy, -1.0);
+    const double e3 = 2.0 * sdlibm_exp_m2pik[3], e4 = 2.0 * sdlibm_exp_m2pik[4];
+    const double e5 = 2.0 * sdlibm_exp_m2pik[kImages + 1];
+    const double aA3 = ti * e3, aB3 = t * e3, aA4 = ti * e4, aB4 = t * e4, aB5 = t * e5;
+    double S = ((ad + 3.0) * aA3) * __builtin_fma(c2, aA3, cy);
+    S = __builtin_fma((ad - 3.0) * aB3, __builtin_fma(c2, aB3, cy), S);
+    S = __builtin_fma(((1.0 - ad) * (ad + 4.0)) * aA4, __builtin_fma(c2, aA4, cy), S);
+    S = __builtin_fma((ad - 4.0) * aB4, __builtin_fma(c2, aB4, cy), S);
+    S = __builtin_fma((ad * (ad - 5.0)) * aB5, __builtin_fma(c2, aB5, cy), S);
+    double far = P / Q + S;
     far = dx < 0 ? -far : far;
     return near + far * (2.0 * kPiSq);
 }

@@ -1,15 +1,15 @@
 // Restarted flexible GMRES in FP64 for A delta = g, with A in its CSR view
 // (AxRow on the CSC index arrays: the pattern is symmetric) and a right
-// preconditioner applied per direction (Jacobi by default, a sparse
-// approximate inverse when one is installed).  Acceptance: the true relative
-// residual ||g - A delta|| / ||g|| <= rel_tol, checked with an explicit SpMV
-// at the end of every cycle; non-convergence returns SDDDST_ERR_SOLVER and
-// the dispatcher falls back to the direct solver.
+// preconditioner applied per direction (Jacobi, pair blocks, or the learned
+// sparse approximate inverse).  Acceptance: the true relative residual
+// ||g - A delta|| / ||g|| <= rel_tol, checked with an explicit SpMV at the
+// end of the solve; non-convergence returns SDDDST_ERR_SOLVER and the
+// dispatcher falls back to the direct solver.
 //
-// Arnoldi uses classical Gram-Schmidt applied twice (CGS2): one kernel
-// computes all j+1 projections at once, so each iteration costs one SpMV,
-// two projection passes and one 8*(j+2)-byte device-to-host copy of the
-// Hessenberg column; the Givens rotations run on the host.
+// Arnoldi uses classical Gram-Schmidt applied twice (CGS2), each pass one
+// kernel over all j+1 basis vectors.  An iteration costs the SpMV, two
+// projection launches and one device-to-host copy of the Hessenberg column;
+// the Givens rotations run on the host.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -120,11 +120,13 @@ __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__res
     __syncthreads();
     for (int v = (int)threadIdx.x; v < nvt; v += 256)
         partial[(size_t)blockIdx.x * pstride + v] = ((wsum[0][v] + wsum[1][v]) + wsum[2][v]) + wsum[3][v];
-    // release the partials before the ticket, acquire them in the last
-    // workgroup (device scope)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // the barrier completes the workgroup's stores; one lane releases them
+    // to the device before taking the ticket, the last workgroup acquires
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
     __syncthreads();
     if (!last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -239,20 +241,27 @@ __global__ void k_resid(int n, const double *__restrict__ g, double *__restrict_
 
 static inline int nb256(int n) { return (n + 255) / 256; }
 
+static int spmv_grid()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, cu = 0;
+        SD_HIP(hipGetDevice(&dev));
+        SD_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double) * kSpmvLdsMax)));
+        cus = cu < 1 ? 1 : (cu > kRedMax ? kRedMax : cu);
+    }
+    return cus;
+}
+
 // y = A x on the CSR view (AxRow): LDS-staged x when it fits.
 static int launch_spmv(Context &c, const double *x, double *y)
 {
     const int n = c.n;
-    static int cus = 0;
     if (n <= kSpmvLdsMax) {
-        if (cus == 0) {
-            int dev = 0, cu = 0;
-            SD_HIP(hipGetDevice(&dev));
-            SD_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
-            SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(sizeof(double) * kSpmvLdsMax)));
-            cus = cu > 0 ? cu : 1;
-        }
+        const int cus = spmv_grid();
+        if (cus < 0) return cus;
         hipLaunchKernelGGL(k_spmv_lds, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n, c.Ap, c.Ai, c.AxRow, x,
                            y);
     } else {
@@ -467,7 +476,6 @@ int Fgmres::solve(Context &c)
             rc = apply(c, V + (size_t)k * N, Z + (size_t)k * N);
             if (rc) return rc;
             if ((rc = launch_spmv(c, Z + (size_t)k * N, w))) return rc;
-            SD_HIP(hipGetLastError());
             rc = arnoldi_orth(c, *this, k + 1, hh.data(), V + (size_t)(k + 1) * N);
             if (rc) return rc;
             const double hnext = hh[k + 1];

@@ -58,7 +58,7 @@ static void free_all(Context &c)
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
                     c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
-                    c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart};
+                    c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);

@@ -80,6 +80,7 @@ struct Context {
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
     int *jwstart = nullptr;                     // [n][ceil(n/64)] position of each mask word's first row
     int *cell = nullptr, *csorted = nullptr;    // cell of each dislocation, dislocations by cell
+    double *csx = nullptr, *csy = nullptr;      // their positions in that order
     int *ccount = nullptr, *cstart = nullptr, *ccursor = nullptr;
     int cell_cap = 0;
     bool pattern_valid = false;

@@ -108,11 +108,18 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int cells, const int *__rest
     if (t == 1023) cstart[cells] = part[1023];
 }
 
-__global__ void k_cell_scatter(int n, const int *__restrict__ cell, int *__restrict__ cursor, int *__restrict__ sorted)
+// Dislocations by cell, with copies of their positions in the same order
+// (the order within a cell is arbitrary: the count only sets bits).
+__global__ void k_cell_scatter(int n, const int *__restrict__ cell, const double *__restrict__ x,
+                               const double *__restrict__ y, int *__restrict__ cursor, int *__restrict__ sorted,
+                               double *__restrict__ sx, double *__restrict__ sy)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    sorted[atomicAdd(&cursor[cell[i]], 1)] = i;
+    const int p = atomicAdd(&cursor[cell[i]], 1);
+    sorted[p] = i;
+    sx[p] = x[i];
+    sy[p] = y[i];
 }
 
 // Count of column j from the 3 x 3 neighbouring cells: the same predicate as
@@ -123,7 +130,9 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
                                                          const double *__restrict__ y, CutParams cp, int G,
                                                          const int *__restrict__ cell,
                                                          const int *__restrict__ cstart,
-                                                         const int *__restrict__ sorted, int *__restrict__ cnt,
+                                                         const int *__restrict__ sorted,
+                                                         const double *__restrict__ sx,
+                                                         const double *__restrict__ sy, int *__restrict__ cnt,
                                                          unsigned long long *__restrict__ mask)
 {
     extern __shared__ unsigned long long lm[];
@@ -138,15 +147,19 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
         const int cx = c % G, cy = c / G;
         for (int oy = -1; oy <= 1; ++oy) {
             const int yy = (cy + oy + G) % G;
-            for (int ox = -1; ox <= 1; ++ox) {
+            // the three cells of this row: one contiguous range unless the
+            // row wraps around the cell edge
+            const bool inner = cx >= 1 && cx <= G - 2;
+            for (int ox = inner ? 0 : -1; ox <= (inner ? 0 : 1); ++ox) {
                 const int q = yy * G + (cx + ox + G) % G;
-                for (int k = cstart[q] + lane; k < cstart[q + 1]; k += 64) {
+                const int k0 = inner ? cstart[q - 1] : cstart[q], k1 = inner ? cstart[q + 2] : cstart[q + 1];
+                for (int k = k0 + lane; k < k1; k += 64) {
                     const int i = sorted[k];
+                    const double xi = sx[k], yi = sy[k];
                     bool in;
                     if (i == j) {
                         in = true;
                     } else {
-                        const double xi = x[i], yi = y[i];
                         const double dx = sdfield::normalize(i > j ? xi - xj : xj - xi);
                         const double dy = sdfield::normalize(i > j ? yi - yj : yj - yi);
                         double r2;
@@ -556,13 +569,16 @@ static int build_pattern(Context &c, int slot)
         }
         if (!c.cell) SD_HIP(hipMalloc(&c.cell, sizeof(int) * n));
         if (!c.csorted) SD_HIP(hipMalloc(&c.csorted, sizeof(int) * n));
+        if (!c.csx) SD_HIP(hipMalloc(&c.csx, sizeof(double) * n));
+        if (!c.csy) SD_HIP(hipMalloc(&c.csy, sizeof(double) * n));
         hipLaunchKernelGGL(k_fill_int, dim3((cells + 255) / 256), dim3(256), 0, c.stream, c.ccount, 0, cells);
         hipLaunchKernelGGL(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, x, c.y, G, c.cell, c.ccount);
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c.stream, cells, c.ccount, c.cstart, c.ccursor);
-        hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.cell, c.ccursor,
-                           c.csorted);
+        hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.cell, x, c.y,
+                           c.ccursor, c.csorted, c.csx, c.csy);
         hipLaunchKernelGGL(k_jac_count_cells, dim3(wblocks), dim3(256), sizeof(unsigned long long) * 4 * words,
-                           c.stream, n, words, x, c.y, cp, G, c.cell, c.cstart, c.csorted, c.colcount, c.jmask);
+                           c.stream, n, words, x, c.y, cp, G, c.cell, c.cstart, c.csorted, c.csx, c.csy, c.colcount,
+                           c.jmask);
     } else {
         hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, words, x, c.y, cp, c.colcount,
                            c.jmask);

@@ -13,25 +13,29 @@
 // exactly zero; that case is detected and compacted afterwards.
 //
 // Stages (all on the context stream):
-//   count  one wavefront per column; lanes sweep the rows 64 at a time with
-//          a cheap r^2 pre-filter before the square root; the ballot of each
-//          sweep is the column's membership bitmask word, stored, so the
-//          predicate runs once per ordered pair;
+//   count  one wavefront per column; with cells at least the pre-filter
+//          radius wide, the candidates are the 3 x 3 neighbouring cells'
+//          dislocations (cell-ordered copies of their positions), else all
+//          rows 64 at a time; a cheap r^2 pre-filter precedes the square
+//          root; the column's membership bitmask is stored, so the predicate
+//          runs once per ordered pair;
 //   scan   one workgroup, exclusive prefix sum into Ap (int64 checked);
 //   fill   one wavefront per column, from the bitmask: prefix popcounts give
 //          each row its slot, rows land in ascending order, the diagonal
-//          position is recorded;
+//          position and each mask word's first position are recorded;
 //   values one wavefront per column over its strictly-lower rows: tau_x and
 //          the taper once per unique pair, kept with the position of the
-//          mirror entry (row j of column i, found by binary search);
-//   raw    h*b_i*b_j*tau_x*taper written at the entry and its mirror.  Stage
-//          II of a step assembles at the same positions as stage I with h/2:
-//          it reuses pattern, tau_x and taper and reruns raw, diag, scale;
-//   diag   one lane per column: the column sum in storage order
-//          (simulation.cpp:349-374), d_j;
+//          mirror entry (row j of column i: the word start plus a popcount);
+//   rawsum one wavefront per column: the unscaled values h*b_i*b_j*tau_x*
+//          taper in storage order (upper entries through their mirrors),
+//          stored coalesced, and their ordered sum
+//          (simulation.cpp:349-374) giving the raw diagonal and d_j;
 //   scale  one wavefront per column: A = row-scaled values (+1 on the
-//          diagonal, simulation.cpp:376-383) and the same matrix row-major
-//          (AxRow: row j of A on the index arrays of column j) for SpMV.
+//          diagonal, simulation.cpp:376-383), the same matrix row-major
+//          (AxRow: row j of A on the index arrays of column j) for SpMV, and
+//          the pair partner of row j.  Stage II of a step assembles at the
+//          same positions as stage I with h/2: its unscaled values and sums
+//          are stage I's times 0.5 exactly, so only d_j and scale rerun.
 //
 // Algorithmic HBM traffic of one assembly: 12*nnz + 40*N + 4 bytes.
 #include <hip/hip_runtime.h>

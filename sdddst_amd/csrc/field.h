@@ -209,61 +209,91 @@ SDLIBM_FN double xy_fast(double dx, double dy)
     return near + far * (2.0 * kPiSq);
 }
 
-// Production form of xy_diff_x(): the near image is the reference's f_dx;
-// the nine far images (offsets +-1..+-3, 5s, -4s, +4s with the weights of
-// AnalyticField.cpp:162-181) take cosh and sinh from one shared exp(2 pi dx)
-// and fixed exp(+-2 pi k), and their denominators cosh - cos are inverted
-// together (products within [1e1, 1e70]).
+// Production form of xy_diff_x() (AnalyticField.cpp:162-181).  tau_x is
+// even in (dx, dy), so with ad = |dx| the sum is
+//   near + sum_{k=1,2,3} [Fd(ad + k) + Fd(k - ad)] + (1 - ad) Fd(ad + 4)
+//        - F(ad + 4) + Fd(4 - ad) + ad Fd(5 - ad) - F(5 - ad)
+// with F(u) = u (c cy - 1) / d^2, Fd(u) = (c cy - 1) / d^2 +
+// u sh (2 pi cy / d^2 - 4 pi (c cy - 1) / d^3), c = cosh(2 pi u),
+// sh = sinh(2 pi u), d = c - cy (times 2 pi^2).  The near image is the
+// reference's f_dx with cosh and sinh formed exactly as glibc forms them
+// (one expm1, and one exp where cosh needs it).  All images take cosh and
+// sinh from t = e^{2 pi ad}; the
+// four within 2.5 cells are summed as one fraction (one division), the five
+// beyond by their expansions in a = 1 / cosh (a <= 3e-7, remainder below
+// 1e-17 of the term):
+//   Fd = a (cy + a c2) - 2 pi u a (cy + 2 a c2),  F = u a (cy + a c2),
+// c2 = 2 cy^2 - 1, a = 2 e^{-2 pi u}.
 SDLIBM_FN double xy_diff_x_fast(double dx, double dy)
 {
     const double cy = sdlibm::cos(kPi * 2.0 * dy);
-    const double near = f_dx(dx, cy, dy);
-    const double a0 = dx * 2.0 * kPi;
-    const double t = sdlibm::exp(sdlibm::fabsd(a0));
+    const double a0 = kPi * 2.0 * dx;
+    const double ax = sdlibm::fabsd(a0);
+    // glibc's sinh from expm1(|x|) (|x| < 22), its cosh from expm1 below
+    // 0.5 ln 2 and from exp above, both exactly as glibc forms them
+    const uint32_t hx = sdlibm::hiword(ax);
+    const double m = sdlibm::expm1(ax);
+    const double sh0 = hx < 0x3e300000u ? ax
+                       : hx < 0x3ff00000u ? 0.5 * (2.0 * m - m * m / (m + 1.0))
+                                          : 0.5 * (m + m / (m + 1.0));
+    double c0, t;
+    if (hx < 0x3fd62e43u) {
+        const double w = 1.0 + m;
+        c0 = 1.0 + (m * m) / (w + w);
+        t = w;
+    } else {
+        t = sdlibm::exp(ax);
+        c0 = 0.5 * t + 0.5 * (1.0 / t);
+    }
     const double ti = 1.0 / t;
-    const double ep = a0 < 0 ? ti : t, em = a0 < 0 ? t : ti;
-    const bool neg = dx < 0;
-    const double s = neg ? 1.0 : -1.0;
-    constexpr int kFar = 9;
-    int off[kFar];
-    double wd[kFar], wf[kFar];
-    for (int k = 1; k < kImages; ++k) {
-        off[2 * k - 2] = -k;
-        off[2 * k - 1] = k;
-        wd[2 * k - 2] = wd[2 * k - 1] = 1.0;
-        wf[2 * k - 2] = wf[2 * k - 1] = 0.0;
+    double near;
+    if (dx * dx + dy * dy > 1e-6) {
+        const double sh = a0 < 0 ? -sh0 : sh0;
+        const double d = c0 - cy;
+        const double d2 = d * d;
+        const double d3 = sdlibm::powi(d, 3);
+        near = ((c0 * cy - 1.0) / d2 + dx * (sh * cy * 2.0 * kPi / d2 - (c0 * cy - 1.0) / d3 * 4.0 * kPi * sh)) *
+               kPi * kPi * 2.0;
+    } else {
+        near = f_dx_taylor(dx, dy);
     }
-    off[6] = neg ? 5 : -5;          // a = dx + 5s: -s (f + dx f_dx)
-    wd[6] = -s * dx;
-    wf[6] = -s;
-    off[7] = neg ? -4 : 4;          // dx - 4s: (1 + s dx) f_dx + s f
-    wd[7] = 1.0 + s * dx;
-    wf[7] = s;
-    off[8] = neg ? 4 : -4;          // dx + 4s: f_dx
-    wd[8] = 1.0;
-    wf[8] = 0.0;
-    double x[kFar], ch[kFar], sh[kFar], d[kFar], pre[kFar];
-    for (int m = 0; m < kFar; ++m) {
-        const int k = off[m];
-        const int ak = k < 0 ? -k : k;
-        const double up = k > 0 ? sdlibm_exp_2pik[ak] : sdlibm_exp_m2pik[ak];
-        const double dn = k > 0 ? sdlibm_exp_m2pik[ak] : sdlibm_exp_2pik[ak];
-        x[m] = dx + (double)k;
-        ch[m] = 0.5 * (ep * up + em * dn);
-        sh[m] = 0.5 * (ep * up - em * dn);
-        d[m] = ch[m] - cy;
-        pre[m] = m ? pre[m - 1] * d[m] : d[m];
+    const double ad = sdlibm::fabsd(dx);
+    // u = ad + k and k - ad for k = 1, 2: Fd = (P d + u sh (2 pi cy d - 4 pi P)) / d^3
+    double N = 0.0, D = 1.0;
+    for (int k = 1; k <= 2; ++k) {
+        const double hp = 0.5 * sdlibm_exp_2pik[k], hm = 0.5 * sdlibm_exp_m2pik[k];
+        for (int side = 0; side < 2; ++side) {
+            const double c = side ? __builtin_fma(ti, hp, t * hm) : __builtin_fma(t, hp, ti * hm);
+            const double sh = side ? __builtin_fma(ti, hp, -(t * hm)) : __builtin_fma(t, hp, -(ti * hm));
+            const double u = side ? (double)k - ad : ad + (double)k;
+            const double d = c - cy;
+            const double P = __builtin_fma(c, cy, -1.0);
+            const double num = __builtin_fma(P, d, (u * sh) * ((2.0 * kPi) * cy * d - (4.0 * kPi) * P));
+            const double d3 = d * d * d;
+            N = __builtin_fma(num, D, N * d3);
+            D *= d3;
+        }
     }
-    double inv = 1.0 / pre[kFar - 1];
-    double far = 0.0;
-    for (int m = kFar - 1; m >= 0; --m) {
-        const double r = m ? inv * pre[m - 1] : inv;
-        inv *= d[m];
-        const double P = ch[m] * cy - 1.0;
-        const double r2 = r * r;
-        const double fdx = P * r2 + x[m] * (sh[m] * cy * (2.0 * kPi) * r2 - P * r2 * r * (4.0 * kPi) * sh[m]);
-        far += wd[m] * fdx + wf[m] * (x[m] * P * r2);
+    double far = N / D;
+    // u >= 2.5: expansions in a = 2 e^{-2 pi u}
+    const double c2 = __builtin_fma(2.0 * cy, cy, -1.0);
+    const double twopi = 2.0 * kPi;
+    const double e3 = 2.0 * sdlibm_exp_m2pik[3], e4 = 2.0 * sdlibm_exp_m2pik[4];
+    const double e5 = 2.0 * sdlibm_exp_m2pik[kImages + 1];
+    const double us[5] = {ad + 3.0, 3.0 - ad, ad + 4.0, 4.0 - ad, 5.0 - ad};
+    const double as[5] = {ti * e3, t * e3, ti * e4, t * e4, t * e5};
+    const double wd[5] = {1.0, 1.0, 1.0 - ad, 1.0, ad};     // weights of Fd
+    const double wf[5] = {0.0, 0.0, -1.0, 0.0, -1.0};       // weights of F
+    double S = 0.0;
+    for (int q = 0; q < 5; ++q) {
+        const double a = as[q], u = us[q];
+        const double g1 = a * __builtin_fma(a, c2, cy);
+        const double g2 = a * __builtin_fma(2.0 * a, c2, cy);
+        const double fd = __builtin_fma(-(twopi * u), g2, g1);
+        S = __builtin_fma(wd[q], fd, S);
+        S = __builtin_fma(wf[q] * u, g1, S);
     }
+    far += S;
     return near + far * (2.0 * kPiSq);
 }
 

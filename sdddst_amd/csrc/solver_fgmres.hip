@@ -75,14 +75,17 @@ __device__ __forceinline__ double wave_sum(double s)
 // hsub (nullable) w first becomes w - sum_v hsub[v] V[v] (stored back, the
 // subtraction of k_orth); then out[v] = V[v] . w for v < nv and, with
 // with_self, out[nv] = w . w; acc (nullable) receives
-// (accumulate ? acc : 0) + out[0..nv).  Each workgroup reduces its rows (wave
+// (accumulate ? acc : 0) + out[0..nv); host_out (nullable, mapped pinned
+// memory) receives out at [0, nvt) and acc at [host_acc_off, +nv).  Each
+// workgroup reduces its rows (wave
 // butterflies, the 4 waves in order) into partial[block][0..nv]; the last
 // workgroup to arrive (atomic ticket) adds the partials in a fixed tree over
 // the block index, so the sums do not depend on scheduling.
 __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__restrict__ V, double *__restrict__ w,
                                               const double *__restrict__ hsub, double *__restrict__ partial,
                                               int pstride, unsigned *__restrict__ ticket, double *__restrict__ out,
-                                              double *__restrict__ acc_out, int accumulate, int with_self)
+                                              double *__restrict__ acc_out, int accumulate, int with_self,
+                                              double *__restrict__ host_out, int host_acc_off)
 {
     __shared__ double wsum[4][kMaxV + 1];
     __shared__ bool last;
@@ -137,7 +140,14 @@ __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__res
         t = wave_sum(t);
         if (lane == 0) {
             out[v] = t;
-            if (acc_out && v < nv) acc_out[v] = accumulate ? acc_out[v] + t : t;
+            const double a = (acc_out && v < nv) ? (accumulate ? acc_out[v] + t : t) : 0.0;
+            if (acc_out && v < nv) acc_out[v] = a;
+            // the host's copy, in mapped pinned memory (read after the stream
+            // synchronises)
+            if (host_out) {
+                host_out[v] = t;
+                if (acc_out && v < nv) host_out[host_acc_off + v] = a;
+            }
         }
     }
     if (threadIdx.x == 0) *ticket = 0u;
@@ -220,13 +230,17 @@ __global__ void k_pair_apply(int n, const double *__restrict__ binv, const int *
 }
 
 // x += sum_j y[j] Z[j]
-__global__ void k_combine(int n, int k, const double *__restrict__ Z, const double *__restrict__ yv,
-                          double *__restrict__ x)
+struct Coefs {
+    double v[kMaxV];
+};
+
+// x += sum_j y[j] Z[j], y passed by value
+__global__ void k_combine(int n, int k, const double *__restrict__ Z, Coefs yv, double *__restrict__ x)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     double acc = x[i];
-    for (int j = 0; j < k; ++j) acc += yv[j] * Z[(size_t)j * n + i];
+    for (int j = 0; j < k; ++j) acc += yv.v[j] * Z[(size_t)j * n + i];
     x[i] = acc;
 }
 
@@ -280,10 +294,11 @@ static inline int proj_blocks(int n)
 // One projection pass (k_proj): with hsub, w -= V hsub first; out[v] = V[v].w
 // for v < nv (and out[nv] = w.w with with_self); acc (nullable) += or = out.
 static void launch_proj(Context &c, Fgmres &f, int nv, const double *V, double *w, const double *hsub,
-                        double *out, double *acc = nullptr, int accumulate = 0, int with_self = 0)
+                        double *out, double *acc = nullptr, int accumulate = 0, int with_self = 0,
+                        double *host_out = nullptr, int host_acc_off = 0)
 {
     hipLaunchKernelGGL(k_proj, dim3(proj_blocks(f.n)), dim3(256), 0, c.stream, f.n, nv, V, w, hsub, f.red, f.m + 2,
-                       f.ticket, out, acc, accumulate, with_self);
+                       f.ticket, out, acc, accumulate, with_self, host_out, host_acc_off);
 }
 
 int Fgmres::init(Context &c)
@@ -303,7 +318,10 @@ int Fgmres::init(Context &c)
     SD_HIP(hipMalloc(&ticket, sizeof(unsigned)));
     SD_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), c.stream));
     SD_HIP(hipMalloc(&hcol, sizeof(double) * 3 * (m + 2)));
-    SD_HIP(hipHostMalloc(&H, sizeof(double) * 2 * (m + 2), hipHostMallocDefault));
+    // results the host reads after a synchronisation: written by the
+    // reductions' last workgroup into mapped, coherent pinned memory
+    SD_HIP(hipHostMalloc(&H, sizeof(double) * 3 * (m + 2), hipHostMallocMapped | hipHostMallocCoherent));
+    SD_HIP(hipHostGetDevicePointer((void **)&Hdev, H, 0));
     return SDDDST_OK;
 }
 
@@ -321,7 +339,7 @@ void Fgmres::release()
         if (p) (void)hipFree(p);
     binv = nullptr;
     bcol = nullptr;
-    V = Z = w = dinv = red = hcol = H = nullptr;
+    V = Z = w = dinv = red = hcol = H = Hdev = nullptr;
     ticket = nullptr;
 }
 
@@ -389,12 +407,12 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *ne
     double *h2 = f.hcol + (f.m + 2);          // second pass (+ w'.w')
     double *hsum = f.hcol + 2 * (f.m + 2);    // h1 + h2
     launch_proj(c, f, nv, f.V, f.w, nullptr, h1, hsum, 0, 0);
-    launch_proj(c, f, nv, f.V, f.w, h1, h2, hsum, 1, 1);
+    // the second pass also writes h2, w'.w' (at [0, nv]) and h (at
+    // [m+2, m+2+nv)) into the mapped pinned buffer
+    launch_proj(c, f, nv, f.V, f.w, h1, h2, hsum, 1, 1, f.Hdev, f.m + 2);
     SD_HIP(hipGetLastError());
-    // one copy: h2 and w'.w' at [0, nv], h at [m+2, m+2+nv)
-    double *hc = hhost + (f.m + 2);
-    SD_HIP(hipMemcpyAsync(hc, h2, sizeof(double) * (f.m + 2 + nv), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
+    const double *hc = f.H;
     for (int v = 0; v < nv; ++v) hhost[v] = hc[f.m + 2 + v];
     double s2 = 0.0;
     for (int v = 0; v < nv; ++v) s2 += hc[v] * hc[v];
@@ -409,11 +427,10 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *ne
         return SDDDST_OK;
     }
     hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, h2, f.w);
-    launch_proj(c, f, 0, f.V, f.w, nullptr, hsum + nv, nullptr, 0, 1);
+    launch_proj(c, f, 0, f.V, f.w, nullptr, hsum + nv, nullptr, 0, 1, f.Hdev);
     SD_HIP(hipGetLastError());
-    SD_HIP(hipMemcpyAsync(hhost + nv, hsum + nv, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
-    hhost[nv] = sqrt(hhost[nv]);
+    hhost[nv] = sqrt(f.H[0]);
     if (next && hhost[nv] > 0)
         hipLaunchKernelGGL(k_scale_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, 1.0 / hhost[nv], f.w, next);
     SD_HIP(hipGetLastError());
@@ -422,9 +439,8 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *ne
 
 static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
-    launch_proj(c, f, 0, f.V, const_cast<double *>(v), nullptr, f.hcol, nullptr, 0, 1);
+    launch_proj(c, f, 0, f.V, const_cast<double *>(v), nullptr, f.hcol, nullptr, 0, 1, f.Hdev);
     SD_HIP(hipGetLastError());
-    SD_HIP(hipMemcpyAsync(f.H, f.hcol, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     *out = sqrt(f.H[0]);
     return SDDDST_OK;
@@ -505,10 +521,10 @@ int Fgmres::solve(Context &c)
             for (int j = i + 1; j < k; ++j) acc -= Hm[(size_t)i * m + j] * y[j];
             y[i] = acc / Hm[(size_t)i * m + i];
         }
-        SD_HIP(hipMemcpyAsync(hcol, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(k_combine, dim3(nb256(N)), dim3(256), 0, c.stream, N, k, Z, hcol, c.delta);
+        Coefs yc{};
+        for (int j = 0; j < k; ++j) yc.v[j] = y[j];
+        hipLaunchKernelGGL(k_combine, dim3(nb256(N)), dim3(256), 0, c.stream, N, k, Z, yc, c.delta);
         SD_HIP(hipGetLastError());
-        SD_HIP(hipStreamSynchronize(c.stream));
     }
     // true residual
     if ((rc = launch_spmv(c, c.delta, w))) return rc;

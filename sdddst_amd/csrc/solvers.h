@@ -28,7 +28,8 @@ struct Fgmres {
     double *dinv = nullptr;  // Jacobi: inverse diagonal of A
     double *red = nullptr;   // reduction scratch
     double *hcol = nullptr;  // device copy of one Hessenberg column
-    double *H = nullptr;     // host (m+1) x m Hessenberg (pinned)
+    double *H = nullptr;     // host copy of reduction results (mapped, coherent pinned memory)
+    double *Hdev = nullptr;  // its device address
     unsigned *ticket = nullptr;  // last-block counter of the fused reductions
     double *binv = nullptr;      // pair-block inverse rows (2 per row)
     int *bcol = nullptr;

@@ -182,6 +182,16 @@ __global__ void k_scale_copy(int n, double a, const double *__restrict__ x, doub
     if (i < n) y[i] = a * x[i];
 }
 
+// y = x / sqrt(*nrm2) (0 when the norm is 0), the norm read on the device
+__global__ void k_scale_by_norm(int n, const double *__restrict__ nrm2, const double *__restrict__ x,
+                                double *__restrict__ y)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const double b = sqrt(*nrm2);
+    y[i] = (b > 0 ? 1.0 / b : 0.0) * x[i];
+}
+
 __global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const double *__restrict__ v,
                                double *__restrict__ z)
 {
@@ -317,10 +327,10 @@ int Fgmres::init(Context &c)
     SD_HIP(hipMalloc(&red, sizeof(double) * (size_t)kRedMax * (m + 2)));
     SD_HIP(hipMalloc(&ticket, sizeof(unsigned)));
     SD_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), c.stream));
-    SD_HIP(hipMalloc(&hcol, sizeof(double) * 3 * (m + 2)));
+    SD_HIP(hipMalloc(&hcol, sizeof(double) * (3 * (m + 2) + 2)));
     // results the host reads after a synchronisation: written by the
     // reductions' last workgroup into mapped, coherent pinned memory
-    SD_HIP(hipHostMalloc(&H, sizeof(double) * 3 * (m + 2), hipHostMallocMapped | hipHostMallocCoherent));
+    SD_HIP(hipHostMalloc(&H, sizeof(double) * (3 * (m + 2) + 2), hipHostMallocMapped | hipHostMallocCoherent));
     SD_HIP(hipHostGetDevicePointer((void **)&Hdev, H, 0));
     return SDDDST_OK;
 }
@@ -455,16 +465,16 @@ int Fgmres::solve(Context &c)
     const int precond = c.precond_override >= 0 ? c.precond_override : c.precond;
     const int max_it = precond == SDDDST_PRECOND_LEARNED ? std::min(c.solver_max_iter, m) : c.solver_max_iter;
     int rc;
-    // delta = 0, r = g
+    // delta = 0, r = g; ||g||^2 lands in the device slot gs and its mapped
+    // copy, and v_0 = g / ||g|| is formed on the device, so the host learns
+    // ||g|| at the first iteration's synchronisation
     SD_HIP(hipMemsetAsync(c.delta, 0, sizeof(double) * N, c.stream));
-    double gnorm;
-    rc = norm2(c, *this, c.g, &gnorm);
-    if (rc) return rc;
+    double *gs = hcol + 3 * (m + 2);
+    launch_proj(c, *this, 0, V, const_cast<double *>(c.g), nullptr, gs, nullptr, 0, 1, Hdev + 3 * (m + 2));
+    hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256(N)), dim3(256), 0, c.stream, N, gs, c.g, V);
+    SD_HIP(hipGetLastError());
+    double gnorm = -1.0;
     c.last_iters = 0;
-    if (gnorm == 0.0) {
-        c.last_rel_res = 0.0;
-        return SDDDST_OK;
-    }
     std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), hh(3 * (m + 2) + 1);
     int total_it = 0;
     double rel = 1.0;
@@ -486,8 +496,10 @@ int Fgmres::solve(Context &c)
         }
         if (total_it >= max_it) break;
         checked = false;
-        hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / beta, r0, V);
-        SD_HIP(hipGetLastError());
+        if (cycle > 0) {
+            hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / beta, r0, V);
+            SD_HIP(hipGetLastError());
+        }
         std::fill(s.begin(), s.end(), 0.0);
         s[0] = beta;
         int k = 0;
@@ -498,6 +510,15 @@ int Fgmres::solve(Context &c)
             if ((rc = launch_spmv(c, Z + (size_t)k * N, w))) return rc;
             rc = arnoldi_orth(c, *this, k + 1, hh.data(), V + (size_t)(k + 1) * N);
             if (rc) return rc;
+            if (gnorm < 0) {
+                // first synchronisation of the solve: ||g|| is on the host
+                gnorm = sqrt(H[3 * (m + 2)]);
+                if (gnorm == 0.0) {
+                    c.last_rel_res = 0.0;
+                    return SDDDST_OK;
+                }
+                s[0] = beta = gnorm;
+            }
             const double hnext = hh[k + 1];
             // apply previous rotations to the new column
             for (int i = 0; i < k; ++i) {

@@ -178,6 +178,61 @@ def test_jacobian_fast_field(n, m, seed):
     assert np.max(np.abs(dv - rdv)) < 1e-13
 
 
+def _assemble(c, h, rebuild):
+    """Assemble with the given rebuild mode (1: full, 2: reuse the pattern of
+    the last build) and download the CSC, d and the diagonal positions."""
+    import ctypes
+    from sdddst_amd._capi import _dptr, _iptr
+    nnz = ctypes.c_int64()
+    c._check(c.lib.sdddst_jacobian_csc(c.h, 0, h, rebuild, None, None, None, None, None, ctypes.byref(nnz)))
+    Ap = np.empty(c.n + 1, dtype=np.int32)
+    Ai = np.empty(nnz.value, dtype=np.int32)
+    Ax = np.empty(nnz.value)
+    dv = np.empty(c.n)
+    dg = np.empty(c.n, dtype=np.int32)
+    c._check(c.lib.sdddst_jacobian_csc(c.h, 0, h, 0, _iptr(Ap), _iptr(Ai), _dptr(Ax), _dptr(dv), _iptr(dg),
+                                       ctypes.byref(nnz)))
+    return Ap, Ai, Ax, dv, dg
+
+
+def _pair_solve(c, h):
+    c.solver_select(S.SOLVER_FGMRES, 1e-10, 300, 40)
+    c.set_preconditioner(S.PRECOND_PAIR)
+    c.residual(h, 0, 0, S.V_INIT, S.V_INIT2)
+    c.solver_factor()
+    return c.solver_solve(download=True)
+
+
+@pytest.mark.parametrize("pc", [0, 40])
+def test_stage_two_reuse_is_bit_identical(pc):
+    # stage II reuses stage I's unsigned values times 0.5: the matrix, d and
+    # the pair blocks equal a fresh assembly at h / 2 bit for bit
+    n, m, h = 1024, 2.0, 3e-5
+    x, y, b = configs.dislocations(n, 61)
+    kw = {}
+    if pc:
+        px, py = configs.point_defects(n, pc, 61)
+        kw = dict(pc=pc, A=1e-4 * 16.0 / np.sqrt(n), KASQR=1.65 * 1.65 * 1e6 / 256.0 * n)
+    out = []
+    for reuse in (True, False):
+        with S.Context(n, cutoff_multiplier=m, field_mode=S.FIELD_FAST, **kw) as c:
+            c.upload_state(0, x, y, b)
+            if pc:
+                c.upload_points(px, py)
+            c.speeds(0, S.V_INIT, 0.0)
+            c.speeds(0, S.V_INIT2, 0.0)
+            if reuse:
+                _assemble(c, h, 1)
+                csc = _assemble(c, 0.5 * h, 2)
+            else:
+                csc = _assemble(c, 0.5 * h, 1)
+            out.append((csc, _pair_solve(c, 0.5 * h)))
+    (a, da), (b_, db) = out
+    for u, v in zip(a, b_):
+        assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+    assert np.array_equal(da.view(np.int64), db.view(np.int64))
+
+
 # -- point defects (config-5 style; A and KASQR scaled as the CLI does) --------
 
 def _pd_setup(n, pc, seed):

@@ -233,6 +233,7 @@ int sdddst_upload_state(sdddst_ctx *ctx, int slot, const double *x, const double
     if (y) SD_HIP(hipMemcpyAsync(c.y, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     if (b) SD_HIP(hipMemcpyAsync(c.b, b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
+    c.order_age = 0;   // new positions: the next speeds call orders them afresh
     // The assembled matrix stays valid.  Reusing its pattern (rebuild = 2) is
     // the caller's assertion about x; new y or b rule it out.
     if (y != nullptr || b != nullptr) c.pattern_valid = false;

@@ -66,6 +66,7 @@ struct Context {
     int *perm = nullptr, *xkey = nullptr, *xkey_sorted = nullptr, *xidx = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
+    int order_age = 0;                  // speeds calls since the order was last computed
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
     // Jacobian (CSC)

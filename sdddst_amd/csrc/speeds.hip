@@ -2,8 +2,9 @@
 // simulation.cpp:86-131) and the tolerance update folded into it
 // (PrecisionHandler::updateTolerance, precision_handler.cpp:72-87).
 //
-// Layout: x, y, b are SoA FP64 arrays of N.  The dislocations are first
-// ordered by x bin (perm, a stable radix sort of the bin of the wrapped x):
+// Layout: x, y, b are SoA FP64 arrays of N.  The dislocations are ordered
+// by x bin (perm, a stable radix sort of the bin of the wrapped x, refreshed
+// every 16 calls and after an upload):
 // tile positions are positions in that order, so most tiles hold only pairs
 // far apart in x.  The N(N-1)/2 unordered pairs are cut into 64x64 tiles
 // (bi <= bj) of the upper triangle; one wavefront owns one tile.  Lane l
@@ -212,6 +213,8 @@ int pair_order_alloc(Context &c)
     return SDDDST_OK;
 }
 
+constexpr int kOrderReuse = 16;
+
 // perm = the dislocations ordered by x bin (index order within a bin).
 static int pair_order(Context &c, const double *x)
 {
@@ -246,7 +249,13 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     const long nfull = (long)nb * (nb - 1) / 2;
     const long units = nfull + (nb + 1) / 2;
     SD_CHECK(units < (1L << 31), SDDDST_ERR_ARG, "too many tiles for N=%d", n);
-    if ((rc = pair_order(c, c.x[slot]))) return rc;
+    // the order is refreshed every kOrderReuse calls: positions move far less
+    // than a bin width between them, and any order gives the same terms
+    if (c.order_age == 0 || c.order_age >= kOrderReuse) {
+        if ((rc = pair_order(c, c.x[slot]))) return rc;
+        c.order_age = 0;
+    }
+    ++c.order_age;
     if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,

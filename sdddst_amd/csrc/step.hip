@@ -142,8 +142,9 @@ __global__ __launch_bounds__(256) void k_strain(int n, const double *__restrict_
     if (threadIdx.x == 0) out->strain = r;
 }
 
-// The same sums as a fixed-order tree (one workgroup: contiguous chunks per
-// lane, wave sums, wave partials in order) for the fast field, whose speeds
+// The same sums as a fixed-order tree (one workgroup: lane t takes elements
+// t, t + 1024, ..., then wave sums and wave partials in order) for the fast
+// field, whose speeds
 // differ from the reference's in the last bits anyway.
 __device__ __forceinline__ double tree_finish(double acc, double *red, int t)
 {
@@ -163,10 +164,9 @@ __global__ __launch_bounds__(1024) void k_speed_sums_tree(int n, const double *_
 {
     __shared__ double red[16];
     const int t = (int)threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int s0 = t * per, s1 = min(n, s0 + per);
     double sa = 0.0, sq = 0.0, sbv = 0.0;
-    for (int k = s0; k < s1; ++k) {
+    // lane t takes t, t + 1024, ...: coalesced loads, a fixed order
+    for (int k = t; k < n; k += 1024) {
         const double vi = v[k];
         sa += __builtin_fabs(vi);
         sq += vi * vi;
@@ -188,10 +188,8 @@ __global__ __launch_bounds__(1024) void k_strain_tree(int n, const double *__res
 {
     __shared__ double red[16];
     const int t = (int)threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int s0 = t * per, s1 = min(n, s0 + per);
     double r = 0.0;
-    for (int k = s0; k < s1; ++k) r += b[k] * (xn[k] - xo[k]);
+    for (int k = t; k < n; k += 1024) r += b[k] * (xn[k] - xo[k]);
     r = tree_finish(r, red, t);
     if (t == 0) out->strain = r;
 }

@@ -166,7 +166,8 @@ __global__ __launch_bounds__(64) void k_e_pair_tiles(const int *__restrict__ tl,
     int bi, bj;
     pairtile::tile_coords(tl[2 * blockIdx.x + 1], bi, bj);
     const SysDesc &d = D[s];
-    pairtile::tile<Mode>(bi, bj, d.n, x + d.off, y + d.off, b + d.off, nullptr, partial + d.poff, minr2 + d.off,
+    __shared__ pairtile::TileLds L;
+    pairtile::tile<Mode>(L, bi, bj, d.n, x + d.off, y + d.off, b + d.off, nullptr, partial + d.poff, minr2 + d.off,
                          r2_keep);
 }
 

@@ -141,9 +141,10 @@ SDLIBM_FN double xy_diff_x(double dx, double dy)
 
 
 // Production form of xy() (see header comment).
-SDLIBM_FN double xy_fast(double dx, double dy)
+SDLIBM_FN double xy_fast(double dx, double dy, const double *__restrict__ sctab = sdlibm_sincos_tab,
+                         const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
 {
-    const double cy = sdlibm::cos(kPi * 2.0 * dy);
+    const double cy = sdlibm::cos(kPi * 2.0 * dy, sctab);
     const double a0 = dx * 2.0 * kPi;
     const double ax = sdlibm::fabsd(a0);
     // Near image: the reference's cosh, evaluated once.  On the exp branch
@@ -159,7 +160,7 @@ SDLIBM_FN double xy_fast(double dx, double dy)
         t = w;
         ti = 1.0 / t;
     } else {
-        t = sdlibm::exp(ax);
+        t = sdlibm::exp(ax, e2tab);
         ti = 1.0 / t;
         c0 = 0.5 * t + 0.5 * ti;
     }

@@ -145,7 +145,7 @@ SDLIBM_FN double exp_special(double tmp, uint64_t sbits, uint64_t ki)
 
 // exp(x + xt) for a tail |xt| << ulp(x)-scale: the tail joins the reduced
 // argument.  exp(x) is exp_tail(x, 0.0), bit for bit.
-SDLIBM_FN double exp_tail(double x, double xt)
+SDLIBM_FN double exp_tail(double x, double xt, const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
 {
     const double InvLn2N = 0x1.71547652b82fep0 * 128, NegLn2hiN = -0x1.62e42fefa0000p-8,
                  NegLn2loN = -0x1.cf79abc9e3b3ap-47, Shift = 0x1.8p52;
@@ -167,8 +167,8 @@ SDLIBM_FN double exp_tail(double x, double xt)
     const double r = fmad(kd, NegLn2loN, fmad(kd, NegLn2hiN, x)) + xt;
     const uint64_t idx = 2 * (ki % 128);
     const uint64_t top = ki << 45;
-    const double tail = asf64(sdlibm_exp2_tab[idx]);
-    const uint64_t sbits = sdlibm_exp2_tab[idx + 1] + top;
+    const double tail = asf64(e2tab[idx]);
+    const uint64_t sbits = e2tab[idx + 1] + top;
     const double r2 = r * r;
     const double tmp = fmad(r2 * r2, fmad(r, C5, C4), fmad(r2, fmad(r, C3, C2), tail + r));
     if (abstop == 0) return detail::exp_special(tmp, sbits, ki);
@@ -176,7 +176,8 @@ SDLIBM_FN double exp_tail(double x, double xt)
     return fmad(scale, tmp, scale);
 }
 
-SDLIBM_FN double exp(double x) { return exp_tail(x, 0.0); }
+// e2tab / sctab: the tables, or a copy of them (a kernel may stage them in LDS)
+SDLIBM_FN double exp(double x, const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab) { return exp_tail(x, 0.0, e2tab); }
 
 // pow(M_E, y) with M_E rounded to double: exp(y * ln(M_E)) with
 // ln(M_E) = 1 + c, c = -5.318e-17, i.e. exp of y plus the tail y*c.
@@ -291,7 +292,7 @@ SDLIBM_FN double do_sin(double x, double dx)
 // argument, one table lookup and the do_cos / do_sin combinations selected
 // per lane, each with the operation order of its branch form, so the value
 // is the same.  (Random arguments put all three ranges in most wavefronts.)
-SDLIBM_FN double cos(double x)
+SDLIBM_FN double cos(double x, const double *__restrict__ sctab = sdlibm_sincos_tab)
 {
     using namespace detail;
     const double ax = fabsd(x);
@@ -320,8 +321,8 @@ SDLIBM_FN double cos(double x)
     // (-sn, -ssn): pick the table pairs by offset and carry do_cos's minus
     // sign on s (B * s = (-B') * s = B' * (-s) exactly)
     const int ka = k + (r2 ? 0 : 2), kb = k + (r2 ? 2 : 0);
-    const double A = sdlibm_sincos_tab[ka], Alo = sdlibm_sincos_tab[ka + 1];
-    const double B = sdlibm_sincos_tab[kb], Blo = sdlibm_sincos_tab[kb + 1];
+    const double A = sctab[ka], Alo = sctab[ka + 1];
+    const double B = sctab[kb], Blo = sctab[kb + 1];
     const double sb = r2 ? s : -s;
     const double res = A + fmad(B, sb, fmad(-A, c, fmad(sb, Blo, Alo)));
     // do_sin's Taylor form for |a| < 0.126 (dx as passed in)

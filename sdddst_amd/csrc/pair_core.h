@@ -38,16 +38,25 @@ namespace pairtile {
 constexpr int kTile = 64;
 constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
 
+// The source tile in LDS, declared once by the calling kernel (one copy per
+// workgroup however many tile variants the kernel instantiates).
+struct TileLds {
+    double x[kTile], y[kTile], b[kTile];
+    int id[kTile];
+};
+
 __device__ __forceinline__ double rotate_down1(double v, int lane)
 {
     // lane l receives the value of lane (l + 1) mod 64
     return __shfl(v, (lane + 1) & 63, 64);
 }
 
+// sct / e2t: the cos and exp tables of the fast field, or a copy in LDS
 template <int Mode>
-__device__ __forceinline__ double field_tau(double dx, double dy)
+__device__ __forceinline__ double field_tau(double dx, double dy, const double *__restrict__ sct,
+                                            const uint64_t *__restrict__ e2t)
 {
-    if constexpr (Mode == SDDDST_FIELD_FAST) return sdfield::xy_fast(dx, dy);
+    if constexpr (Mode == SDDDST_FIELD_FAST) return sdfield::xy_fast(dx, dy, sct, e2t);
     else return sdfield::xy(dx, dy);
 }
 
@@ -71,7 +80,8 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
 // rounding residue (below 1e-17).
 template <int Mode, bool Perm>
 __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int io, int jo, double sxm, double sym,
-                                            double sbm, bool lower_by_lane, double &r2)
+                                            double sbm, bool lower_by_lane, double &r2,
+                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t)
 {
     const bool lower = (Perm && Mode != SDDDST_FIELD_FAST) ? io < jo : (Perm ? true : lower_by_lane);
     const double d = xi - sxm, e = yi - sym;
@@ -85,21 +95,25 @@ __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int
         dx = sdfield::normalize(lower ? d : -d);
         dy = sdfield::normalize(lower ? e : -e);
     }
-    const double t = bI * sbm * field_tau<Mode>(dx, dy);
+    const double t = bI * sbm * field_tau<Mode>(dx, dy, sct, e2t);
     r2 = dx * dx + dy * dy;
     return lower ? t : -t;
 }
 
 // Called by one whole 64-lane workgroup.  Full: both blocks lie inside n
-// and bi < bj, so every pair of the tile counts.
+// and bi < bj, so every pair of the tile counts.  sct / e2t (fast field
+// only) may point at LDS copies of the tables written before the call: the
+// tile's barrier orders them before the first read.
 template <int Mode, bool Perm = false, bool Full = false>
-__device__ __forceinline__ void tile(int bi, int bj, int n, const double *__restrict__ x,
+__device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
                                      const int *__restrict__ perm, double *__restrict__ partial,
-                                     unsigned long long *__restrict__ minr2, double r2_keep)
+                                     unsigned long long *__restrict__ minr2, double r2_keep,
+                                     const double *__restrict__ sct = sdlibm_sincos_tab,
+                                     const uint64_t *__restrict__ e2t = sdlibm_exp2_tab)
 {
-    __shared__ double sx[kTile], sy[kTile], sb[kTile];
-    __shared__ int sid[kTile];
+    double *sx = L.x, *sy = L.y, *sb = L.b;
+    int *sid = L.id;
     const int l = (int)threadIdx.x;
     const int p = bi * kTile + l;              // target position
     const int q = bj * kTile + l;              // source position held in slot l
@@ -124,7 +138,7 @@ __device__ __forceinline__ void tile(int bi, int bj, int n, const double *__rest
         const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
         double r2;
         // without Perm the target has the lower index in every tile it meets
-        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2);
+        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2, sct, e2t);
         if (valid) {
             row += t;
             col -= t;
@@ -162,13 +176,15 @@ __device__ __forceinline__ void strict_coords(int t, int &bi, int &bj)
 // difference is negated and the term enters the row with a minus sign and
 // the column with a plus sign.  Called by one whole 64-lane workgroup.
 template <int Mode, bool Perm = false>
-__device__ __forceinline__ void diag_tile(int bk, int n, const double *__restrict__ x, const double *__restrict__ y,
+__device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const double *__restrict__ x,
+                                          const double *__restrict__ y,
                                           const double *__restrict__ b, const int *__restrict__ perm,
                                           double *__restrict__ partial, unsigned long long *__restrict__ minr2,
-                                          double r2_keep)
+                                          double r2_keep, const double *__restrict__ sct = sdlibm_sincos_tab,
+                                          const uint64_t *__restrict__ e2t = sdlibm_exp2_tab)
 {
-    __shared__ double sx[kTile], sy[kTile], sb[kTile];
-    __shared__ int sid[kTile];
+    double *sx = L.x, *sy = L.y, *sb = L.b;
+    int *sid = L.id;
     const int l = (int)threadIdx.x;
     const int p = bk * kTile + l;
     const int io = Perm ? (p < n ? perm[p] : 0) : p;
@@ -187,7 +203,7 @@ __device__ __forceinline__ void diag_tile(int bk, int n, const double *__restric
         const bool valid = (p < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
         const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
         double r2;
-        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], m > l, r2);
+        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], m > l, r2, sct, e2t);
         if (valid) {
             row += t;
             col -= t;

@@ -58,18 +58,36 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
                                                    unsigned long long *__restrict__ minr2, double r2_keep)
 {
     const int u = (int)blockIdx.x;
+    __shared__ pairtile::TileLds L;
+    // The fast field's table lookups (cos and exp, per lane at data-dependent
+    // indices) read an LDS copy: 32-bit LDS addressing instead of 64-bit
+    // global address arithmetic per lookup.  The copy is written here and
+    // ordered before its first read by the tile's own barrier.
+    const double *sct = sdlibm_sincos_tab;
+    const uint64_t *e2t = sdlibm_exp2_tab;
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        constexpr int nsc = (int)(sizeof(sdlibm_sincos_tab) / sizeof(double));
+        constexpr int ne2 = (int)(sizeof(sdlibm_exp2_tab) / sizeof(uint64_t));
+        __shared__ double lsc[nsc];
+        __shared__ uint64_t le2[ne2];
+        for (int k = (int)threadIdx.x; k < nsc; k += kTile) lsc[k] = sdlibm_sincos_tab[k];
+        for (int k = (int)threadIdx.x; k < ne2; k += kTile) le2[k] = sdlibm_exp2_tab[k];
+        sct = lsc;
+        e2t = le2;
+    }
     if (u < nfull) {
         int bi, bj;
         pairtile::strict_coords(u, bi, bj);
-        if ((bj + 1) * kTile <= n) pairtile::tile<Mode, true, true>(bi, bj, n, x, y, b, perm, partial, minr2, r2_keep);
-        else pairtile::tile<Mode, true, false>(bi, bj, n, x, y, b, perm, partial, minr2, r2_keep);
+        if ((bj + 1) * kTile <= n)
+            pairtile::tile<Mode, true, true>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
+        else pairtile::tile<Mode, true, false>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
         return;
     }
     const int bk = 2 * (u - nfull);
-    pairtile::diag_tile<Mode, true>(bk, n, x, y, b, perm, partial, minr2, r2_keep);
+    pairtile::diag_tile<Mode, true>(L, bk, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
     if (bk + 1 < nb) {
         __syncthreads();
-        pairtile::diag_tile<Mode, true>(bk + 1, n, x, y, b, perm, partial, minr2, r2_keep);
+        pairtile::diag_tile<Mode, true>(L, bk + 1, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
     }
 }
 

@@ -96,6 +96,9 @@ struct Context {
     double *Araw = nullptr, *sraw = nullptr;
     bool raw_valid = false;
     double raw_h = 0;
+    // step size of the off-diagonal unsigned values in Araw (written by the
+    // values pass of a pattern build, or by k_jac_raw for another h)
+    double araw_h = __builtin_nan("");   // NaN: none
     int *partner = nullptr;
     double *strength = nullptr;
     bool partner_valid = false;

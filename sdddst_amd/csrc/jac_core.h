@@ -136,7 +136,8 @@ __device__ __forceinline__ int row_position(int words, const unsigned long long 
 // tau_x and the taper of every strictly-lower entry of column j (row i > j),
 // the position of its mirror (row j in column i) and the within-cutoff flag
 // of both positions.  Mode FAST takes the production derivative
-// (xy_diff_x_fast), the other modes the reference's expression.
+// (xy_diff_x_fast), the other modes the reference's expression.  With raw,
+// also the unsigned value for step size h at the entry and at its mirror.
 template <int Mode>
 __device__ __forceinline__ void values_column(int j, const double *__restrict__ x, const double *__restrict__ y,
                                               const CutParams &cp, const int *__restrict__ Ap,
@@ -144,12 +145,15 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
                                               double *__restrict__ tau, double *__restrict__ mult,
                                               int *__restrict__ mirror, unsigned char *__restrict__ core,
                                               int words = 0, const unsigned long long *__restrict__ mask = nullptr,
-                                              const int *__restrict__ wstart = nullptr)
+                                              const int *__restrict__ wstart = nullptr,
+                                              const double *__restrict__ b = nullptr, double h = 0.0,
+                                              double *__restrict__ raw = nullptr, int *__restrict__ flags = nullptr)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int d = diag[j];
     const int end = Ap[j + 1];
     const double xj = x[j], yj = y[j];
+    const double bj = raw ? b[j] : 0.0;
     for (int k = d + 1 + lane; k < end; k += 64) {
         const int i = Ai[k];
         // mirror: row j in column i (two loads with the word starts, else a
@@ -163,11 +167,22 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
             const double s = sdlibm::sqrt(r2) - cp.rc;
             m = sdlibm::exp(-(s * s) * cp.inv_rc2);
         }
-        if constexpr (Mode == SDDDST_FIELD_FAST) tau[k] = sdfield::xy_diff_x_fast(dx, dy);
-        else tau[k] = sdfield::xy_diff_x(dx, dy);
+        double t;
+        if constexpr (Mode == SDDDST_FIELD_FAST) t = sdfield::xy_diff_x_fast(dx, dy);
+        else t = sdfield::xy_diff_x(dx, dy);
+        tau[k] = t;
         mult[k] = m;
         mirror[k] = mk;
-        mirror[mk] = k;    // the upper entry's mirror: this lower entry
+        if (raw) {
+            // the unsigned value h * b_i * b_j * tau * taper in the operation
+            // order of simulation.cpp:337-343; the upper copy (row j of column
+            // i, simulation.cpp:260-268) is the same product.  An exact zero
+            // is flagged: its upper copy must not exist.
+            const double v = h * b[i] * bj * t * m;
+            raw[k] = v;
+            raw[mk] = v;
+            if (v == 0.0) atomicOr(flags, 1);
+        }
         const unsigned char inside = r2 > cp.rc2 ? 0 : 1;
         core[k] = inside;
         core[mk] = inside;

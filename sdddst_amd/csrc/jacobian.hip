@@ -241,8 +241,9 @@ __global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsign
     jac::fill_column(j, words, mask + (size_t)j * words, Ap[j], Ai, diag + j, wstart + (size_t)j * words);
 }
 
-// tau_x and the taper of every strictly-lower entry (row i > column j) and
-// the position of its mirror (row j in column i).
+// tau_x and the taper of every strictly-lower entry (row i > column j), the
+// position of its mirror (row j in column i), and the unsigned value for step
+// size h at both positions.
 template <int Mode>
 __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                     CutParams cp, const int *__restrict__ Ap,
@@ -250,11 +251,13 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
                                                     double *__restrict__ tau, double *__restrict__ mult,
                                                     int *__restrict__ mirror, unsigned char *__restrict__ core,
                                                     int words, const unsigned long long *__restrict__ mask,
-                                                    const int *__restrict__ wstart)
+                                                    const int *__restrict__ wstart, const double *__restrict__ b,
+                                                    double h, double *__restrict__ raw, int *__restrict__ flags)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    jac::values_column<Mode>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart);
+    jac::values_column<Mode>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart, b, h, raw,
+                             flags);
 }
 
 // simulation.cpp:269-324: the point-defect sum tmp of column j (defects in
@@ -313,49 +316,28 @@ __global__ __launch_bounds__(256) void k_jac_scale(int n, const int *__restrict_
     jac::scale_column(j, Ap, Ai, diag, dvec, Ax, AxRow);
 }
 
-// Fused pass 1 (simulation.cpp:236-374 without the row scaling): one
-// wavefront per column j computes the column's unscaled values in storage
-// order -- upper entries (i, j), i < j, as the copy of the lower entry (j, i)
-// of column i, h * b_j * b_i * tau * taper at that entry (simulation.cpp:
-// 260-268); the diagonal -pd * h; lower entries h * b_i * b_j * tau * taper
-// (simulation.cpp:337-343) -- stores them coalesced, and lane 0 adds them in
-// storage order from LDS (simulation.cpp:351-362), giving s = -sum, the raw
-// diagonal, and d_j (363-373).  An exactly zero lower value is flagged (its
-// upper copy must not exist; the caller then takes the compacting path).
+// Pass 2 of the reference (simulation.cpp:349-374) over the unsigned values
+// already in storage order (Araw, written at both positions of every pair by
+// the values pass or by k_jac_raw): one wavefront per column j stages the
+// column in LDS with coalesced loads, the diagonal taking -pd * h, and lane 0
+// adds it in storage order, giving s = -sum, the raw diagonal, and d_j.
 constexpr int kSumChunk = 512;
-__global__ __launch_bounds__(256) void k_jac_rawsum(int n, const double *__restrict__ b, double h,
-                                                    const int *__restrict__ Ap, const int *__restrict__ Ai,
-                                                    const int *__restrict__ diag, const double *__restrict__ tau,
-                                                    const double *__restrict__ mult, const int *__restrict__ mirror,
-                                                    const double *__restrict__ pdtmp, double *__restrict__ Araw,
-                                                    double *__restrict__ sraw, double *__restrict__ dvec,
-                                                    int *__restrict__ flags)
+__global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *__restrict__ Ap,
+                                                    const int *__restrict__ diag, const double *__restrict__ pdtmp,
+                                                    double *__restrict__ Araw, double *__restrict__ sraw,
+                                                    double *__restrict__ dvec)
 {
     __shared__ double buf[4][kSumChunk];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int j = (int)(blockIdx.x * 4 + wave);
     if (j >= n) return;
     const int beg = Ap[j], end = Ap[j + 1], d = diag[j];
-    const double bj = b[j];
-    const double pd = pdtmp ? pdtmp[j] : 0.0;
+    const double pdh = -(pdtmp ? pdtmp[j] : 0.0) * h;
     double *lb = buf[wave];
     double subSum = 0;
     for (int c0 = beg; c0 < end; c0 += kSumChunk) {
         const int c1 = min(end, c0 + kSumChunk);
-        for (int k = c0 + lane; k < c1; k += 64) {
-            double v;
-            if (k < d) {
-                const int kk = mirror[k];
-                v = h * bj * b[Ai[k]] * tau[kk] * mult[kk];
-            } else if (k == d) {
-                v = -pd * h;
-            } else {
-                v = h * b[Ai[k]] * bj * tau[k] * mult[k];
-                if (v == 0.0) atomicOr(flags, 1);
-            }
-            lb[k - c0] = v;
-            Araw[k] = v;
-        }
+        for (int k = c0 + lane; k < c1; k += 64) lb[k - c0] = k == d ? pdh : Araw[k];
         // the wavefront's LDS stores are visible to its own later loads
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -527,6 +509,7 @@ static int ensure_capacity(Context &c, int64_t need)
     }
     c.cap = 0;
     c.raw_valid = false;
+    c.araw_h = __builtin_nan("");
     const size_t sizes[] = {sizeof(int),    sizeof(double), sizeof(double),        sizeof(double),
                             sizeof(double), sizeof(int),    sizeof(unsigned char), sizeof(double)};
     for (int k = 0; k < 8; ++k) {
@@ -540,8 +523,9 @@ static int ensure_capacity(Context &c, int64_t need)
     return SDDDST_OK;
 }
 
-// Pattern, tau_x, taper and mirror positions for the positions in slot.
-static int build_pattern(Context &c, int slot)
+// Pattern, tau_x, taper and mirror positions for the positions in slot, and
+// the off-diagonal unsigned values for step size h (with the zero flag).
+static int build_pattern(Context &c, int slot, double h)
 {
     const int n = c.n;
     const CutParams cp = cut_params(c);
@@ -601,13 +585,17 @@ static int build_pattern(Context &c, int slot)
     hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag,
                        c.jwstart);
     SD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_FAST>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap,
-                           c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart);
+                           c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart, c.b, h,
+                           c.Araw, c.d_flags);
     else
         hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_EXACT>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp,
-                           c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart);
+                           c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart, c.b, h,
+                           c.Araw, c.d_flags);
     SD_HIP(hipGetLastError());
+    c.araw_h = h;
     if (c.pc > 0) {
         hipLaunchKernelGGL(k_jac_pd, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.pc, x, c.y, c.b, c.px,
                            c.py, c.p.A, c.p.KASQR, cp, c.jpd);
@@ -631,7 +619,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     int rc;
     if (c.timing && (rc = timer_begin(c, c.t_jac))) return rc;
     if (!reuse_pattern || !c.pattern_valid) {
-        rc = build_pattern(c, slot);
+        rc = build_pattern(c, slot, h);
         if (rc) return rc;
         c.raw_valid = false;
     }
@@ -642,10 +630,17 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
         hipLaunchKernelGGL(k_jac_dvec, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.sraw, factor, c.dvec);
         SD_HIP(hipGetLastError());
     } else {
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
-        hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
-                           c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec,
-                           c.d_flags);
+        if (c.araw_h != h) {
+            // the pattern is reused with another step size: the unsigned
+            // values again, from tau_x and the taper (with the zero flag)
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
+            hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
+                               c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Araw, c.d_flags);
+            SD_HIP(hipGetLastError());
+            c.araw_h = h;
+        }
+        hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, h, c.Ap, c.diag,
+                           c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec);
         SD_HIP(hipGetLastError());
         int flag = 0;
         SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -203,10 +203,12 @@ def _pair_solve(c, h):
     return c.solver_solve(download=True)
 
 
-@pytest.mark.parametrize("pc", [0, 40])
-def test_stage_two_reuse_is_bit_identical(pc):
+@pytest.mark.parametrize("pc, ratio", [(0, 0.5), (40, 0.5), (0, 0.3), (40, 1.0)])
+def test_stage_two_reuse_is_bit_identical(pc, ratio):
     # stage II reuses stage I's unsigned values times 0.5: the matrix, d and
-    # the pair blocks equal a fresh assembly at h / 2 bit for bit
+    # the pair blocks equal a fresh assembly at h / 2 bit for bit; a reuse of
+    # the pattern at another step size (0.3 h: the values again from tau_x and
+    # the taper; h: the values of the build) equals a fresh assembly as well
     n, m, h = 1024, 2.0, 3e-5
     x, y, b = configs.dislocations(n, 61)
     kw = {}
@@ -223,10 +225,10 @@ def test_stage_two_reuse_is_bit_identical(pc):
             c.speeds(0, S.V_INIT2, 0.0)
             if reuse:
                 _assemble(c, h, 1)
-                csc = _assemble(c, 0.5 * h, 2)
+                csc = _assemble(c, ratio * h, 2)
             else:
-                csc = _assemble(c, 0.5 * h, 1)
-            out.append((csc, _pair_solve(c, 0.5 * h)))
+                csc = _assemble(c, ratio * h, 1)
+            out.append((csc, _pair_solve(c, ratio * h)))
     (a, da), (b_, db) = out
     for u, v in zip(a, b_):
         assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))

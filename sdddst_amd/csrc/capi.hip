@@ -58,7 +58,7 @@ static void free_all(Context &c)
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
                     c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
-                    c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy};
+                    c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
@@ -111,6 +111,7 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
         (rc = dalloc(&c.Ap, (size_t)n + 1)) || (rc = dalloc(&c.diag, n)) || (rc = dalloc(&c.colcount, n)) ||
         (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, 1)) || (rc = dalloc(&c.jpd, n)) ||
         (rc = dalloc(&c.sraw, n)) || (rc = dalloc(&c.partner, n)) || (rc = dalloc(&c.strength, n)) ||
+        (rc = dalloc(&c.colinfo, n)) ||
         (rc = pair_order_alloc(c)))
         return rc;
     SD_HIP(hipHostMalloc((void **)&c.hs, sizeof(DevScalars), hipHostMallocDefault));

@@ -94,6 +94,10 @@ struct Context {
     // pass 2 of the reference (sraw, n) at step size raw_h, and the pair
     // partners of the scaled matrix written by the scaling pass
     double *Araw = nullptr, *sraw = nullptr;
+    // per column j, written with d_j: {(1 + d_j) / 2, |s_j (1 + d_j) / 2 + 1|}
+    // (the row scale and the scaled diagonal's magnitude, as the scaling
+    // pass forms them)
+    double2 *colinfo = nullptr;
     bool raw_valid = false;
     double raw_h = 0;
     // step size of the off-diagonal unsigned values in Araw (written by the

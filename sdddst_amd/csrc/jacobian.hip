@@ -325,7 +325,7 @@ constexpr int kSumChunk = 512;
 __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *__restrict__ Ap,
                                                     const int *__restrict__ diag, const double *__restrict__ pdtmp,
                                                     double *__restrict__ Araw, double *__restrict__ sraw,
-                                                    double *__restrict__ dvec)
+                                                    double *__restrict__ dvec, double2 *__restrict__ colinfo)
 {
     __shared__ double buf[4][kSumChunk];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -362,11 +362,14 @@ __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *
         dv = 0.;
     }
     dvec[j] = dv;
+    const double Dj = (1.0 + dv) * 0.5;
+    colinfo[j] = make_double2(Dj, __builtin_fabs(subSum * Dj + 1.0));
 }
 
 // d_j for the column sums s * factor (factor a power of two: the unscaled
 // values and their ordered sums scale exactly).
-__global__ void k_jac_dvec(int n, const double *__restrict__ sraw, double factor, double *__restrict__ dvec)
+__global__ void k_jac_dvec(int n, const double *__restrict__ sraw, double factor, double *__restrict__ dvec,
+                           double2 *__restrict__ colinfo)
 {
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j >= n) return;
@@ -381,6 +384,8 @@ __global__ void k_jac_dvec(int n, const double *__restrict__ sraw, double factor
         dv = 0.;
     }
     dvec[j] = dv;
+    const double Dj = (1.0 + dv) * 0.5;
+    colinfo[j] = make_double2(Dj, __builtin_fabs(s * Dj + 1.0));
 }
 
 // Fused pass 2 (simulation.cpp:376-383) for column j with the unscaled values
@@ -388,42 +393,60 @@ __global__ void k_jac_dvec(int n, const double *__restrict__ sraw, double factor
 // +1 on the diagonal; and the pair partner of row j from the same values
 // (the rule of partner_row in precond_core.h: the j' maximising
 // |A_jj' A_j'j| / (|A_jj| |A_j'j'|), ties to the smaller j', with |A_j'j'|
-// formed as the scaling forms it).
+// formed as the scaling forms it).  The row scales and diagonal magnitudes
+// come packed per column (one 16-byte gather per entry), and each lane
+// issues the loads of four entries before their arithmetic, so the
+// index -> gather latency chain is paid once per four entries.
+constexpr int kScaleUnroll = 4;
 __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__restrict__ Ap,
                                                            const int *__restrict__ Ai,
                                                            const int *__restrict__ diag,
-                                                           const double *__restrict__ dvec,
-                                                           const double *__restrict__ Araw,
-                                                           const double *__restrict__ sraw, double factor,
+                                                           const double2 *__restrict__ colinfo,
+                                                           const double *__restrict__ Araw, double factor,
                                                            double *__restrict__ Ax, double *__restrict__ AxRow,
                                                            int *__restrict__ partner, double *__restrict__ strength)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    const double Dj = (1.0 + dvec[j]) * 0.5;
+    const double2 cj = colinfo[j];
+    const double Dj = cj.x, ajj = cj.y;
     const int dpos = diag[j];
-    const double ajj = __builtin_fabs(sraw[j] * factor * Dj + 1.0);
+    const int end = Ap[j + 1];
     double best = -1.0;
     int bj = -1;
-    for (int k = Ap[j] + lane; k < Ap[j + 1]; k += 64) {
-        const double raw = Araw[k] * factor;
-        const int i = Ai[k];
-        const double Di = (1.0 + dvec[i]) * 0.5;
-        double a = raw * Di;
-        double r = raw * Dj;
-        if (k == dpos) {
-            a += 1.0;
-            r += 1.0;
+    for (int k0 = Ap[j] + lane; k0 < end; k0 += 64 * kScaleUnroll) {
+        double raw[kScaleUnroll];
+        int ii[kScaleUnroll];
+#pragma unroll
+        for (int u = 0; u < kScaleUnroll; ++u) {
+            const int k = k0 + 64 * u;
+            raw[u] = k < end ? Araw[k] : 0.0;
+            ii[u] = k < end ? Ai[k] : j;
         }
-        Ax[k] = a;
-        AxRow[k] = r;
-        if (i != j) {
-            const double aii = __builtin_fabs(sraw[i] * factor * Di + 1.0);
-            const double st = __builtin_fabs(r * a) / (ajj * aii);
-            if (st > best || (st == best && i < bj)) {
-                best = st;
-                bj = i;
+        double2 ci[kScaleUnroll];
+#pragma unroll
+        for (int u = 0; u < kScaleUnroll; ++u) ci[u] = colinfo[ii[u]];
+#pragma unroll
+        for (int u = 0; u < kScaleUnroll; ++u) {
+            const int k = k0 + 64 * u;
+            if (k >= end) continue;
+            const double r0 = raw[u] * factor;
+            const int i = ii[u];
+            double a = r0 * ci[u].x;
+            double r = r0 * Dj;
+            if (k == dpos) {
+                a += 1.0;
+                r += 1.0;
+            }
+            Ax[k] = a;
+            AxRow[k] = r;
+            if (i != j) {
+                const double st = __builtin_fabs(r * a) / (ajj * ci[u].y);
+                if (st > best || (st == best && i < bj)) {
+                    best = st;
+                    bj = i;
+                }
             }
         }
     }
@@ -627,7 +650,8 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     double factor = 1.0;
     if (reuse_pattern && c.raw_valid && c.jac_symmetric && h == 0.5 * c.raw_h) {
         factor = 0.5;
-        hipLaunchKernelGGL(k_jac_dvec, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.sraw, factor, c.dvec);
+        hipLaunchKernelGGL(k_jac_dvec, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.sraw, factor, c.dvec,
+                           c.colinfo);
         SD_HIP(hipGetLastError());
     } else {
         if (c.araw_h != h) {
@@ -640,7 +664,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             c.araw_h = h;
         }
         hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, h, c.Ap, c.diag,
-                           c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec);
+                           c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec, c.colinfo);
         SD_HIP(hipGetLastError());
         int flag = 0;
         SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -667,8 +691,8 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             return SDDDST_OK;
         }
     }
-    hipLaunchKernelGGL(k_jac_scale_partner, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec,
-                       c.Araw, c.sraw, factor, c.Ax, c.AxRow, c.partner, c.strength);
+    hipLaunchKernelGGL(k_jac_scale_partner, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag,
+                       c.colinfo, c.Araw, factor, c.Ax, c.AxRow, c.partner, c.strength);
     SD_HIP(hipGetLastError());
     c.partner_valid = true;
     if (c.timing && (rc = timer_end(c, c.t_jac))) return rc;

@@ -320,8 +320,10 @@ __global__ __launch_bounds__(256) void k_jac_scale(int n, const int *__restrict_
 // already in storage order (Araw, written at both positions of every pair by
 // the values pass or by k_jac_raw): one wavefront per column j stages the
 // column in LDS with coalesced loads, the diagonal taking -pd * h, and lane 0
-// adds it in storage order, giving s = -sum, the raw diagonal, and d_j.
+// adds it in storage order, giving s = -sum, the raw diagonal, and d_j.  The
+// next chunk's loads are in flight while lane 0 adds the current one.
 constexpr int kSumChunk = 512;
+constexpr int kSumPer = kSumChunk / 64;
 __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *__restrict__ Ap,
                                                     const int *__restrict__ diag, const double *__restrict__ pdtmp,
                                                     double *__restrict__ Araw, double *__restrict__ sraw,
@@ -335,13 +337,27 @@ __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *
     const double pdh = -(pdtmp ? pdtmp[j] : 0.0) * h;
     double *lb = buf[wave];
     double subSum = 0;
+    double nx[kSumPer];
+#pragma unroll
+    for (int u = 0; u < kSumPer; ++u) {
+        const int k = beg + lane + 64 * u;
+        nx[u] = k < end ? (k == d ? pdh : Araw[k]) : 0.0;
+    }
     for (int c0 = beg; c0 < end; c0 += kSumChunk) {
         const int c1 = min(end, c0 + kSumChunk);
-        for (int k = c0 + lane; k < c1; k += 64) lb[k - c0] = k == d ? pdh : Araw[k];
+#pragma unroll
+        for (int u = 0; u < kSumPer; ++u) lb[lane + 64 * u] = nx[u];
         // the wavefront's LDS stores are visible to its own later loads
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (c1 < end) {
+#pragma unroll
+            for (int u = 0; u < kSumPer; ++u) {
+                const int k = c1 + lane + 64 * u;
+                nx[u] = k < end ? (k == d ? pdh : Araw[k]) : 0.0;
+            }
+        }
         if (lane == 0)
             for (int t = 0; t < c1 - c0; ++t) subSum += lb[t];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

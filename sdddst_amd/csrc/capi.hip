@@ -58,7 +58,8 @@ static void free_all(Context &c)
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
                     c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
-                    c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo};
+                    c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo,
+                    c.ytc, c.yts};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
@@ -231,7 +232,11 @@ int sdddst_upload_state(sdddst_ctx *ctx, int slot, const double *x, const double
     SD_CHECK(n == c.n, SDDDST_ERR_ARG, "n=%d does not match the context (%d)", n, c.n);
     SD_CHECK(x != nullptr, SDDDST_ERR_ARG, "x is NULL");
     SD_HIP(hipMemcpyAsync(c.x[slot], x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
-    if (y) SD_HIP(hipMemcpyAsync(c.y, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    if (y) {
+        SD_HIP(hipMemcpyAsync(c.y, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        const int rc = launch_ytrig(c);
+        if (rc) return rc;
+    }
     if (b) SD_HIP(hipMemcpyAsync(c.b, b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     SD_HIP(hipStreamSynchronize(c.stream));
     c.order_age = 0;   // new positions: the next speeds call orders them afresh

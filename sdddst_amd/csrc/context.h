@@ -67,6 +67,7 @@ struct Context {
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int order_age = 0;                  // speeds calls since the order was last computed
+    double *ytc = nullptr, *yts = nullptr;   // cos, sin of 2 pi y per dislocation (set with y)
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
     // Jacobian (CSC)
@@ -147,6 +148,7 @@ void compute_cutoff(Context &c);
 // kernel launchers (return SDDDST status)
 int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol);
 int pair_order_alloc(Context &c);   // buffers of the x ordering of the pair tiles
+int launch_ytrig(Context &c);       // cos / sin of 2 pi y after an upload of y
 int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern);
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
 int launch_newton_update(Context &c, int slot);

@@ -140,11 +140,25 @@ SDLIBM_FN double xy_diff_x(double dx, double dy)
 }
 
 
-// Production form of xy() (see header comment).
-SDLIBM_FN double xy_fast(double dx, double dy, const double *__restrict__ sctab = sdlibm_sincos_tab,
-                         const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
+// cos and sin of 2 pi y for y wrapped into [-0.5, 0.5] (exactly: y - rint(y)).
+SDLIBM_FN void ytrig(double y, double &c, double &s)
 {
-    const double cy = sdlibm::cos(kPi * 2.0 * dy, sctab);
+    const double a = kPi * 2.0 * (y - __builtin_rint(y));
+    c = sdlibm::cos(a);
+    s = sdlibm::sin(a);
+}
+
+// cos(2 pi (y_i - y_j)) from the two dislocations' ytrig values (angle
+// addition; symmetric in i and j bit for bit).  Within a few ulps of the
+// reference's cos(2 pi dy): used only where the field is insensitive to it
+// (|dx| >= kTrigMinDx, where cosh(2 pi dx) - cos(2 pi dy) >= 2.7).
+SDLIBM_FN double cos_diff(double ci, double si, double cj, double sj) { return __builtin_fma(ci, cj, si * sj); }
+constexpr double kTrigMinDx = 0.2;
+
+// Production form of xy() (see header comment) with cos(2 pi dy) supplied
+// by the caller; xy_fast() below computes it as the reference does.
+SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
+{
     const double a0 = dx * 2.0 * kPi;
     const double ax = sdlibm::fabsd(a0);
     // Near image: the reference's cosh, evaluated once.  On the exp branch
@@ -208,6 +222,13 @@ SDLIBM_FN double xy_fast(double dx, double dy, const double *__restrict__ sctab 
     double far = P / Q + S;
     far = dx < 0 ? -far : far;
     return near + far * (2.0 * kPiSq);
+}
+
+// Production form of xy() (see header comment).
+SDLIBM_FN double xy_fast(double dx, double dy, const double *__restrict__ sctab = sdlibm_sincos_tab,
+                         const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
+{
+    return xy_fast_cy(dx, dy, sdlibm::cos(kPi * 2.0 * dy, sctab), e2tab);
 }
 
 // Production form of xy_diff_x() (AnalyticField.cpp:162-181).  tau_x is

@@ -29,6 +29,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/sdddst_amd.h"
 #include "field.h"
 
@@ -42,6 +44,7 @@ constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
 // workgroup however many tile variants the kernel instantiates).
 struct TileLds {
     double x[kTile], y[kTile], b[kTile];
+    double ca[kTile], sa[kTile];   // cos, sin of 2 pi y (far tiles)
     int id[kTile];
 };
 
@@ -78,10 +81,14 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
 // exactly, so for it the target-minus-source term already equals the
 // lower-first term with its sign; dx = +-0 differs only by the far images'
 // rounding residue (below 1e-17).
-template <int Mode, bool Perm>
+//
+// Trig (fast field, far tiles): cos(2 pi dy) from the dislocations' cos and
+// sin of 2 pi y by angle addition (cai, sai: target; caj, saj: source).
+template <int Mode, bool Perm, bool Trig = false>
 __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int io, int jo, double sxm, double sym,
                                             double sbm, bool lower_by_lane, double &r2,
-                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t)
+                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t,
+                                            double cai = 0.0, double sai = 0.0, double caj = 0.0, double saj = 0.0)
 {
     const bool lower = (Perm && Mode != SDDDST_FIELD_FAST) ? io < jo : (Perm ? true : lower_by_lane);
     const double d = xi - sxm, e = yi - sym;
@@ -95,7 +102,9 @@ __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int
         dx = sdfield::normalize(lower ? d : -d);
         dy = sdfield::normalize(lower ? e : -e);
     }
-    const double t = bI * sbm * field_tau<Mode>(dx, dy, sct, e2t);
+    double t;
+    if constexpr (Trig) t = bI * sbm * sdfield::xy_fast_cy(dx, dy, sdfield::cos_diff(cai, sai, caj, saj), e2t);
+    else t = bI * sbm * field_tau<Mode>(dx, dy, sct, e2t);
     r2 = dx * dx + dy * dy;
     return lower ? t : -t;
 }
@@ -103,14 +112,19 @@ __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int
 // Called by one whole 64-lane workgroup.  Full: both blocks lie inside n
 // and bi < bj, so every pair of the tile counts.  sct / e2t (fast field
 // only) may point at LDS copies of the tables written before the call: the
-// tile's barrier orders them before the first read.
+// tile's barrier orders them before the first read.  With ytc / yts (the cos
+// and sin of 2 pi y per dislocation; fast field, full tiles) a tile whose
+// two blocks lie at least kTrigMinDx apart in wrapped x -- decided from the
+// current positions, the same for every lane -- takes cos(2 pi dy) by angle
+// addition instead of evaluating cos per pair.
 template <int Mode, bool Perm = false, bool Full = false>
 __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
                                      const int *__restrict__ perm, double *__restrict__ partial,
                                      unsigned long long *__restrict__ minr2, double r2_keep,
                                      const double *__restrict__ sct = sdlibm_sincos_tab,
-                                     const uint64_t *__restrict__ e2t = sdlibm_exp2_tab)
+                                     const uint64_t *__restrict__ e2t = sdlibm_exp2_tab,
+                                     const double *__restrict__ ytc = nullptr, const double *__restrict__ yts = nullptr)
 {
     double *sx = L.x, *sy = L.y, *sb = L.b;
     int *sid = L.id;
@@ -119,42 +133,81 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
     const int q = bj * kTile + l;              // source position held in slot l
     const int io = Perm ? (p < n ? perm[p] : 0) : p;
     const int jo = Perm ? (q < n ? perm[q] : 0) : q;
-    sx[l] = q < n ? x[jo] : 0.0;
+    const double xs = q < n ? x[jo] : 0.0;
+    sx[l] = xs;
     sy[l] = q < n ? y[jo] : 0.0;
     sb[l] = q < n ? b[jo] : 0.0;
     sid[l] = jo;
     const double xi = p < n ? x[io] : 0.0;
     const double yi = p < n ? y[io] : 0.0;
     const double bI = p < n ? b[io] : 0.0;
-    __syncthreads();
-    const bool diag = (bi == bj);
-    double row = 0.0, col = 0.0;
-    for (int s = 0; s < kTile; ++s) {
-        const int m = (l + s) & 63;
-        const int j = bj * kTile + m;
-        const bool valid = Full || ((p < n) && (j < n) && (!diag || l < m));
-        // the source's original index: for the lower-first rule (exact field)
-        // and for the rare tolerance candidates
-        const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
-        double r2;
-        // without Perm the target has the lower index in every tile it meets
-        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2, sct, e2t);
-        if (valid) {
-            row += t;
-            col -= t;
-            if (r2 < r2_keep) {
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
-                atomicMin(&minr2[io], bits);
-                atomicMin(&minr2[Perm ? sid[m] : j], bits);
+    bool far = false;
+    double cai = 0.0, sai = 0.0;
+    if constexpr (Mode == SDDDST_FIELD_FAST && Full) {
+        if (ytc) {
+            // wrapped-x intervals of the two blocks; the smallest circular
+            // distance between them bounds |dx| of every pair of the tile
+            const double tw = xi - __builtin_rint(xi), sw = xs - __builtin_rint(xs);
+            double tmin = tw, tmax = tw, smin = sw, smax = sw;
+            for (int off = 32; off > 0; off >>= 1) {
+                tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
+                tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
+                smin = fmin(smin, __shfl_xor(smin, off, 64));
+                smax = fmax(smax, __shfl_xor(smax, off, 64));
+            }
+            const double lo = tmin - smax, hi = tmax - smin;
+            const double mind = (lo <= 0.0 && hi >= 0.0) ? 0.0 : (hi < 0.0 ? fmin(-hi, 1.0 + lo) : fmin(lo, 1.0 - hi));
+            far = __builtin_amdgcn_readfirstlane((int)(mind >= sdfield::kTrigMinDx)) != 0;
+            if (far) {
+                L.ca[l] = ytc[jo];
+                L.sa[l] = yts[jo];
+                cai = ytc[io];
+                sai = yts[io];
             }
         }
-        col = rotate_down1(col, l);
     }
-    if (diag) {
-        if (p < n) partial[(size_t)bi * n + p] = row + col;
+    __syncthreads();
+    const bool diag = (bi == bj);
+    auto sweep = [&](auto trig) {
+        constexpr bool T = decltype(trig)::value;
+        double row = 0.0, col = 0.0;
+        for (int s = 0; s < kTile; ++s) {
+            const int m = (l + s) & 63;
+            const int j = bj * kTile + m;
+            const bool valid = Full || ((p < n) && (j < n) && (!diag || l < m));
+            // the source's original index: for the lower-first rule (exact
+            // field) and for the rare tolerance candidates
+            const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
+            double r2, t;
+            // without Perm the target has the lower index in every tile it meets
+            if constexpr (T)
+                t = pair_term<Mode, Perm, true>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2, sct, e2t, cai, sai,
+                                                L.ca[m], L.sa[m]);
+            else
+                t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2, sct, e2t);
+            if (valid) {
+                row += t;
+                col -= t;
+                if (r2 < r2_keep) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
+                    atomicMin(&minr2[io], bits);
+                    atomicMin(&minr2[Perm ? sid[m] : j], bits);
+                }
+            }
+            col = rotate_down1(col, l);
+        }
+        if (diag) {
+            if (p < n) partial[(size_t)bi * n + p] = row + col;
+        } else {
+            if (p < n) partial[(size_t)bj * n + p] = row;
+            if (q < n) partial[(size_t)bi * n + q] = col;
+        }
+    };
+    if constexpr (Mode == SDDDST_FIELD_FAST && Full) {
+        if (far) sweep(std::true_type{});
+        else sweep(std::false_type{});
     } else {
-        if (p < n) partial[(size_t)bj * n + p] = row;
-        if (q < n) partial[(size_t)bi * n + q] = col;
+        sweep(std::false_type{});
     }
 }
 

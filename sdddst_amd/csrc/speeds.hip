@@ -55,7 +55,8 @@ template <int Mode>
 __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, const double *__restrict__ x,
                                                    const double *__restrict__ y, const double *__restrict__ b,
                                                    const int *__restrict__ perm, double *__restrict__ partial,
-                                                   unsigned long long *__restrict__ minr2, double r2_keep)
+                                                   unsigned long long *__restrict__ minr2, double r2_keep,
+                                                   const double *__restrict__ ytc, const double *__restrict__ yts)
 {
     const int u = (int)blockIdx.x;
     __shared__ pairtile::TileLds L;
@@ -79,7 +80,8 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
         int bi, bj;
         pairtile::strict_coords(u, bi, bj);
         if ((bj + 1) * kTile <= n)
-            pairtile::tile<Mode, true, true>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
+            pairtile::tile<Mode, true, true>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t, ytc,
+                                             yts);
         else pairtile::tile<Mode, true, false>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
         return;
     }
@@ -89,6 +91,18 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
         __syncthreads();
         pairtile::diag_tile<Mode, true>(L, bk + 1, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
     }
+}
+
+// cos and sin of 2 pi y per dislocation (y wrapped first), for the far
+// tiles of the fast field.  y changes only by upload.
+__global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict__ yc, double *__restrict__ ys)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double cv, sv;
+    sdfield::ytrig(y[i], cv, sv);
+    yc[i] = cv;
+    ys[i] = sv;
 }
 
 // Sort keys of the pair-tile order: the bin of the wrapped x among nb equal
@@ -228,6 +242,15 @@ int pair_order_alloc(Context &c)
                                               key_bits(c.nb), c.stream));
     c.sort_tmp_bytes = bytes > 0 ? bytes : 1;
     SD_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes));
+    for (double **p : {&c.ytc, &c.yts}) SD_HIP(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
+    return SDDDST_OK;
+}
+
+int launch_ytrig(Context &c)
+{
+    if (c.n <= 0) return SDDDST_OK;
+    hipLaunchKernelGGL(k_ytrig, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.y, c.ytc, c.yts);
+    SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
@@ -278,11 +301,11 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
-                           pairtile::tolerance_r2(minPrecSqr));
+                           pairtile::tolerance_r2(minPrecSqr), c.ytc, c.yts);
     else
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
-                           pairtile::tolerance_r2(minPrecSqr));
+                           pairtile::tolerance_r2(minPrecSqr), c.ytc, c.yts);
     SD_HIP(hipGetLastError());
     if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.perm, c.partial,

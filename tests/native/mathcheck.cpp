@@ -38,6 +38,20 @@ void mc_batch1(int fn, long n, const double *x, double *out)
     }
 }
 
+// The pair kernel's far-tile term: xy_fast with cos(2 pi dy) from the two
+// dislocations' ytrig values (angle addition), for positions (xi, yi), (xj, yj).
+void mc_batch_pair_trig(long n, const double *xi, const double *yi, const double *xj, const double *yj,
+                        double *out)
+{
+    for (long k = 0; k < n; ++k) {
+        double ci, si, cj, sj;
+        sdfield::ytrig(yi[k], ci, si);
+        sdfield::ytrig(yj[k], cj, sj);
+        const double dx = sdfield::wrap_fast(xi[k] - xj[k]), dy = sdfield::wrap_fast(yi[k] - yj[k]);
+        out[k] = sdfield::xy_fast_cy(dx, dy, sdfield::cos_diff(ci, si, cj, sj));
+    }
+}
+
 // fn = 0 xy, 1 xy_fast, 2 xy_diff_x, 3 xy_diff_x_fast
 void mc_batch2(int fn, long n, const double *dx, const double *dy, double *out)
 {

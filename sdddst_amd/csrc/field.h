@@ -155,30 +155,10 @@ SDLIBM_FN void ytrig(double y, double &c, double &s)
 SDLIBM_FN double cos_diff(double ci, double si, double cj, double sj) { return __builtin_fma(ci, cj, si * sj); }
 constexpr double kTrigMinDx = 0.2;
 
-// Production form of xy() (see header comment) with cos(2 pi dy) supplied
-// by the caller; xy_fast() below computes it as the reference does.
-SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
+// The nine far images of xy_fast (see below) as sg(dx) F(|dx|) without the
+// sign and the factor 2 pi^2, from t = e^{2 pi |dx|} and ti = 1 / t.
+SDLIBM_FN double far_images(double dx, double cy, double t, double ti)
 {
-    const double a0 = dx * 2.0 * kPi;
-    const double ax = sdlibm::fabsd(a0);
-    // Near image: the reference's cosh, evaluated once.  On the exp branch
-    // 0.5 / t is taken as 0.5 * (1 / t): scaling by 2^-1 commutes with
-    // rounding while 0.5 / t stays normal (t < 2^1021, |a0| < 707; a wrapped
-    // dx gives t <= e^pi), so the value is the same and one division serves
-    // both uses.
-    double c0, t, ti;
-    if (sdlibm::hiword(ax) < 0x3fd62e43u) {
-        const double m = sdlibm::expm1(ax);
-        const double w = 1.0 + m;
-        c0 = 1.0 + (m * m) / (w + w);
-        t = w;
-        ti = 1.0 / t;
-    } else {
-        t = sdlibm::exp(ax, e2tab);
-        ti = 1.0 / t;
-        c0 = 0.5 * t + 0.5 * ti;
-    }
-    const double near = (dx * dx + dy * dy > 1e-6) ? f_given_cosh(dx, c0, cy) : f_taylor(dx, dy);
     // Far images in terms of ad = |dx| and the sign sg of dx (every image
     // of the reference's sum is odd in dx, so the sum is sg * F(ad)):
     //   dx -+ k (k = 1..3) -> sg * ((ad + k) fA_k + (ad - k) fB_k)
@@ -219,9 +199,79 @@ SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__r
     S = __builtin_fma(((1.0 - ad) * (ad + 4.0)) * aA4, __builtin_fma(c2, aA4, cy), S);
     S = __builtin_fma((ad - 4.0) * aB4, __builtin_fma(c2, aB4, cy), S);
     S = __builtin_fma((ad * (ad - 5.0)) * aB5, __builtin_fma(c2, aB5, cy), S);
-    double far = P / Q + S;
+    return P / Q + S;
+}
+
+// Production form of xy() (see header comment) with cos(2 pi dy) supplied
+// by the caller; xy_fast() below computes it as the reference does.
+SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__restrict__ e2tab = sdlibm_exp2_tab)
+{
+    const double a0 = dx * 2.0 * kPi;
+    const double ax = sdlibm::fabsd(a0);
+    // Near image: the reference's cosh, evaluated once.  On the exp branch
+    // 0.5 / t is taken as 0.5 * (1 / t): scaling by 2^-1 commutes with
+    // rounding while 0.5 / t stays normal (t < 2^1021, |a0| < 707; a wrapped
+    // dx gives t <= e^pi), so the value is the same and one division serves
+    // both uses.
+    double c0, t, ti;
+    if (sdlibm::hiword(ax) < 0x3fd62e43u) {
+        const double m = sdlibm::expm1(ax);
+        const double w = 1.0 + m;
+        c0 = 1.0 + (m * m) / (w + w);
+        t = w;
+        ti = 1.0 / t;
+    } else {
+        t = sdlibm::exp(ax, e2tab);
+        ti = 1.0 / t;
+        c0 = 0.5 * t + 0.5 * ti;
+    }
+    const double near = (dx * dx + dy * dy > 1e-6) ? f_given_cosh(dx, c0, cy) : f_taylor(dx, dy);
+    double far = far_images(dx, cy, t, ti);
     far = dx < 0 ? -far : far;
     return near + far * (2.0 * kPiSq);
+}
+
+// The production field for a pair with |dx| >= kTrigMinDx (far-tile pairs of
+// the pair kernel): cos(2 pi dy) and t = e^{2 pi |dx|}, ti = e^{-2 pi |dx|}
+// supplied by the caller (from per-dislocation factors, within a few ulps),
+// cosh(2 pi dx) = (t + ti) / 2 as the reference forms it on its exp branch.
+SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
+{
+    const double c0 = 0.5 * t + 0.5 * ti;
+    const double near = f_given_cosh(dx, c0, cy);
+    double far = far_images(dx, cy, t, ti);
+    far = dx < 0 ? -far : far;
+    return near + far * (2.0 * kPiSq);
+}
+
+// Per-dislocation factors of the far pairs: the wrapped x w = x - rint(x)
+// (exact) and e^{+-2 pi w}.
+SDLIBM_FN void xexp(double x, double &w, double &ep, double &em)
+{
+    w = x - __builtin_rint(x);
+    const double a = kPi * 2.0 * w;
+    ep = sdlibm::exp(a);
+    em = sdlibm::exp(-a);
+}
+
+// Circular distance |z - rint(z)| (exact) of a difference of wrapped x.
+SDLIBM_FN double circ(double z) { return sdlibm::fabsd(z - __builtin_rint(z)); }
+
+// The production field of a far pair (circ(wi - wj) >= kTrigMinDx) from the
+// two dislocations' wrapped x, exp factors and cos(2 pi dy): dx = s - rint(s)
+// with s = wi - wj, and e^{2 pi dx} = e^{2 pi wi} e^{-2 pi wj} e^{-2 pi k}
+// (k = rint(s) in {-1, 0, 1}), each exponential within a few ulps.  Swapping
+// the two dislocations negates s, k and dx exactly and exchanges the two
+// products, so the term is odd bit for bit.
+SDLIBM_FN double far_pair(double wi, double wj, double epi, double emi, double epj, double emj, double cy)
+{
+    const double s = wi - wj;
+    const double k = __builtin_rint(s);
+    const double dx = s - k;
+    const double gk = k == 0.0 ? 1.0 : (k > 0.0 ? sdlibm_exp_m2pik[1] : sdlibm_exp_2pik[1]);
+    const double gmk = k == 0.0 ? 1.0 : (k > 0.0 ? sdlibm_exp_2pik[1] : sdlibm_exp_m2pik[1]);
+    const double pp = (epi * emj) * gk, pm = (emi * epj) * gmk;
+    return dx >= 0.0 ? xy_far(dx, cy, pp, pm) : xy_far(dx, cy, pm, pp);
 }
 
 // Production form of xy() (see header comment).

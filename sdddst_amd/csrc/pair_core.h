@@ -44,9 +44,23 @@ constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
 // workgroup however many tile variants the kernel instantiates).
 struct TileLds {
     double x[kTile], y[kTile], b[kTile];
-    double ca[kTile], sa[kTile];   // cos, sin of 2 pi y (far tiles)
+    // far-pair factors of the source slots: cos, sin of 2 pi y, wrapped x
+    // and e^{+-2 pi x}
+    double ca[kTile], sa[kTile], w[kTile], ep[kTile], em[kTile];
     int id[kTile];
 };
+
+// Per-dislocation far-pair factors of the fast field (null members: every
+// pair takes the reference-form path).
+struct FarData {
+    const double *ca = nullptr, *sa = nullptr;   // cos, sin of 2 pi y
+    const double *w = nullptr, *ep = nullptr, *em = nullptr;   // x - rint(x), e^{+-2 pi w}
+};
+
+// How a tile's pairs are evaluated: every pair in the reference form (all
+// circular x distances provably below kTrigMinDx, or no far-pair factors),
+// every pair in the far form (all provably at or above it), or per pair.
+enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2 };
 
 __device__ __forceinline__ double rotate_down1(double v, int lane)
 {
@@ -81,14 +95,10 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
 // exactly, so for it the target-minus-source term already equals the
 // lower-first term with its sign; dx = +-0 differs only by the far images'
 // rounding residue (below 1e-17).
-//
-// Trig (fast field, far tiles): cos(2 pi dy) from the dislocations' cos and
-// sin of 2 pi y by angle addition (cai, sai: target; caj, saj: source).
-template <int Mode, bool Perm, bool Trig = false>
+template <int Mode, bool Perm>
 __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int io, int jo, double sxm, double sym,
                                             double sbm, bool lower_by_lane, double &r2,
-                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t,
-                                            double cai = 0.0, double sai = 0.0, double caj = 0.0, double saj = 0.0)
+                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t)
 {
     const bool lower = (Perm && Mode != SDDDST_FIELD_FAST) ? io < jo : (Perm ? true : lower_by_lane);
     const double d = xi - sxm, e = yi - sym;
@@ -102,29 +112,99 @@ __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int
         dx = sdfield::normalize(lower ? d : -d);
         dy = sdfield::normalize(lower ? e : -e);
     }
-    double t;
-    if constexpr (Trig) t = bI * sbm * sdfield::xy_fast_cy(dx, dy, sdfield::cos_diff(cai, sai, caj, saj), e2t);
-    else t = bI * sbm * field_tau<Mode>(dx, dy, sct, e2t);
+    const double t = bI * sbm * field_tau<Mode>(dx, dy, sct, e2t);
     r2 = dx * dx + dy * dy;
     return lower ? t : -t;
+}
+
+// The far-pair factors of one dislocation (a target lane's registers).
+struct FarSelf {
+    double ca = 0.0, sa = 0.0, w = 0.0, ep = 0.0, em = 0.0;
+};
+
+__device__ __forceinline__ FarSelf far_load(const FarData &f, int i)
+{
+    FarSelf r;
+    r.ca = f.ca[i];
+    r.sa = f.sa[i];
+    r.w = f.w[i];
+    r.ep = f.ep[i];
+    r.em = f.em[i];
+    return r;
+}
+
+__device__ __forceinline__ void far_stage(TileLds &L, int l, const FarSelf &v)
+{
+    L.ca[l] = v.ca;
+    L.sa[l] = v.sa;
+    L.w[l] = v.w;
+    L.ep[l] = v.ep;
+    L.em[l] = v.em;
+}
+
+// The tile kind from the wrapped x of the target lanes (tw) and of the source
+// slots (sw), NaN for lanes outside the system; the same on every lane.  The
+// differences s = tw_i - sw_j of the tile lie in [lo, hi] (rounding is
+// monotone), and circ() peaks at half-integers and vanishes at integers, so
+// its extremes over [lo, hi] bound every pair's circ(s).
+__device__ __forceinline__ int tile_kind(double tw, double sw)
+{
+    double tmin = tw, tmax = tw, smin = sw, smax = sw;
+    for (int off = 32; off > 0; off >>= 1) {
+        tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
+        tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
+        smin = fmin(smin, __shfl_xor(smin, off, 64));
+        smax = fmax(smax, __shfl_xor(smax, off, 64));
+    }
+    const double lo = tmin - smax, hi = tmax - smin;
+    const double clo = sdfield::circ(lo), chi = sdfield::circ(hi);
+    const bool has_int = (lo <= 0.0 && hi >= 0.0) || (lo <= 1.0 && hi >= 1.0) || (lo <= -1.0 && hi >= -1.0);
+    const bool has_half = (lo <= 0.5 && hi >= 0.5) || (lo <= -0.5 && hi >= -0.5);
+    const double mind = has_int ? 0.0 : fmin(clo, chi);
+    const double maxd = has_half ? 0.5 : fmax(clo, chi);
+    const int kind = mind >= sdfield::kTrigMinDx ? kFarTile : (maxd < sdfield::kTrigMinDx ? kNearTile : kMixedTile);
+    return __builtin_amdgcn_readfirstlane(kind);
+}
+
+// One pair of a far or mixed tile: the far form when circ(w_i - w_j) >=
+// kTrigMinDx (always in a far tile), else the reference form.  r2 is set
+// for the reference form only (far pairs are never tolerance candidates:
+// kTrigMinDx^2 exceeds every r2_keep the caller passes on this path).
+template <int Mode, bool Perm, int Kind>
+__device__ __forceinline__ double tile_term(double xi, double yi, double bI, int io, int jm, const TileLds &L, int m,
+                                            const FarSelf &fi, bool lower_by_lane, double &r2,
+                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t)
+{
+    if constexpr (Kind == kNearTile) {
+        return pair_term<Mode, Perm>(xi, yi, bI, io, jm, L.x[m], L.y[m], L.b[m], lower_by_lane, r2, sct, e2t);
+    } else {
+        const double wj = L.w[m];
+        const bool far = Kind == kFarTile || sdfield::circ(fi.w - wj) >= sdfield::kTrigMinDx;
+        if (far) {
+            r2 = 1.0;
+            const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
+            return bI * L.b[m] * sdfield::far_pair(fi.w, wj, fi.ep, fi.em, L.ep[m], L.em[m], cy);
+        }
+        return pair_term<Mode, Perm>(xi, yi, bI, io, jm, L.x[m], L.y[m], L.b[m], lower_by_lane, r2, sct, e2t);
+    }
 }
 
 // Called by one whole 64-lane workgroup.  Full: both blocks lie inside n
 // and bi < bj, so every pair of the tile counts.  sct / e2t (fast field
 // only) may point at LDS copies of the tables written before the call: the
-// tile's barrier orders them before the first read.  With ytc / yts (the cos
-// and sin of 2 pi y per dislocation; fast field, full tiles) a tile whose
-// two blocks lie at least kTrigMinDx apart in wrapped x -- decided from the
-// current positions, the same for every lane -- takes cos(2 pi dy) by angle
-// addition instead of evaluating cos per pair.
+// tile's barrier orders them before the first read.  With far-pair factors
+// (fast field) a pair whose circular x distance is at least kTrigMinDx takes
+// the far form (cos by angle addition, the exponentials from per-dislocation
+// factors); the choice is made per pair, so a pair's value does not depend
+// on the tile that holds it, and the tile's x intervals decide once whether
+// all, none or some of its pairs are far.
 template <int Mode, bool Perm = false, bool Full = false>
 __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
                                      const int *__restrict__ perm, double *__restrict__ partial,
                                      unsigned long long *__restrict__ minr2, double r2_keep,
                                      const double *__restrict__ sct = sdlibm_sincos_tab,
-                                     const uint64_t *__restrict__ e2t = sdlibm_exp2_tab,
-                                     const double *__restrict__ ytc = nullptr, const double *__restrict__ yts = nullptr)
+                                     const uint64_t *__restrict__ e2t = sdlibm_exp2_tab, FarData fd = FarData())
 {
     double *sx = L.x, *sy = L.y, *sb = L.b;
     int *sid = L.id;
@@ -133,43 +213,29 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
     const int q = bj * kTile + l;              // source position held in slot l
     const int io = Perm ? (p < n ? perm[p] : 0) : p;
     const int jo = Perm ? (q < n ? perm[q] : 0) : q;
-    const double xs = q < n ? x[jo] : 0.0;
-    sx[l] = xs;
+    sx[l] = q < n ? x[jo] : 0.0;
     sy[l] = q < n ? y[jo] : 0.0;
     sb[l] = q < n ? b[jo] : 0.0;
     sid[l] = jo;
     const double xi = p < n ? x[io] : 0.0;
     const double yi = p < n ? y[io] : 0.0;
     const double bI = p < n ? b[io] : 0.0;
-    bool far = false;
-    double cai = 0.0, sai = 0.0;
-    if constexpr (Mode == SDDDST_FIELD_FAST && Full) {
-        if (ytc) {
-            // wrapped-x intervals of the two blocks; the smallest circular
-            // distance between them bounds |dx| of every pair of the tile
-            const double tw = xi - __builtin_rint(xi), sw = xs - __builtin_rint(xs);
-            double tmin = tw, tmax = tw, smin = sw, smax = sw;
-            for (int off = 32; off > 0; off >>= 1) {
-                tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
-                tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
-                smin = fmin(smin, __shfl_xor(smin, off, 64));
-                smax = fmax(smax, __shfl_xor(smax, off, 64));
-            }
-            const double lo = tmin - smax, hi = tmax - smin;
-            const double mind = (lo <= 0.0 && hi >= 0.0) ? 0.0 : (hi < 0.0 ? fmin(-hi, 1.0 + lo) : fmin(lo, 1.0 - hi));
-            far = __builtin_amdgcn_readfirstlane((int)(mind >= sdfield::kTrigMinDx)) != 0;
-            if (far) {
-                L.ca[l] = ytc[jo];
-                L.sa[l] = yts[jo];
-                cai = ytc[io];
-                sai = yts[io];
+    int kind = kNearTile;
+    FarSelf fi;
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        if (fd.w) {
+            const double nan = __builtin_nan("");
+            kind = tile_kind(p < n ? fd.w[io] : nan, q < n ? fd.w[jo] : nan);
+            if (kind != kNearTile) {
+                far_stage(L, l, q < n ? far_load(fd, jo) : FarSelf());
+                if (p < n) fi = far_load(fd, io);
             }
         }
     }
     __syncthreads();
     const bool diag = (bi == bj);
-    auto sweep = [&](auto trig) {
-        constexpr bool T = decltype(trig)::value;
+    auto sweep = [&](auto kind_tag) {
+        constexpr int K = decltype(kind_tag)::value;
         double row = 0.0, col = 0.0;
         for (int s = 0; s < kTile; ++s) {
             const int m = (l + s) & 63;
@@ -178,17 +244,13 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
             // the source's original index: for the lower-first rule (exact
             // field) and for the rare tolerance candidates
             const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
-            double r2, t;
+            double r2;
             // without Perm the target has the lower index in every tile it meets
-            if constexpr (T)
-                t = pair_term<Mode, Perm, true>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2, sct, e2t, cai, sai,
-                                                L.ca[m], L.sa[m]);
-            else
-                t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], true, r2, sct, e2t);
+            const double t = tile_term<Mode, Perm, K>(xi, yi, bI, io, jm, L, m, fi, true, r2, sct, e2t);
             if (valid) {
                 row += t;
                 col -= t;
-                if (r2 < r2_keep) {
+                if (K != kFarTile && r2 < r2_keep) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
                     atomicMin(&minr2[io], bits);
                     atomicMin(&minr2[Perm ? sid[m] : j], bits);
@@ -203,11 +265,12 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
             if (q < n) partial[(size_t)bi * n + q] = col;
         }
     };
-    if constexpr (Mode == SDDDST_FIELD_FAST && Full) {
-        if (far) sweep(std::true_type{});
-        else sweep(std::false_type{});
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        if (kind == kFarTile) sweep(std::integral_constant<int, kFarTile>{});
+        else if (kind == kMixedTile) sweep(std::integral_constant<int, kMixedTile>{});
+        else sweep(std::integral_constant<int, kNearTile>{});
     } else {
-        sweep(std::false_type{});
+        sweep(std::integral_constant<int, kNearTile>{});
     }
 }
 
@@ -234,7 +297,7 @@ __device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const doubl
                                           const double *__restrict__ b, const int *__restrict__ perm,
                                           double *__restrict__ partial, unsigned long long *__restrict__ minr2,
                                           double r2_keep, const double *__restrict__ sct = sdlibm_sincos_tab,
-                                          const uint64_t *__restrict__ e2t = sdlibm_exp2_tab)
+                                          const uint64_t *__restrict__ e2t = sdlibm_exp2_tab, FarData fd = FarData())
 {
     double *sx = L.x, *sy = L.y, *sb = L.b;
     int *sid = L.id;
@@ -248,29 +311,52 @@ __device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const doubl
     sy[l] = yi;
     sb[l] = bI;
     sid[l] = io;
-    __syncthreads();
-    double row = 0.0, col = 0.0;
-    for (int s = 1; s <= kTile / 2; ++s) {
-        const int m = (l + s) & 63;
-        const int j = bk * kTile + m;
-        const bool valid = (p < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
-        const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
-        double r2;
-        const double t = pair_term<Mode, Perm>(xi, yi, bI, io, jm, sx[m], sy[m], sb[m], m > l, r2, sct, e2t);
-        if (valid) {
-            row += t;
-            col -= t;
-            if (r2 < r2_keep) {
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
-                atomicMin(&minr2[io], bits);
-                atomicMin(&minr2[Perm ? sid[m] : j], bits);
+    // a block against itself: never all far (its x interval overlaps itself);
+    // mixed when its members lie kTrigMinDx or more apart
+    int kind = kNearTile;
+    FarSelf fi;
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        if (fd.w) {
+            const double wv = p < n ? fd.w[io] : __builtin_nan("");
+            kind = tile_kind(wv, wv);
+            if (kind != kNearTile) {
+                if (p < n) fi = far_load(fd, io);
+                far_stage(L, l, fi);
             }
         }
-        col = rotate_down1(col, l);
     }
-    // after sweeps 1..32 the column sum of source m sits in lane (m - 33) mod 64
-    const double cs = __shfl(col, (l + 31) & 63, 64);
-    if (p < n) partial[(size_t)bk * n + p] = row + cs;
+    __syncthreads();
+    auto sweep = [&](auto kind_tag) {
+        constexpr int K = decltype(kind_tag)::value;
+        double row = 0.0, col = 0.0;
+        for (int s = 1; s <= kTile / 2; ++s) {
+            const int m = (l + s) & 63;
+            const int j = bk * kTile + m;
+            const bool valid = (p < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
+            const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
+            double r2;
+            const double t = tile_term<Mode, Perm, K>(xi, yi, bI, io, jm, L, m, fi, m > l, r2, sct, e2t);
+            if (valid) {
+                row += t;
+                col -= t;
+                if (r2 < r2_keep) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
+                    atomicMin(&minr2[io], bits);
+                    atomicMin(&minr2[Perm ? sid[m] : j], bits);
+                }
+            }
+            col = rotate_down1(col, l);
+        }
+        // after sweeps 1..32 the column sum of source m sits in lane (m - 33) mod 64
+        const double cs = __shfl(col, (l + 31) & 63, 64);
+        if (p < n) partial[(size_t)bk * n + p] = row + cs;
+    };
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        if (kind == kMixedTile) sweep(std::integral_constant<int, kMixedTile>{});
+        else sweep(std::integral_constant<int, kNearTile>{});
+    } else {
+        sweep(std::integral_constant<int, kNearTile>{});
+    }
 }
 
 // r^2 below which a pair can change a tolerance: 400 prec^2 with a margin

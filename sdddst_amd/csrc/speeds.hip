@@ -56,7 +56,7 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
                                                    const double *__restrict__ y, const double *__restrict__ b,
                                                    const int *__restrict__ perm, double *__restrict__ partial,
                                                    unsigned long long *__restrict__ minr2, double r2_keep,
-                                                   const double *__restrict__ ytc, const double *__restrict__ yts)
+                                                   pairtile::FarData fd)
 {
     const int u = (int)blockIdx.x;
     __shared__ pairtile::TileLds L;
@@ -80,21 +80,21 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
         int bi, bj;
         pairtile::strict_coords(u, bi, bj);
         if ((bj + 1) * kTile <= n)
-            pairtile::tile<Mode, true, true>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t, ytc,
-                                             yts);
-        else pairtile::tile<Mode, true, false>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
+            pairtile::tile<Mode, true, true>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t, fd);
+        else pairtile::tile<Mode, true, false>(L, bi, bj, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t, fd);
         return;
     }
     const int bk = 2 * (u - nfull);
-    pairtile::diag_tile<Mode, true>(L, bk, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
+    pairtile::diag_tile<Mode, true>(L, bk, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t, fd);
     if (bk + 1 < nb) {
         __syncthreads();
-        pairtile::diag_tile<Mode, true>(L, bk + 1, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t);
+        pairtile::diag_tile<Mode, true>(L, bk + 1, n, x, y, b, perm, partial, minr2, r2_keep, sct, e2t, fd);
     }
 }
 
-// cos and sin of 2 pi y per dislocation (y wrapped first), for the far
-// tiles of the fast field.  y changes only by upload.
+// Far-pair factors of the fast field: cos and sin of 2 pi y per dislocation
+// (y wrapped first; y changes only by upload), and the wrapped x with
+// e^{+-2 pi x} (per speeds call).
 __global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict__ yc, double *__restrict__ ys)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -103,6 +103,18 @@ __global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict_
     sdfield::ytrig(y[i], cv, sv);
     yc[i] = cv;
     ys[i] = sv;
+}
+
+__global__ void k_xexp(int n, const double *__restrict__ x, double *__restrict__ w, double *__restrict__ ep,
+                       double *__restrict__ em)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double wv, pv, mv;
+    sdfield::xexp(x[i], wv, pv, mv);
+    w[i] = wv;
+    ep[i] = pv;
+    em[i] = mv;
 }
 
 // Sort keys of the pair-tile order: the bin of the wrapped x among nb equal
@@ -242,7 +254,8 @@ int pair_order_alloc(Context &c)
                                               key_bits(c.nb), c.stream));
     c.sort_tmp_bytes = bytes > 0 ? bytes : 1;
     SD_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes));
-    for (double **p : {&c.ytc, &c.yts}) SD_HIP(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
+    for (double **p : {&c.ytc, &c.yts, &c.xw, &c.xep, &c.xem})
+        SD_HIP(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
     return SDDDST_OK;
 }
 
@@ -297,15 +310,29 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         c.order_age = 0;
     }
     ++c.order_age;
+    // far-pair factors (fast field): every pair at circular x distance >=
+    // kTrigMinDx takes the far form, which skips the tolerance candidates, so
+    // they are used only when no such pair can be one
+    pairtile::FarData fd;
+    const double r2_keep = pairtile::tolerance_r2(minPrecSqr);
+    if (c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kTrigMinDx * sdfield::kTrigMinDx) {
+        hipLaunchKernelGGL(k_xexp, dim3(blocks), dim3(threads), 0, c.stream, n, c.x[slot], c.xw, c.xep, c.xem);
+        SD_HIP(hipGetLastError());
+        fd.ca = c.ytc;
+        fd.sa = c.yts;
+        fd.w = c.xw;
+        fd.ep = c.xep;
+        fd.em = c.xem;
+    }
     if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
-                           pairtile::tolerance_r2(minPrecSqr), c.ytc, c.yts);
+                           pairtile::tolerance_r2(minPrecSqr), pairtile::FarData());
     else
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
-                           pairtile::tolerance_r2(minPrecSqr), c.ytc, c.yts);
+                           pairtile::tolerance_r2(minPrecSqr), fd);
     SD_HIP(hipGetLastError());
     if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.perm, c.partial,

@@ -38,17 +38,18 @@ void mc_batch1(int fn, long n, const double *x, double *out)
     }
 }
 
-// The pair kernel's far-tile term: xy_fast with cos(2 pi dy) from the two
-// dislocations' ytrig values (angle addition), for positions (xi, yi), (xj, yj).
-void mc_batch_pair_trig(long n, const double *xi, const double *yi, const double *xj, const double *yj,
-                        double *out)
+// The pair kernel's far-pair term (circ(wi - wj) >= kTrigMinDx): cos by
+// angle addition and the exponentials from per-dislocation factors.
+void mc_batch_pair_far(long n, const double *xi, const double *yi, const double *xj, const double *yj,
+                       double *out)
 {
     for (long k = 0; k < n; ++k) {
-        double ci, si, cj, sj;
+        double ci, si, cj, sj, wi, epi, emi, wj, epj, emj;
         sdfield::ytrig(yi[k], ci, si);
         sdfield::ytrig(yj[k], cj, sj);
-        const double dx = sdfield::wrap_fast(xi[k] - xj[k]), dy = sdfield::wrap_fast(yi[k] - yj[k]);
-        out[k] = sdfield::xy_fast_cy(dx, dy, sdfield::cos_diff(ci, si, cj, sj));
+        sdfield::xexp(xi[k], wi, epi, emi);
+        sdfield::xexp(xj[k], wj, epj, emj);
+        out[k] = sdfield::far_pair(wi, wj, epi, emi, epj, emj, sdfield::cos_diff(ci, si, cj, sj));
     }
 }
 

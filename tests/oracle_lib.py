@@ -132,7 +132,7 @@ def mathcheck():
     lib = ctypes.CDLL(MATHCHECK_SO)
     lib.mc_batch1.argtypes = [ctypes.c_int, L, DP, DP]
     lib.mc_batch2.argtypes = [ctypes.c_int, L, DP, DP, DP]
-    lib.mc_batch_pair_trig.argtypes = [L, DP, DP, DP, DP, DP]
+    lib.mc_batch_pair_far.argtypes = [L, DP, DP, DP, DP, DP]
     lib.mc_normalize.restype = D
     lib.mc_normalize.argtypes = [D]
     lib.mc_powi.restype = D

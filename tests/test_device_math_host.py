@@ -83,13 +83,14 @@ def test_field_fast_and_derivative():
     assert np.max(np.abs(df - xyd) / (np.abs(xyd) + 1.0)) < 1e-14
 
 
-def test_far_tile_angle_addition_term():
-    # the pair kernel's far tiles (wrapped |dx| >= 0.2) take cos(2 pi dy) as
-    # cos a_i cos a_j + sin a_i sin a_j from per-dislocation values: within
-    # 8e-15 of the reference's term relative to |tau| + 1 (3e6-sample max
-    # observed 5.6e-15; at |dx| >= 0.12 it would be 1.3e-14), and odd under
-    # swapping the two dislocations bit for bit, which the x-ordered tiles
-    # rely on
+def test_far_pair_term():
+    # the pair kernel's far pairs (circular |dx| >= 0.2) take cos(2 pi dy) as
+    # cos a_i cos a_j + sin a_i sin a_j and e^{+-2 pi dx} as products of
+    # per-dislocation e^{+-2 pi x}: within 1.2e-14 of the reference's term
+    # relative to |tau| + 1 (3e6-sample max observed 7.8e-15, 99.99th
+    # percentile 4.2e-15; at |dx| >= 0.12 the max would be 2.1e-14), and odd
+    # under swapping the two dislocations bit for bit, so a pair's value does
+    # not depend on which one a tile holds as the target
     mc = O.mathcheck()
     rng = np.random.default_rng(12)
     n = 400_000
@@ -97,8 +98,8 @@ def test_far_tile_angle_addition_term():
     xi[: n // 4] += rng.integers(-3, 4, n // 4)
     yi[: n // 4] += rng.integers(-3, 4, n // 4)
     out, swp = np.empty(n), np.empty(n)
-    mc.mc_batch_pair_trig(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(out))
-    mc.mc_batch_pair_trig(n, O.dp(xj), O.dp(yj), O.dp(xi), O.dp(yi), O.dp(swp))
+    mc.mc_batch_pair_far(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(out))
+    mc.mc_batch_pair_far(n, O.dp(xj), O.dp(yj), O.dp(xi), O.dp(yi), O.dp(swp))
     dx, dy = np.empty(n), np.empty(n)
     ex, ey = xi - xj, yi - yj
     mc.mc_batch1(6, n, O.dp(ex), O.dp(dx))
@@ -107,9 +108,8 @@ def test_far_tile_angle_addition_term():
     far = np.abs(dx) >= 0.2
     assert far.mean() > 0.55
     err = np.abs(out - ref) / (np.abs(ref) + 1.0)
-    assert err[far].max() < 8e-15
-    half = np.abs(np.abs(dx) - 0.5) < 1e-12   # wrap tie: +0.5 for both orders
-    assert np.array_equal(out[far & ~half].view(np.int64), (-swp[far & ~half]).view(np.int64))
+    assert err[far].max() < 1.2e-14
+    assert np.array_equal(out[far].view(np.int64), (-swp[far]).view(np.int64))
 
 
 def test_integer_powers_correctly_rounded():

@@ -156,8 +156,12 @@ SDLIBM_FN double cos_diff(double ci, double si, double cj, double sj) { return _
 constexpr double kTrigMinDx = 0.2;
 
 // The nine far images of xy_fast (see below) as sg(dx) F(|dx|) without the
-// sign and the factor 2 pi^2, from t = e^{2 pi |dx|} and ti = 1 / t.
-SDLIBM_FN double far_images(double dx, double cy, double t, double ti)
+// sign and the factor 2 pi^2, from t = e^{2 pi |dx|} and ti = 1 / t: the
+// four within 2.5 cells as the fraction P / Q, the five beyond as the sum S.
+struct FarParts {
+    double P, Q, S;
+};
+SDLIBM_FN FarParts far_image_parts(double dx, double cy, double t, double ti)
 {
     // Far images in terms of ad = |dx| and the sign sg of dx (every image
     // of the reference's sum is odd in dx, so the sum is sg * F(ad)):
@@ -199,7 +203,13 @@ SDLIBM_FN double far_images(double dx, double cy, double t, double ti)
     S = __builtin_fma(((1.0 - ad) * (ad + 4.0)) * aA4, __builtin_fma(c2, aA4, cy), S);
     S = __builtin_fma((ad - 4.0) * aB4, __builtin_fma(c2, aB4, cy), S);
     S = __builtin_fma((ad * (ad - 5.0)) * aB5, __builtin_fma(c2, aB5, cy), S);
-    return P / Q + S;
+    return FarParts{P, Q, S};
+}
+
+SDLIBM_FN double far_images(double dx, double cy, double t, double ti)
+{
+    const FarParts f = far_image_parts(dx, cy, t, ti);
+    return f.P / f.Q + f.S;
 }
 
 // Production form of xy() (see header comment) with cos(2 pi dy) supplied
@@ -231,17 +241,20 @@ SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__r
     return near + far * (2.0 * kPiSq);
 }
 
-// The production field for a pair with |dx| >= kTrigMinDx (far-tile pairs of
-// the pair kernel): cos(2 pi dy) and t = e^{2 pi |dx|}, ti = e^{-2 pi |dx|}
+// The production field for a pair with |dx| >= kTrigMinDx (far pairs of the
+// pair kernel): cos(2 pi dy) and t = e^{2 pi |dx|}, ti = e^{-2 pi |dx|}
 // supplied by the caller (from per-dislocation factors, within a few ulps),
-// cosh(2 pi dx) = (t + ti) / 2 as the reference forms it on its exp branch.
+// cosh(2 pi dx) = (t + ti) / 2.  The near image dx (c cy - 1) / (c - cy)^2
+// and the four inner far images P / Q share one division (cosh - cos >= 0.9
+// here, so the near image needs no care against cancellation).
 SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
 {
     const double c0 = 0.5 * t + 0.5 * ti;
-    const double near = f_given_cosh(dx, c0, cy);
-    double far = far_images(dx, cy, t, ti);
-    far = dx < 0 ? -far : far;
-    return near + far * (2.0 * kPiSq);
+    const double d = c0 - cy;
+    const double A = dx * (c0 * cy - 1.0), B = d * d;
+    const FarParts f = far_image_parts(dx, cy, t, ti);
+    const double P = dx < 0 ? -f.P : f.P, S = dx < 0 ? -f.S : f.S;
+    return (__builtin_fma(A, f.Q, P * B) / (B * f.Q) + S) * (2.0 * kPiSq);
 }
 
 // Per-dislocation factors of the far pairs: the wrapped x w = x - rint(x)
@@ -271,7 +284,8 @@ SDLIBM_FN double far_pair(double wi, double wj, double epi, double emi, double e
     const double gk = k == 0.0 ? 1.0 : (k > 0.0 ? sdlibm_exp_m2pik[1] : sdlibm_exp_2pik[1]);
     const double gmk = k == 0.0 ? 1.0 : (k > 0.0 ? sdlibm_exp_2pik[1] : sdlibm_exp_m2pik[1]);
     const double pp = (epi * emj) * gk, pm = (emi * epj) * gmk;
-    return dx >= 0.0 ? xy_far(dx, cy, pp, pm) : xy_far(dx, cy, pm, pp);
+    const bool pos = dx >= 0.0;
+    return xy_far(dx, cy, pos ? pp : pm, pos ? pm : pp);
 }
 
 // Production form of xy() (see header comment).

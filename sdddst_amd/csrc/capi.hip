@@ -2,6 +2,8 @@
 // stream ownership, and the thin calls into the kernel launchers.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -63,6 +65,7 @@ static void free_all(Context &c)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
+    if (c.sig) (void)hipHostFree(c.sig);
     for (Context::Timer *t : {&c.t_pair, &c.t_speeds, &c.t_jac}) {
         for (hipEvent_t e : t->begin) (void)hipEventDestroy(e);
         for (hipEvent_t e : t->end) (void)hipEventDestroy(e);
@@ -94,6 +97,16 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
     SD_HIP(hipGetDeviceCount(&ndev));
     SD_CHECK(ndev > 0, SDDDST_ERR_HIP, "no HIP device");
     SD_HIP(hipGetDevice(&c.device));
+    // The step waits on the device for small results many times per step
+    // (Krylov coefficients, step acceptance): spin on completion instead of
+    // yielding the thread, which otherwise adds a wake-up latency to every
+    // wait.  Best effort: a runtime already active with other flags keeps them.
+    {
+        unsigned flags = 0;
+        if (hipGetDeviceFlags(&flags) == hipSuccess && (flags & hipDeviceScheduleMask) != hipDeviceScheduleSpin)
+            (void)hipSetDeviceFlags((flags & ~hipDeviceScheduleMask) | hipDeviceScheduleSpin);
+        (void)hipGetLastError();
+    }
     c.n = n;
     c.pc = pc;
     c.p = *params;
@@ -116,6 +129,9 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
         (rc = pair_order_alloc(c)))
         return rc;
     SD_HIP(hipHostMalloc((void **)&c.hs, sizeof(DevScalars), hipHostMallocDefault));
+    SD_HIP(hipHostMalloc((void **)&c.sig, sizeof(HostSignal), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(c.sig, 0, sizeof(HostSignal));
+    SD_HIP(hipHostGetDevicePointer((void **)&c.sig_dev, c.sig, 0));
     SD_HIP(hipMemsetAsync(c.dvec, 0, sizeof(double) * n, c.stream));
     SD_HIP(hipMemsetAsync(c.x[0], 0, sizeof(double) * n, c.stream));
     rc = launch_reset_tolerance(c);
@@ -517,8 +533,7 @@ int sdddst_error_control(sdddst_ctx *ctx, double h, double hmax, int *accept, do
     Context &c = ctx->c;
     int rc = launch_error_control(c, h, hmax);
     if (rc) return rc;
-    SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
+    if ((rc = host_wait(c, c.ds, (int)(sizeof(DevScalars) / 8), c.hs))) return rc;
     if (accept) *accept = c.hs->accept;
     if (h_next) *h_next = c.hs->h_next;
     if (max_ratio) *max_ratio = c.hs->max_ratio;
@@ -533,8 +548,7 @@ int sdddst_speed_sums(sdddst_ctx *ctx, int buf, double *sum_abs, double *sum_sq,
     SD_CHECK(valid_buf(buf), SDDDST_ERR_ARG, "bad buffer");
     int rc = launch_speed_sums(c, buf);
     if (rc) return rc;
-    SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
+    if ((rc = host_wait(c, c.ds, (int)(sizeof(DevScalars) / 8), c.hs))) return rc;
     if (sum_abs) *sum_abs = c.hs->sum_abs;
     if (sum_sq) *sum_sq = c.hs->sum_sq;
     if (sum_bv) *sum_bv = c.hs->sum_bv;
@@ -548,8 +562,7 @@ int sdddst_strain_increment(sdddst_ctx *ctx, int old_slot, int new_slot, double 
     SD_CHECK(valid_slot(old_slot) && valid_slot(new_slot) && out, SDDDST_ERR_ARG, "bad arguments");
     int rc = launch_strain(c, old_slot, new_slot);
     if (rc) return rc;
-    SD_HIP(hipMemcpyAsync(c.hs, c.ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
+    if ((rc = host_wait(c, c.ds, (int)(sizeof(DevScalars) / 8), c.hs))) return rc;
     *out = c.hs->strain;
     return SDDDST_OK;
 }

@@ -45,6 +45,17 @@ struct DevScalars {
     double sum_abs, sum_sq, sum_bv, strain;
 };
 
+// Completion signal of the host waits: a one-lane kernel copies a few small
+// device results into this block of host-coherent mapped memory and then
+// publishes a sequence number with a system-scope release; the host polls
+// the number instead of blocking in the runtime.
+struct HostSignal {
+    uint64_t seq;
+    uint64_t pad;
+    uint64_t payload[14];
+};
+static_assert(sizeof(DevScalars) % 8 == 0 && sizeof(DevScalars) <= 14 * 8, "DevScalars fits the signal payload");
+
 struct Context {
     int n = 0, pc = 0;
     sdddst_params p{};
@@ -114,6 +125,9 @@ struct Context {
     // controller scalars
     DevScalars *ds = nullptr;
     DevScalars *hs = nullptr;           // pinned host mirror
+    HostSignal *sig = nullptr;          // host view of the completion signal (mapped, coherent)
+    HostSignal *sig_dev = nullptr;      // its device address
+    uint64_t sig_seq = 0;
     // solver
     int solver = SDDDST_SOLVER_DENSE_LU;
     double solver_tol = 1e-10;
@@ -150,6 +164,11 @@ void compute_cutoff(Context &c);
 int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol);
 int pair_order_alloc(Context &c);   // buffers of the x ordering of the pair tiles
 int launch_ytrig(Context &c);       // cos / sin of 2 pi y after an upload of y
+// Waits until the work enqueued on the stream so far is done, copying `words`
+// 64-bit words from device address src (if any) to dst on the way: a signal
+// kernel publishes them in HostSignal and the host polls it (a bounded spin,
+// then hipStreamSynchronize, which also reports device errors).
+int host_wait(Context &c, const void *src = nullptr, int words = 0, void *dst = nullptr);
 int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern);
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
 int launch_newton_update(Context &c, int slot);

@@ -615,8 +615,7 @@ static int build_pattern(Context &c, int slot, double h)
     int rc = launch_scan(c, n, c.colcount, c.Ap, d_total);
     if (rc) return rc;
     long long total = 0;
-    SD_HIP(hipMemcpyAsync(&total, d_total, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
+    if ((rc = host_wait(c, d_total, 1, &total))) return rc;
     SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
     rc = ensure_capacity(c, total);
     if (rc) return rc;
@@ -682,9 +681,9 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
         hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, h, c.Ap, c.diag,
                            c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec, c.colinfo);
         SD_HIP(hipGetLastError());
-        int flag = 0;
-        SD_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        SD_HIP(hipStreamSynchronize(c.stream));
+        int flags2[2] = {0, 0};   // d_flags[0..1] as one word; [0] is the zero-value flag
+        if ((rc = host_wait(c, c.d_flags, 1, flags2))) return rc;
+        const int flag = flags2[0];
         c.jac_symmetric = flag == 0;
         c.raw_valid = flag == 0;
         c.raw_h = h;

@@ -166,8 +166,8 @@ __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__res
             out[v] = t;
             const double a = (acc_out && v < nv) ? (accumulate ? acc_out[v] + t : t) : 0.0;
             if (acc_out && v < nv) acc_out[v] = a;
-            // the host's copy, in mapped pinned memory (read after the stream
-            // synchronises)
+            // the host's copy, in mapped pinned memory (read after the host's
+            // completion wait)
             if (host_out) {
                 host_out[v] = t;
                 if (acc_out && v < nv) host_out[host_acc_off + v] = a;
@@ -445,7 +445,8 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *ne
     // [m+2, m+2+nv)) into the mapped pinned buffer
     launch_proj(c, f, nv, f.V, f.w, h1, h2, hsum, 1, 1, f.Hdev, f.m + 2);
     SD_HIP(hipGetLastError());
-    SD_HIP(hipStreamSynchronize(c.stream));
+    int rc = host_wait(c);
+    if (rc) return rc;
     const double *hc = f.H;
     for (int v = 0; v < nv; ++v) hhost[v] = hc[f.m + 2 + v];
     double s2 = 0.0;
@@ -463,7 +464,7 @@ static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *ne
     hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, h2, f.w);
     launch_proj(c, f, 0, f.V, f.w, nullptr, hsum + nv, nullptr, 0, 1, f.Hdev);
     SD_HIP(hipGetLastError());
-    SD_HIP(hipStreamSynchronize(c.stream));
+    if ((rc = host_wait(c))) return rc;
     hhost[nv] = sqrt(f.H[0]);
     if (next && hhost[nv] > 0)
         hipLaunchKernelGGL(k_scale_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, 1.0 / hhost[nv], f.w, next);
@@ -475,7 +476,8 @@ static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
     launch_proj(c, f, 0, f.V, const_cast<double *>(v), nullptr, f.hcol, nullptr, 0, 1, f.Hdev);
     SD_HIP(hipGetLastError());
-    SD_HIP(hipStreamSynchronize(c.stream));
+    const int rc = host_wait(c);
+    if (rc) return rc;
     *out = sqrt(f.H[0]);
     return SDDDST_OK;
 }

@@ -764,8 +764,7 @@ static int spai_forward(Context &c, const Fgmres &f)
     SD_HIP(hipGetLastError());
     if ((rc = launch_scan(c, n, s.cnt, s.Cp, s.d_total))) return rc;
     long long ne = 0;
-    SD_HIP(hipMemcpyAsync(&ne, s.d_total, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
+    if ((rc = host_wait(c, s.d_total, 1, &ne))) return rc;
     if ((rc = spai_alloc_edges(s, ne))) return rc;
     s.ne = ne;
     hipLaunchKernelGGL(k_core_fill, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.jcore, f.bcol, s.Cp,

@@ -12,6 +12,9 @@
 //                  with the fast field.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstring>
+
 #include "context.h"
 #include "field.h"
 #include "step_core.h"
@@ -194,6 +197,15 @@ __global__ __launch_bounds__(1024) void k_strain_tree(int n, const double *__res
     if (t == 0) out->strain = r;
 }
 
+// One lane: copy the payload, then publish seq at system scope (the copies
+// and every earlier kernel's results become visible to the host first).
+__global__ void k_signal(const uint64_t *__restrict__ src, int words, HostSignal *__restrict__ sig, uint64_t seq)
+{
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < words; ++k) sig->payload[k] = src[k];
+    __hip_atomic_store(&sig->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_fill(double *p, double v, int n)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -246,6 +258,31 @@ int launch_strain(Context &c, int old_slot, int new_slot)
     else
         hipLaunchKernelGGL(k_strain, dim3(1), dim3(256), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
     SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int host_wait(Context &c, const void *src, int words, void *dst)
+{
+    SD_CHECK(words >= 0 && words <= 14 && (words == 0 || (src && dst)), SDDDST_ERR_ARG, "bad host_wait payload");
+    const uint64_t seq = ++c.sig_seq;
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, c.stream, static_cast<const uint64_t *>(src), words, c.sig_dev,
+                       seq);
+    SD_HIP(hipGetLastError());
+    // poll for up to 50 ms (a step's waits are microseconds to a few ms), then
+    // let the runtime block: that also surfaces a failed kernel as an error
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned spins = 0;
+    while (__atomic_load_n(&c.sig->seq, __ATOMIC_ACQUIRE) < seq) {
+        __builtin_ia32_pause();
+        if ((++spins & 4095u) == 0 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+            SD_HIP(hipStreamSynchronize(c.stream));
+            SD_CHECK(__atomic_load_n(&c.sig->seq, __ATOMIC_ACQUIRE) >= seq, SDDDST_ERR_HIP,
+                     "device completion signal %llu not seen", (unsigned long long)seq);
+            break;
+        }
+    }
+    if (words) std::memcpy(dst, const_cast<const uint64_t *>(c.sig->payload), sizeof(uint64_t) * words);
     return SDDDST_OK;
 }
 

@@ -384,16 +384,18 @@ class Context:
         self._check(self.lib.sdddst_libm_eval(self.h, fn, len(x), _dptr(x), _dptr(out)))
         return out
 
-    def timing(self, on: bool):
-        timing(self.h, on)
+    def timing(self, on: bool, pair_only: bool = False):
+        timing(self.h, on, pair_only)
 
     def kernel_stats(self, kernel: str):
         return kernel_stats(self.h, kernel)
 
 
-def timing(ctx_handle, on: bool):
+def timing(ctx_handle, on, pair_only: bool = False):
+    """Start (clearing the counts) or stop the event timers; pair_only
+    records only the pair kernel's events."""
     lib = load_library()
-    rc = lib.sdddst_timing(ctx_handle, int(on))
+    rc = lib.sdddst_timing(ctx_handle, (2 if pair_only else 1) if on else 0)
     if rc != OK:
         raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
 

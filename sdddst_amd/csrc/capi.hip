@@ -596,7 +596,8 @@ int sdddst_timing(sdddst_ctx *ctx, int enable)
         t->total_ms = 0;
         t->count = 0;
     }
-    c.timing = enable != 0;
+    SD_CHECK(enable >= 0 && enable <= 2, SDDDST_ERR_ARG, "timing mode %d (0 off, 1 all, 2 pair kernel)", enable);
+    c.timing = enable == 1 ? (kTimePair | kTimeSpeeds | kTimeJacobian) : enable == 2 ? kTimePair : 0;
     return SDDDST_OK;
 }
 

@@ -56,6 +56,8 @@ struct HostSignal {
 };
 static_assert(sizeof(DevScalars) % 8 == 0 && sizeof(DevScalars) <= 14 * 8, "DevScalars fits the signal payload");
 
+constexpr int kTimePair = 1, kTimeSpeeds = 2, kTimeJacobian = 4;
+
 struct Context {
     int n = 0, pc = 0;
     sdddst_params p{};
@@ -143,7 +145,9 @@ struct Context {
     double last_rel_res = 0;
     // timing: recorded event pairs per kernel kind, read back on request
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timing = false;
+    // timers recording: kTimePair | kTimeSpeeds | kTimeJacobian (each event
+    // pair also leaves the stream idle for a few microseconds)
+    int timing = 0;
     struct Timer {
         std::vector<hipEvent_t> begin, end;
         int used = 0;

@@ -655,7 +655,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     const int n = c.n;
     const int wblocks = (n + 3) / 4;
     int rc;
-    if (c.timing && (rc = timer_begin(c, c.t_jac))) return rc;
+    if ((c.timing & kTimeJacobian) && (rc = timer_begin(c, c.t_jac))) return rc;
     if (!reuse_pattern || !c.pattern_valid) {
         rc = build_pattern(c, slot, h);
         if (rc) return rc;
@@ -700,7 +700,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec,
                                c.Ax, c.AxRow);
             SD_HIP(hipGetLastError());
-            if (c.timing && (rc = timer_end(c, c.t_jac))) return rc;
+            if ((c.timing & kTimeJacobian) && (rc = timer_end(c, c.t_jac))) return rc;
             c.jac_valid = true;
             c.jac_h = h;
             return SDDDST_OK;
@@ -710,7 +710,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
                        c.colinfo, c.Araw, factor, c.Ax, c.AxRow, c.partner, c.strength);
     SD_HIP(hipGetLastError());
     c.partner_valid = true;
-    if (c.timing && (rc = timer_end(c, c.t_jac))) return rc;
+    if ((c.timing & kTimeJacobian) && (rc = timer_end(c, c.t_jac))) return rc;
     c.jac_valid = true;
     c.jac_h = h;
     return SDDDST_OK;

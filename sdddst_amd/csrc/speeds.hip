@@ -288,13 +288,13 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     const int threads = 256;
     const int blocks = (n + threads - 1) / threads;
     int rc;
-    if (c.timing && (rc = timer_begin(c, c.t_speeds))) return rc;
+    if ((c.timing & kTimeSpeeds) && (rc = timer_begin(c, c.t_speeds))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_ORDERED) {
-        if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
+        if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
         hipLaunchKernelGGL(k_speeds_ordered, dim3(blocks), dim3(threads), 0, c.stream, n, c.pc, c.x[slot], c.y, c.b,
                            c.px, c.py, c.p.A, c.p.KASQR, sigma, c.tol, minPrecSqr, update_tol, c.v[out_buf]);
         SD_HIP(hipGetLastError());
-        if (c.timing && ((rc = timer_end(c, c.t_pair)) || (rc = timer_end(c, c.t_speeds)))) return rc;
+        if ((((c.timing & kTimePair) && (rc = timer_end(c, c.t_pair))) || ((c.timing & kTimeSpeeds) && (rc = timer_end(c, c.t_speeds))))) return rc;
         return SDDDST_OK;
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(threads), 0, c.stream, c.minr2, kInfBits, n);
@@ -324,7 +324,7 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         fd.ep = c.xep;
         fd.em = c.xem;
     }
-    if (c.timing && (rc = timer_begin(c, c.t_pair))) return rc;
+    if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
@@ -334,13 +334,13 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
                            pairtile::tolerance_r2(minPrecSqr), fd);
     SD_HIP(hipGetLastError());
-    if (c.timing && (rc = timer_end(c, c.t_pair))) return rc;
+    if ((c.timing & kTimePair) && (rc = timer_end(c, c.t_pair))) return rc;
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.perm, c.partial,
                        c.x[slot],
                        c.y, c.b, c.px, c.py, c.p.A, c.p.KASQR, sigma, c.minr2, c.tol, minPrecSqr, update_tol,
                        c.v[out_buf]);
     SD_HIP(hipGetLastError());
-    if (c.timing && (rc = timer_end(c, c.t_speeds))) return rc;
+    if ((c.timing & kTimeSpeeds) && (rc = timer_end(c, c.t_speeds))) return rc;
     return SDDDST_OK;
 }
 

@@ -64,7 +64,53 @@ def parse():
                    help="also time this many steps of the direct-solver path (0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--trace-gaps", metavar="DIR",
+                   help="no GPU work: summarise the device idle time of a rocprofv3 --kernel-trace CSV "
+                        "output directory (gaps between kernels over the last 60 %% of the trace)")
     return p.parse_args()
+
+
+def trace_gaps(d):
+    """Device idle time between kernels in a rocprofv3 kernel trace: the
+    union of kernel intervals over the last 60 % of the dispatches (the
+    timed steps of a bench run) against their span, and the gaps grouped by
+    the kernel that ends before them."""
+    import collections
+    import csv
+    import glob
+    import re
+    f = glob.glob(os.path.join(d, "*kernel_trace.csv"))[0]
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in csv.DictReader(open(f)))
+    t0 = iv[int(len(iv) * 0.4)][0]
+
+    def short(name):
+        m = re.search(r"(k_\w+|__amd_rocclr_\w+|rocprim)", name)
+        return m.group(1) if m else name[:40]
+
+    busy, gaps = 0, collections.defaultdict(lambda: [0, 0])
+    cur_s = cur_e = None
+    last = ""
+    for st, en, nm in iv:
+        if st < t0:
+            continue
+        if cur_e is None:
+            cur_s, cur_e = st, en
+        elif st > cur_e:
+            busy += cur_e - cur_s
+            g = gaps[short(last)]
+            g[0] += 1
+            g[1] += st - cur_e
+            cur_s, cur_e = st, en
+        else:
+            cur_e = max(cur_e, en)
+        last = nm if en >= cur_e else last
+    busy += cur_e - cur_s
+    span = iv[-1][1] - t0
+    print(f"span {span / 1e6:.2f} ms, kernels busy {busy / 1e6:.2f} ms, idle {(span - busy) / 1e6:.2f} ms "
+          f"({100 * (span - busy) / span:.1f} %)")
+    print("idle after kernel: count, total ms, mean us")
+    for k, (c, t) in sorted(gaps.items(), key=lambda kv: -kv[1][1])[:15]:
+        print(f"  {k:28s} {c:6d} {t / 1e6:8.2f} {t / c / 1e3:8.2f}")
 
 
 def make_sim(args, solver, x, y, b, precond=None):
@@ -199,14 +245,16 @@ def cpu_baseline(x, y, b, seconds):
     done_pairs = 0
     elapsed = 0.0
     i0 = 0
-    while elapsed < seconds and i0 < n:
+    while elapsed < seconds:
         i1 = min(n, i0 + rows)
         cnt = ctypes.c_long()
         t0 = time.perf_counter()
         lib.orc_speeds_rows_sample(n, O.dp(x), O.dp(y), O.dp(b), i0, i1, ctypes.byref(cnt))
         elapsed += time.perf_counter() - t0
         done_pairs += cnt.value
-        i0 = i1 + n // 64      # spread the sample over the triangle
+        # spread the sample over the triangle, wrapping around: the stride
+        # visits every 16-row block in turn
+        i0 = (i1 + n // 64) % n
     pairs_per_s = done_pairs / elapsed
     step_s = 5 * pairs_total / pairs_per_s
     return {"value": 1.0 / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
@@ -253,7 +301,8 @@ def large_run(args):
                        "final_time": st.sim_time, "final_stress": rate * st.sim_time},
             "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": tflops,
                          "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
-                         "traffic": None, "pairs_per_launch": pairs, "avg_launch_ms": avg, "launches": pair_cnt},
+                         "traffic": None, "pairs_per_launch": pairs, "avg_launch_ms": avg, "launches": pair_cnt,
+                         "launch_sampling": "HIP events on one pair-kernel launch in 4 of the timed run"},
             "pairs_per_s": pairs / (avg * 1e-3),
             "solver_iterations_per_solve": st.solver_total_iters / max(1, st.solver_total_solves),
             "solver_fallbacks_total": st.solver_fallbacks,
@@ -308,6 +357,9 @@ def m4_run(args):
 
 def main():
     args = parse()
+    if args.trace_gaps:
+        trace_gaps(args.trace_gaps)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -388,7 +440,8 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": achieved_tflops,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": traffic, "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
-                     "avg_launch_ms": avg_pair_ms, "launches": pair_cnt},
+                     "avg_launch_ms": avg_pair_ms, "launches": pair_cnt,
+                     "launch_sampling": "HIP events on one pair-kernel launch in 4 of the timed run"},
         "pairs_per_s": pairs / (avg_pair_ms * 1e-3),
         "speeds_eval_ms": sp_ms / max(sp_cnt, 1),
         "jacobian_ms": jac_ms / max(jac_cnt, 1),

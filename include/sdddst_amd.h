@@ -211,8 +211,8 @@ int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double
 
 /* Kernel timing with HIP events on the context stream (no host syncs while
  * recording).  enable = 1 clears and starts recording every timer, 2 only the
- * pair kernel's (each recorded event pair leaves the stream idle for a few
- * microseconds), 0 stops.  kernel is
+ * pair kernel's, on one launch in four (each recorded event pair leaves the
+ * stream idle for a few microseconds), 0 stops.  kernel is
  * "pair" (the pair-speed tile kernel), "speeds" (one whole speed evaluation)
  * or "jacobian" (one whole assembly); count and total_ms cover the launches
  * recorded since the last enable. */

@@ -143,6 +143,8 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
 // Event pairs grow in chunks; elapsed times are read only in timer_collect.
 int timer_begin(Context &c, Context::Timer &t)
 {
+    t.active = t.calls++ % t.every == 0;
+    if (!t.active) return SDDDST_OK;
     if (t.used == (int)t.begin.size()) {
         if (t.used >= 4096) {
             int rc = timer_collect(c, t);
@@ -161,6 +163,7 @@ int timer_begin(Context &c, Context::Timer &t)
 
 int timer_end(Context &c, Context::Timer &t)
 {
+    if (!t.active) return SDDDST_OK;
     SD_HIP(hipEventRecord(t.end[t.used], c.stream));
     t.used++;
     return SDDDST_OK;
@@ -387,7 +390,9 @@ int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, in
     }
     if (dVec) SD_HIP(hipMemcpyAsync(dVec, c.dvec, sizeof(double) * c.n, hipMemcpyDeviceToHost, c.stream));
     if (diag) SD_HIP(hipMemcpyAsync(diag, c.diag, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
-    SD_HIP(hipStreamSynchronize(c.stream));
+    // wait only for copies to the host: an assembly alone stays queued ahead
+    // of the factorisation the caller enqueues next
+    if (Ap || dVec || diag) SD_HIP(hipStreamSynchronize(c.stream));
     return SDDDST_OK;
 }
 
@@ -591,12 +596,15 @@ int sdddst_timing(sdddst_ctx *ctx, int enable)
     CTX_CHECK(ctx);
     Context &c = ctx->c;
     SD_HIP(hipStreamSynchronize(c.stream));
+    SD_CHECK(enable >= 0 && enable <= 2, SDDDST_ERR_ARG, "timing mode %d (0 off, 1 all, 2 pair kernel)", enable);
     for (Context::Timer *t : {&c.t_pair, &c.t_speeds, &c.t_jac}) {
         t->used = 0;
         t->total_ms = 0;
         t->count = 0;
+        t->calls = 0;
+        t->every = 1;
     }
-    SD_CHECK(enable >= 0 && enable <= 2, SDDDST_ERR_ARG, "timing mode %d (0 off, 1 all, 2 pair kernel)", enable);
+    if (enable == 2) c.t_pair.every = 4;
     c.timing = enable == 1 ? (kTimePair | kTimeSpeeds | kTimeJacobian) : enable == 2 ? kTimePair : 0;
     return SDDDST_OK;
 }

@@ -153,6 +153,9 @@ struct Context {
         int used = 0;
         double total_ms = 0;
         int count = 0;
+        int every = 1;          // record one call in `every`
+        long calls = 0;
+        bool active = false;    // this call is recorded
     } t_pair, t_speeds, t_jac;
 };
 

@@ -105,18 +105,6 @@ __global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict_
     ys[i] = sv;
 }
 
-__global__ void k_xexp(int n, const double *__restrict__ x, double *__restrict__ w, double *__restrict__ ep,
-                       double *__restrict__ em)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    double wv, pv, mv;
-    sdfield::xexp(x[i], wv, pv, mv);
-    w[i] = wv;
-    ep[i] = pv;
-    em[i] = mv;
-}
-
 // Sort keys of the pair-tile order: the bin of the wrapped x among nb equal
 // bins, and the index (a stable radix sort keeps equal bins in index order,
 // so the order is a function of the positions alone).
@@ -229,10 +217,21 @@ __global__ void k_speeds_ordered(int n, int pc, const double *__restrict__ x, co
     if (update_tol) tol[i] = t;
 }
 
-__global__ void k_fill_u64(unsigned long long *p, unsigned long long val, int n)
+// Start of a speeds call: the tolerance candidates cleared, and with w (the
+// fast field's far pairs) the wrapped x and e^{+-2 pi x} of each dislocation.
+__global__ void k_speeds_prep(int n, unsigned long long *__restrict__ minr2, const double *__restrict__ x,
+                              double *__restrict__ w, double *__restrict__ ep, double *__restrict__ em)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i < n) p[i] = val;
+    if (i >= n) return;
+    minr2[i] = kInfBits;
+    if (w) {
+        double wv, pv, mv;
+        sdfield::xexp(x[i], wv, pv, mv);
+        w[i] = wv;
+        ep[i] = pv;
+        em[i] = mv;
+    }
 }
 
 }  // namespace
@@ -297,8 +296,6 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         if ((((c.timing & kTimePair) && (rc = timer_end(c, c.t_pair))) || ((c.timing & kTimeSpeeds) && (rc = timer_end(c, c.t_speeds))))) return rc;
         return SDDDST_OK;
     }
-    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(threads), 0, c.stream, c.minr2, kInfBits, n);
-    SD_HIP(hipGetLastError());
     const int nb = c.nb;
     const long nfull = (long)nb * (nb - 1) / 2;
     const long units = nfull + (nb + 1) / 2;
@@ -315,9 +312,11 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     // they are used only when no such pair can be one
     pairtile::FarData fd;
     const double r2_keep = pairtile::tolerance_r2(minPrecSqr);
-    if (c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kTrigMinDx * sdfield::kTrigMinDx) {
-        hipLaunchKernelGGL(k_xexp, dim3(blocks), dim3(threads), 0, c.stream, n, c.x[slot], c.xw, c.xep, c.xem);
-        SD_HIP(hipGetLastError());
+    const bool far = c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kTrigMinDx * sdfield::kTrigMinDx;
+    hipLaunchKernelGGL(k_speeds_prep, dim3(blocks), dim3(threads), 0, c.stream, n, c.minr2, c.x[slot],
+                       far ? c.xw : nullptr, c.xep, c.xem);
+    SD_HIP(hipGetLastError());
+    if (far) {
         fd.ca = c.ytc;
         fd.sa = c.yts;
         fd.w = c.xw;

@@ -28,6 +28,39 @@ namespace {
 constexpr int kRedMax = 256;  // most workgroups of one projection launch
 constexpr int kMaxV = 64;   // most vectors one dots launch handles (restart + 1)
 
+// Stop codes of a device-driven restart cycle.  The kernels of an iteration
+// run while the cycle runs; the closing kernels (solution update and true
+// residual) once it has converged, broken down or used its iteration budget.
+constexpr int kRunning = 0, kConverged = 1, kBreakdown = 2, kNeedNorm = 3, kZeroRhs = 4, kBudget = 5;
+constexpr int kGateIter = 0, kGateClose = 1;
+
+__device__ __forceinline__ bool gated(const int *gate, int mode)
+{
+    if (!gate) return false;
+    const int s = *gate;
+    return mode == kGateIter ? s != kRunning : !(s == kConverged || s == kBreakdown || s == kBudget);
+}
+
+}  // namespace
+
+// The cycle's Givens state on the device (one lane updates it per iteration).
+struct KrylovDev {
+    double gnorm;             // ||g||
+    double scale;             // 1 / h_{k+1,k} of the last step
+    double s[kMaxV + 1];      // rotated right-hand side
+    double cs[kMaxV], sn[kMaxV];
+    double Hm[kMaxV * kMaxV]; // R of the rotated Hessenberg matrix, row-major
+    int k, kmax, stop;
+};
+
+// What the host reads after a wait.
+struct KrylovHost {
+    double gnorm;
+    int k, stop;
+};
+
+namespace {
+
 // One row's dot product by one wavefront: lane l adds the products of
 // entries l, l + 64, ... in order, and the lanes are combined by a fixed
 // butterfly.  Each lane issues the value and index loads of eight of its
@@ -60,13 +93,15 @@ __device__ __forceinline__ double row_dot(int b, int e, int lane, const int *__r
 // y = A x, one wavefront per row.
 __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                               const double *__restrict__ Av, const double *__restrict__ x,
-                                              double *__restrict__ y)
+                                              double *__restrict__ y, const int *__restrict__ gate, int gmode,
+                                              const double *__restrict__ gsub)
 {
+    if (gated(gate, gmode)) return;
     const int lane = (int)(threadIdx.x & 63);
     const int r = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (r >= n) return;
     const double s = row_dot(Ap[r], Ap[r + 1], lane, Ai, Av, [&](int i) { return x[i]; });
-    if (lane == 0) y[r] = s;
+    if (lane == 0) y[r] = gsub ? gsub[r] - s : s;
 }
 
 // The same product with x staged in LDS (systems up to kSpmvLdsMax rows):
@@ -77,15 +112,17 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap,
 constexpr int kSpmvLdsMax = 20480;   // 160 KiB of FP64
 __global__ __launch_bounds__(1024) void k_spmv_lds(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                    const double *__restrict__ Av, const double *__restrict__ x,
-                                                   double *__restrict__ y)
+                                                   double *__restrict__ y, const int *__restrict__ gate, int gmode,
+                                                   const double *__restrict__ gsub)
 {
+    if (gated(gate, gmode)) return;
     extern __shared__ double xs[];
     for (int i = (int)threadIdx.x; i < n; i += 1024) xs[i] = x[i];
     __syncthreads();
     const int lane = (int)(threadIdx.x & 63);
     for (int r = (int)(blockIdx.x * 16 + (threadIdx.x >> 6)); r < n; r += (int)gridDim.x * 16) {
         const double s = row_dot(Ap[r], Ap[r + 1], lane, Ai, Av, [&](int i) { return xs[i]; });
-        if (lane == 0) y[r] = s;
+        if (lane == 0) y[r] = gsub ? gsub[r] - s : s;
     }
 }
 
@@ -93,6 +130,57 @@ __device__ __forceinline__ double wave_sum(double s)
 {
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
+}
+
+// Iteration k of the cycle after the two projection passes (h2 with
+// ||w'||^2 at h2[nv], h = h1 + h2 in hs): the new norm ||w'||^2 - ||h2||^2
+// (or hn_given, taken explicitly by the host when the second pass removed
+// more than a quarter of ||w'||), the previous rotations applied to the new
+// column, the new rotation, the residual estimate and the stop decision.
+// One lane; hh is LDS scratch of kMaxV + 1 doubles.
+__device__ void arnoldi_update(KrylovDev *__restrict__ st, KrylovHost *__restrict__ kh, const double *h2,
+                               const double *hs, int k, double halftol, double hn_given, double *hh)
+{
+    const int nv = k + 1;
+    double hn = hn_given;
+    if (hn_given < 0.0) {
+        double s2 = 0.0;
+        for (int v = 0; v < nv; ++v) s2 += h2[v] * h2[v];
+        const double ww = h2[nv];
+        if (s2 > ww * 0.0625) {
+            st->stop = kNeedNorm;
+            kh->k = k;
+            kh->stop = kNeedNorm;
+            __threadfence_system();
+            return;
+        }
+        hn = sqrt(ww - s2);
+    }
+    for (int v = 0; v < nv; ++v) hh[v] = hs[v];
+    hh[nv] = hn;
+    for (int i = 0; i < k; ++i) {
+        const double t = st->cs[i] * hh[i] + st->sn[i] * hh[i + 1];
+        hh[i + 1] = -st->sn[i] * hh[i] + st->cs[i] * hh[i + 1];
+        hh[i] = t;
+    }
+    const double denom = hypot(hh[k], hh[k + 1]);
+    const double ck = denom > 0 ? hh[k] / denom : 1.0, sk = denom > 0 ? hh[k + 1] / denom : 0.0;
+    st->cs[k] = ck;
+    st->sn[k] = sk;
+    hh[k] = denom;
+    const double sk1 = -sk * st->s[k];
+    st->s[k + 1] = sk1;
+    st->s[k] = ck * st->s[k];
+    for (int i = 0; i <= k; ++i) st->Hm[i * kMaxV + k] = hh[i];
+    st->k = k + 1;
+    st->scale = hn > 0 ? 1.0 / hn : 0.0;
+    int stop = kRunning;
+    if (fabs(sk1) / st->gnorm <= halftol || hn == 0.0) stop = hn == 0.0 ? kBreakdown : kConverged;
+    else if (k + 1 >= st->kmax) stop = kBudget;
+    st->stop = stop;
+    kh->k = k + 1;
+    kh->stop = stop;
+    __threadfence_system();
 }
 
 // One classical Gram-Schmidt pass of the Arnoldi step in one launch.  With
@@ -109,8 +197,11 @@ __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__res
                                               const double *__restrict__ hsub, double *__restrict__ partial,
                                               int pstride, unsigned *__restrict__ ticket, double *__restrict__ out,
                                               double *__restrict__ acc_out, int accumulate, int with_self,
-                                              double *__restrict__ host_out, int host_acc_off)
+                                              double *__restrict__ host_out, int host_acc_off,
+                                              const int *__restrict__ gate, int gmode, KrylovDev *__restrict__ st,
+                                              KrylovHost *__restrict__ kh, double halftol)
 {
+    if (gated(gate, gmode)) return;   // every workgroup: the ticket stays untouched
     __shared__ double wsum[4][kMaxV + 1];
     __shared__ bool last;
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -175,6 +266,12 @@ __global__ __launch_bounds__(256) void k_proj(int n, int nv, const double *__res
         }
     }
     if (threadIdx.x == 0) *ticket = 0u;
+    // the second pass of an Arnoldi step: the cycle's update from h2 (out,
+    // with ||w'||^2 at out[nv]) and h (acc_out), by one lane of this workgroup
+    if (st) {
+        __syncthreads();
+        if (threadIdx.x == 0) arnoldi_update(st, kh, out, acc_out, nv - 1, halftol, -1.0, &wsum[0][0]);
+    }
 }
 
 // w -= sum_v h[v] V[v]   (h from the latest projection pass in hp)
@@ -188,32 +285,10 @@ __global__ void k_orth(int n, int nv, const double *__restrict__ V, const double
     w[i] = acc;
 }
 
-// y = a * (w - sum_v h[v] V[v]): the second Gram-Schmidt subtraction and the
-// normalisation of the next basis vector in one pass (w is left unchanged).
-__global__ void k_orth_scale(int n, int nv, const double *__restrict__ V, const double *__restrict__ hp,
-                             const double *__restrict__ w, double a, double *__restrict__ y)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    double acc = w[i];
-    for (int v = 0; v < nv; ++v) acc -= hp[v] * V[(size_t)v * n + i];
-    y[i] = a * acc;
-}
-
 __global__ void k_scale_copy(int n, double a, const double *__restrict__ x, double *__restrict__ y)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) y[i] = a * x[i];
-}
-
-// y = x / sqrt(*nrm2) (0 when the norm is 0), the norm read on the device
-__global__ void k_scale_by_norm(int n, const double *__restrict__ nrm2, const double *__restrict__ x,
-                                double *__restrict__ y)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    const double b = sqrt(*nrm2);
-    y[i] = (b > 0 ? 1.0 / b : 0.0) * x[i];
 }
 
 __global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const double *__restrict__ v,
@@ -264,25 +339,74 @@ __global__ void k_pair_apply(int n, const double *__restrict__ binv, const int *
 }
 
 // x += sum_j y[j] Z[j]
-struct Coefs {
-    double v[kMaxV];
-};
-
-// x += sum_j y[j] Z[j], y passed by value
-__global__ void k_combine(int n, int k, const double *__restrict__ Z, Coefs yv, double *__restrict__ x)
+// Start of a restart cycle: v_0 = r / beta with beta = sqrt(*beta2), and
+// s = (beta, 0, ...); in cycle 0 (r = g) beta is also ||g||.
+__global__ void k_cycle_start(int n, const double *__restrict__ r, const double *__restrict__ beta2,
+                              double *__restrict__ v0, KrylovDev *__restrict__ st, KrylovHost *__restrict__ kh,
+                              int first, int kmax)
 {
+    const double beta = sqrt(*beta2);
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) v0[i] = (beta > 0 ? 1.0 / beta : 0.0) * r[i];
+    if (i != 0) return;
+    if (first) {
+        st->gnorm = beta;
+        kh->gnorm = beta;
+    }
+    st->s[0] = beta;
+    st->k = 0;
+    st->kmax = kmax;
+    st->stop = st->gnorm == 0.0 ? kZeroRhs : kRunning;
+    kh->k = 0;
+    kh->stop = st->stop;
+    __threadfence_system();
+}
+
+// The step resumed with an explicit norm (the host's path for kNeedNorm).
+__global__ void k_arnoldi_step(KrylovDev *__restrict__ st, KrylovHost *__restrict__ kh, const double *__restrict__ hcol,
+                               int m, int k, double halftol, double hn_given)
+{
+    __shared__ double hh[kMaxV + 1];
+    if (threadIdx.x != 0) return;
+    st->stop = kRunning;
+    arnoldi_update(st, kh, hcol + (m + 2), hcol + 2 * (m + 2), k, halftol, hn_given, hh);
+}
+
+// y = scale * (w - sum_v h2[v] V[v]): the next basis vector, scale from the
+// step; skipped once the cycle has stopped.
+__global__ void k_orth_scale_dev(int n, int nv, const double *__restrict__ V, const double *__restrict__ hp,
+                                 const double *__restrict__ w, const KrylovDev *__restrict__ st,
+                                 double *__restrict__ y)
+{
+    if (st->stop != kRunning) return;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    double acc = w[i];
+    for (int v = 0; v < nv; ++v) acc -= hp[v] * V[(size_t)v * n + i];
+    y[i] = st->scale * acc;
+}
+
+// x += sum_j y[j] Z[j] once the cycle has ended, with R y = s solved by back
+// substitution by one lane of each workgroup (k <= 63: a few hundred flops).
+__global__ void k_combine_dev(int n, const double *__restrict__ Z, const KrylovDev *__restrict__ st,
+                              double *__restrict__ x)
+{
+    __shared__ double y[kMaxV];
+    if (gated(&st->stop, kGateClose)) return;
+    const int k = st->k;
+    if (threadIdx.x == 0) {
+        for (int i = k - 1; i >= 0; --i) {
+            double acc = st->s[i];
+            for (int j = i + 1; j < k; ++j) acc -= st->Hm[i * kMaxV + j] * y[j];
+            y[i] = acc / st->Hm[i * kMaxV + i];
+        }
+    }
+    __syncthreads();
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     double acc = x[i];
-    for (int j = 0; j < k; ++j) acc += yv.v[j] * Z[(size_t)j * n + i];
+    for (int j = 0; j < k; ++j) acc += y[j] * Z[(size_t)j * n + i];
     x[i] = acc;
-}
-
-// r = g - r (r holds A x on entry)
-__global__ void k_resid(int n, const double *__restrict__ g, double *__restrict__ r)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i < n) r[i] = g[i] - r[i];
 }
 
 }  // namespace
@@ -304,16 +428,18 @@ static int spmv_grid()
 }
 
 // y = A x on the CSR view (AxRow): LDS-staged x when it fits.
-static int launch_spmv(Context &c, const double *x, double *y)
+static int launch_spmv(Context &c, const double *x, double *y, const int *gate = nullptr, int gmode = kGateIter,
+                       const double *gsub = nullptr)
 {
     const int n = c.n;
     if (n <= kSpmvLdsMax) {
         const int cus = spmv_grid();
         if (cus < 0) return cus;
         hipLaunchKernelGGL(k_spmv_lds, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n, c.Ap, c.Ai, c.AxRow, x,
-                           y);
+                           y, gate, gmode, gsub);
     } else {
-        hipLaunchKernelGGL(k_spmv, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.AxRow, x, y);
+        hipLaunchKernelGGL(k_spmv, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.AxRow, x, y, gate,
+                           gmode, gsub);
     }
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
@@ -329,10 +455,13 @@ static inline int proj_blocks(int n)
 // for v < nv (and out[nv] = w.w with with_self); acc (nullable) += or = out.
 static void launch_proj(Context &c, Fgmres &f, int nv, const double *V, double *w, const double *hsub,
                         double *out, double *acc = nullptr, int accumulate = 0, int with_self = 0,
-                        double *host_out = nullptr, int host_acc_off = 0)
+                        double *host_out = nullptr, int host_acc_off = 0, const int *gate = nullptr,
+                        int gmode = kGateIter, double halftol = -1.0)
 {
+    // halftol >= 0: the second pass of an Arnoldi step, which also updates the cycle
     hipLaunchKernelGGL(k_proj, dim3(proj_blocks(f.n)), dim3(256), 0, c.stream, f.n, nv, V, w, hsub, f.red, f.m + 2,
-                       f.ticket, out, acc, accumulate, with_self, host_out, host_acc_off);
+                       f.ticket, out, acc, accumulate, with_self, host_out, host_acc_off, gate, gmode,
+                       halftol >= 0.0 ? f.kst : nullptr, f.khdev, halftol);
 }
 
 int Fgmres::init(Context &c)
@@ -356,6 +485,10 @@ int Fgmres::init(Context &c)
     // reductions' last workgroup into mapped, coherent pinned memory
     SD_HIP(hipHostMalloc(&H, sizeof(double) * (3 * (m + 2) + 2), hipHostMallocMapped | hipHostMallocCoherent));
     SD_HIP(hipHostGetDevicePointer((void **)&Hdev, H, 0));
+    SD_HIP(hipMalloc(&kst, sizeof(KrylovDev)));
+    SD_HIP(hipMemsetAsync(kst, 0, sizeof(KrylovDev), c.stream));
+    SD_HIP(hipHostMalloc(&kh, sizeof(KrylovHost), hipHostMallocMapped | hipHostMallocCoherent));
+    SD_HIP(hipHostGetDevicePointer((void **)&khdev, kh, 0));
     return SDDDST_OK;
 }
 
@@ -369,6 +502,10 @@ void Fgmres::release()
     if (hcol) (void)hipFree(hcol);
     if (H) (void)hipHostFree(H);
     if (ticket) (void)hipFree(ticket);
+    if (kst) (void)hipFree(kst);
+    if (kh) (void)hipHostFree(kh);
+    kst = nullptr;
+    kh = khdev = nullptr;
     for (void *p : {(void *)binv, (void *)bcol})
         if (p) (void)hipFree(p);
     binv = nullptr;
@@ -421,57 +558,6 @@ int Fgmres::apply(Context &c, const double *v, double *z)
     return SDDDST_OK;
 }
 
-// CGS2: two passes of (h = V^T w, w -= V h) accumulated into hsum, then
-// hsum[nv] = ||w||; one copy of the column to the host.
-// The second pass also returns ||w||^2 of its input, so the new norm is
-// ||w - V h2||^2 = ||w||^2 - ||h2||^2 (V orthonormal); h2 is tiny after the
-// first pass, and when it is not (||h2||^2 > ||w||^2 / 16) the norm is
-// computed explicitly.
-// Arnoldi step for the new vector w against V[0..nv): CGS2 (two projection
-// passes, the first subtraction fused into the second pass).  On return
-// hhost[0..nv) = h = h1 + h2 and hhost[nv] = ||w - V h||, from
-// ||w'||^2 - ||h2||^2 (w' = w - V h1, orthogonal to V up to h2) unless the
-// second pass removed more than a quarter of ||w'||, when the norm is taken
-// explicitly.  next (nullable) receives (w - V h) / hhost[nv] when that is
-// positive; w then holds w' (or w - V h after the explicit norm).
-static int arnoldi_orth(Context &c, Fgmres &f, int nv, double *hhost, double *next)
-{
-    const int n = f.n;
-    double *h1 = f.hcol;                      // first pass
-    double *h2 = f.hcol + (f.m + 2);          // second pass (+ w'.w')
-    double *hsum = f.hcol + 2 * (f.m + 2);    // h1 + h2
-    launch_proj(c, f, nv, f.V, f.w, nullptr, h1, hsum, 0, 0);
-    // the second pass also writes h2, w'.w' (at [0, nv]) and h (at
-    // [m+2, m+2+nv)) into the mapped pinned buffer
-    launch_proj(c, f, nv, f.V, f.w, h1, h2, hsum, 1, 1, f.Hdev, f.m + 2);
-    SD_HIP(hipGetLastError());
-    int rc = host_wait(c);
-    if (rc) return rc;
-    const double *hc = f.H;
-    for (int v = 0; v < nv; ++v) hhost[v] = hc[f.m + 2 + v];
-    double s2 = 0.0;
-    for (int v = 0; v < nv; ++v) s2 += hc[v] * hc[v];
-    const double ww = hc[nv];
-    if (s2 <= ww * 0.0625) {
-        const double hn = sqrt(ww - s2);
-        hhost[nv] = hn;
-        if (next && hn > 0)
-            hipLaunchKernelGGL(k_orth_scale, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, h2, f.w, 1.0 / hn,
-                               next);
-        SD_HIP(hipGetLastError());
-        return SDDDST_OK;
-    }
-    hipLaunchKernelGGL(k_orth, dim3(nb256(n)), dim3(256), 0, c.stream, n, nv, f.V, h2, f.w);
-    launch_proj(c, f, 0, f.V, f.w, nullptr, hsum + nv, nullptr, 0, 1, f.Hdev);
-    SD_HIP(hipGetLastError());
-    if ((rc = host_wait(c))) return rc;
-    hhost[nv] = sqrt(f.H[0]);
-    if (next && hhost[nv] > 0)
-        hipLaunchKernelGGL(k_scale_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, 1.0 / hhost[nv], f.w, next);
-    SD_HIP(hipGetLastError());
-    return SDDDST_OK;
-}
-
 static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 {
     launch_proj(c, f, 0, f.V, const_cast<double *>(v), nullptr, f.hcol, nullptr, 0, 1, f.Hdev);
@@ -482,6 +568,18 @@ static int norm2(Context &c, Fgmres &f, const double *v, double *out)
     return SDDDST_OK;
 }
 
+// One restart cycle is driven by the device: the host enqueues a batch of
+// iterations (z_k = M v_k, w = A z_k, two projection passes -- the first
+// subtraction fused into the second, whose last workgroup also updates the
+// Givens state -- and the next basis vector) and the cycle's closing kernels
+// (back substitution with delta += Z y, the true residual g - A delta and
+// its norm), then waits once.  Iteration kernels
+// after the cycle stops, and the closing kernels while it still runs, return
+// at once, so a solve that converges within a batch costs one wait.  Only an
+// Arnoldi step whose second pass removed more than a quarter of ||w'|| comes
+// back to the host, for an explicit norm.
+constexpr int kBatch = 2;
+
 int Fgmres::solve(Context &c)
 {
     const int N = n;
@@ -491,104 +589,97 @@ int Fgmres::solve(Context &c)
     const int precond = c.precond_override >= 0 ? c.precond_override : c.precond;
     const int max_it = precond == SDDDST_PRECOND_LEARNED ? std::min(c.solver_max_iter, m) : c.solver_max_iter;
     int rc;
-    // delta = 0, r = g; ||g||^2 lands in the device slot gs and its mapped
-    // copy, and v_0 = g / ||g|| is formed on the device, so the host learns
-    // ||g|| at the first iteration's synchronisation
+    // delta = 0, r = g: ||g||^2 into the device slot gs, v_0 = g / ||g||
     SD_HIP(hipMemsetAsync(c.delta, 0, sizeof(double) * N, c.stream));
     double *gs = hcol + 3 * (m + 2);
-    launch_proj(c, *this, 0, V, const_cast<double *>(c.g), nullptr, gs, nullptr, 0, 1, Hdev + 3 * (m + 2));
-    hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256(N)), dim3(256), 0, c.stream, N, gs, c.g, V);
+    double *rn2 = hcol + 3 * (m + 2) + 1;   // ||g - A delta||^2 of a cycle's end (device; H[0] on the host)
+    launch_proj(c, *this, 0, V, const_cast<double *>(c.g), nullptr, gs, nullptr, 0, 1);
     SD_HIP(hipGetLastError());
-    double gnorm = -1.0;
     c.last_iters = 0;
-    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), hh(3 * (m + 2) + 1);
     int total_it = 0;
-    double rel = 1.0;
-    bool checked = false;   // rel is the true residual of the current delta
-    const double *r0 = c.g;
+    double gnorm = -1.0, rel = 1.0, beta = 0.0;
+    const double *h1 = hcol, *h2 = hcol + (m + 2);
+    double *hsum = hcol + 2 * (m + 2);
+    const int *stop = &kst->stop;
     for (int cycle = 0;; ++cycle) {
-        double beta = gnorm;   // delta = 0 in cycle 0: r = g
-        if (cycle > 0) {
-            // r = g - A delta, the true residual of the last cycle's delta
-            if ((rc = launch_spmv(c, c.delta, w))) return rc;
-            hipLaunchKernelGGL(k_resid, dim3(nb256(N)), dim3(256), 0, c.stream, N, c.g, w);
-            SD_HIP(hipGetLastError());
-            r0 = w;
-            rc = norm2(c, *this, r0, &beta);
-            if (rc) return rc;
-            rel = beta / gnorm;
-            checked = true;
-            if (rel <= tol) break;
-        }
-        if (total_it >= max_it) break;
-        checked = false;
-        if (cycle > 0) {
-            hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / beta, r0, V);
-            SD_HIP(hipGetLastError());
-        }
-        std::fill(s.begin(), s.end(), 0.0);
-        s[0] = beta;
-        int k = 0;
-        for (; k < m && total_it < max_it; ++k, ++total_it) {
-            // z_k = M^-1 v_k ; w = A z_k
-            rc = apply(c, V + (size_t)k * N, Z + (size_t)k * N);
-            if (rc) return rc;
-            if ((rc = launch_spmv(c, Z + (size_t)k * N, w))) return rc;
-            rc = arnoldi_orth(c, *this, k + 1, hh.data(), V + (size_t)(k + 1) * N);
-            if (rc) return rc;
-            if (gnorm < 0) {
-                // first synchronisation of the solve: ||g|| is on the host
-                gnorm = sqrt(H[3 * (m + 2)]);
-                if (gnorm == 0.0) {
-                    c.last_rel_res = 0.0;
-                    return SDDDST_OK;
-                }
-                s[0] = beta = gnorm;
+        const int kmax = std::min(m, max_it - total_it);
+        if (kmax <= 0) break;
+        // v_0 = r / ||r||: r = g in cycle 0, else the last cycle's true residual (in w)
+        hipLaunchKernelGGL(k_cycle_start, dim3(nb256(N)), dim3(256), 0, c.stream, N, cycle == 0 ? c.g : w,
+                           cycle == 0 ? gs : rn2, V, kst, khdev, cycle == 0 ? 1 : 0, kmax);
+        SD_HIP(hipGetLastError());
+        int k = 0;   // iterations of this cycle the host has seen completed
+        bool ended = false;
+        // the first batch as long as this context's recent solves (rounded),
+        // later ones kBatch: iterations past convergence return at once but
+        // still cost their launches
+        int batch = std::max(1, std::min(m, (int)(iter_mean + 0.5)));
+        while (!ended) {
+            const int B = std::min(batch, kmax - k);
+            batch = kBatch;
+            for (int b = 0; b < B; ++b) {
+                const int kk = k + b;
+                double *vk = V + (size_t)kk * N, *zk = Z + (size_t)kk * N;
+                if ((rc = apply(c, vk, zk))) return rc;
+                if ((rc = launch_spmv(c, zk, w, stop))) return rc;
+                launch_proj(c, *this, kk + 1, V, w, nullptr, const_cast<double *>(h1), hsum, 0, 0, nullptr, 0, stop);
+                launch_proj(c, *this, kk + 1, V, w, h1, const_cast<double *>(h2), hsum, 1, 1, nullptr, 0, stop,
+                            kGateIter, 0.5 * tol);
+                hipLaunchKernelGGL(k_orth_scale_dev, dim3(nb256(N)), dim3(256), 0, c.stream, N, kk + 1, V, h2, w, kst,
+                                   V + (size_t)(kk + 1) * N);
+                SD_HIP(hipGetLastError());
             }
-            const double hnext = hh[k + 1];
-            // apply previous rotations to the new column
-            for (int i = 0; i < k; ++i) {
-                const double t = cs[i] * hh[i] + sn[i] * hh[i + 1];
-                hh[i + 1] = -sn[i] * hh[i] + cs[i] * hh[i + 1];
-                hh[i] = t;
-            }
-            const double denom = hypot(hh[k], hh[k + 1]);
-            cs[k] = denom > 0 ? hh[k] / denom : 1.0;
-            sn[k] = denom > 0 ? hh[k + 1] / denom : 0.0;
-            hh[k] = denom;
-            hh[k + 1] = 0.0;
-            s[k + 1] = -sn[k] * s[k];
-            s[k] = cs[k] * s[k];
-            for (int i = 0; i <= k; ++i) Hm[(size_t)i * m + k] = hh[i];
-            if (fabs(s[k + 1]) / gnorm <= 0.5 * tol || hnext == 0.0) {
+            // the cycle's end, when it came: delta += Z y and the true residual
+            hipLaunchKernelGGL(k_combine_dev, dim3(nb256(N)), dim3(256), 0, c.stream, N, Z, kst, c.delta);
+            if ((rc = launch_spmv(c, c.delta, w, stop, kGateClose, c.g))) return rc;
+            launch_proj(c, *this, 0, V, w, nullptr, rn2, nullptr, 0, 1, Hdev, 0, stop, kGateClose);
+            SD_HIP(hipGetLastError());
+            if ((rc = host_wait(c))) return rc;
+            const volatile KrylovHost *vkh = kh;
+            KrylovHost st;
+            st.gnorm = vkh->gnorm;
+            st.k = vkh->k;
+            st.stop = vkh->stop;
+            if (gnorm < 0) gnorm = st.gnorm;
+            SD_CHECK(st.k >= k && (st.k > k || st.stop != kRunning || B == 0), SDDDST_ERR_SOLVER,
+                     "FGMRES: no progress in a batch (iteration %d, state %d)", k, st.stop);
+            total_it += st.k - k;
+            k = st.k;
+            switch (st.stop) {
+            case kZeroRhs:
+                c.last_iters = 0;
+                c.last_rel_res = 0.0;
+                return SDDDST_OK;
+            case kNeedNorm: {
+                // iteration k: w -= V h2, its norm taken explicitly, then the
+                // step resumed with it (the batch's later kernels returned)
+                hipLaunchKernelGGL(k_orth, dim3(nb256(N)), dim3(256), 0, c.stream, N, k + 1, V, h2, w);
+                SD_HIP(hipGetLastError());
+                double hn;
+                if ((rc = norm2(c, *this, w, &hn))) return rc;
+                hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, c.stream, kst, khdev, hcol, m, k, 0.5 * tol,
+                                   hn);
+                if (hn > 0)
+                    hipLaunchKernelGGL(k_scale_copy, dim3(nb256(N)), dim3(256), 0, c.stream, N, 1.0 / hn, w,
+                                       V + (size_t)(k + 1) * N);
+                SD_HIP(hipGetLastError());
                 ++k;
                 ++total_it;
                 break;
             }
+            case kRunning:
+                break;
+            default:   // converged estimate, breakdown or budget: the closing kernels ran
+                beta = sqrt(H[0]);
+                rel = beta / gnorm;
+                ended = true;
+            }
         }
-        // back substitution H y = s
-        for (int i = k - 1; i >= 0; --i) {
-            double acc = s[i];
-            for (int j = i + 1; j < k; ++j) acc -= Hm[(size_t)i * m + j] * y[j];
-            y[i] = acc / Hm[(size_t)i * m + i];
-        }
-        Coefs yc{};
-        for (int j = 0; j < k; ++j) yc.v[j] = y[j];
-        hipLaunchKernelGGL(k_combine, dim3(nb256(N)), dim3(256), 0, c.stream, N, k, Z, yc, c.delta);
-        SD_HIP(hipGetLastError());
-    }
-    // the true residual, unless the last check already measured this delta
-    if (!checked) {
-        if ((rc = launch_spmv(c, c.delta, w))) return rc;
-        hipLaunchKernelGGL(k_resid, dim3(nb256(N)), dim3(256), 0, c.stream, N, c.g, w);
-        SD_HIP(hipGetLastError());
-        double rn;
-        rc = norm2(c, *this, w, &rn);
-        if (rc) return rc;
-        rel = rn / gnorm;
+        if (rel <= tol || total_it >= max_it) break;
     }
     c.last_iters = total_it;
     c.last_rel_res = rel;
+    iter_mean = 0.875 * iter_mean + 0.125 * total_it;
     if (!(c.last_rel_res <= tol)) {
         set_error("FGMRES: relative residual %.3e after %d iterations", c.last_rel_res, total_it);
         return SDDDST_ERR_SOLVER;

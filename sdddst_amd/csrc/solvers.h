@@ -20,6 +20,9 @@ struct DenseLu {
 
 // Restarted flexible GMRES (FP64) on the CSR view of A with a
 // right preconditioner; see solver_fgmres.hip.
+struct KrylovDev;    // state of a device-driven restart cycle (solver_fgmres.hip)
+struct KrylovHost;   // what the host reads of it (mapped memory)
+
 struct Fgmres {
     int n = 0, m = 0;
     double *V = nullptr;     // (m+1) x n Krylov basis
@@ -33,6 +36,10 @@ struct Fgmres {
     unsigned *ticket = nullptr;  // last-block counter of the fused reductions
     double *binv = nullptr;      // pair-block inverse rows (2 per row)
     int *bcol = nullptr;
+    KrylovDev *kst = nullptr;    // device cycle state
+    KrylovHost *kh = nullptr;    // its host-read part (mapped, coherent pinned memory)
+    KrylovHost *khdev = nullptr; // device address of kh
+    double iter_mean = 3.0;      // running mean of iterations per solve (first batch size)
     int init(Context &c);
     int factor(Context &c);
     int pair_factor(Context &c);                         // pair-block inverse of A

@@ -244,6 +244,49 @@ def test_fgmres_long_restart_cycle():
     c.close()
 
 
+def test_fgmres_restart_cycles_converge():
+    # restart 2: every solve that needs more than two iterations runs several
+    # device-driven cycles (each closing with delta += Z y and the true
+    # residual, the next starting from it); no fallback
+    n = 1024
+    c, x, y, b = _jacobian_ctx(n, 1.0, 3, 1e-6)
+    rng = np.random.default_rng(5)
+    c.upload_speed(S.V_INIT, np.zeros(n))
+    c.upload_speed(S.V_SPEED, np.zeros(n))
+    c.upload_state(S.SLOT_SCRATCH_A, x + rng.normal(size=n))
+    c.upload_state(S.SLOT_SCRATCH_B, x)
+    c.residual(1e-6, S.SLOT_SCRATCH_A, S.SLOT_SCRATCH_B, S.V_SPEED, S.V_INIT)
+    c.solver_select(S.SOLVER_DENSE_LU)
+    c.solver_factor()
+    d_lu = c.solver_solve(download=True)
+    c.solver_select(S.SOLVER_FGMRES, 1e-10, 300, 2)
+    c.set_preconditioner(S.PRECOND_JACOBI)
+    c.solver_factor()
+    d = c.solver_solve(download=True)
+    it, rr, fb = c.solver_stats()
+    assert fb == 0 and rr <= 1e-10 and it > 2
+    assert np.linalg.norm(d - d_lu) <= 1e-8 * np.linalg.norm(d_lu)
+    c.close()
+
+
+def test_fgmres_zero_right_hand_side():
+    # g = 0 (x_new = x_old, zero speeds): delta = 0 after no iteration
+    n = 512
+    c, x, y, b = _jacobian_ctx(n, 1.0, 4, 1e-6)
+    c.upload_speed(S.V_INIT, np.zeros(n))
+    c.upload_speed(S.V_SPEED, np.zeros(n))
+    c.upload_state(S.SLOT_SCRATCH_A, x)
+    c.upload_state(S.SLOT_SCRATCH_B, x)
+    gd = c.residual(1e-6, S.SLOT_SCRATCH_A, S.SLOT_SCRATCH_B, S.V_SPEED, S.V_INIT, download=True)
+    assert not np.any(gd)
+    c.solver_select(S.SOLVER_FGMRES, 1e-10, 100, 20)
+    c.solver_factor()
+    d = c.solver_solve(download=True)
+    it, rr, fb = c.solver_stats()
+    assert not np.any(d) and it == 0 and rr == 0.0 and fb == 0
+    c.close()
+
+
 @pytest.mark.parametrize("precond", [S.PRECOND_JACOBI, S.PRECOND_PAIR])
 def test_fgmres_preconditioners(precond):
     # large h makes close pairs dominate: the pair-block preconditioner must

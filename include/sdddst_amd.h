@@ -209,6 +209,18 @@ int sdddst_field_eval(sdddst_ctx *ctx, int which, int64_t n, const double *dx, c
  * the device math against the host C library. */
 int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double *out);
 
+/* EV analysis of the positions in cfg_slot (the point defects uploaded to the
+ * context included): replaces SpectralDecompositor::decompose
+ * (spectral_decompositor.cpp:63-201 -- the dense Hessian b_i b_j dtau/dx with
+ * its diagonal, then LAPACKE_dsyevd) and the per-mode columns of
+ * TimeSeriesProcessor::run (time_series_processor.cpp:40-58, 75-82).
+ * eigenvalues[n] ascending; eigenvectors[n * n] row i = eigenvector i (the
+ * reference's getEigenVectorElement(i, j)); pn[i] = 1 / sum_j v_ij^4,
+ * sum_b[i] = sum_j v_ij b_j, gan[i] = sum_b[i]^2 / eigenvalues[i].  Any
+ * output may be NULL.  Needs n * n * 8 bytes of device memory. */
+int sdddst_ev_decompose(sdddst_ctx *ctx, int cfg_slot, double *eigenvalues, double *eigenvectors, double *pn,
+                        double *gan, double *sum_b);
+
 /* Kernel timing with HIP events on the context stream (no host syncs while
  * recording).  enable = 1 clears and starts recording every timer, 2 only the
  * pair kernel's, on one launch in four (each recorded event pair leaves the

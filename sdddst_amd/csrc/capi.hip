@@ -591,6 +591,14 @@ int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double
     return libm_eval(ctx->c, fn, n, x, out);
 }
 
+int sdddst_ev_decompose(sdddst_ctx *ctx, int cfg_slot, double *eigenvalues, double *eigenvectors, double *pn,
+                        double *gan, double *sum_b)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(valid_slot(cfg_slot), SDDDST_ERR_ARG, "bad slot");
+    return ev_decompose(ctx->c, cfg_slot, eigenvalues, eigenvectors, pn, gan, sum_b);
+}
+
 int sdddst_timing(sdddst_ctx *ctx, int enable)
 {
     CTX_CHECK(ctx);

@@ -171,6 +171,9 @@ void compute_cutoff(Context &c);
 int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_tol);
 int pair_order_alloc(Context &c);   // buffers of the x ordering of the pair tiles
 int launch_ytrig(Context &c);       // cos / sin of 2 pi y after an upload of y
+// The EV analysis of the positions in slot (ev_analysis.hip); outputs nullable.
+int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors, double *pn, double *gan,
+                 double *sum_b);
 // Waits until the work enqueued on the stream so far is done, copying `words`
 // 64-bit words from device address src (if any) to dst on the way: a signal
 // kernel publishes them in HostSignal and the host polls it (a bounded spin,

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include "StressProtocols/stress_protocol.h"
+#include "data_time_series.h"
 #include "simulation_data.h"
 
 namespace sdddstCore {
@@ -26,6 +28,12 @@ public:
     ~ProjectParser();
 
     std::shared_ptr<SimulationData> getSimulationData();
+    // The EV analysis: its snapshot series (--file-list, --pd-configuration;
+    // null when either is missing) and its stress protocol (a new
+    // FixedRateProtocol; like the reference it aborts without
+    // --fixed-rate-external-stress).
+    std::shared_ptr<DataTimeSeries> getDataTimeSeries();
+    StressProtocol *getStressProtocol();
     ProgramType getPType() const { return pType; }
     static void printLicense();
     static std::string helpText();

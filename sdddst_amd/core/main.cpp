@@ -1,11 +1,11 @@
 // The sdddst executable (main.cpp of the reference): parse the command line,
-// run the simulation on the device.  The eigenvalue analysis
-// (time_series_processor.cpp) is not part of the device path.
+// run the simulation or the eigenvalue analysis of a time series on the device.
 #include <exception>
 #include <iostream>
 
 #include "../../include/sdddst/project_parser.h"
 #include "../../include/sdddst/simulation.h"
+#include "../../include/sdddst/time_series_processor.h"
 
 int main(int argc, char **argv)
 {
@@ -19,8 +19,14 @@ int main(int argc, char **argv)
             return 1;
         }
     } else if (parser.getPType() == sdddstCore::EV_ANALYZATION) {
-        std::cerr << "sdddst: the eigenvalue analysis is not available in this build" << std::endl;
-        return 3;
+        try {
+            sdddstEV::TimeSeriesProcessor processor(parser.getSimulationData(), parser.getDataTimeSeries(),
+                                                    parser.getStressProtocol());
+            processor.run();
+        } catch (const std::exception &e) {
+            std::cerr << "sdddst: " << e.what() << std::endl;
+            return 1;
+        }
     }
     return 0;
 }

@@ -327,9 +327,34 @@ void ProjectParser::processInput()
         sD->solverRelTolerance = asDouble("solver-tolerance");
         if (has("spai-weights")) sD->spaiWeightsPath = asString("spai-weights");
     } else if (!has("simulation")) {
+        // project_parser.cpp:298-319
+        sD = std::make_shared<SimulationData>();
         if (!has("result-ev-file")) fail("result-ev-file is not defined! Exiting...", 1);
+        sD->eVAnalResultFile = asString("result-ev-file");
         pType = EV_ANALYZATION;
+        if (has("calculate-ev-deriv")) sD->calculateDerivativeEVAnal = true;
+        if (has("eigenvector-to-write")) sD->numberOfEigenVecToWrite = asInt("eigenvector-to-write");
+        if (has("write-correlation-matrices")) sD->writeCorrelMatrices = asInt("write-correlation-matrices");
     }
+}
+
+// project_parser.cpp:330-338
+std::shared_ptr<DataTimeSeries> ProjectParser::getDataTimeSeries()
+{
+    if (pType == EV_ANALYZATION && has("pd-configuration") && has("file-list"))
+        return std::make_shared<DataTimeSeries>(asString("file-list"), asString("pd-configuration"));
+    return nullptr;
+}
+
+// project_parser.cpp:340-348: the analysis takes a fixed-rate protocol only
+StressProtocol *ProjectParser::getStressProtocol()
+{
+    if (has("fixed-rate-external-stress")) {
+        auto *fpr = new FixedRateProtocol();
+        fpr->setRate(asDouble("fixed-rate-external-stress"));
+        return fpr;
+    }
+    abort();
 }
 
 }  // namespace sdddstCore

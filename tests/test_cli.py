@@ -71,9 +71,13 @@ def test_conflicting_stress_protocols(tmp_path):
     assert "Only one stress protocol" in r.stderr
 
 
-def test_ev_analysis_is_reported_unavailable(tmp_path):
+def test_ev_analysis_requires_fixed_rate_and_series(tmp_path):
+    # project_parser.cpp:340-348: without a fixed-rate protocol the reference
+    # aborts; without the snapshot list there is nothing to analyse
     r = run("-c", "--ev-analyzation", "--result-ev-file", str(tmp_path / "ev"))
-    assert r.returncode == 3 and "not available" in r.stderr
+    assert r.returncode == -6
+    r = run("-c", "--ev-analyzation", "--result-ev-file", str(tmp_path / "ev"), "--fixed-rate-external-stress", "1")
+    assert r.returncode == 1 and "--file-list and --pd-configuration are required" in r.stderr
 
 
 def test_simulation_fails_loudly_without_device(tmp_path):

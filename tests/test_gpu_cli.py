@@ -73,3 +73,46 @@ def test_cli_point_defects_fixed_rate_match_oracle(tmp_path):
     xr, _, _ = configs.read_dconf(str(out))
     assert np.max(np.abs(xr - o.x())) < 1e-9
     o.close()
+
+
+def test_cli_ev_analysis_series(tmp_path):
+    # TimeSeriesProcessor::run (time_series_processor.cpp:21-93): one gzip
+    # text record per snapshot -- time, eigenvalues, participation numbers,
+    # s^2 / lambda, speeds, x, s, the first K eigenvectors -- with the
+    # device decomposition's values at six significant digits
+    import gzip
+
+    import sdddst_amd as S
+
+    n, pc, K = 64, 8, 3
+    px, py = configs.point_defects(n, pc, 41)
+    pfile, lst, out = tmp_path / "pd.fconf", tmp_path / "list.txt", tmp_path / "ev.gz"
+    with open(pfile, "w") as f:
+        for u, v in zip(px, py):
+            f.write("%.16e %.16e\n" % (u, v))
+    snaps, lines = [], []
+    for s, t in enumerate([0.0, 1.5e-3]):
+        x, y, b = configs.dislocations(n, 40 + s)
+        d = tmp_path / ("s%d.dconf" % s)
+        configs.write_dconf(str(d), x, y, b)
+        lines.append("%r %s" % (t, d))
+        snaps.append((t,) + configs.read_dconf(str(d)))
+    lst.write_text("\n".join(lines) + "\n")
+    r = _cli("--ev-analyzation", "--result-ev-file", str(out), "--file-list", str(lst), "--pd-configuration",
+             str(pfile), "--fixed-rate-external-stress", "2", "--eigenvector-to-write", str(K))
+    assert r.stdout.split() == ["0", "0.0015"]
+    rows = [np.array(list(map(float, ln.split()))) for ln in gzip.open(out, "rt").read().splitlines()]
+    assert len(rows) == 2
+    for (t, x, y, b), row in zip(snaps, rows):
+        assert len(row) == 1 + 6 * n + K * n and np.all(np.isfinite(row))
+        assert row[0] == pytest.approx(t)
+        with S.Context(n, pc=pc) as c:
+            c.upload_state(0, x, y, b)
+            c.upload_points(px, py)
+            ev = c.ev_decompose(0)
+        lam = ev["eigenvalues"]
+        np.testing.assert_allclose(row[1:1 + n], lam, rtol=1e-5, atol=1e-6 * np.abs(lam).max())
+        np.testing.assert_allclose(row[1 + n:1 + 2 * n], ev["pn"], rtol=1e-5)
+        np.testing.assert_allclose(row[1 + 4 * n:1 + 5 * n], x, rtol=1e-5, atol=1e-6)
+        vecs = row[1 + 6 * n:].reshape(K, n)
+        np.testing.assert_allclose(np.abs(vecs), np.abs(ev["eigenvectors"][:K]), rtol=1e-4, atol=1e-6)

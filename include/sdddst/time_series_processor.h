@@ -1,8 +1,8 @@
 // The EV analysis of a time series (time_series_processor.h of the
 // reference): per snapshot the eigen-decomposition of the dense Hessian on
-// the device (sdddst_ev_decompose) and the reference's gzip-compressed text
-// output.  The eigenvalue derivatives (--calculate-ev-deriv) need the second
-// field derivative, which this build does not provide: run() rejects them.
+// the device (sdddst_ev_analysis, with the eigenvalue derivatives of
+// --calculate-ev-deriv when asked for) and the reference's gzip-compressed
+// text output.
 #pragma once
 
 #include <memory>

@@ -221,6 +221,18 @@ int sdddst_libm_eval(sdddst_ctx *ctx, int fn, int64_t n, const double *x, double
 int sdddst_ev_decompose(sdddst_ctx *ctx, int cfg_slot, double *eigenvalues, double *eigenvectors, double *pn,
                         double *gan, double *sum_b);
 
+/* sdddst_ev_decompose plus the eigenvalue derivatives of the reference's
+ * --calculate-ev-deriv option (time_series_processor.cpp:92-111 with
+ * TimeSeriesProcessor::calculateHessianDerivative, :205-246): given the
+ * speeds[n] of the configuration (host array) and the fixed-rate protocol's
+ * stress_rate sigma', deriv[m] = sum_ij v_mi v_mj sum_k speeds_k T_ijk /
+ * sigma' over the reference's third-derivative tensor T (built on
+ * AnalyticField::xy_diff_x2, AnalyticField.cpp:183-202).  speeds and deriv
+ * are both given or both NULL (then this is sdddst_ev_decompose).  Needs
+ * 3 n * n * 8 bytes of device memory with the derivatives. */
+int sdddst_ev_analysis(sdddst_ctx *ctx, int cfg_slot, const double *speeds, double stress_rate, double *eigenvalues,
+                       double *eigenvectors, double *pn, double *gan, double *sum_b, double *deriv);
+
 /* Kernel timing with HIP events on the context stream (no host syncs while
  * recording).  enable = 1 clears and starts recording every timer, 2 only the
  * pair kernel's, on one launch in four (each recorded event pair leaves the

@@ -42,6 +42,13 @@ double ref_xy_diff_x(double dx, double dy)
     return p->xy_diff_x(dx, dy);
 }
 
+double ref_xy_diff_x2(double dx, double dy)
+{
+    AnalyticField f;
+    Field *p = &f;
+    return p->xy_diff_x2(dx, dy);
+}
+
 void ref_xy_batch(long n, const double *dx, const double *dy, double *out, double *out_dx)
 {
     AnalyticField f;

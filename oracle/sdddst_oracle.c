@@ -14,9 +14,10 @@
  * with exponents 3..7, sqrt) goes to the host glibc, as in the reference.
  *
  * Parity pins (tests/test_oracle.py):
- *   - orc_xy / orc_xy_diff_x / precision handler: bit-identical to the
- *     reference's own AnalyticField / PrecisionHandler built into
- *     oracle/_ref (when it is present) and to the committed golden vectors;
+ *   - orc_xy / orc_xy_diff_x / orc_xy_diff_x2 / precision handler:
+ *     bit-identical to the reference's own AnalyticField / PrecisionHandler
+ *     built into oracle/_ref (when it is present) and to the committed
+ *     golden vectors;
  *   - closed forms: y-periodic image sum, odd symmetry, small-r limit,
  *     Taylor-branch continuity, central differences (mpmath);
  *   - speeds: N=2 dipole, sum rule, brute-force long-double sums;
@@ -161,6 +162,56 @@ double orc_xy_diff_x(double dx, double dy)
            fwall(dx + Nf, cos2piy, dy) +
            fwall_dx(dx - Nf, cos2piy, dy) +
            gsum_dx(ORC_ANALYTIC_FIELD_N - 1, dx, cos2piy, dy);
+}
+
+/* AnalyticField.cpp:95-106 : second x-derivative of fwall (no small-r
+ * branch: the reference divides by (cos 2 pi dy - cosh 2 pi dx)^4 for every
+ * image). */
+static double fwall_dx2(double dx, double cos2piy, double dy)
+{
+    double ch = cosh(M_PI * 2.0 * dx);
+    double sh = sinh(M_PI * 2.0 * dx);
+    double c4y = cos(4.0 * M_PI * dy);
+    double ch2 = cosh(M_PI * 4.0 * dx);
+    double sh2 = sinh(M_PI * 4.0 * dx);
+    double num = sh * cos2piy * (pow(cos2piy, 2.0) - 2.0) +
+                 2.0 * M_PI * dx * pow(sh, 2.0) * (c4y - 2.0) -
+                 M_PI * dx * pow(ch, 3.0) * cos2piy +
+                 0.5 * M_PI * dx * ch * cos2piy * (2.0 * ch2 + c4y - 5.0) +
+                 pow(ch, 2.0) * (2.0 * M_PI * dx - sh * cos2piy) +
+                 sh2;
+    return (4.0 * M_PI * num) / pow(cos2piy - ch, 4.0);
+}
+
+/* AnalyticField.cpp:122-132 */
+static double gsum_dx2(int images, double dx, double cos2piy, double dy)
+{
+    double acc = fwall_dx2(dx, cos2piy, dy);
+    for (int k = 1; k <= images; ++k)
+        acc = acc + fwall_dx2(dx - (double)k, cos2piy, dy) + fwall_dx2(dx + (double)k, cos2piy, dy);
+    return acc;
+}
+
+/* AnalyticField.cpp:183-202 : d^2 tau_xy / dx^2 (the eigenvalue-derivative
+ * option of the EV analysis). */
+double orc_xy_diff_x2(double dx, double dy)
+{
+    const double Nf = (double)ORC_ANALYTIC_FIELD_N;
+    double cos2piy = cos(M_PI * 2.0 * dy);
+    if (dx < 0) {
+        return -2.0 * fwall_dx(dx + Nf + 1.0, cos2piy, dy) -
+               dx * fwall_dx2(dx + Nf + 1.0, cos2piy, dy) +
+               fwall_dx2(dx - Nf, cos2piy, dy) * (1.0 + dx) +
+               2.0 * fwall_dx(dx - Nf, cos2piy, dy) +
+               fwall_dx2(dx + Nf, cos2piy, dy) +
+               gsum_dx2(ORC_ANALYTIC_FIELD_N - 1, dx, cos2piy, dy);
+    }
+    return 2.0 * fwall_dx(dx - Nf - 1.0, cos2piy, dy) +
+           dx * fwall_dx2(dx - Nf - 1.0, cos2piy, dy) +
+           fwall_dx2(dx + Nf, cos2piy, dy) * (1.0 - dx) -
+           2.0 * fwall_dx(dx + Nf, cos2piy, dy) +
+           fwall_dx2(dx - Nf, cos2piy, dy) +
+           gsum_dx2(ORC_ANALYTIC_FIELD_N - 1, dx, cos2piy, dy);
 }
 
 /* utility.h:42-55 : point-defect helpers. */

@@ -98,6 +98,7 @@ _SIGNATURES = {
     "sdddst_field_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]),
     "sdddst_libm_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp]),
     "sdddst_ev_decompose": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
+    "sdddst_ev_analysis": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
     "sdddst_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "sdddst_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip, _dp]),
 }
@@ -385,17 +386,27 @@ class Context:
         self._check(self.lib.sdddst_libm_eval(self.h, fn, len(x), _dptr(x), _dptr(out)))
         return out
 
-    def ev_decompose(self, slot: int, vectors: bool = True) -> dict:
+    def ev_decompose(self, slot: int, vectors: bool = True, speeds=None, stress_rate: float = 1.0) -> dict:
         """EV analysis of the positions in slot (SpectralDecompositor::decompose
         and the per-mode columns of TimeSeriesProcessor::run): ascending
         eigenvalues, eigenvectors as rows (None unless vectors), participation
-        numbers, s_i = sum_j v_ij b_j and s_i^2 / lambda_i."""
+        numbers, s_i = sum_j v_ij b_j and s_i^2 / lambda_i; with speeds, also
+        the eigenvalue derivatives over stress_rate ("deriv")."""
         n = self.n
         lam, pn, gan, sb = np.empty(n), np.empty(n), np.empty(n), np.empty(n)
         vec = np.empty((n, n)) if vectors else None
-        self._check(self.lib.sdddst_ev_decompose(self.h, slot, _dptr(lam), _dptr(vec) if vectors else None,
-                                                 _dptr(pn), _dptr(gan), _dptr(sb)))
-        return {"eigenvalues": lam, "eigenvectors": vec, "pn": pn, "gan": gan, "sum_b": sb}
+        if speeds is None:
+            self._check(self.lib.sdddst_ev_decompose(self.h, slot, _dptr(lam), _dptr(vec) if vectors else None,
+                                                     _dptr(pn), _dptr(gan), _dptr(sb)))
+            return {"eigenvalues": lam, "eigenvectors": vec, "pn": pn, "gan": gan, "sum_b": sb}
+        sp = np.ascontiguousarray(speeds, dtype=np.float64)
+        if sp.shape != (n,):
+            raise ValueError(f"speeds must have {n} entries")
+        deriv = np.empty(n)
+        self._check(self.lib.sdddst_ev_analysis(self.h, slot, _dptr(sp), float(stress_rate), _dptr(lam),
+                                                _dptr(vec) if vectors else None, _dptr(pn), _dptr(gan), _dptr(sb),
+                                                _dptr(deriv)))
+        return {"eigenvalues": lam, "eigenvectors": vec, "pn": pn, "gan": gan, "sum_b": sb, "deriv": deriv}
 
     def timing(self, on: bool, pair_only: bool = False):
         timing(self.h, on, pair_only)

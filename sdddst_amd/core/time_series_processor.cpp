@@ -4,7 +4,9 @@
 // record in the reference's column order and number formatting (a default
 // std::ostream: six significant digits):
 //   time, eigenvalues, participation numbers, s_i^2 / lambda_i, speeds, x,
-//   s_i = sum_j v_ij b_j, the first numberOfEigenVecToWrite eigenvectors
+//   s_i = sum_j v_ij b_j, the first numberOfEigenVecToWrite eigenvectors and,
+//   with calculateDerivativeEVAnal, the eigenvalue derivatives (zeros unless
+//   the protocol is fixed-rate-stress, time_series_processor.cpp:92-111)
 // or, with writeCorrelMatrices = W > 0, the first three groups followed per
 // mode by the three W x W histograms of v_i^2 v_j^2 over the pair's (dx, dy)
 // (same-sign pairs, opposite-sign pairs, all pairs).
@@ -58,9 +60,6 @@ TimeSeriesProcessor::TimeSeriesProcessor(std::shared_ptr<sdddstCore::SimulationD
 
 void TimeSeriesProcessor::run()
 {
-    if (sD->calculateDerivativeEVAnal)
-        throw std::runtime_error("EV analysis: the eigenvalue derivatives (--calculate-ev-deriv) are not available in "
-                                 "this build");
     if (!dTS) throw std::runtime_error("EV analysis: --file-list and --pd-configuration are required");
     gzFile f = gzopen(sD->eVAnalResultFile.c_str(), "wb");
     if (!f) throw std::runtime_error("EV analysis: cannot open " + sD->eVAnalResultFile);
@@ -85,8 +84,24 @@ void TimeSeriesProcessor::run()
             check(sdddst_ctx_create(n, pc, &p, &ctx.h), "sdddst_ctx_create");
             check(sdddst_upload_state(ctx.h, 0, x.data(), y.data(), b.data(), n), "upload");
             if (pc > 0) check(sdddst_upload_points(ctx.h, px.data(), py.data(), pc), "upload points");
-            std::vector<double> lam(n), vec((size_t)n * n), pn(n), gan(n), sum(n);
-            check(sdddst_ev_decompose(ctx.h, 0, lam.data(), vec.data(), pn.data(), gan.data(), sum.data()),
+            std::vector<double> lam(n), vec((size_t)n * n), pn(n), gan(n), sum(n), speeds, deriv;
+            const bool plain = sD->writeCorrelMatrices == 0;
+            if (plain) {
+                speeds.resize(n);
+                check(sdddst_speeds(ctx.h, 0, SDDDST_V_SCRATCH,
+                                    sD->externalStressProtocol->getStress(sdddstCore::StressProtocolStepType::Original),
+                                    0, speeds.data()),
+                      "speeds");
+            }
+            // the derivatives need the fixed-rate protocol's sigma'; other
+            // protocols write zeros
+            const bool with_deriv = plain && sD->calculateDerivativeEVAnal &&
+                                    sD->externalStressProtocol->getType() == "fixed-rate-stress";
+            if (plain && sD->calculateDerivativeEVAnal) deriv.assign(n, 0.0);
+            check(sdddst_ev_analysis(ctx.h, 0, with_deriv ? speeds.data() : nullptr,
+                                     with_deriv ? sD->externalStressProtocol->getStressDerivative(sD->simTime) : 1.0,
+                                     lam.data(), vec.data(), pn.data(), gan.data(), sum.data(),
+                                     with_deriv ? deriv.data() : nullptr),
                   "EV decomposition");
             auto v = [&](int mode, int j) { return vec[(size_t)mode * n + j]; };
             std::ostringstream out;
@@ -94,17 +109,13 @@ void TimeSeriesProcessor::run()
             for (int i = 0; i < n; ++i) out << " " << lam[i];   // EV
             for (int i = 0; i < n; ++i) out << " " << pn[i];    // PN
             for (int i = 0; i < n; ++i) out << " " << gan[i];   // 1/Gael (GAN)
-            if (sD->writeCorrelMatrices == 0) {
-                std::vector<double> speeds(n);
-                check(sdddst_speeds(ctx.h, 0, SDDDST_V_SCRATCH,
-                                    sD->externalStressProtocol->getStress(sdddstCore::StressProtocolStepType::Original),
-                                    0, speeds.data()),
-                      "speeds");
+            if (plain) {
                 for (int i = 0; i < n; ++i) out << " " << speeds[i];   // speeds
                 for (int i = 0; i < n; ++i) out << " " << x[i];        // X coordinates
                 for (int i = 0; i < n; ++i) out << " " << sum[i];      // SUM
                 for (int i = 0; i < sD->numberOfEigenVecToWrite && i < n; ++i)
                     for (int j = 0; j < n; ++j) out << " " << v(i, j);
+                for (const double d : deriv) out << " " << d;         // eigenvalue derivatives
                 out << "\n";
                 write(f, out.str());
                 continue;

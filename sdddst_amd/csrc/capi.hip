@@ -599,6 +599,16 @@ int sdddst_ev_decompose(sdddst_ctx *ctx, int cfg_slot, double *eigenvalues, doub
     return ev_decompose(ctx->c, cfg_slot, eigenvalues, eigenvectors, pn, gan, sum_b);
 }
 
+int sdddst_ev_analysis(sdddst_ctx *ctx, int cfg_slot, const double *speeds, double stress_rate, double *eigenvalues,
+                       double *eigenvectors, double *pn, double *gan, double *sum_b, double *deriv)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(valid_slot(cfg_slot), SDDDST_ERR_ARG, "bad slot");
+    SD_CHECK((speeds == nullptr) == (deriv == nullptr), SDDDST_ERR_ARG,
+             "EV analysis: speeds and deriv go together");
+    return ev_decompose(ctx->c, cfg_slot, eigenvalues, eigenvectors, pn, gan, sum_b, speeds, stress_rate, deriv);
+}
+
 int sdddst_timing(sdddst_ctx *ctx, int enable)
 {
     CTX_CHECK(ctx);

@@ -172,8 +172,10 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
 int pair_order_alloc(Context &c);   // buffers of the x ordering of the pair tiles
 int launch_ytrig(Context &c);       // cos / sin of 2 pi y after an upload of y
 // The EV analysis of the positions in slot (ev_analysis.hip); outputs nullable.
+// With speeds (host) and deriv both given, also the eigenvalue derivatives
+// deriv[m] = v_m^T C v_m / rate of the reference's fixed-rate option.
 int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors, double *pn, double *gan,
-                 double *sum_b);
+                 double *sum_b, const double *speeds = nullptr, double rate = 1.0, double *deriv = nullptr);
 // Waits until the work enqueued on the stream so far is done, copying `words`
 // 64-bit words from device address src (if any) to dst on the way: a signal
 // kernel publishes them in HostSignal and the host polls it (a bounded spin,

@@ -14,6 +14,17 @@
 //   observables    time_series_processor.cpp:40-58, 75-82: participation
 //                  number 1 / sum_j v_ij^4, s_i = sum_j v_ij b_j and
 //                  s_i^2 / lambda_i.
+//   derivatives    time_series_processor.cpp:92-111, 205-246: with the
+//                  third-derivative tensor T_ijk of calculateHessianDerivative
+//                  and speeds s, the reference writes per mode
+//                  sum_ij v_i v_j sum_k s_k T_ijk / sigma'.  The only nonzero
+//                  T are T_iii = -b_i sum_l b_l X_il, T_iik = b_i b_k X_ik,
+//                  T_iji = b_i b_j X_ij and T_ijj = -b_i b_j X_ij, with
+//                  X_ij = xy_diff_x2(wrap(x_i - x_j), wrap(y_i - y_j)), so
+//                  C_ij = sum_k s_k T_ijk = (s_i - s_j) b_i b_j X_ij off the
+//                  diagonal and C_ii = -sum_{k != i} C_ik: the dense C, one
+//                  dgemm W = C V and a dot per mode (O(n^3) where the
+//                  reference's loop is O(n^4)).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -60,6 +71,51 @@ __global__ void k_hessian_diag(int n, int pc, const double *__restrict__ x, cons
     H[(size_t)i * n + i] = d;
 }
 
+// C_ij = (s_i - s_j) b_i b_j xy_diff_x2(wrap(x_i - x_j), wrap(y_i - y_j))
+// for i != j; one thread per ordered pair, as the reference evaluates them.
+__global__ void k_deriv_pairs(int n, const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ b, const double *__restrict__ s, double *__restrict__ C)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (i >= n || j >= n || i == j) return;
+    const double dx = sdfield::normalize(x[i] - x[j]);
+    const double dy = sdfield::normalize(y[i] - y[j]);
+    C[(size_t)i * n + j] = (s[i] - s[j]) * (b[i] * b[j] * sdfield::xy_diff_x2(dx, dy));
+}
+
+// C_ii = -sum_{k != i} C_ik, one thread per row.
+__global__ void k_deriv_diag(int n, double *__restrict__ C)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const double *row = C + (size_t)i * n;
+    double acc = 0.0;
+    for (int k = 0; k < n; ++k)
+        if (k != i) acc += row[k];
+    C[(size_t)i * n + i] = -acc;
+}
+
+// deriv[m] = (sum_i W_mi V_mi) / rate with W = C V: row m of W (row-major)
+// is C applied to eigenvector m.  One workgroup per mode.
+__global__ __launch_bounds__(256) void k_deriv_modes(int n, const double *__restrict__ V,
+                                                     const double *__restrict__ W, double rate,
+                                                     double *__restrict__ deriv)
+{
+    __shared__ double r[256];
+    const int mode = (int)blockIdx.x, t = (int)threadIdx.x;
+    const double *v = V + (size_t)mode * n, *w = W + (size_t)mode * n;
+    double a = 0.0;
+    for (int j = t; j < n; j += 256) a += w[j] * v[j];
+    r[t] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) r[t] += r[t + off];
+        __syncthreads();
+    }
+    if (t == 0) deriv[mode] = r[0] / rate;
+}
+
 // Per mode (one workgroup each, fixed-order reductions): 1 / sum_j v^4,
 // s = sum_j v_j b_j and s^2 / lambda.
 __global__ __launch_bounds__(256) void k_mode_observables(int n, const double *__restrict__ V,
@@ -98,11 +154,12 @@ __global__ __launch_bounds__(256) void k_mode_observables(int n, const double *_
 }  // namespace
 
 int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors, double *pn, double *gan,
-                 double *sum_b)
+                 double *sum_b, const double *speeds, double rate, double *deriv)
 {
     const int n = c.n;
     const size_t nn = (size_t)n * n;
-    double *H = nullptr, *lam = nullptr, *E = nullptr, *obs = nullptr;
+    const bool with_deriv = speeds != nullptr && deriv != nullptr;
+    double *H = nullptr, *lam = nullptr, *E = nullptr, *obs = nullptr, *C = nullptr, *W = nullptr, *sp = nullptr;
     int *info = nullptr;
     rocblas_handle handle = nullptr;
     int rc = SDDDST_OK;
@@ -117,7 +174,10 @@ int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors
         return SDDDST_ERR_NOMEM;
     }
     if (hipMalloc(&lam, sizeof(double) * n) != hipSuccess || hipMalloc(&E, sizeof(double) * n) != hipSuccess ||
-        hipMalloc(&obs, sizeof(double) * 3 * n) != hipSuccess || hipMalloc(&info, sizeof(int)) != hipSuccess) {
+        hipMalloc(&obs, sizeof(double) * 4 * n) != hipSuccess || hipMalloc(&info, sizeof(int)) != hipSuccess ||
+        (with_deriv && (hipMalloc(&C, sizeof(double) * nn) != hipSuccess ||
+                        hipMalloc(&W, sizeof(double) * nn) != hipSuccess ||
+                        hipMalloc(&sp, sizeof(double) * n) != hipSuccess))) {
         (void)hipGetLastError();
         set_error("EV analysis: device allocation failed");
         rc = SDDDST_ERR_NOMEM;
@@ -155,12 +215,43 @@ int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors
         rc = SDDDST_ERR_HIP;
         goto done;
     }
+    if (with_deriv) {
+        if (hipMemcpyAsync(sp, speeds, sizeof(double) * n, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+            set_error("EV analysis: speeds upload failed");
+            rc = SDDDST_ERR_HIP;
+            goto done;
+        }
+        const dim3 blk(32, 8), grd((n + 31) / 32, (n + 7) / 8);
+        hipLaunchKernelGGL(k_deriv_pairs, grd, blk, 0, c.stream, n, c.x[slot], c.y, c.b, sp, C);
+        hipLaunchKernelGGL(k_deriv_diag, dim3((n + 127) / 128), dim3(128), 0, c.stream, n, C);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("EV analysis: derivative launch failed");
+            rc = SDDDST_ERR_HIP;
+            goto done;
+        }
+        // column-major views: C is symmetric, H holds eigenvector m in column
+        // m, so W = C H has C v_m in column m (row m of the row-major array)
+        const double one = 1.0, zero = 0.0;
+        if (rocblas_dgemm(handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, C, n, H, n, &zero,
+                          W, n) != rocblas_status_success) {
+            set_error("EV analysis: rocblas_dgemm failed");
+            rc = SDDDST_ERR_SOLVER;
+            goto done;
+        }
+        hipLaunchKernelGGL(k_deriv_modes, dim3(n), dim3(256), 0, c.stream, n, H, W, rate, obs + 3 * n);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("EV analysis: derivative launch failed");
+            rc = SDDDST_ERR_HIP;
+            goto done;
+        }
+    }
     if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
         (eigenvalues && hipMemcpyAsync(eigenvalues, lam, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream)) ||
         (eigenvectors && hipMemcpyAsync(eigenvectors, H, sizeof(double) * nn, hipMemcpyDeviceToHost, c.stream)) ||
         (pn && hipMemcpyAsync(pn, obs, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream)) ||
         (sum_b && hipMemcpyAsync(sum_b, obs + n, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream)) ||
         (gan && hipMemcpyAsync(gan, obs + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream)) ||
+        (with_deriv && hipMemcpyAsync(deriv, obs + 3 * n, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream)) ||
         hipStreamSynchronize(c.stream) != hipSuccess) {
         set_error("EV analysis: copy to the host failed");
         rc = SDDDST_ERR_HIP;
@@ -172,7 +263,7 @@ int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors
     }
 done:
     if (handle) rocblas_destroy_handle(handle);
-    for (void *p : {(void *)H, (void *)lam, (void *)E, (void *)obs, (void *)info})
+    for (void *p : {(void *)H, (void *)lam, (void *)E, (void *)obs, (void *)info, (void *)C, (void *)W, (void *)sp})
         if (p) (void)hipFree(p);
     return rc;
 }

@@ -139,6 +139,62 @@ SDLIBM_FN double xy_diff_x(double dx, double dy)
            f_dx(dx - N, cy, dy) + g;
 }
 
+// cos(p) for |p| <= 2 pi (sdlibm::cos covers the first two quadrants
+// glibc's reduction forms, n = round(|p| 2/pi) <= 2) in glibc's way: for
+// n = 3 and 4 the argument a + da = |p| - n pi/2 (high part exact by
+// Sterbenz, the constant's low part carried in da), then sin(a + da) for
+// n = 3 and cos(a + da) for n = 4 by the table method.
+SDLIBM_FN double cos_2pi(double p)
+{
+    const double ap = sdlibm::fabsd(p);
+    if (sdlibm::hiword(ap) < 0x400368fdu) return sdlibm::cos(p);   // |p| < 2.426265
+    const double t = ap * 0x1.45f306dc9c883p-1 + 0x1.8p52;
+    const int n = (int)(t - 0x1.8p52);
+    if (n <= 2) return sdlibm::cos(p);
+    const double hi = n == 3 ? 0x1.2d97c7f3321d2p2 : 0x1.921fb54442d18p2;   // 3 pi/2, 2 pi
+    const double lo = n == 3 ? 0x1.a79394c9e8a0ap-53 : 0x1.1a62633145c07p-52;
+    const double r = ap - hi;
+    const double a = r - lo;
+    const double da = (r - a) - lo;
+    return n == 3 ? sdlibm::detail::do_sin(a, da) : sdlibm::detail::do_cos(a, da);
+}
+
+// AnalyticField.cpp:95-106: the second x-derivative of f, as the reference
+// writes it (no small-r branch, and without the factor 2 pi^2 that f and
+// f_dx carry).
+SDLIBM_FN double f_dx2(double dx, double cos2piy, double dy)
+{
+    const double ch = sdlibm::cosh(kPi * 2.0 * dx);
+    const double sh = sdlibm::sinh(kPi * 2.0 * dx);
+    const double c4y = cos_2pi(4.0 * kPi * dy);
+    const double ch2 = sdlibm::cosh(kPi * 4.0 * dx);
+    const double sh2 = sdlibm::sinh(kPi * 4.0 * dx);
+    const double num = sh * cos2piy * (cos2piy * cos2piy - 2.0) + 2.0 * kPi * dx * (sh * sh) * (c4y - 2.0) -
+                       kPi * dx * sdlibm::powi(ch, 3) * cos2piy +
+                       0.5 * kPi * dx * ch * cos2piy * (2.0 * ch2 + c4y - 5.0) + (ch * ch) * (2.0 * kPi * dx - sh * cos2piy) +
+                       sh2;
+    return (4.0 * kPi * num) / sdlibm::powi(cos2piy - ch, 4);
+}
+
+// AnalyticField.cpp:183-202: the reference's d^2 tau_xy / dx^2 (images of
+// f_dx2, boundary images through f_dx as well), used only by the
+// eigenvalue-derivative option of the EV analysis.
+SDLIBM_FN double xy_diff_x2(double dx, double dy)
+{
+    const double cy = sdlibm::cos(kPi * 2.0 * dy);
+    const double N = (double)kImages;
+    double g = f_dx2(dx, cy, dy);
+    for (int k = 1; k < kImages; ++k) g = g + f_dx2(dx - (double)k, cy, dy) + f_dx2(dx + (double)k, cy, dy);
+    if (dx < 0) {
+        const double a = dx + N + 1.0;
+        return -2.0 * f_dx(a, cy, dy) - dx * f_dx2(a, cy, dy) + f_dx2(dx - N, cy, dy) * (1.0 + dx) +
+               2.0 * f_dx(dx - N, cy, dy) + f_dx2(dx + N, cy, dy) + g;
+    }
+    const double a = dx - N - 1.0;
+    return 2.0 * f_dx(a, cy, dy) + dx * f_dx2(a, cy, dy) + f_dx2(dx + N, cy, dy) * (1.0 - dx) -
+           2.0 * f_dx(dx + N, cy, dy) + f_dx2(dx - N, cy, dy) + g;
+}
+
 
 // cos and sin of 2 pi y for y wrapped into [-0.5, 0.5] (exactly: y - rint(y)).
 SDLIBM_FN void ytrig(double y, double &c, double &s)

@@ -1,6 +1,7 @@
 """Write the golden vectors of tests/golden/ from the reference's own
 translation units (oracle/_ref/libsdddst_ref.so, built by `make -C oracle`
-from the reference sources): AnalyticField::xy / xy_diff_x at fixed points,
+from the reference sources): AnalyticField::xy / xy_diff_x / xy_diff_x2 at
+fixed points,
 the reference's pair loop (simulation.cpp:86-131 order over
 AnalyticField::xy) at N=256, PrecisionHandler sequences and the stress
 protocols.  Inputs are seeded; outputs are stored bit-exactly in .npz.
@@ -42,6 +43,9 @@ def main():
     xyd = np.empty_like(dx)
     ref.ref_xy_batch(len(dx), O.dp(dx), O.dp(dy), O.dp(xy), O.dp(xyd))
     np.savez(os.path.join(HERE, "field_points.npz"), dx=dx, dy=dy, xy=xy, xy_diff_x=xyd)
+    # AnalyticField::xy_diff_x2 (the EV analysis' eigenvalue derivatives)
+    xyd2 = np.array([ref.ref_xy_diff_x2(float(u), float(v)) for u, v in zip(dx, dy)])
+    np.savez(os.path.join(HERE, "field_points_dx2.npz"), dx=dx, dy=dy, xy_diff_x2=xyd2)
 
     n = 256
     x, y, b = configs.dislocations(n, 1)

@@ -16,6 +16,8 @@ double mc_normalize(double d) { return sdfield::normalize(d); }
 double mc_xy(double dx, double dy) { return sdfield::xy(dx, dy); }
 double mc_xy_fast(double dx, double dy) { return sdfield::xy_fast(dx, dy); }
 double mc_xy_diff_x(double dx, double dy) { return sdfield::xy_diff_x(dx, dy); }
+double mc_xy_diff_x2(double dx, double dy) { return sdfield::xy_diff_x2(dx, dy); }
+double mc_cos_2pi(double p) { return sdfield::cos_2pi(p); }
 double mc_pd_force_dx(double dx, double dy, double A, double K) { return sdfield::pd_force_dx(dx, dy, A, K); }
 
 // Batched forms: fn = 0 cos, 1 sin, 2 exp, 3 expm1, 4 cosh, 5 sinh, 6 normalize
@@ -53,14 +55,15 @@ void mc_batch_pair_far(long n, const double *xi, const double *yi, const double 
     }
 }
 
-// fn = 0 xy, 1 xy_fast, 2 xy_diff_x, 3 xy_diff_x_fast
+// fn = 0 xy, 1 xy_fast, 2 xy_diff_x, 3 xy_diff_x_fast, 4 xy_diff_x2
 void mc_batch2(int fn, long n, const double *dx, const double *dy, double *out)
 {
     for (long i = 0; i < n; ++i) {
         if (fn == 0) out[i] = sdfield::xy(dx[i], dy[i]);
         else if (fn == 1) out[i] = sdfield::xy_fast(dx[i], dy[i]);
         else if (fn == 2) out[i] = sdfield::xy_diff_x(dx[i], dy[i]);
-        else out[i] = sdfield::xy_diff_x_fast(dx[i], dy[i]);
+        else if (fn == 3) out[i] = sdfield::xy_diff_x_fast(dx[i], dy[i]);
+        else out[i] = sdfield::xy_diff_x2(dx[i], dy[i]);
     }
 }
 }

@@ -49,6 +49,7 @@ def oracle():
         "orc_normalize": (D, [D]),
         "orc_xy": (D, [D, D]),
         "orc_xy_diff_x": (D, [D, D]),
+        "orc_xy_diff_x2": (D, [D, D]),
         "orc_new_step_size": (D, [D, D]),
         "orc_ph_update_tolerance": (None, [DP, D, D, ctypes.c_uint]),
         "orc_max_error_ratio": (D, [ctypes.c_int, DP, DP, DP, IP]),
@@ -105,6 +106,7 @@ def reference():
         "ref_normalize": (D, [D]),
         "ref_xy": (D, [D, D]),
         "ref_xy_diff_x": (D, [D, D]),
+        "ref_xy_diff_x2": (D, [D, D]),
         "ref_xy_batch": (None, [L, DP, DP, DP, DP]),
         "ref_speeds": (None, [ctypes.c_int, DP, DP, DP, D, DP]),
         "ref_precision_sequence": (D, [ctypes.c_uint, D, ctypes.c_int, DP, ctypes.POINTER(ctypes.c_uint),
@@ -135,6 +137,8 @@ def mathcheck():
     lib.mc_batch_pair_far.argtypes = [L, DP, DP, DP, DP, DP]
     lib.mc_normalize.restype = D
     lib.mc_normalize.argtypes = [D]
+    lib.mc_cos_2pi.restype = D
+    lib.mc_cos_2pi.argtypes = [D]
     lib.mc_powi.restype = D
     lib.mc_powi.argtypes = [D, ctypes.c_int]
     _mc = lib

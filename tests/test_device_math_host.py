@@ -83,6 +83,38 @@ def test_field_fast_and_derivative():
     assert np.max(np.abs(df - xyd) / (np.abs(xyd) + 1.0)) < 1e-14
 
 
+def test_field_second_derivative():
+    # the EV analysis' xy_diff_x2 (AnalyticField.cpp:183-202) against the
+    # oracle: the same glibc-identical cosh, sinh and cos (cos(4 pi dy) over
+    # the quadrants |p| <= 2 pi), integer powers through the correctly
+    # rounded powi.  Where glibc's pow(x, 3.0) is an ulp off the correctly
+    # rounded cube, the reference's numerator, which cancels as r -> 0
+    # (no small-r branch), magnifies that ulp: ~1e-10 at r = 0.008, below
+    # 1e-14 beyond r = 0.05
+    rng = np.random.default_rng(13)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, 20_000), rng.uniform(-2e-2, 2e-2, 4_000)])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, 20_000), rng.uniform(-2e-2, 2e-2, 4_000)])
+    lib = O.oracle()
+    want = np.array([lib.orc_xy_diff_x2(u, v) for u, v in zip(dx, dy)])
+    got = _mc2(4, dx, dy)
+    assert np.mean(got.view(np.int64) == want.view(np.int64)) > 0.995
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1.0)
+    assert rel[np.hypot(dx, dy) > 0.05].max() < 1e-14
+    assert rel.max() < 1e-9
+
+
+def test_cos_over_two_pi_matches_glibc():
+    # cos(4 pi dy) of xy_diff_x2: |p| up to 2 pi, quadrant boundaries included
+    rng = np.random.default_rng(14)
+    p = np.concatenate([4 * np.pi * rng.uniform(-0.5, 0.5, 200_000),
+                        np.array([np.pi, -np.pi, 2 * np.pi, -2 * np.pi, 1.25 * np.pi, 1.75 * np.pi, 2.426265,
+                                  np.nextafter(1.25 * np.pi, 0), np.nextafter(1.75 * np.pi, 9)])])
+    mc = O.mathcheck()
+    got = np.array([mc.mc_cos_2pi(float(v)) for v in p])
+    want = O.libm_batch(0, p)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+
+
 def test_far_pair_term():
     # the pair kernel's far pairs (circular |dx| >= 0.1) take cos(2 pi dy) as
     # cos a_i cos a_j + sin a_i sin a_j and e^{+-2 pi dx} as products of

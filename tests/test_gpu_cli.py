@@ -116,3 +116,22 @@ def test_cli_ev_analysis_series(tmp_path):
         np.testing.assert_allclose(row[1 + 4 * n:1 + 5 * n], x, rtol=1e-5, atol=1e-6)
         vecs = row[1 + 6 * n:].reshape(K, n)
         np.testing.assert_allclose(np.abs(vecs), np.abs(ev["eigenvectors"][:K]), rtol=1e-4, atol=1e-6)
+    # --calculate-ev-deriv (time_series_processor.cpp:92-111): the same
+    # records with the eigenvalue derivatives appended, from the speeds under
+    # the fixed-rate stress 2 t (point-defect constants scaled by the
+    # snapshot's size, data_time_series.cpp) and sigma' = 2
+    out2 = tmp_path / "ev_deriv.gz"
+    r = _cli("--ev-analyzation", "--result-ev-file", str(out2), "--file-list", str(lst), "--pd-configuration",
+             str(pfile), "--fixed-rate-external-stress", "2", "--eigenvector-to-write", str(K),
+             "--calculate-ev-deriv")
+    rows2 = [np.array(list(map(float, ln.split()))) for ln in gzip.open(out2, "rt").read().splitlines()]
+    assert len(rows2) == 2
+    for (t, x, y, b), row, row2 in zip(snaps, rows, rows2):
+        assert len(row2) == len(row) + n and np.array_equal(row2[:len(row)], row)
+        with S.Context(n, pc=pc, A=1e-4 * 16.0 / np.sqrt(n), KASQR=1.65 * 1.65 * 1e6 / 256.0 * n) as c:
+            c.upload_state(0, x, y, b)
+            c.upload_points(px, py)
+            v = c.speeds(0, S.V_SCRATCH, sigma=2.0 * t, update_tol=False)
+            np.testing.assert_allclose(row[1 + 3 * n:1 + 4 * n], v, rtol=1e-5, atol=1e-6 * np.abs(v).max())
+            d = c.ev_decompose(0, speeds=v, stress_rate=2.0)["deriv"]
+        np.testing.assert_allclose(row2[len(row):], d, rtol=1e-5, atol=1e-6 * np.abs(d).max())

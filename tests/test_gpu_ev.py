@@ -1,7 +1,8 @@
 """EV analysis (spectral_decompositor.cpp:63-201, time_series_processor.cpp:
-40-82): the device Hessian, eigen-decomposition and per-mode observables
-against a numpy restatement on the oracle's field derivative and point-defect
-term, with numpy's symmetric eigensolver."""
+40-111, 205-246): the device Hessian, eigen-decomposition, per-mode
+observables and eigenvalue derivatives against a numpy restatement on the
+oracle's field derivatives and point-defect term, with numpy's symmetric
+eigensolver."""
 import numpy as np
 import pytest
 
@@ -43,6 +44,80 @@ def oracle_hessian(x, y, b, px=None, py=None):
             d -= H[j, i]
         H[i, i] = d
     return H
+
+
+def _x2_matrix(x, y):
+    """X_ij = xy_diff_x2(wrap(x_i - x_j), wrap(y_i - y_j)) for i != j."""
+    lib = O.oracle()
+    n = len(x)
+    X = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if i != j:
+                X[i, j] = lib.orc_xy_diff_x2(lib.orc_normalize(float(x[i] - x[j])),
+                                             lib.orc_normalize(float(y[i] - y[j])))
+    return X
+
+
+def oracle_deriv_matrix(x, y, b, s):
+    """C_ij = sum_k s_k T_ijk over calculateHessianDerivative's tensor
+    (time_series_processor.cpp:205-246), reduced: (s_i - s_j) b_i b_j X_ij
+    off the diagonal, minus the row's sum on it."""
+    C = (s[:, None] - s[None, :]) * np.outer(b, b) * _x2_matrix(x, y)
+    np.fill_diagonal(C, 0.0)
+    np.fill_diagonal(C, -C.sum(axis=1))
+    return C
+
+
+def test_deriv_matrix_reduces_the_reference_tensor():
+    # the reference's per-mode sum over i, j, k of v_i v_j s_k T_ijk / sigma',
+    # with T built term by term as calculateHessianDerivative builds it,
+    # against the reduced matrix
+    n = 9
+    x, y, b = configs.dislocations(n, 5)
+    s = np.random.default_rng(6).uniform(-1, 1, n)
+    X = _x2_matrix(x, y)
+    T = np.zeros((n, n, n))
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                if i == j == k:
+                    T[i, j, k] = -b[i] * sum(b[m] * X[i, m] for m in range(n) if m != i)
+                elif i == j:
+                    T[i, j, k] = b[i] * b[k] * X[i, k]
+                elif i == k:
+                    T[i, j, k] = b[i] * b[j] * X[i, j]
+                elif j == k:
+                    T[i, j, k] = -b[i] * b[j] * X[i, j]
+    C = oracle_deriv_matrix(x, y, b, s)
+    np.testing.assert_allclose(np.einsum("ijk,k->ij", T, s), C, rtol=1e-12, atol=1e-12 * np.abs(C).max())
+    _, V = np.linalg.eigh(oracle_hessian(x, y, b))
+    rate = 0.7
+    ref = np.array([sum(V[i, m] * V[j, m] / rate * (T[i, j] @ s) for i in range(n) for j in range(n))
+                    for m in range(n)])
+    np.testing.assert_allclose(np.einsum("im,ij,jm->m", V, C, V) / rate, ref, rtol=1e-10,
+                               atol=1e-12 * np.abs(ref).max())
+
+
+def test_deriv_is_the_eigenvalue_slope_along_the_speeds():
+    # C is the derivative of the Hessian (no point defects) along x + e s, so
+    # v^T C v is the slope of the eigenvalue; the reference's xy_diff_x2
+    # lacks the field's factor 2 pi^2 (AnalyticField.cpp:95-106), which the
+    # slope carries
+    n = 24
+    x, y, b = configs.dislocations(n, 7)
+    s = np.random.default_rng(8).uniform(-1, 1, n)
+    lam, V = np.linalg.eigh(oracle_hessian(x, y, b))
+    d = np.einsum("im,ij,jm->m", V, oracle_deriv_matrix(x, y, b, s), V) * 2.0 * np.pi ** 2
+    e = 1e-6
+    lp = np.linalg.eigvalsh(oracle_hessian(x + e * s, y, b))
+    lm = np.linalg.eigvalsh(oracle_hessian(x - e * s, y, b))
+    fd = (lp - lm) / (2 * e)
+    scale = np.abs(lam).max()
+    gaps = np.minimum(np.r_[np.inf, np.diff(lam)], np.r_[np.diff(lam), np.inf])
+    sep = gaps > 1e-3 * scale
+    assert sep.sum() > n // 2
+    assert np.abs(fd - d)[sep].max() <= 1e-5 * np.abs(d).max()
 
 
 def test_oracle_hessian_is_symmetric_and_translation_invariant():
@@ -99,3 +174,28 @@ def test_ev_decompose_matches_oracle(pc):
         k = np.argmin(np.abs(lam))
         assert abs(lam[k]) <= 1e-10 * scale
         assert abs(V[k].sum()) / np.sqrt(n) > 1 - 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pc", [0, 24])
+def test_ev_derivatives_match_oracle(pc):
+    # the device's C (xy_diff_x2 per ordered pair), dgemm and per-mode dots
+    # against the oracle's C on the device's own eigenvectors
+    n, rate = 128, 2.5
+    x, y, b = configs.dislocations(n, 33)
+    s = np.random.default_rng(34).uniform(-1, 1, n)
+    kw = dict(pc=pc) if pc else {}
+    with S.Context(n, **kw) as c:
+        c.upload_state(0, x, y, b)
+        if pc:
+            px, py = configs.point_defects(n, pc, 33)
+            c.upload_points(px, py)
+        plain = c.ev_decompose(0)
+        out = c.ev_decompose(0, speeds=s, stress_rate=rate)
+    for k in ("eigenvalues", "eigenvectors", "pn", "gan", "sum_b"):
+        assert np.array_equal(out[k], plain[k])
+    V = out["eigenvectors"]
+    C = oracle_deriv_matrix(x, y, b, s)
+    want = np.einsum("mi,ij,mj->m", V, C, V) / rate
+    atol = 1e-12 * np.abs(C).sum(axis=1).max() / rate
+    np.testing.assert_allclose(out["deriv"], want, rtol=1e-10, atol=atol)

@@ -28,6 +28,13 @@ def test_golden_field_points():
     assert _bits_equal(xyd, g["xy_diff_x"])
 
 
+def test_golden_field_points_dx2():
+    g = np.load(os.path.join(GOLDEN, "field_points_dx2.npz"))
+    lib = O.oracle()
+    got = np.array([lib.orc_xy_diff_x2(float(u), float(v)) for u, v in zip(g["dx"], g["dy"])])
+    assert _bits_equal(got, g["xy_diff_x2"])
+
+
 def test_golden_speeds_reference_loop():
     g = np.load(os.path.join(GOLDEN, "speeds_n256_seed1.npz"))
     v0, _ = O.speeds(g["x"], g["y"], g["b"], sigma=0.0)
@@ -91,6 +98,15 @@ def test_reference_field_bit_identical(ref):
     assert _bits_equal(a, ra) and _bits_equal(b, rb)
 
 
+def test_reference_xy_diff_x2_bit_identical(ref):
+    rng = np.random.default_rng(5)
+    lib = O.oracle()
+    pts = np.concatenate([rng.uniform(-0.5, 0.5, (20000, 2)), rng.uniform(-2e-3, 2e-3, (4000, 2))])
+    got = np.array([lib.orc_xy_diff_x2(u, v) for u, v in pts])
+    want = np.array([ref.ref_xy_diff_x2(u, v) for u, v in pts])
+    assert _bits_equal(got, want)
+
+
 def test_reference_normalize(ref):
     lib = O.oracle()
     for v in np.concatenate([np.random.default_rng(4).uniform(-5, 5, 20000),
@@ -141,6 +157,40 @@ def test_xy_diff_x_equals_derivative(dx, dy):
     d = float(mpmath.diff(lambda t: _xy_mp(t, dy), mpmath.mpf(dx)))
     v = O.oracle().orc_xy_diff_x(dx, dy)
     assert abs(v - d) <= 1e-9 * (1.0 + abs(d))
+
+
+@pytest.mark.parametrize("dx,dy", [(0.1, 0.2), (-0.3, 0.05)])
+def test_xy_diff_x2_against_wall_derivatives(dx, dy):
+    # AnalyticField.cpp:95-106 leaves out the factor 2 pi^2 that f and f_dx
+    # carry, so every f_dx2 term of xy_diff_x2 is the wall's second
+    # derivative over 2 pi^2, while the two boundary terms 2 f_dx keep it:
+    # with d2 the second derivative of the image structure and e its part
+    # 2 (f'(dx-4) - f'(dx+5)) for dx < 0, 2 (f'(dx-5) - f'(dx+4)) otherwise,
+    # the reference's value is (d2 - e) / (2 pi^2) + e; walls and their
+    # derivatives from the y-image sum in mpmath
+    F = lambda a: _wall_mp(a, dy)  # noqa: E731
+    F1 = lambda a: mpmath.diff(F, mpmath.mpf(a), 1)  # noqa: E731
+    d2 = mpmath.diff(lambda t: _xy_mp(t, dy), mpmath.mpf(dx), 2)
+    extra = 2 * (F1(dx - 4) - F1(dx + 5)) if dx < 0 else 2 * (F1(dx - 5) - F1(dx + 4))
+    want = float((d2 - extra) / (2 * mpmath.pi ** 2) + extra)
+    v = O.oracle().orc_xy_diff_x2(dx, dy)
+    assert abs(v - want) <= 1e-13 * (1.0 + abs(want))
+    # the factor itself: 2 pi^2 xy_diff_x2 is the central difference of the
+    # oracle's first derivative up to the boundary terms (1e-8 here)
+    lib, h = O.oracle(), 1e-5
+    fd = (lib.orc_xy_diff_x(dx + h, dy) - lib.orc_xy_diff_x(dx - h, dy)) / (2 * h)
+    assert abs(v * 2.0 * np.pi ** 2 - fd) <= 1e-8 * (1.0 + abs(fd))
+
+
+def test_xy_diff_x2_symmetries():
+    # second x-derivative of a field odd in x and even in y: odd in x, even in y
+    rng = np.random.default_rng(12)
+    lib = O.oracle()
+    for _ in range(2000):
+        dx, dy = rng.uniform(-0.5, 0.5, 2)
+        a = lib.orc_xy_diff_x2(dx, dy)
+        assert abs(lib.orc_xy_diff_x2(-dx, dy) + a) <= 1e-11 * (1 + abs(a))
+        assert abs(lib.orc_xy_diff_x2(dx, -dy) - a) <= 1e-11 * (1 + abs(a))
 
 
 def test_xy_symmetries():

@@ -17,7 +17,8 @@ network), a few steps of the direct-solver path and the batched ensemble of
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precond learned]
 Prints one JSON line.  `python bench.py --copy-calibration` runs only a plain
 device copy of known size (the reference kernel for scaling the HBM byte
-counters of a profiler run).
+counters of a profiler run); `python bench.py --ev-analysis N` times only the
+EV analysis of one N-dislocation snapshot.
 """
 from __future__ import annotations
 
@@ -64,6 +65,9 @@ def parse():
                    help="also time this many steps of the direct-solver path (0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--ev-analysis", type=int, metavar="N", default=0,
+                   help="only time the EV analysis of one N-dislocation snapshot (Hessian, eigen-decomposition, "
+                        "observables, eigenvalue derivatives) and print its own JSON line")
     p.add_argument("--trace-gaps", metavar="DIR",
                    help="no GPU work: summarise the device idle time of a rocprofv3 --kernel-trace CSV "
                         "output directory (gaps between kernels over the last 60 %% of the trace)")
@@ -263,6 +267,62 @@ def cpu_baseline(x, y, b, seconds):
             "pairs_per_s": pairs_per_s, "cpu": platform.processor() or platform.machine()}
 
 
+def ev_cpu_baseline(x, y, b, seconds_cap=60.0):
+    """The CPU side of one EV snapshot as the reference runs it, minus the
+    eigenvalue-derivative loop: the Hessian from the oracle's tau_x (numpy
+    diagonal) and LAPACK's symmetric eigensolver through numpy, on the
+    threads numpy's BLAS uses.  The reference's derivative loop is O(N^4)
+    (N^4 tensor terms per snapshot) and is not timed."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib as O
+    n = len(x)
+    t0 = time.perf_counter()
+    I, J = np.tril_indices(n, -1)
+    d, e = x[I] - x[J], y[I] - y[J]
+    _, xyd = O.xy_batch(d - np.floor(d + 0.5), e - np.floor(e + 0.5))
+    H = np.zeros((n, n))
+    H[I, J] = b[I] * b[J] * xyd
+    H[J, I] = H[I, J]
+    np.fill_diagonal(H, -H.sum(axis=1))
+    t1 = time.perf_counter()
+    np.linalg.eigh(H)
+    t2 = time.perf_counter()
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count()
+    return {"hessian_s": t1 - t0, "eigh_s": t2 - t1, "kind": "port", "cores": threads,
+            "sample": f"oracle tau_x Hessian of the same N={n} snapshot + numpy.linalg.eigh (LAPACK syevd); "
+                      "derivative loop (O(N^4)) not timed"}
+
+
+def ev_run(args):
+    """One EV snapshot on the device: sdddst_ev_decompose (dense Hessian,
+    rocSOLVER dsyevd, per-mode observables) and sdddst_ev_analysis with the
+    eigenvalue derivatives; one untimed call first (library set-up), then
+    the mean of three."""
+    import sdddst_amd as S
+    from sdddst_amd import configs
+    n = args.ev_analysis
+    x, y, b = configs.dislocations(n, 2)
+    with S.Context(n) as c:
+        c.upload_state(0, x, y, b)
+        v = c.speeds(0, S.V_SCRATCH, sigma=0.0, update_tol=False)
+        c.ev_decompose(0, speeds=v, stress_rate=1.0)
+        times = {}
+        for label, kw in (("decompose_s", {}), ("with_derivatives_s", dict(speeds=v, stress_rate=1.0))):
+            t0 = time.perf_counter()
+            for _ in range(3):
+                out = c.ev_decompose(0, **kw)
+            times[label] = (time.perf_counter() - t0) / 3
+    rec = {"ev_analysis": {"n": n, "data": "synthetic (SplitMix64 seed 2 positions, balanced +/-1 Burgers vectors)",
+                           **times, "eigenvalue_range": [float(out["eigenvalues"][0]), float(out["eigenvalues"][-1])],
+                           "reference_derivative_terms": float(n) ** 4}}
+    if not args.no_cpu_baseline:
+        rec["ev_analysis"]["cpu_baseline"] = ev_cpu_baseline(x, y, b)
+    print(json.dumps(rec), flush=True)
+
+
 def large_run(args):
     """N = 65536 dislocations (seed 5) + 2048 point defects (seed 5) with
     A and KASQR scaled as the command line scales them, m = 2, precision
@@ -362,6 +422,12 @@ def main():
         return
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.ev_analysis > 0:
+        if args.gpus != 1:
+            print("bench.py: this build measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        ev_run(args)
         return
     if args.workload == "n16384-m4-zero-learned":
         if args.gpus != 1:

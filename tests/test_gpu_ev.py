@@ -199,3 +199,22 @@ def test_ev_derivatives_match_oracle(pc):
     want = np.einsum("mi,ij,mj->m", V, C, V) / rate
     atol = 1e-12 * np.abs(C).sum(axis=1).max() / rate
     np.testing.assert_allclose(out["deriv"], want, rtol=1e-10, atol=atol)
+
+
+@pytest.mark.gpu
+def test_ev_analysis_rejects_speeds_without_outputs():
+    # speeds and deriv go together; a lone one is an argument error, not a
+    # silent decomposition without derivatives
+    import ctypes
+    n = 32
+    x, y, b = configs.dislocations(n, 35)
+    with S.Context(n) as c:
+        c.upload_state(0, x, y, b)
+        lam = np.empty(n)
+        sp = np.zeros(n)
+        dp = ctypes.POINTER(ctypes.c_double)
+        rc = c.lib.sdddst_ev_analysis(c.h, 0, sp.ctypes.data_as(dp), 1.0, lam.ctypes.data_as(dp), None, None, None,
+                                      None, None)
+        assert rc != 0 and b"speeds and deriv go together" in c.lib.sdddst_last_error()
+        out = c.ev_decompose(0, vectors=False)
+        assert np.all(np.diff(out["eigenvalues"]) >= 0)

@@ -51,7 +51,9 @@ __global__ void k_hessian_pairs(int n, const double *__restrict__ x, const doubl
     H[(size_t)j * n + i] = t;
 }
 
-// The diagonal, one thread per row, in the reference's operation order.
+// The diagonal, one thread per row, in the reference's operation order.  The
+// lower row's entries are read from column i (H[j][i] holds the same bits as
+// H[i][j]), so neighbouring threads read neighbouring addresses.
 __global__ void k_hessian_diag(int n, int pc, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ b, const double *__restrict__ px,
                                const double *__restrict__ py, double A, double K, double *__restrict__ H)
@@ -59,7 +61,7 @@ __global__ void k_hessian_diag(int n, int pc, const double *__restrict__ x, cons
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     double subSum = 0;
-    for (int j = 0; j < i; ++j) subSum += H[(size_t)i * n + j];
+    for (int j = 0; j < i; ++j) subSum += H[(size_t)j * n + i];
     double d = 0.0;
     d -= subSum;
     for (int l = 0; l < pc; ++l) {

@@ -967,6 +967,34 @@ void orc_sim_get_speeds(const orc_sim *s, double *initSpeed, double *initSpeed2)
     if (initSpeed2) memcpy(initSpeed2, s->initSpeed2, (size_t)s->n * sizeof(double));
 }
 
+/* Stage I positions of the last step (the big step, simulation.cpp:482-488). */
+void orc_sim_get_big_x(const orc_sim *s, double *out) { memcpy(out, s->bigx, (size_t)s->n * sizeof(double)); }
+
+/* Entry points for the mathematical pins of the stepping path (tests only).
+ * orc_sim_residual: one calculateG (simulation.cpp:133-193) of the stage-I
+ * kind at the given new/old positions, with dVec and the old speeds given
+ * (so the residual can be differentiated with them held fixed); the new
+ * speeds are computed at newx as the stepping path computes them. */
+void orc_sim_residual(orc_sim *s, double stepsize, const double *newx, const double *oldx, const double *dVec,
+                      const double *v_old, double *g_out)
+{
+    memcpy(s->dVec, dVec, (size_t)s->n * sizeof(double));
+    memcpy(s->initSpeed, v_old, (size_t)s->n * sizeof(double));
+    calculate_g(s, stepsize, newx, oldx, 0, 0, 0, ORC_ORIGINAL, ORC_END_BIG);
+    memcpy(g_out, s->g, (size_t)s->n * sizeof(double));
+}
+
+/* orc_sim_stage: one integrate (simulation.cpp:62-84) of stage-I kind from
+ * the simulation's current positions with step size h: newx receives the
+ * end positions, dVec_out (nullable) the d vector of the assembly. */
+int orc_sim_stage(orc_sim *s, double h, double *newx, double *dVec_out)
+{
+    memcpy(newx, s->x, (size_t)s->n * sizeof(double));
+    int rc = integrate(s, h, newx, s->x, 0, 1, ORC_ORIGINAL, ORC_END_BIG);
+    if (dVec_out) memcpy(dVec_out, s->dVec, (size_t)s->n * sizeof(double));
+    return rc;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Batched evaluations for the tests                                        */
 /* ---------------------------------------------------------------------- */

@@ -12,7 +12,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# SDDDST_ORACLE_SO points the tests at another build of the oracle (a copy
+# with a planted mistake, to see which pins catch it)
+ORACLE_SO = os.environ.get("SDDDST_ORACLE_SO", os.path.join(ORACLE_DIR, "liboracle.so"))
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libsdddst_ref.so")
 MATHCHECK_SO = os.path.join(ROOT, "tests", "native", "libmathcheck.so")
 
@@ -78,6 +80,9 @@ def oracle():
         "orc_sim_log_rows": (L, [VP]),
         "orc_sim_get_log": (None, [VP, DP]),
         "orc_sim_get_speeds": (None, [VP, DP, DP]),
+        "orc_sim_get_big_x": (None, [VP, DP]),
+        "orc_sim_residual": (None, [VP, D, DP, DP, DP, DP, DP]),
+        "orc_sim_stage": (ctypes.c_int, [VP, D, DP, DP]),
         "orc_libm_batch": (None, [ctypes.c_int, L, DP, DP]),
         "orc_xy_batch": (None, [L, DP, DP, DP, DP]),
         "orc_speeds_abs": (None, [ctypes.c_int, DP, DP, DP, DP]),
@@ -261,6 +266,26 @@ class OracleSim:
         out = np.empty(self.n)
         self.lib.orc_sim_get_x(self.h, dp(out))
         return out
+
+    def big_x(self):
+        out = np.empty(self.n)
+        self.lib.orc_sim_get_big_x(self.h, dp(out))
+        return out
+
+    def residual(self, h, newx, oldx, dvec, v_old):
+        """One calculateG of the stage-I kind with dVec and the old speeds given."""
+        g = np.empty(self.n)
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (newx, oldx, dvec, v_old)]
+        self.lib.orc_sim_residual(self.h, h, *(dp(a) for a in args), dp(g))
+        return g
+
+    def stage(self, h):
+        """One integrate of the stage-I kind from the current positions:
+        returns (end positions, d vector)."""
+        newx, dv = np.empty(self.n), np.empty(self.n)
+        if self.lib.orc_sim_stage(self.h, h, dp(newx), dp(dv)) != 0:
+            raise RuntimeError("oracle integrate failed")
+        return newx, dv
 
     def log(self):
         rows = self.lib.orc_sim_log_rows(self.h)

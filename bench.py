@@ -49,8 +49,9 @@ def parse():
     p.add_argument("--solver", default="fgmres", choices=["fgmres", "dense-lu", "auto", "umfpack"])
     p.add_argument("--field-mode", default="fast", choices=["fast", "exact"])
     p.add_argument("--precond", default="learned", choices=["pair", "jacobi", "learned"])
-    p.add_argument("--compare-steps", type=int, default=200,
-                   help="steps of the same workload with the pair-block preconditioner (0: skip)")
+    p.add_argument("--compare-steps", type=int, default=None,
+                   help="steps of the same workload with the pair-block preconditioner, from t = 0 like the "
+                        "headline (default: --steps, so both rates cover the same window; 0: skip)")
     p.add_argument("--ensemble-steps", type=int, default=500,
                    help="steps per realization of the 256-realization ensemble (0: skip)")
     p.add_argument("--workload", default="n16384-m2-const0.02", choices=["n16384-m2-const0.02",
@@ -61,17 +62,25 @@ def parse():
                         "learned preconditioner, 100 steps from t = 0, no warm-up)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run only a 512 MiB device-to-device copy kernel and exit")
-    p.add_argument("--plain-steps", type=int, default=2,
+    p.add_argument("--plain-steps", type=int, default=5,
                    help="also time this many steps of the direct-solver path (0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--assembly-reps", type=int, default=10,
+                   help="full Jacobian assemblies of the headline positions timed on their own (0: skip)")
+    p.add_argument("--assembly-only", action="store_true",
+                   help="run only the Jacobian assembly leg and print its record")
+    p.add_argument("--cpu-seconds", type=float, default=8.0,
+                   help="seconds of the pair-speed row sample of the CPU baseline")
     p.add_argument("--ev-analysis", type=int, metavar="N", default=0,
                    help="only time the EV analysis of one N-dislocation snapshot (Hessian, eigen-decomposition, "
                         "observables, eigenvalue derivatives) and print its own JSON line")
     p.add_argument("--trace-gaps", metavar="DIR",
                    help="no GPU work: summarise the device idle time of a rocprofv3 --kernel-trace CSV "
                         "output directory (gaps between kernels over the last 60 %% of the trace)")
-    return p.parse_args()
+    a = p.parse_args()
+    if a.compare_steps is None:
+        a.compare_steps = a.steps
+    return a
 
 
 def trace_gaps(d):
@@ -163,26 +172,78 @@ def valu_record():
         return None
 
 
-def execution_fields(pairs, avg_ms):
-    """Hardware use of the pair kernel beside the cost-model roofline: the
-    fast field needs fewer FP64 operations per pair than the reference's
-    expression tree that the 745-flop cost model counts, so frac can pass 1;
-    these fields give the FP64 flops the kernel executes (FMA = 2) against
-    the FP64 peak, and its VALU instructions against the VALU issue rate
-    (one 64-lane instruction per CU per clock = FP64 peak / 2 lane-ops/s)."""
+def pair_roofline(pairs, avg_ms, traffic, traffic_rec):
+    """The pair kernel's roofline record.  The kernel is bound by the FP64
+    VALU pipe (arithmetic intensity about N * 160 / 24 flop/byte), so
+    `achieved` counts the FP64 flops the kernel executes per launch (the
+    counter profile's ADD + MUL + 2 FMA + TRANS FP64 instructions per pair,
+    one lane per pair) over its average launch time, against the MI355X FP64
+    vector peak.  The reference expression tree's cost model (745 flops per
+    pair: 165 add/sub/mul + 10 div x 15 + 10 cosh x 40 + 1 cos x 30) is kept
+    beside it as `cost_model_tflops`: the production field needs far fewer
+    operations per pair than that tree, so the cost model's rate passes the
+    peak and is a pair rate, not a hardware figure.  `valu_issue_frac` is
+    VALU instructions per second against the issue rate of one 64-lane
+    instruction per CU per clock (= FP64 peak / 2 lane-ops/s)."""
     rec = valu_record()
-    out = {"cost_model": "745 FP64 flops per pair: 165 add/sub/mul + 10 div x 15 + 10 cosh x 40 + 1 cos x 30"}
-    if not rec:
-        return out
     t = avg_ms * 1e-3
-    flops = rec["fp64_flops_per_pair"]
-    insts = rec["valu_insts_per_pair"]
-    out.update({"executed_fp64_flops_per_pair": flops,
-                "fp64_pipe_frac": flops * pairs / t / (FP64_PEAK_TFLOPS * 1e12),
-                "valu_insts_per_pair": insts,
-                "valu_issue_frac": insts * pairs / t / (FP64_PEAK_TFLOPS * 1e12 / 2),
-                "execution_source": rec.get("source")})
+    out = {"bound": "fp64-valu", "kernel": "k_pair_tiles", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "achieved": None, "frac": None, "traffic": traffic, "pairs_per_launch": pairs, "avg_launch_ms": avg_ms,
+           "cost_model_flops_per_pair": FLOPS_PER_PAIR,
+           "cost_model_tflops": FLOPS_PER_PAIR * pairs / t / 1e12,
+           "cost_model": "745 FP64 flops per pair: 165 add/sub/mul + 10 div x 15 + 10 cosh x 40 + 1 cos x 30"}
+    if traffic_rec:
+        out["traffic_source"] = traffic_rec.get("source")
+    if rec:
+        flops = rec["fp64_flops_per_pair"]
+        insts = rec["valu_insts_per_pair"]
+        achieved = flops * pairs / t / 1e12
+        out.update({"achieved": achieved, "frac": achieved / FP64_PEAK_TFLOPS,
+                    "executed_fp64_flops_per_pair": flops, "valu_insts_per_pair": insts,
+                    "valu_issue_frac": insts * pairs / t / (FP64_PEAK_TFLOPS * 1e12 / 2),
+                    "execution_source": rec.get("source")})
     return out
+
+
+def assembly_run(n=16384, m=2.0, reps=10):
+    """One full Jacobian assembly (calculateJacobian, simulation.cpp:236-384:
+    count, scan, fill, values, column sums, d, scaling) of the headline
+    workload's initial positions, timed with HIP events on the context's
+    stream around each assembly (host read-back of nnz included), and
+    compared with the compulsory bytes 12 nnz + 40 N + 4 (write Ap, Ai, Ax,
+    dVec and the diagonal positions; read x, y, b).  The HBM bytes the
+    kernels actually move come from the committed counter profile."""
+    import sdddst_amd as S
+    from sdddst_amd import configs
+    x, y, b = configs.dislocations(n, 2)
+    with S.Context(n, cutoff_multiplier=m, field_mode=S.FIELD_FAST) as c:
+        c.upload_state(0, x, y, b)
+        nnz = c.jacobian_nnz(0, 1e-6)
+        c.synchronize()
+        c.timing(True)
+        for _ in range(reps):
+            c.jacobian_nnz(0, 1e-6)
+        c.synchronize()
+        cnt, ms = c.kernel_stats("jacobian")
+        c.timing(False)
+    avg = ms / max(cnt, 1)
+    alg = 12 * nnz + 40 * n + 4
+    rec = {"workload": f"N={n}, m={m:g}, seed 2 positions, fast field, full rebuild", "nnz": nnz,
+           "algorithmic_bytes": alg, "avg_assembly_ms": avg, "assemblies": cnt,
+           "achieved_gbs": alg / (avg * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
+           "frac": alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    path = os.path.join(HERE, "profiles", "jacobian_n16384_m2_traffic.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+        rec.update({"hbm_bytes_per_assembly": prof["hbm_bytes_per_assembly"],
+                    "kernel_busy_ms": prof.get("kernel_busy_ms"),
+                    "hbm_gbs_over_kernel_time": prof["hbm_bytes_per_assembly"] / (prof["kernel_busy_ms"] * 1e-3) / 1e9
+                    if prof.get("kernel_busy_ms") else None,
+                    "traffic_source": prof["source"]})
+    except (OSError, ValueError, KeyError):
+        pass
+    return rec
 
 
 def ensemble_run(steps):
@@ -232,18 +293,55 @@ def timed_steps(sim, k, label):
     return t1 - t0, s1.succesful_steps - s0.succesful_steps, s1.failed_steps - s0.failed_steps
 
 
+def host_cpu():
+    """Model, thread count and C library of the host the baseline runs on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count()
+    libc, ver = platform.libc_ver()
+    return {"model": model, "nproc": os.cpu_count(), "threads_available": threads, "libc": f"{libc} {ver}"}
+
+
 def cpu_baseline(x, y, b, seconds):
-    """Oracle (the CPU restatement of the reference, 1 thread) on a bounded
-    sample: the per-pair speed work of a row sample at full N, scaled to the
-    5 speed evaluations of one successful step with constant stress.  The
-    Jacobian assemblies and the sparse factorisations the reference also
-    performs per step are not timed (no UMFPACK on the host), so this is an
-    upper bound on the reference's steps/s."""
+    """The oracle (the CPU restatement of the reference, 1 thread, -O2, no
+    contraction) on bounded samples, on this host:
+    - the small configuration in full: N = 256 (seed 1), m = 1, zero stress,
+      precision 1e-6, 100 successful steps, dense LU standing in for UMFPACK
+      (the reference's own CPU-runnable case);
+    - the headline workload (N = 16384, m = 2, constant stress 0.02): the
+      pair terms of a row sample at full N, scaled to the 5 speed evaluations
+      of a successful step, plus one full Jacobian assembly timed, times the
+      3 assemblies of a step.  The sparse factorisations and solves
+      (UMFPACK, absent from the host) are not timed, so `value` is an upper
+      bound on the reference's step rate there."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
-    import oracle_lib as O
-    n = len(x)
-    lib = O.oracle()
     import ctypes
+
+    import oracle_lib as O
+    from sdddst_amd import configs
+    lib = O.oracle()
+    # small configuration, in full
+    x1, y1, b1 = configs.dislocations(256, 1)
+    t0 = time.perf_counter()
+    s1 = O.OracleSim(x1, y1, b1, cutoff_multiplier=1.0, prec=1e-6, h0=1e-6, stress="zero")
+    while s1.succesful_steps < 100:
+        s1.step()
+    c1_wall = time.perf_counter() - t0
+    c1 = {"steps_per_s": 100 / c1_wall, "wall_s": c1_wall, "failed_steps": int(s1.failed_steps),
+          "sim_time": s1.time}
+    s1.close()
+    # headline workload: pair-term row sample
+    n = len(x)
     pairs_total = n * (n - 1) / 2
     rows = 16
     done_pairs = 0
@@ -260,11 +358,19 @@ def cpu_baseline(x, y, b, seconds):
         # visits every 16-row block in turn
         i0 = (i1 + n // 64) % n
     pairs_per_s = done_pairs / elapsed
-    step_s = 5 * pairs_total / pairs_per_s
+    # one full assembly of the headline matrix
+    t0 = time.perf_counter()
+    Ap = O.jacobian(x, y, b, 2.0, 1e-6, cap=n * 700)[0]
+    asm_s = time.perf_counter() - t0
+    step_s = 5 * pairs_total / pairs_per_s + 3 * asm_s
     return {"value": 1.0 / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle pair terms of {done_pairs} pairs (row sample of N={n}) in {elapsed:.1f} s "
-                      f"= {pairs_per_s:.3e} pairs/s; x5 evaluations per step; assembly and solve not timed",
-            "pairs_per_s": pairs_per_s, "cpu": platform.processor() or platform.machine()}
+            "sample": f"headline workload: oracle pair terms of {done_pairs} pairs (row sample of N={n}) in "
+                      f"{elapsed:.1f} s = {pairs_per_s:.3e} pairs/s, x5 evaluations per step, plus one full "
+                      f"oracle Jacobian assembly ({int(Ap[-1])} entries) in {asm_s:.1f} s, x3 per step; "
+                      "factorisation and solves not timed (no UMFPACK on the host); and the N=256 configuration "
+                      f"run in full ({c1_wall:.1f} s)",
+            "pairs_per_s": pairs_per_s, "assembly_s": asm_s, "speeds_eval_s": pairs_total / pairs_per_s,
+            "config1_n256_full_run": c1, "cpu": host_cpu()}
 
 
 def ev_cpu_baseline(x, y, b, seconds_cap=60.0):
@@ -350,7 +456,6 @@ def large_run(args):
     sim.close()
     pairs = n * (n - 1) / 2
     avg = pair_ms / max(pair_cnt, 1)
-    tflops = FLOPS_PER_PAIR * pairs / (avg * 1e-3) / 1e12
     line = {"metric": "successful implicit steps/s at N=65536 + 2048 point defects, 1e-6 precision",
             "value": ok / wall, "unit": "steps/s", "n_gpus": 1, "steps": ok, "warmup": 0,
             "ms_per_step": wall / ok * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -359,10 +464,8 @@ def large_run(args):
                        "precision": 1e-6, "stress": f"fixed-rate {rate:g}", "initial_step": 1e-6,
                        "preconditioner": args.precond, "failed_steps_in_timed_region": failed,
                        "final_time": st.sim_time, "final_stress": rate * st.sim_time},
-            "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": tflops,
-                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
-                         "traffic": None, "pairs_per_launch": pairs, "avg_launch_ms": avg, "launches": pair_cnt,
-                         "launch_sampling": "HIP events on one pair-kernel launch in 4 of the timed run"},
+            "roofline": dict(pair_roofline(pairs, avg, None, None), launches=pair_cnt,
+                             launch_sampling="HIP events on one pair-kernel launch in 4 of the timed run"),
             "pairs_per_s": pairs / (avg * 1e-3),
             "solver_iterations_per_solve": st.solver_total_iters / max(1, st.solver_total_solves),
             "solver_fallbacks_total": st.solver_fallbacks,
@@ -422,6 +525,9 @@ def main():
         return
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.assembly_only:
+        print(json.dumps({"assembly": assembly_run(args.n, 2.0, max(1, args.assembly_reps))}))
         return
     if args.ev_analysis > 0:
         if args.gpus != 1:
@@ -485,7 +591,6 @@ def main():
     traffic, traffic_rec = traffic_record()
     pairs = n * (n - 1) / 2
     avg_pair_ms = pair_ms / max(pair_cnt, 1)
-    achieved_tflops = FLOPS_PER_PAIR * pairs / (avg_pair_ms * 1e-3) / 1e12
     line = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -503,11 +608,8 @@ def main():
                    "stress": 0.02, "initial_step": 1e-6, "solver": solver_name, "preconditioner": args.precond,
                    "field_mode": args.field_mode,
                    "failed_steps_in_timed_region": failed},
-        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_pair_tiles", "achieved": achieved_tflops,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-                     "traffic": traffic, "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
-                     "avg_launch_ms": avg_pair_ms, "launches": pair_cnt,
-                     "launch_sampling": "HIP events on one pair-kernel launch in 4 of the timed run"},
+        "roofline": dict(pair_roofline(pairs, avg_pair_ms, traffic, traffic_rec), launches=pair_cnt,
+                         launch_sampling="HIP events on one pair-kernel launch in 4 of the timed run"),
         "pairs_per_s": pairs / (avg_pair_ms * 1e-3),
         "speeds_eval_ms": sp_ms / max(sp_cnt, 1),
         "jacobian_ms": jac_ms / max(jac_cnt, 1),
@@ -516,9 +618,9 @@ def main():
         "solver_fallbacks_total": st.solver_fallbacks,
         "solver_precond_fallbacks_total": st.solver_precond_fallbacks,
     }
-    if traffic_rec:
-        line["roofline"]["traffic_source"] = traffic_rec.get("source")
-    line["roofline"].update(execution_fields(pairs, avg_pair_ms))
+    if args.assembly_reps > 0:
+        log("Jacobian assembly")
+        line["assembly"] = assembly_run(n, 2.0, args.assembly_reps)
     if args.compare_steps > 0 and args.solver == "fgmres" and args.precond != "pair":
         log("pair-block preconditioner (the method without the network)")
         sim3 = make_sim(args, "fgmres", x, y, b, precond="pair")

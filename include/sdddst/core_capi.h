@@ -78,6 +78,10 @@ int sdddst_sim_flush_log(sdddst_sim *s);
 int sdddst_sim_write_dconf(sdddst_sim *s, const char *path);
 const char *sdddst_sim_solver_name(sdddst_sim *s);
 const char *sdddst_core_last_error(void);
+/* The Field virtuals of AnalyticField (Field.h:34-36, AnalyticField.cpp:
+ * 145-202) per point, through the class: which = 0 xy, 1 xy_diff_x,
+ * 2 xy_diff_x2 (the bindings' Field.xy / Field.xy_diff_x, bindings.cpp). */
+int sdddst_core_field_eval(int which, int n, const double *dx, const double *dy, double *out);
 /* The device context the simulation runs on (for sdddst_timing etc.). */
 struct sdddst_ctx *sdddst_sim_device_context(sdddst_sim *s);
 

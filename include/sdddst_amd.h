@@ -198,10 +198,11 @@ int sdddst_strain_increment(sdddst_ctx *ctx, int old_slot, int new_slot, double 
 
 int sdddst_synchronize(sdddst_ctx *ctx);
 
-/* AnalyticField::xy / xy_diff_x (AnalyticField.cpp:145-181) over arrays, on
- * the device, with the context's field mode (which = 0 xy, 1 xy_diff_x;
- * which = 2 evaluates the production form of xy regardless of the mode).
- * Used to compare per-pair values with the reference's Field virtuals. */
+/* AnalyticField::xy / xy_diff_x / xy_diff_x2 (AnalyticField.cpp:145-202,
+ * Field.h:34-36) over arrays, on the device: which = 0 xy, 1 xy_diff_x,
+ * 3 xy_diff_x2 (the reference's expression trees); which = 2 evaluates the
+ * production form of xy.  Backs the AnalyticField class methods and compares
+ * per-pair values with the reference's Field virtuals. */
 int sdddst_field_eval(sdddst_ctx *ctx, int which, int64_t n, const double *dx, const double *dy, double *out);
 
 /* Device elementary functions (cos, sin, exp, expm1, cosh, sinh, sqrt,

@@ -22,7 +22,13 @@
  *     Taylor-branch continuity, central differences (mpmath);
  *   - speeds: N=2 dipole, sum rule, brute-force long-double sums;
  *   - Jacobian: brute-force predicate set equality, column-sum identity;
- *   - dense LU solve: residual check.
+ *   - dense LU solve: residual check;
+ *   - calculateG / integrate / passes 2-3 / the three stages
+ *     (tests/test_oracle_stepping.py): the assembled matrix equals central
+ *     differences of the residual at an infinite cutoff (with and without
+ *     point defects), the speed weights sum to one, a dipole's linear mode
+ *     follows the theta-scheme with theta = (1+d)/2, local error order 3,
+ *     second-order convergence to a DOP853 solution, N = 1 closed forms.
  * The dense LU solver below stands in for UMFPACK (absent from this image);
  * its solves are pinned only by residual, "parity unpinned" against UMFPACK.
  */
@@ -628,6 +634,29 @@ int orc_dense_solve_csc(int n, const int *Ap, const int *Ai, const double *Ax, c
 enum { ORC_ORIGINAL = 0, ORC_END_BIG = 1, ORC_END_FIRST_SMALL = 2, ORC_END_SECOND_SMALL = 3 };
 enum { ORC_ZERO = 0, ORC_FIXED_RATE = 1, ORC_SPRING = 2, ORC_CONSTANT = 3 };
 
+/* The run-loop settings of SimulationData (simulation_data.h:84-254) that
+ * Simulation::run and stage III read (simulation.cpp:417-433, 553-633). */
+typedef struct {
+    int isTimeLimit;
+    double timeLimit;
+    int isStrainIncreaseLimit;
+    double strainIncreaseLimit;
+    int isStepCountLimit;
+    long stepCountLimit;
+    int countAvalanches;
+    long avalancheTriggerLimit;
+    double avalancheSpeedThreshold;
+    int isSpeedLimit;
+    double speedLimit;
+    int isStressLimit;
+    double stressLimit;
+    int isSpeedThresholdForCutoffChange;
+    double speedThresholdForCutoffChange;
+    int isSaveSubConfigs;
+    long subConfigDelay, subConfigDelayDuringAvalanche;
+    long remainingFinalSteps;
+} orc_run_opts;
+
 typedef struct {
     int n, pc;
     double *x, *y, *b;                       /* dislocations (state)        */
@@ -666,6 +695,14 @@ typedef struct {
     double *logrows;
     long logcap, loglen;
     double *ysave;
+    /* run loop (simulation.cpp:417-433, 553-633) */
+    orc_run_opts run;
+    long avalancheCount;
+    int inAvalanche, finish, inFinal;
+    long subconfigDistanceCounter;
+    /* sub-configurations written: simulation time and positions of each */
+    double *subTimes, *subX;
+    long subLen, subCap;
 } orc_sim;
 
 static double protocol_get(const orc_sim *s, int type)
@@ -790,6 +827,23 @@ static void log_row(orc_sim *s, const double row[12])
     s->loglen++;
 }
 
+static void sub_record(orc_sim *s)
+{
+    if (s->subLen >= s->subCap) {
+        long nc = s->subCap ? s->subCap * 2 : 16;
+        double *tt = (double *)realloc(s->subTimes, (size_t)nc * sizeof(double));
+        if (!tt) return;
+        s->subTimes = tt;
+        double *xx = (double *)realloc(s->subX, (size_t)nc * (size_t)s->n * sizeof(double));
+        if (!xx) return;
+        s->subX = xx;
+        s->subCap = nc;
+    }
+    s->subTimes[s->subLen] = s->simTime;
+    memcpy(s->subX + (size_t)s->subLen * (size_t)s->n, s->x, (size_t)s->n * sizeof(double));
+    s->subLen++;
+}
+
 /* simulation.cpp:435-650 ; returns 1 on a successful step, 0 on a failed
  * step, <0 on error.  Log column 9 (wall time) is recorded as 0. */
 int orc_sim_step(orc_sim *s)
@@ -829,6 +883,7 @@ int orc_sim_step(orc_sim *s)
     if (s->ph.maxErrorRatioSqr < 1.0) {
         s->succesfulStep = 1;
         s->initSpeedCalculationIsNeeded = 1;
+        if (s->inFinal) s->run.remainingFinalSteps--;
         if (s->calculateStrain) {
             s->totalAccumulatedStrainIncrease += strain_increment(n, s->b, s->x, s->fsx);
             s->totalAccumulatedStrainIncrease += strain_increment(n, s->b, s->fsx, s->ssx);
@@ -849,13 +904,41 @@ int orc_sim_step(orc_sim *s)
         sim_speeds(s, s->x, s->initSpeed);
         s->initSpeedCalculationIsNeeded = 0;
         s->sumAvgSpeed = sum_abs(s->initSpeed, n) / (double)n;
+        /* simulation.cpp:553-555 */
+        if (s->sumAvgSpeed < s->run.speedLimit && s->run.isSpeedLimit) s->finish = 1;
         s->vsquare = sum_sq(s->initSpeed, n);
+        /* simulation.cpp:559-570 */
+        if (s->run.countAvalanches) {
+            if (s->inAvalanche && s->sumAvgSpeed < s->run.avalancheSpeedThreshold) {
+                s->avalancheCount++;
+                s->inAvalanche = 0;
+            } else if (s->sumAvgSpeed > s->run.avalancheSpeedThreshold) {
+                s->inAvalanche = 1;
+            }
+        }
         s->energyAccum += (s->vsquare2 + s->vsquare) * 0.5 * s->stepSize * 0.5;
+        const double loggedCutOff = s->cut.cutOff;
+        /* simulation.cpp:603-611 */
+        if (s->run.isStressLimit && protocol_get(s, ORC_ORIGINAL) > s->run.stressLimit) s->finish = 1;
+        if (s->run.isSpeedThresholdForCutoffChange && s->run.speedThresholdForCutoffChange > s->sumAvgSpeed) {
+            s->cutOffMultiplier = 1e20;
+            orc_cutoff(n, s->cutOffMultiplier, &s->cut.cutOff, &s->cut.cutOffSqr, &s->cut.onePerCutOffSqr);
+        }
         s->energy += s->energyAccum;
         double row[12] = {s->simTime, (double)s->succesfulSteps, (double)s->failedSteps, s->ph.maxErrorRatioSqr,
-                          s->sumAvgSpeed, s->cut.cutOff, orderParameter, protocol_get(s, ORC_ORIGINAL), 0.0,
+                          s->sumAvgSpeed, loggedCutOff, orderParameter, protocol_get(s, ORC_ORIGINAL), 0.0,
                           s->calculateStrain ? s->totalAccumulatedStrainIncrease : NAN, s->vsquare, s->energy};
         log_row(s, row);
+        /* simulation.cpp:619-633: the file is named by the simulation time */
+        if (s->run.isSaveSubConfigs) {
+            if ((!s->inAvalanche && s->run.subConfigDelay >= s->subconfigDistanceCounter) ||
+                (s->inAvalanche && s->run.subConfigDelayDuringAvalanche >= s->subconfigDistanceCounter)) {
+                s->subconfigDistanceCounter = 0;
+                sub_record(s);
+            } else {
+                s->subconfigDistanceCounter++;
+            }
+        }
     } else {
         s->failedSteps++;
     }
@@ -941,6 +1024,7 @@ void orc_sim_destroy(orc_sim *s)
     free(s->Ap); free(s->Ai); free(s->Ax); free(s->indexes);
     free(s->ph.tol); free(s->ph.err);
     free(s->logrows);
+    free(s->subTimes); free(s->subX);
     free(s);
 }
 
@@ -967,8 +1051,10 @@ void orc_sim_get_speeds(const orc_sim *s, double *initSpeed, double *initSpeed2)
     if (initSpeed2) memcpy(initSpeed2, s->initSpeed2, (size_t)s->n * sizeof(double));
 }
 
-/* Stage I positions of the last step (the big step, simulation.cpp:482-488). */
+/* Stage I positions of the last step (the big step, simulation.cpp:482-488)
+ * and stage II's (the first small step, 494-499). */
 void orc_sim_get_big_x(const orc_sim *s, double *out) { memcpy(out, s->bigx, (size_t)s->n * sizeof(double)); }
+void orc_sim_get_fs_x(const orc_sim *s, double *out) { memcpy(out, s->fsx, (size_t)s->n * sizeof(double)); }
 
 /* Entry points for the mathematical pins of the stepping path (tests only).
  * orc_sim_residual: one calculateG (simulation.cpp:133-193) of the stage-I
@@ -987,6 +1073,54 @@ void orc_sim_residual(orc_sim *s, double stepsize, const double *newx, const dou
 /* orc_sim_stage: one integrate (simulation.cpp:62-84) of stage-I kind from
  * the simulation's current positions with step size h: newx receives the
  * end positions, dVec_out (nullable) the d vector of the assembly. */
+/* Simulation::run (simulation.cpp:417-433) with the given settings: step
+ * while every active limit allows, then the post-relax steps.  Returns the
+ * number of attempts made, <0 on error. */
+long orc_sim_run(orc_sim *s, const orc_run_opts *o)
+{
+    s->run = *o;
+    long attempts = 0;
+    while (((s->run.isTimeLimit && s->simTime < s->run.timeLimit) || !s->run.isTimeLimit) &&
+           ((s->run.isStrainIncreaseLimit && s->totalAccumulatedStrainIncrease < s->run.strainIncreaseLimit) ||
+            !s->run.isStrainIncreaseLimit) &&
+           ((s->run.isStepCountLimit && s->succesfulSteps < s->run.stepCountLimit) || !s->run.isStepCountLimit) &&
+           ((s->run.countAvalanches && s->avalancheCount < s->run.avalancheTriggerLimit) || !s->run.countAvalanches) &&
+           !s->finish) {
+        if (orc_sim_step(s) < 0) return -1;
+        attempts++;
+    }
+    while (s->run.remainingFinalSteps > 0) {
+        s->inFinal = 1;
+        if (orc_sim_step(s) < 0) return -1;
+        attempts++;
+    }
+    return attempts;
+}
+
+/* Run-loop state: [avalancheCount, inAvalanche, finish, remainingFinalSteps,
+ * cutOff, cutOffMultiplier, subconfigDistanceCounter]. */
+void orc_sim_run_state(const orc_sim *s, double *out)
+{
+    out[0] = (double)s->avalancheCount;
+    out[1] = s->inAvalanche;
+    out[2] = s->finish;
+    out[3] = (double)s->run.remainingFinalSteps;
+    out[4] = s->cut.cutOff;
+    out[5] = s->cutOffMultiplier;
+    out[6] = (double)s->subconfigDistanceCounter;
+}
+
+/* Sub-configurations recorded so far: times (and positions, n per record)
+ * into the caller's arrays of cap records; returns the count. */
+long orc_sim_subconfigs(const orc_sim *s, double *times, double *xs, long cap)
+{
+    for (long k = 0; k < s->subLen && k < cap; ++k) {
+        if (times) times[k] = s->subTimes[k];
+        if (xs) memcpy(xs + (size_t)k * (size_t)s->n, s->subX + (size_t)k * (size_t)s->n, (size_t)s->n * sizeof(double));
+    }
+    return s->subLen;
+}
+
 int orc_sim_stage(orc_sim *s, double h, double *newx, double *dVec_out)
 {
     memcpy(newx, s->x, (size_t)s->n * sizeof(double));

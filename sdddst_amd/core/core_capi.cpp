@@ -251,6 +251,24 @@ const char *sdddst_sim_solver_name(sdddst_sim *s)
 
 const char *sdddst_core_last_error(void) { return core_error.c_str(); }
 
+int sdddst_core_field_eval(int which, int n, const double *dx, const double *dy, double *out)
+{
+    try {
+        if (which < 0 || which > 2 || n < 0 || (n > 0 && (!dx || !dy || !out))) {
+            core_error = "bad arguments";
+            return SDDDST_ERR_ARG;
+        }
+        AnalyticField f;
+        Field &tau = f;
+        for (int i = 0; i < n; ++i)
+            out[i] = which == 0 ? tau.xy(dx[i], dy[i]) : which == 1 ? tau.xy_diff_x(dx[i], dy[i])
+                                                                    : tau.xy_diff_x2(dx[i], dy[i]);
+        return SDDDST_OK;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 sdddst_ctx *sdddst_sim_device_context(sdddst_sim *s) { return s->sim->deviceContext(); }
 
 }  // extern "C"

@@ -2,8 +2,6 @@
 // src/Fields/AnalyticField.cpp:134-202).  AnalyticField's per-point methods
 // evaluate the device implementation (the same code the kernels run) through
 // a small private context; Simulation never calls them per pair.
-#include <math.h>
-
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -47,4 +45,4 @@ AnalyticField::AnalyticField() : Field() {}
 AnalyticField::~AnalyticField() {}
 double AnalyticField::xy(double dx, double dy) { return eval_point(0, dx, dy); }
 double AnalyticField::xy_diff_x(double dx, double dy) { return eval_point(1, dx, dy); }
-double AnalyticField::xy_diff_x2(double, double) { return NAN; }
+double AnalyticField::xy_diff_x2(double dx, double dy) { return eval_point(3, dx, dy); }

@@ -16,7 +16,8 @@ __global__ void k_field_eval(int which, int64_t n, const double *__restrict__ dx
     double r;
     if (which == 0) r = sdfield::xy(dx[i], dy[i]);
     else if (which == 1) r = sdfield::xy_diff_x(dx[i], dy[i]);
-    else r = sdfield::xy_fast(dx[i], dy[i]);
+    else if (which == 2) r = sdfield::xy_fast(dx[i], dy[i]);
+    else r = sdfield::xy_diff_x2(dx[i], dy[i]);
     out[i] = r;
 }
 
@@ -45,7 +46,7 @@ __global__ void k_libm_eval(int fn, int64_t n, const double *__restrict__ x, dou
 
 int field_eval(Context &c, int which, int64_t n, const double *dx, const double *dy, double *out)
 {
-    SD_CHECK(which >= 0 && which <= 2 && n >= 0 && dx && dy && out, SDDDST_ERR_ARG, "bad arguments");
+    SD_CHECK(which >= 0 && which <= 3 && n >= 0 && dx && dy && out, SDDDST_ERR_ARG, "bad arguments");
     if (n == 0) return SDDDST_OK;
     double *d = nullptr;
     SD_HIP(hipMalloc(&d, sizeof(double) * 3 * n));

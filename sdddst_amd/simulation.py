@@ -98,6 +98,7 @@ def load_core():
         "sdddst_sim_solver_name": (ctypes.c_char_p, [vp]),
         "sdddst_core_last_error": (ctypes.c_char_p, []),
         "sdddst_sim_device_context": (ctypes.c_void_p, [vp]),
+        "sdddst_core_field_eval": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -105,6 +106,19 @@ def load_core():
         f.argtypes = args
     _core = lib
     return lib
+
+
+def analytic_field(which: int, dx, dy) -> np.ndarray:
+    """AnalyticField's Field virtuals through the C++ class (which = 0 xy,
+    1 xy_diff_x, 2 xy_diff_x2), per point."""
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    out = np.empty_like(dx)
+    lib = load_core()
+    if lib.sdddst_core_field_eval(which, len(dx), dx.ctypes.data_as(_dp), dy.ctypes.data_as(_dp),
+                                  out.ctypes.data_as(_dp)) != 0:
+        raise SdddstError(lib.sdddst_core_last_error().decode())
+    return out
 
 
 def default_options() -> SimOptions:

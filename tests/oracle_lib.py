@@ -81,6 +81,10 @@ def oracle():
         "orc_sim_get_log": (None, [VP, DP]),
         "orc_sim_get_speeds": (None, [VP, DP, DP]),
         "orc_sim_get_big_x": (None, [VP, DP]),
+        "orc_sim_get_fs_x": (None, [VP, DP]),
+        "orc_sim_run": (L, [VP, ctypes.c_void_p]),
+        "orc_sim_run_state": (None, [VP, DP]),
+        "orc_sim_subconfigs": (L, [VP, DP, DP, L]),
         "orc_sim_residual": (None, [VP, D, DP, DP, DP, DP, DP]),
         "orc_sim_stage": (ctypes.c_int, [VP, D, DP, DP]),
         "orc_libm_batch": (None, [ctypes.c_int, L, DP, DP]),
@@ -220,6 +224,19 @@ def cutoff(n, m):
     return a.value, b.value, c.value
 
 
+class RunOpts(ctypes.Structure):
+    """orc_run_opts: the run-loop settings (simulation.cpp:417-433, 553-633)."""
+    _fields_ = [("isTimeLimit", ctypes.c_int), ("timeLimit", D),
+                ("isStrainIncreaseLimit", ctypes.c_int), ("strainIncreaseLimit", D),
+                ("isStepCountLimit", ctypes.c_int), ("stepCountLimit", L),
+                ("countAvalanches", ctypes.c_int), ("avalancheTriggerLimit", L), ("avalancheSpeedThreshold", D),
+                ("isSpeedLimit", ctypes.c_int), ("speedLimit", D),
+                ("isStressLimit", ctypes.c_int), ("stressLimit", D),
+                ("isSpeedThresholdForCutoffChange", ctypes.c_int), ("speedThresholdForCutoffChange", D),
+                ("isSaveSubConfigs", ctypes.c_int), ("subConfigDelay", L), ("subConfigDelayDuringAvalanche", L),
+                ("remainingFinalSteps", L)]
+
+
 class OracleSim:
     """The oracle's full stepping driver (dense LU stands in for UMFPACK)."""
 
@@ -270,6 +287,51 @@ class OracleSim:
     def big_x(self):
         out = np.empty(self.n)
         self.lib.orc_sim_get_big_x(self.h, dp(out))
+        return out
+
+    def run(self, time_limit=None, strain_limit=None, step_count_limit=None, avalanche_limit=None,
+            avalanche_threshold=1e-3, speed_limit=None, stress_limit=None, cutoff_inf_threshold=None,
+            sub_configs=False, sub_delay=5, sub_delay_avalanche=1, post_relax=0):
+        """Simulation::run with the command line's settings; returns the
+        number of attempts."""
+        o = RunOpts()
+        for value, value_field, flag_field in ((time_limit, "timeLimit", "isTimeLimit"),
+                                (strain_limit, "strainIncreaseLimit", "isStrainIncreaseLimit"),
+                                (step_count_limit, "stepCountLimit", "isStepCountLimit"),
+                                (avalanche_limit, "avalancheTriggerLimit", "countAvalanches"),
+                                (speed_limit, "speedLimit", "isSpeedLimit"),
+                                (stress_limit, "stressLimit", "isStressLimit"),
+                                (cutoff_inf_threshold, "speedThresholdForCutoffChange",
+                                 "isSpeedThresholdForCutoffChange")):
+            if value is not None:
+                setattr(o, flag_field, 1)
+                setattr(o, value_field, value)
+        o.avalancheSpeedThreshold = avalanche_threshold
+        o.isSaveSubConfigs = int(sub_configs)
+        o.subConfigDelay = sub_delay
+        o.subConfigDelayDuringAvalanche = sub_delay_avalanche
+        o.remainingFinalSteps = post_relax
+        r = self.lib.orc_sim_run(self.h, ctypes.addressof(o))
+        if r < 0:
+            raise RuntimeError("oracle run failed")
+        return r
+
+    def run_state(self):
+        out = np.empty(7)
+        self.lib.orc_sim_run_state(self.h, dp(out))
+        keys = ("avalanche_count", "in_avalanche", "finish", "remaining_final_steps", "cutoff", "cutoff_multiplier",
+                "subconfig_distance_counter")
+        return dict(zip(keys, out.tolist()))
+
+    def subconfigs(self):
+        k = self.lib.orc_sim_subconfigs(self.h, None, None, 0)
+        t, xs = np.empty(k), np.empty((k, self.n))
+        self.lib.orc_sim_subconfigs(self.h, dp(t), dp(xs), k)
+        return t, xs
+
+    def fs_x(self):
+        out = np.empty(self.n)
+        self.lib.orc_sim_get_fs_x(self.h, dp(out))
         return out
 
     def residual(self, h, newx, oldx, dvec, v_old):

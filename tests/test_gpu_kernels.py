@@ -84,6 +84,37 @@ def test_field_xy_fast_abs_error(ctx256):
     assert err.max() < 4e-15
 
 
+def test_field_xy_diff_x2_golden(ctx256):
+    # AnalyticField::xy_diff_x2 (AnalyticField.cpp:95-132, 183-202) against
+    # the golden vectors made from the reference's own translation unit; below
+    # r = 0.05 the reference's numerator (no small-r branch) magnifies one-ulp
+    # differences of pow(x, 3.0) against a correctly rounded cube
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "field_points_dx2.npz"))
+    dev = ctx256.field_eval(3, g["dx"], g["dy"])
+    ref = g["xy_diff_x2"]
+    r = np.hypot(g["dx"], g["dy"])
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(dev), fin)
+    rel = np.abs(dev[fin] - ref[fin]) / np.maximum(np.abs(ref[fin]), 1.0)
+    far = r[fin] > 0.05
+    assert far.sum() > 1000
+    assert rel[far].max() < 1e-13
+    assert rel.max() < 1e-8
+    assert np.mean(dev[fin].view(np.int64) == ref[fin].view(np.int64)) > 0.99
+
+
+def test_analytic_field_class_methods(ctx256):
+    # the drop-in's AnalyticField::xy / xy_diff_x / xy_diff_x2 (Field.h:34-36)
+    # evaluate the device field: same bits as the array evaluation
+    from sdddst_amd.simulation import analytic_field
+    dx, dy = _pairs(2000, 4)
+    for which, dev_which in ((0, 0), (1, 1), (2, 3)):
+        a = analytic_field(which, dx, dy)
+        d = ctx256.field_eval(dev_which, dx, dy)
+        assert np.array_equal(a.view(np.int64), d.view(np.int64))
+
+
 # -- K1 pair speeds --------------------------------------------------------
 
 def _speeds_dev(n, seed, mode, sigma=0.0, cutoff=1.0):
@@ -147,7 +178,8 @@ def test_speeds_sum_rule_and_determinism_n16384():
 
 # -- K2 Jacobian -----------------------------------------------------------
 
-@pytest.mark.parametrize("n,m,seed", [(256, 1.0, 1), (300, 1.0, 4), (1024, 2.0, 7), (64, 1.0, 3), (4096, 1.0, 9), (4096, 2.0, 10)])
+@pytest.mark.parametrize("n,m,seed", [(256, 1.0, 1), (300, 1.0, 4), (1024, 2.0, 7), (64, 1.0, 3), (4096, 1.0, 9),
+                                      (4096, 2.0, 10), (4096, 4.0, 11)])
 def test_jacobian_pattern_bit_exact(n, m, seed):
     x, y, b = configs.dislocations(n, seed)
     h = 1e-4
@@ -161,6 +193,25 @@ def test_jacobian_pattern_bit_exact(n, m, seed):
     rel = np.abs(Ax - rAx) / np.maximum(np.abs(rAx), 1e-300)
     assert np.mean(Ax.view(np.int64) == rAx.view(np.int64)) > 0.995
     assert rel.max() < 1e-12
+    assert np.max(np.abs(dv - rdv)) < 1e-13
+
+
+def test_jacobian_full_size_n16384_m2():
+    # the headline workload's matrix (N = 16384, m = 2, seed 2): pattern,
+    # diagonal positions and values against the oracle's column-by-column
+    # assembly (simulation.cpp:236-384), exact field
+    n, m, h = 16384, 2.0, 1e-4
+    x, y, b = configs.dislocations(n, 2)
+    with _ctx(n, cutoff_multiplier=m, field_mode=S.FIELD_EXACT) as c:
+        c.upload_state(0, x, y, b)
+        Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)
+    rAp, rAi, rAx, rdv, ridx, raw = O.jacobian(x, y, b, m, h, cap=n * 700)
+    assert rAp[-1] > 1.0e7
+    assert np.array_equal(Ap, rAp)
+    assert np.array_equal(Ai, rAi)
+    assert np.array_equal(dg, ridx)
+    assert np.mean(Ax.view(np.int64) == rAx.view(np.int64)) > 0.995
+    assert np.max(np.abs(Ax - rAx) / np.maximum(np.abs(rAx), 1e-300)) < 1e-12
     assert np.max(np.abs(dv - rdv)) < 1e-13
 
 

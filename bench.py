@@ -469,7 +469,9 @@ def large_run(args):
             "pairs_per_s": pairs / (avg * 1e-3),
             "solver_iterations_per_solve": st.solver_total_iters / max(1, st.solver_total_solves),
             "solver_fallbacks_total": st.solver_fallbacks,
-            "solver_precond_fallbacks_total": st.solver_precond_fallbacks}
+            "solver_precond_fallbacks_total": st.solver_precond_fallbacks,
+            "solver_strong_retries_total": st.solver_strong_retries,
+            "attempts_rejected_for_unconverged_solve": st.aborted_attempts}
     print(json.dumps(line))
 
 

@@ -59,6 +59,8 @@ typedef struct sdddst_sim_state {
     double solver_last_rel_residual;
     long long solver_total_iters, solver_total_solves;
     int solver_precond_fallbacks;  /* learned solves retried with pair blocks */
+    long long aborted_attempts;    /* attempts rejected for an unconverged solve */
+    int solver_strong_retries;     /* solves retried with the strong-row block */
 } sdddst_sim_state;
 
 void sdddst_sim_default_options(sdddst_sim_options *o);

@@ -59,8 +59,13 @@ public:
     double getEnergy() const { return energy; }
     // Wall time spent per successful step in the last run() (seconds).
     double lastRunWallTime() const { return lastRunWall; }
+    // Attempts rejected because a Newton solve did not converge and N is
+    // above the dense direct solver's limit (no counterpart in the
+    // reference, whose UMFPACK always solves).
+    long long abortedAttemptCount() const { return abortedAttempts; }
 
 private:
+    void rejectAbortedAttempt();
     void integrateSlots(double stepsize, int newSlot, int oldSlot, bool useSpeed2, bool calculateInitSpeed,
                         StressProtocolStepType origin, StressProtocolStepType end, bool samePositions = false);
     void calculateGSlots(double stepsize, int newSlot, int oldSlot, bool useSpeed2, bool calculateInitSpeed,
@@ -85,6 +90,7 @@ private:
     double vsquare1;
     double vsquare2;
     double lastRunWall;
+    long long abortedAttempts = 0;
 
     std::shared_ptr<SimulationData> sD;
     std::unique_ptr<PrecisionHandler> pH;

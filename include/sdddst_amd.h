@@ -38,6 +38,9 @@ extern "C" {
 #define SDDDST_ERR_SOLVER (-4)   /* factorisation / solve failure         */
 #define SDDDST_ERR_NOMEM (-5)    /* device allocation failed              */
 #define SDDDST_ERR_UNSUPPORTED (-6)
+#define SDDDST_ERR_NOT_CONVERGED (-7) /* iterative solve above the direct solver's size limit
+                                         did not reach its tolerance; delta holds the last
+                                         iterate and the state is otherwise intact */
 
 #define SDDDST_SLOT_DISLOCATIONS 0
 #define SDDDST_SLOT_BIG_STEP 1
@@ -68,6 +71,9 @@ extern "C" {
 #define SDDDST_PRECOND_JACOBI 0   /* 1 / A_ii                                              */
 #define SDDDST_PRECOND_PAIR 1     /* exact inverse of 2x2 blocks of mutually strongest pairs */
 #define SDDDST_PRECOND_LEARNED 2  /* sparse approximate inverse on A's pattern from the net  */
+#define SDDDST_PRECOND_STRONG 3   /* pair blocks + dense LU of the rows with |A_ii - 1| or an
+                                     off-diagonal |A_ij| above 0.25 (at most 32768 rows);
+                                     also the retry of any solve the others do not converge */
 
 typedef struct sdddst_ctx sdddst_ctx;
 
@@ -149,6 +155,22 @@ int sdddst_solver_stats(sdddst_ctx *ctx, int *last_iters, double *last_rel_resid
  * cycle and were solved again with the pair-block preconditioner (no
  * counterpart in the reference). */
 int sdddst_solver_precond_fallbacks(sdddst_ctx *ctx, int *count);
+/* The most recent iterative-solver attempts (up to 1024 kept, oldest first,
+ * at most max_records returned; a call with max_records > 0 clears the log,
+ * one with 0 only reads the counts): seven
+ * doubles per attempt {preconditioner (-1: the direct solver), iterations,
+ * relative residual, ||g||, outcome (0 converged, 1 not converged, 2 direct
+ * solve after it), h of the Jacobian, rows of the strong-row block (0 for
+ * other preconditioners)}; strong (nullable) receives the count of
+ * solves retried with the strong-row block preconditioner
+ * (SDDDST_PRECOND_STRONG) before the direct solver.  No counterpart in
+ * the reference (UMFPACK solves directly). */
+int sdddst_solver_history(sdddst_ctx *ctx, double *out, int max_records, int *count, int *strong);
+/* Largest N for which an FGMRES solve that does not converge falls back to
+ * the dense direct solver (8 N^2 bytes of device memory); above it such a
+ * solve returns SDDDST_ERR_NOT_CONVERGED (sdddstCore::Simulation then
+ * rejects the attempt and divides the step size by 8).  Default 32768. */
+int sdddst_solver_set_direct_limit(sdddst_ctx *ctx, int max_n);
 
 /* ---- Learned preconditioner (SDDDST_PRECOND_LEARNED) ------------------
  * No counterpart in the reference, which solves A delta = g with UMFPACK

@@ -12,6 +12,7 @@ from ._capi import (  # noqa: F401
     PRECOND_JACOBI,
     PRECOND_LEARNED,
     PRECOND_PAIR,
+    PRECOND_STRONG,
     SLOT_BIG_STEP,
     SLOT_DISLOCATIONS,
     SLOT_FIRST_SMALL,

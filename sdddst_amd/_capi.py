@@ -21,7 +21,7 @@ SLOT_DISLOCATIONS, SLOT_BIG_STEP, SLOT_FIRST_SMALL, SLOT_SECOND_SMALL, SLOT_SCRA
 V_INIT, V_INIT2, V_SPEED, V_SPEED2, V_SCRATCH = range(5)
 FIELD_FAST, FIELD_EXACT, FIELD_ORDERED = 0, 1, 2
 SOLVER_DENSE_LU, SOLVER_FGMRES, SOLVER_HOST = 0, 1, 2
-PRECOND_JACOBI, PRECOND_PAIR, PRECOND_LEARNED = 0, 1, 2
+PRECOND_JACOBI, PRECOND_PAIR, PRECOND_LEARNED, PRECOND_STRONG = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -78,6 +78,8 @@ _SIGNATURES = {
     "sdddst_solver_solve": (ctypes.c_int, [_vp, _dp]),
     "sdddst_solver_stats": (ctypes.c_int, [_vp, _ip, _dp, _ip, _i64p, _i64p]),
     "sdddst_solver_precond_fallbacks": (ctypes.c_int, [_vp, _ip]),
+    "sdddst_solver_history": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _ip, _ip]),
+    "sdddst_solver_set_direct_limit": (ctypes.c_int, [_vp, ctypes.c_int]),
     "sdddst_spai_param_count": (ctypes.c_int, []),
     "sdddst_spai_set_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
     "sdddst_spai_get_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
@@ -312,6 +314,16 @@ class Context:
         self._check(self.lib.sdddst_solver_precond_fallbacks(self.h, ctypes.byref(v)))
         return v.value
 
+    def solver_history(self, max_records: int = 1024):
+        """Recent iterative-solver attempts as an array of rows {precond,
+        iterations, relative residual, ||g||, outcome, h, strong-block
+        rows} (clears the log),
+        and the count of solves retried with the strong-row block."""
+        return solver_history(self.h, max_records)
+
+    def set_direct_limit(self, max_n: int):
+        self._check(self.lib.sdddst_solver_set_direct_limit(self.h, max_n))
+
     # -- learned preconditioner ------------------------------------------
     def spai_param_count(self) -> int:
         return int(self.lib.sdddst_spai_param_count())
@@ -432,3 +444,15 @@ def kernel_stats(ctx_handle, kernel: str):
     if rc != OK:
         raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
     return cnt.value, ms.value
+
+
+
+def solver_history(ctx_handle, max_records: int = 1024):
+    """sdddst_solver_history on a context handle: (records [k, 7], strong-row retries)."""
+    lib = load_library()
+    out = np.empty((max_records, 7))
+    k, ext = ctypes.c_int(), ctypes.c_int()
+    rc = lib.sdddst_solver_history(ctx_handle, _dptr(out), max_records, ctypes.byref(k), ctypes.byref(ext))
+    if rc != OK:
+        raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
+    return out[:k.value].copy(), ext.value

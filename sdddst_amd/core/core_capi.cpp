@@ -204,6 +204,11 @@ int sdddst_sim_get_state(sdddst_sim *s, sdddst_sim_state *o)
     int pf = 0;
     sdddst_solver_precond_fallbacks(s->sim->deviceContext(), &pf);
     o->solver_precond_fallbacks = pf;
+    o->aborted_attempts = s->sim->abortedAttemptCount();
+    int sr = 0, cnt = 0;
+    // counts only (max_records 0): the records stay for sdddst_solver_history
+    (void)sdddst_solver_history(s->sim->deviceContext(), nullptr, 0, &cnt, &sr);
+    o->solver_strong_retries = sr;
     return SDDDST_OK;
 }
 

@@ -91,7 +91,7 @@ const Spec kSpecs[] = {
      "Write correl matrices and EV, PN into the result file only, with the given sampling size"},
     {"linear-solver", STR, "auto", "Device options",
      "auto (UMFPACK when it loads, else dense LU) | umfpack | dense-lu | fgmres"},
-    {"preconditioner", STR, "pair", "Device options", "FGMRES preconditioner: pair | jacobi | learned"},
+    {"preconditioner", STR, "pair", "Device options", "FGMRES preconditioner: pair | jacobi | learned | strong"},
     {"field-mode", STR, "fast", "Device options",
      "fast (production) | exact (reference pair terms) | ordered (reference summation order too)"},
     {"solver-tolerance", DBL, "1e-10", "Device options", "FGMRES relative residual"},
@@ -318,6 +318,7 @@ void ProjectParser::processInput()
         if (pc == "pair") sD->preconditioner = SDDDST_PRECOND_PAIR;
         else if (pc == "jacobi") sD->preconditioner = SDDDST_PRECOND_JACOBI;
         else if (pc == "learned") sD->preconditioner = SDDDST_PRECOND_LEARNED;
+        else if (pc == "strong") sD->preconditioner = SDDDST_PRECOND_STRONG;
         else fail("the argument ('" + pc + "') for option '--preconditioner' is invalid", -1);
         const std::string fm = asString("field-mode");
         if (fm == "fast") sD->fieldMode = SDDDST_FIELD_FAST;

@@ -270,10 +270,19 @@ void Simulation::calculateSparseFormForJacobian()
 }
 
 // simulation.cpp:393-396
+namespace {
+// A Newton solve the device solver could not bring to its tolerance with no
+// direct solver allowed at this size: the attempt ends there.
+struct AttemptAborted {
+};
+}  // namespace
+
 void Simulation::solveEQSys()
 {
     if (solver != LinearSolver::Umfpack) {
-        check(sdddst_solver_solve(ctx, nullptr), "solve");
+        const int rc = sdddst_solver_solve(ctx, nullptr);
+        if (rc == SDDDST_ERR_NOT_CONVERGED) throw AttemptAborted();
+        check(rc, "solve");
         return;
     }
     (void)umf->solve(sD->Ap, sD->Ai, sD->Ax, sD->x, sD->g.data(), sD->Numeric);
@@ -404,10 +413,31 @@ void Simulation::run()
 // simulation.cpp:435-441
 bool Simulation::step()
 {
-    stepStageI();
-    stepStageII();
-    stepStageIII();
+    try {
+        stepStageI();
+        stepStageII();
+        stepStageIII();
+    } catch (const AttemptAborted &) {
+        rejectAbortedAttempt();
+    }
     return succesfulStep;
+}
+
+// An attempt whose linear solve did not converge (the reference's UMFPACK
+// would have solved it; here FGMRES stagnates on the indefinite system of a
+// far too large step and N is above the dense solver's limit) is rejected
+// like a step over the error bound (simulation.cpp:638-649), with the step
+// size divided by 8 since no error estimate exists.  The positions of the
+// attempt are discarded; the state of the last accepted step is untouched.
+void Simulation::rejectAbortedAttempt()
+{
+    succesfulStep = false;
+    sD->failedSteps++;
+    abortedAttempts++;
+    check(sdddst_reset_tolerance(ctx), "reset tolerance");
+    pH->reset();
+    sD->stepSize *= 0.125;
+    if (sD->isMaxStepSizeLimit && sD->maxStepSizeLimit < sD->stepSize) sD->stepSize = sD->maxStepSizeLimit;
 }
 
 void Simulation::writeLogLine(const std::string &line) { sD->standardOutputLog << line; }

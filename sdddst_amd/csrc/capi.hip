@@ -427,7 +427,7 @@ int sdddst_solver_select(sdddst_ctx *ctx, int solver, double rel_tol, int max_it
 int sdddst_set_preconditioner(sdddst_ctx *ctx, int precond)
 {
     CTX_CHECK(ctx);
-    SD_CHECK(precond >= SDDDST_PRECOND_JACOBI && precond <= SDDDST_PRECOND_LEARNED, SDDDST_ERR_ARG,
+    SD_CHECK(precond >= SDDDST_PRECOND_JACOBI && precond <= SDDDST_PRECOND_STRONG, SDDDST_ERR_ARG,
              "unknown preconditioner %d", precond);
     ctx->c.precond = precond;
     return SDDDST_OK;
@@ -469,6 +469,32 @@ int sdddst_solver_precond_fallbacks(sdddst_ctx *ctx, int *count)
     CTX_CHECK(ctx);
     SD_CHECK(count, SDDDST_ERR_ARG, "count is required");
     *count = ctx->c.precond_fallbacks;
+    return SDDDST_OK;
+}
+
+int sdddst_solver_history(sdddst_ctx *ctx, double *out, int max_records, int *count, int *strong)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    SD_CHECK(count && max_records >= 0 && (out || max_records == 0), SDDDST_ERR_ARG, "bad arguments");
+    const int k = c.solve_log_len < max_records ? c.solve_log_len : max_records;
+    // the k most recent records, oldest first
+    const int first = (c.solve_log_next - k + Context::kSolveLogMax) % Context::kSolveLogMax;
+    for (int r = 0; r < k; ++r) {
+        const double *src = c.solve_log[(first + r) % Context::kSolveLogMax];
+        for (int f = 0; f < Context::kSolveLogFields; ++f) out[Context::kSolveLogFields * r + f] = src[f];
+    }
+    *count = k;
+    if (strong) *strong = c.strong_solves;
+    if (max_records > 0) c.solve_log_len = 0;
+    return SDDDST_OK;
+}
+
+int sdddst_solver_set_direct_limit(sdddst_ctx *ctx, int max_n)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(max_n >= 0, SDDDST_ERR_ARG, "negative limit");
+    ctx->c.direct_max_n = max_n;
     return SDDDST_OK;
 }
 

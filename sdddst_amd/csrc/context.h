@@ -142,7 +142,29 @@ struct Context {
     Fgmres *gm = nullptr;
     int last_iters = 0, fallbacks = 0;
     int64_t total_iters = 0, total_solves = 0;
-    double last_rel_res = 0;
+    double last_rel_res = 0, last_gnorm = 0;
+    int strong_solves = 0;              // solves retried with the strong-row block preconditioner
+    int direct_max_n = 32768;           // largest N the dense direct fallback may factor (8 N^2 bytes)
+    // the most recent iterative-solver attempts, oldest first once full:
+    // {preconditioner, iterations, relative residual, ||g||, outcome, h,
+    // rows of the strong-row block} with outcome 0 converged, 1 not
+    // converged, 2 solved by the direct solver after it
+    static constexpr int kSolveLogMax = 1024, kSolveLogFields = 7;
+    double solve_log[kSolveLogMax][kSolveLogFields];
+    int solve_log_len = 0, solve_log_next = 0;
+    void log_solve(int pc, int iters, double rel, double gnorm, int outcome, int strong_rows = 0)
+    {
+        double *r = solve_log[solve_log_next];
+        r[0] = pc;
+        r[1] = iters;
+        r[2] = rel;
+        r[3] = gnorm;
+        r[4] = outcome;
+        r[5] = jac_h;
+        r[6] = strong_rows;
+        solve_log_next = (solve_log_next + 1) % kSolveLogMax;
+        if (solve_log_len < kSolveLogMax) ++solve_log_len;
+    }
     // timing: recorded event pairs per kernel kind, read back on request
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // timers recording: kTimePair | kTimeSpeeds | kTimeJacobian (each event

@@ -40,21 +40,52 @@ int solver_solve(Context &c)
         int rc = c.gm->solve(c);
         c.total_iters += c.last_iters;
         c.total_solves += 1;
+        c.log_solve(c.precond, c.last_iters, c.last_rel_res, c.last_gnorm, rc == SDDDST_OK ? 0 : 1);
+        const int pair = c.precond == SDDDST_PRECOND_JACOBI ? SDDDST_PRECOND_JACOBI : SDDDST_PRECOND_PAIR;
         if (rc == SDDDST_ERR_SOLVER && c.precond == SDDDST_PRECOND_LEARNED) {
             // the pair blocks are part of the learned preconditioner's factor
             c.precond_fallbacks++;
-            c.precond_override = SDDDST_PRECOND_PAIR;
+            c.precond_override = pair;
             rc = c.gm->solve(c);
             c.precond_override = -1;
             c.total_iters += c.last_iters;
+            c.log_solve(pair, c.last_iters, c.last_rel_res, c.last_gnorm, rc == SDDDST_OK ? 0 : 1);
+        }
+        if (rc == SDDDST_ERR_SOLVER && c.precond != SDDDST_PRECOND_STRONG) {
+            // the pair blocks stagnate on indefinite systems (a step size
+            // far too large for the configuration): once more with the
+            // strong-row block, factored once for this matrix, before the
+            // direct solver
+            rc = c.gm->strong_factor(c);
+            if (rc == SDDDST_OK) {
+                c.precond_override = SDDDST_PRECOND_STRONG;
+                rc = c.gm->solve(c);
+                c.precond_override = -1;
+                c.total_iters += c.last_iters;
+                c.log_solve(SDDDST_PRECOND_STRONG, c.last_iters, c.last_rel_res, c.last_gnorm,
+                            rc == SDDDST_OK ? 0 : 1, c.gm->strong.nh);
+            } else {
+                // too many strong rows (or the block is singular): the
+                // direct solver decides
+                c.log_solve(SDDDST_PRECOND_STRONG, 0, 1.0, c.last_gnorm, 1, c.gm->strong.nh);
+                rc = SDDDST_ERR_SOLVER;
+            }
+            c.strong_solves++;
         }
         if (rc == SDDDST_ERR_SOLVER) {
-            // no convergence: fall back to the direct solver
+            // no convergence: fall back to the direct solver, which holds the
+            // matrix dense (8 N^2 bytes) and so is allowed only up to a size
+            SD_CHECK(c.n <= c.direct_max_n, SDDDST_ERR_NOT_CONVERGED,
+                     "FGMRES did not reach the relative residual %.1e (%.3e after %d iterations) and N = %d is "
+                     "above the dense direct solver's limit %d", c.solver_tol, c.last_rel_res, c.last_iters, c.n,
+                     c.direct_max_n);
             c.fallbacks++;
             if (!c.lu) c.lu = new DenseLu();
             rc = c.lu->factor(c);
             if (rc) return rc;
-            return c.lu->solve(c);
+            rc = c.lu->solve(c);
+            c.log_solve(-1, 0, 0.0, c.last_gnorm, 2);
+            return rc;
         }
         return rc;
     }

@@ -510,6 +510,8 @@ void Fgmres::release()
         if (p) (void)hipFree(p);
     binv = nullptr;
     bcol = nullptr;
+    strong.release();
+    strong_valid = false;
     V = Z = w = dinv = red = hcol = H = Hdev = nullptr;
     ticket = nullptr;
 }
@@ -535,6 +537,8 @@ int Fgmres::factor(Context &c)
 {
     int rc = init(c);
     if (rc) return rc;
+    strong_valid = false;
+    if (c.precond == SDDDST_PRECOND_STRONG) return strong_factor(c);
     if (c.precond == SDDDST_PRECOND_JACOBI) {
         hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
         SD_HIP(hipGetLastError());
@@ -555,6 +559,23 @@ int Fgmres::apply(Context &c, const double *v, double *z)
     else
         hipLaunchKernelGGL(k_pair_apply, dim3(nb256(n)), dim3(256), 0, c.stream, n, binv, bcol, v, z);
     SD_HIP(hipGetLastError());
+    if (precond == SDDDST_PRECOND_STRONG) return strong.apply(c, v, z);
+    return SDDDST_OK;
+}
+
+// Threshold of the strong-row block: rows with |A_ii - 1| or an off-diagonal
+// |A_ij| above it are solved exactly.  At 0.25 the first attempt of the
+// N = 65536 point-defect run keeps about a quarter of the rows; FGMRES then
+// converges in tens of iterations where the pair blocks alone stagnate.
+constexpr double kStrongThreshold = 0.25;
+
+int Fgmres::strong_factor(Context &c)
+{
+    if (strong_valid) return SDDDST_OK;
+    int rc = pair_factor(c);
+    if (rc) return rc;
+    if ((rc = strong.factor(c, kStrongThreshold))) return rc;
+    strong_valid = true;
     return SDDDST_OK;
 }
 
@@ -649,6 +670,7 @@ int Fgmres::solve(Context &c)
             case kZeroRhs:
                 c.last_iters = 0;
                 c.last_rel_res = 0.0;
+                c.last_gnorm = 0.0;
                 return SDDDST_OK;
             case kNeedNorm: {
                 // iteration k: w -= V h2, its norm taken explicitly, then the
@@ -679,6 +701,7 @@ int Fgmres::solve(Context &c)
     }
     c.last_iters = total_it;
     c.last_rel_res = rel;
+    c.last_gnorm = gnorm;
     iter_mean = 0.875 * iter_mean + 0.125 * total_it;
     if (!(c.last_rel_res <= tol)) {
         set_error("FGMRES: relative residual %.3e after %d iterations", c.last_rel_res, total_it);

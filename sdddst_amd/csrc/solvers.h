@@ -18,6 +18,25 @@ struct DenseLu {
     void release();
 };
 
+// Strong-row block (solver_strong.hip): a dense LU of A restricted to the
+// rows H whose diagonal is far from 1 or that hold a large off-diagonal
+// entry, applied on top of the pair-block inverse of the other rows.
+struct StrongBlock {
+    static constexpr int kMaxRows = 32768;   // 8 GiB of FP64 factor
+    int n_alloc = 0, nh = 0, nh_alloc = 0;
+    int *flag = nullptr;       // n: row in H
+    int *pos = nullptr;        // n + 1: exclusive scan of flag (row index in H)
+    long long *total = nullptr;
+    double *hv = nullptr;      // n: right-hand side / solution on H
+    double *D = nullptr;       // nh x nh column-major LU factors
+    int64_t *piv = nullptr, *info = nullptr;
+    rocblas_handle handle = nullptr;
+    int factor(Context &c, double thr);
+    int apply(Context &c, const double *v, double *z);   // z holds the pair-block result on entry
+    void release_rows();
+    void release();
+};
+
 // Restarted flexible GMRES (FP64) on the CSR view of A with a
 // right preconditioner; see solver_fgmres.hip.
 struct KrylovDev;    // state of a device-driven restart cycle (solver_fgmres.hip)
@@ -40,7 +59,10 @@ struct Fgmres {
     KrylovHost *kh = nullptr;    // its host-read part (mapped, coherent pinned memory)
     KrylovHost *khdev = nullptr; // device address of kh
     double iter_mean = 3.0;      // running mean of iterations per solve (first batch size)
+    StrongBlock strong;          // strong-row block of the current matrix
+    bool strong_valid = false;   // factored for the current matrix
     int init(Context &c);
+    int strong_factor(Context &c);                       // pair blocks + strong-row block, once per matrix
     int factor(Context &c);
     int pair_factor(Context &c);                         // pair-block inverse of A
     int apply(Context &c, const double *v, double *z);   // z = M^-1 v

@@ -18,7 +18,7 @@ CORE_PATH = os.path.join(_HERE, "libsdddstCore.so")
 PROTOCOLS = {"zero": 0, "fixed-rate": 1, "spring": 2, "constant": 3}
 SOLVERS = {"auto": 0, "umfpack": 1, "dense-lu": 2, "fgmres": 3}
 FIELD_MODES = {"fast": 0, "exact": 1, "ordered": 2}
-PRECONDITIONERS = {"jacobi": 0, "pair": 1, "learned": 2}
+PRECONDITIONERS = {"jacobi": 0, "pair": 1, "learned": 2, "strong": 3}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -66,6 +66,8 @@ class SimState(ctypes.Structure):
         ("solver_total_iters", ctypes.c_longlong),
         ("solver_total_solves", ctypes.c_longlong),
         ("solver_precond_fallbacks", ctypes.c_int),
+        ("aborted_attempts", ctypes.c_longlong),
+        ("solver_strong_retries", ctypes.c_int),
     ]
 
 
@@ -146,6 +148,8 @@ class StepState:
     solver_total_iters: int
     solver_total_solves: int
     solver_precond_fallbacks: int
+    aborted_attempts: int = 0
+    solver_strong_retries: int = 0
 
 
 class Simulation:
@@ -214,7 +218,8 @@ class Simulation:
         return StepState(s.sim_time, s.step_size, s.strain, s.energy, s.max_error_ratio_sqr, s.sum_avg_speed,
                          s.cutoff, s.succesful_steps, s.failed_steps, bool(s.last_step_succeeded),
                          s.last_step_wall_s, s.solver_last_iters, s.solver_fallbacks, s.solver_last_rel_residual,
-                         s.solver_total_iters, s.solver_total_solves, s.solver_precond_fallbacks)
+                         s.solver_total_iters, s.solver_total_solves, s.solver_precond_fallbacks,
+                         s.aborted_attempts, s.solver_strong_retries)
 
     def get_time(self) -> float:
         return self.state().sim_time

@@ -117,6 +117,22 @@ int sdddst_upload_points(sdddst_ctx *ctx, const double *px, const double *py, in
  * v_out (nullable) receives a host copy. */
 int sdddst_speeds(sdddst_ctx *ctx, int cfg_slot, int out_buf, double sigma, int update_tol, double *v_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): v = the pair speeds of n dislocations plus b*sigma, as
+ * sdddst_speeds computes them without point defects and without a
+ * tolerance update, from caller-owned DEVICE arrays x, y, b (n each) into
+ * the caller-owned device array v (n), enqueued on `stream` (a hipStream_t;
+ * NULL: the null stream); returns once enqueued.  field_mode is
+ * SDDDST_FIELD_FAST or SDDDST_FIELD_EXACT.  No context: partial sums and
+ * per-dislocation factors live in a static device array of the library (a
+ * fixed 8.6 MiB per GPU, held until the process ends), so n <= 8192
+ * (SDDDST_ERR_ARG above).  The call allocates nothing; every call on one GPU
+ * shares that array, so calls that overlap on one GPU race.  Tiles follow
+ * index order, so results equal sdddst_speeds' to rounding, not bit for bit. */
+int sdddst_speeds_noalloc(const double *x, const double *y, const double *b, int n, double sigma, int field_mode,
+                          double *v, void *stream);
+
 /* PrecisionHandler state: tolerance download / reset (precision_handler.cpp:62-70). */
 int sdddst_get_tolerance(sdddst_ctx *ctx, double *tol_out, int n);
 int sdddst_reset_tolerance(sdddst_ctx *ctx);

@@ -79,6 +79,7 @@ _SIGNATURES = {
     "sdddst_solver_stats": (ctypes.c_int, [_vp, _ip, _dp, _ip, _i64p, _i64p]),
     "sdddst_solver_precond_fallbacks": (ctypes.c_int, [_vp, _ip]),
     "sdddst_solver_history": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _ip, _ip]),
+    "sdddst_speeds_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "sdddst_solver_set_direct_limit": (ctypes.c_int, [_vp, ctypes.c_int]),
     "sdddst_spai_param_count": (ctypes.c_int, []),
     "sdddst_spai_set_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),

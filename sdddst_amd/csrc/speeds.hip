@@ -35,6 +35,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "context.h"
 #include "field.h"
 #include "pair_core.h"
@@ -343,4 +345,110 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     return SDDDST_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Pair speeds without allocation (sdddst_speeds_noalloc).  The context path
+// above keeps its partial sums, order and far-pair factors in buffers the
+// context allocates; this variant keeps them in one file-scope device array,
+// `scratch`, sized for at most kNoAllocMaxN dislocations, so a call allocates
+// nothing.  Host code addresses the array through the pointer that
+// hipGetSymbolAddress returns for the current device (taken on the first
+// call on each device), never through its name.  Tiles follow index order
+// (no x-bin sort, whose temporary storage would be one more buffer): the
+// same pair terms, another fixed summation order.
+namespace {
+
+constexpr int kNoAllocMaxN = 8192;
+constexpr int kNoAllocMaxBlocks = kNoAllocMaxN / kTile;
+// layout of scratch: partial sums [nb][n], then five per-dislocation FP64
+// arrays (cos 2 pi y, sin 2 pi y, wrapped x, e^{+-2 pi w}), the tolerance
+// candidates (u64) and the order (int)
+constexpr size_t kNoAllocPartialBytes = sizeof(double) * (size_t)kNoAllocMaxBlocks * kNoAllocMaxN;
+constexpr size_t kNoAllocVecBytes = sizeof(double) * (size_t)kNoAllocMaxN;
+constexpr size_t kNoAllocBytes = kNoAllocPartialBytes + 6 * kNoAllocVecBytes + sizeof(int) * (size_t)kNoAllocMaxN;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxDevices];   // device address of scratch, per device
+
+__global__ void k_iota(int n, int *__restrict__ perm)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) perm[i] = i;
+}
+
+int noalloc_scratch(unsigned char **out)
+{
+    int dev = 0;
+    SD_HIP(hipGetDevice(&dev));
+    SD_CHECK(dev >= 0 && dev < kMaxDevices, SDDDST_ERR_ARG, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        SD_CHECK(e == hipSuccess && p, SDDDST_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                 hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return SDDDST_OK;
+}
+
+}  // namespace
+
+int speeds_noalloc(const double *x, const double *y, const double *b, int n, double sigma, int field_mode,
+                   double *v, hipStream_t stream)
+{
+    SD_CHECK(n >= 2 && n <= kNoAllocMaxN, SDDDST_ERR_ARG,
+             "sdddst_speeds_noalloc: n = %d outside [2, %d], the size of its static scratch", n, kNoAllocMaxN);
+    SD_CHECK(x && y && b && v, SDDDST_ERR_ARG, "sdddst_speeds_noalloc: NULL array");
+    SD_CHECK(field_mode == SDDDST_FIELD_FAST || field_mode == SDDDST_FIELD_EXACT, SDDDST_ERR_ARG,
+             "sdddst_speeds_noalloc: field mode %d (fast or exact)", field_mode);
+    unsigned char *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    double *partial = reinterpret_cast<double *>(base);
+    double *vec = reinterpret_cast<double *>(base + kNoAllocPartialBytes);
+    double *ytc = vec, *yts = vec + kNoAllocMaxN, *xw = vec + 2 * kNoAllocMaxN, *xep = vec + 3 * kNoAllocMaxN,
+           *xem = vec + 4 * kNoAllocMaxN;
+    unsigned long long *minr2 = reinterpret_cast<unsigned long long *>(vec + 5 * kNoAllocMaxN);
+    int *perm = reinterpret_cast<int *>(base + kNoAllocPartialBytes + 6 * kNoAllocVecBytes);
+    const int nb = (n + kTile - 1) / kTile;
+    const long nfull = (long)nb * (nb - 1) / 2;
+    const long units = nfull + (nb + 1) / 2;
+    const int blocks = (n + 255) / 256;
+    // no tolerance update: no pair is a candidate (r2_keep = 0), so every
+    // pair at a far circular x distance may take the far form
+    const bool far = field_mode == SDDDST_FIELD_FAST;
+    hipLaunchKernelGGL(k_iota, dim3(blocks), dim3(256), 0, stream, n, perm);
+    if (far) hipLaunchKernelGGL(k_ytrig, dim3(blocks), dim3(256), 0, stream, n, y, ytc, yts);
+    hipLaunchKernelGGL(k_speeds_prep, dim3(blocks), dim3(256), 0, stream, n, minr2, x, far ? xw : nullptr, xep, xem);
+    SD_HIP(hipGetLastError());
+    if (far) {
+        pairtile::FarData fd;
+        fd.ca = ytc;
+        fd.sa = yts;
+        fd.w = xw;
+        fd.ep = xep;
+        fd.em = xem;
+        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)units), dim3(kTile), 0, stream, n, nb,
+                           (int)nfull, x, y, b, perm, partial, minr2, 0.0, fd);
+    } else {
+        hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, stream, n, nb,
+                           (int)nfull, x, y, b, perm, partial, minr2, 0.0, pairtile::FarData());
+    }
+    SD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(256), 0, stream, n, nb, 0, perm, partial, x, y, b,
+                       nullptr, nullptr, 0.0, 0.0, sigma, minr2, nullptr, 0.0, 0, v);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
 }  // namespace sdamd
+
+extern "C" int sdddst_speeds_noalloc(const double *x, const double *y, const double *b, int n, double sigma,
+                                     int field_mode, double *v, void *stream)
+{
+    return sdamd::speeds_noalloc(x, y, b, n, sigma, field_mode, v, static_cast<hipStream_t>(stream));
+}

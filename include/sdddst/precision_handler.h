@@ -6,6 +6,8 @@
 #define SDDDST_CORE_PRECISION_HANDLER_H
 
 #include <string>
+#include <utility>
+#include <vector>
 
 namespace sdddstCore {
 
@@ -17,8 +19,13 @@ public:
     void setSize(unsigned int size);
     unsigned long getSize() const;
 
-    // precision_handler.cpp:62-70 (scalar part; the device resets its arrays)
+    // precision_handler.cpp:62-70 (the device resets its own arrays)
     void reset();
+    // precision_handler.cpp:72-101 on host arrays of size getSize(), for
+    // callers of the class API; the simulator updates its tolerance and
+    // error on the device.
+    void updateTolerance(const double &distanceSqr, const unsigned int &ID);
+    void updateError(const double &error, const unsigned int &ID);
     // precision_handler.cpp:103-119, with the host C library's pow.
     double getNewStepSize(const double &oldStepSize) const;
 
@@ -39,6 +46,8 @@ private:
     double minPrecisitySqr;
     double maxErrorRatioSqr;
     unsigned int selectedID;
+    std::vector<std::pair<double, double>> toleranceAndError;
+    bool hostTracking = false;
 };
 
 }  // namespace sdddstCore

@@ -14,6 +14,7 @@
 #include "precision_handler.h"
 #include "simulation_data.h"
 #include "StressProtocols/stress_protocol.h"
+#include "../sdddst_amd.h"
 
 struct sdddst_ctx;
 
@@ -63,6 +64,10 @@ public:
     // above the dense direct solver's limit (no counterpart in the
     // reference, whose UMFPACK always solves).
     long long abortedAttemptCount() const { return abortedAttempts; }
+    // Carries settings the caller changed in the shared SimulationData
+    // (cutoff multiplier, precision, point-defect constants, iteration
+    // count, field mode) to the device before the next step.
+    void applyDataChanges();
 
 private:
     void rejectAbortedAttempt();
@@ -91,6 +96,7 @@ private:
     double vsquare2;
     double lastRunWall;
     long long abortedAttempts = 0;
+    sdddst_params pushed{};   // what the device context holds
 
     std::shared_ptr<SimulationData> sD;
     std::unique_ptr<PrecisionHandler> pH;

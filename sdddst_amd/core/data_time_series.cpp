@@ -4,17 +4,20 @@
 
 #include <cmath>
 #include <fstream>
+#include <sstream>
 
 #include "../../include/sdddst/constants.h"
 
+// The list holds one "time path" pair per line; blank lines are skipped.
 DataTimeSeries::DataTimeSeries(std::string dconfList, std::string fconf) : fconfPath(fconf)
 {
-    std::ifstream dcs(dconfList);
-    while (dcs) {
-        std::string t, path;
-        if (!(dcs >> t) || t.empty()) break;
-        dcs >> path;
-        dconfs.emplace_back(std::stod(t), path);
+    std::ifstream list(dconfList);
+    for (std::string line; std::getline(list, line);) {
+        std::istringstream fields(line);
+        std::string time, path;
+        if (!(fields >> time)) continue;
+        fields >> path;
+        dconfs.emplace_back(std::stod(time), path);
     }
 }
 

@@ -1,8 +1,10 @@
 // PrecisionHandler scalar state (reference: src/precision_handler.cpp:28-151).
 #include <math.h>
 
+#include <iostream>
 #include <string>
 
+#include "../../include/sdddst/constants.h"
 #include "../../include/sdddst/precision_handler.h"
 
 using namespace sdddstCore;
@@ -14,10 +16,53 @@ PrecisionHandler::PrecisionHandler()
 
 PrecisionHandler::~PrecisionHandler() {}
 
-void PrecisionHandler::setSize(unsigned int s) { size = s; }
+void PrecisionHandler::setSize(unsigned int s)
+{
+    size = s;
+    // entries kept, new ones at (prec^2, 0) (precision_handler.cpp:42-55)
+    toleranceAndError.resize(s, std::make_pair(minPrecisitySqr, 0.0));
+}
 unsigned long PrecisionHandler::getSize() const { return size; }
 
-void PrecisionHandler::reset() { maxErrorRatioSqr = 0; }
+void PrecisionHandler::reset()
+{
+    maxErrorRatioSqr = 0;
+    // the host arrays are touched only by callers of updateTolerance /
+    // updateError; the simulator's own arrays live on the device
+    if (hostTracking)
+        for (auto &te : toleranceAndError) te = std::make_pair(minPrecisitySqr, 0.0);
+}
+
+// precision_handler.cpp:72-87
+void PrecisionHandler::updateTolerance(const double &distanceSqr, const unsigned int &ID)
+{
+    hostTracking = true;
+    const double candidate = distanceSqr * 0.25 * 1e-2;
+    double &tol = toleranceAndError.at(ID).first;
+    if (candidate < minPrecisitySqr && candidate < tol) {
+        if (candidate == 0.0) {
+            tol = EPS;
+            std::cout << "Two dislocations are in the same place!\n";
+        } else {
+            tol = candidate;
+        }
+    }
+}
+
+// precision_handler.cpp:89-101
+void PrecisionHandler::updateError(const double &error, const unsigned int &ID)
+{
+    hostTracking = true;
+    std::pair<double, double> &te = toleranceAndError.at(ID);
+    if (te.second < error) {
+        te.second = error;
+        const double ratio = error * error / te.first;
+        if (ratio > maxErrorRatioSqr) {
+            maxErrorRatioSqr = ratio;
+            selectedID = ID;
+        }
+    }
+}
 
 double PrecisionHandler::getNewStepSize(const double &oldStepSize) const
 {

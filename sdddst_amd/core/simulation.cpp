@@ -54,6 +54,7 @@ Simulation::Simulation(std::shared_ptr<SimulationData> _sD)
 
     sdddst_params p{sD->cutOffMultiplier, sD->prec, sD->A, sD->KASQR, (int)sD->ic, sD->fieldMode};
     check(sdddst_ctx_create((int)sD->dc, (int)sD->pc, &p, &ctx), "sdddst_ctx_create");
+    pushed = p;
     std::vector<double> x(sD->dc), y(sD->dc), b(sD->dc);
     for (unsigned i = 0; i < sD->dc; ++i) {
         x[i] = sD->dislocations[i].x;
@@ -122,6 +123,21 @@ void Simulation::pushParams()
 {
     sdddst_params p{sD->cutOffMultiplier, sD->prec, sD->A, sD->KASQR, (int)sD->ic, sD->fieldMode};
     check(sdddst_set_params(ctx, &p), "set params");
+    pushed = p;
+}
+
+// The SimulationData is shared with the caller (the bindings expose its
+// fields as properties): settings changed since the last push reach the
+// device context's copy and the precision handler before the next step.
+void Simulation::applyDataChanges()
+{
+    const sdddst_params p{sD->cutOffMultiplier, sD->prec, sD->A, sD->KASQR, (int)sD->ic, sD->fieldMode};
+    if (p.cutoff_multiplier == pushed.cutoff_multiplier && p.precision == pushed.precision && p.A == pushed.A &&
+        p.KASQR == pushed.KASQR && p.iteration_count == pushed.iteration_count && p.field_mode == pushed.field_mode)
+        return;
+    if (p.precision != pushed.precision) pH->setMinPrecisity(sD->prec);
+    sD->updateCutOff();
+    pushParams();
 }
 
 void Simulation::uploadPositions(int slot, const std::vector<Dislocation> &in)

@@ -45,23 +45,42 @@ SimulationData::SimulationData(const std::string &dislocationDataFilePath, const
 
 SimulationData::~SimulationData() { deleteDislocationCountRelatedData(); }
 
-// simulation_data.cpp:119-150: whitespace-separated {x y b} triplets.
+namespace {
+// The configuration files are plain whitespace-separated numbers, `width`
+// per record ({x y b} for dislocations, {x y} for point defects; the
+// reference reads them at simulation_data.cpp:119-150, 163-188).  Every
+// token is converted as a decimal double; a token that is not one, or a
+// count that is not a whole number of records, is an error naming the file.
+std::vector<double> readRecords(const std::string &path, size_t width, const char *what)
+{
+    std::ifstream in(path);
+    if (!in.is_open()) throw std::runtime_error(std::string("Cannot open ") + what + " file: " + path);
+    std::vector<double> values;
+    for (std::string token; in >> token;) {
+        size_t used = 0;
+        double v = 0;
+        try {
+            v = std::stod(token, &used);
+        } catch (const std::exception &) {
+            used = 0;
+        }
+        if (used != token.size())
+            throw std::runtime_error(std::string(what) + " file " + path + ": '" + token + "' is not a number");
+        values.push_back(v);
+    }
+    if (values.size() % width != 0)
+        throw std::runtime_error(std::string(what) + " file " + path + ": " + std::to_string(values.size()) +
+                                 " numbers are not whole records of " + std::to_string(width));
+    return values;
+}
+}  // namespace
+
 void SimulationData::readDislocationDataFromFile(const std::string &path)
 {
     if (path.empty()) return;
-    std::ifstream in(path);
-    if (!in.is_open()) throw std::runtime_error("Cannot open dislocation data file: " + path);
-    std::vector<Dislocation> d;
-    while (!in.eof()) {
-        std::string data;
-        in >> data;
-        if (data == "") break;
-        Dislocation tmp;
-        tmp.x = std::stod(data);
-        in >> tmp.y;
-        in >> tmp.b;
-        d.push_back(tmp);
-    }
+    const std::vector<double> v = readRecords(path, 3, "dislocation data");
+    std::vector<Dislocation> d(v.size() / 3);
+    for (size_t i = 0; i < d.size(); ++i) d[i] = Dislocation{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
     setDislocations(d);
 }
 
@@ -74,24 +93,15 @@ void SimulationData::writeDislocationDataToFile(const std::string &path)
     for (auto &i : dislocations) out << i.x << " " << i.y << " " << i.b << "\n";
 }
 
-// simulation_data.cpp:163-188
 void SimulationData::readPointDefectDataFromFile(const std::string &path)
 {
     pc = 0;
     points.clear();
     if (path.empty()) return;
-    std::ifstream in(path);
-    if (!in.is_open()) throw std::runtime_error("Cannot open fixpoints data file: " + path);
-    while (!in.eof()) {
-        std::string data;
-        in >> data;
-        if (data == "") break;
-        PointDefect tmp;
-        tmp.x = std::stod(data);
-        in >> tmp.y;
-        points.push_back(tmp);
-        pc++;
-    }
+    const std::vector<double> v = readRecords(path, 2, "fixpoints data");
+    std::vector<PointDefect> p(v.size() / 2);
+    for (size_t i = 0; i < p.size(); ++i) p[i] = PointDefect{v[2 * i], v[2 * i + 1]};
+    setPointDefects(p);
 }
 
 // simulation_data.cpp:190-201

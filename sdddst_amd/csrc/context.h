@@ -81,6 +81,7 @@ struct Context {
     size_t sort_tmp_bytes = 0;
     int order_age = 0;                  // speeds calls since the order was last computed
     double *ytc = nullptr, *yts = nullptr;   // cos, sin of 2 pi y per dislocation (set with y)
+    double *yw = nullptr;                    // y - rint(y) per dislocation (set with y)
     double *xw = nullptr, *xep = nullptr, *xem = nullptr;   // x - rint(x), e^{+-2 pi (x - rint(x))}
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;

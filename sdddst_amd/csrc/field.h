@@ -253,16 +253,25 @@ SDLIBM_FN FarParts far_image_parts(double dx, double cy, double t, double ti)
     // |u| >= 2.5 (k = 3 both sides, the weighted 4th pair, the 5th): with
     // a = 1 / cosh(2 pi u) <= 3e-7, (c cy - 1) / (c - cy)^2 =
     // a cy + a^2 (2 cy^2 - 1) + O(a^3), and a = 2 e^{-2 pi |u|} to within
-    // e^{-4 pi |u|} <= 3e-14 relative: the omitted parts are below 1e-19.
+    // e^{-4 pi |u|} <= 3e-14 relative.  With a = ti e_k (images ad + k) or
+    // t e_k (images k - ad), e_k = 2 e^{-2 pi k}, the weighted first-order
+    // sum is ti p1(ad) + t p2(ad) for two quadratics in ad:
+    //   p1 = e3 (ad + 3) + e4 (1 - ad)(ad + 4)
+    //   p2 = e3 (ad - 3) + e4 (ad - 4) + e5 ad (ad - 5)
+    // and of the second-order terms only k = 3 reaches 1e-16 (a^2 <= 1e-13);
+    // those of k = 4, 5 (a^2 <= 3e-19) and the third-order ones are omitted,
+    // below 1e-17 of the field after the factor 2 pi^2.
+    constexpr double e3 = 2.0 * sdlibm_exp_m2pik[3], e4 = 2.0 * sdlibm_exp_m2pik[4];
+    constexpr double e5 = 2.0 * sdlibm_exp_m2pik[kImages + 1];
+    constexpr double p10 = 3.0 * e3 + 4.0 * e4, p11 = e3 - 3.0 * e4, p12 = -e4;
+    constexpr double p20 = -3.0 * e3 - 4.0 * e4, p21 = e3 + e4 - 5.0 * e5, p22 = e5;
+    constexpr double e3sq = e3 * e3;
+    const double p1 = __builtin_fma(__builtin_fma(p12, ad, p11), ad, p10);
+    const double p2 = __builtin_fma(__builtin_fma(p22, ad, p21), ad, p20);
+    const double first = __builtin_fma(ti, p1, t * p2);
+    const double second = e3sq * __builtin_fma(ti * ti, ad + 3.0, (t * t) * (ad - 3.0));
     const double c2 = __builtin_fma(2.0 * cy, cy, -1.0);
-    const double e3 = 2.0 * sdlibm_exp_m2pik[3], e4 = 2.0 * sdlibm_exp_m2pik[4];
-    const double e5 = 2.0 * sdlibm_exp_m2pik[kImages + 1];
-    const double aA3 = ti * e3, aB3 = t * e3, aA4 = ti * e4, aB4 = t * e4, aB5 = t * e5;
-    double S = ((ad + 3.0) * aA3) * __builtin_fma(c2, aA3, cy);
-    S = __builtin_fma((ad - 3.0) * aB3, __builtin_fma(c2, aB3, cy), S);
-    S = __builtin_fma(((1.0 - ad) * (ad + 4.0)) * aA4, __builtin_fma(c2, aA4, cy), S);
-    S = __builtin_fma((ad - 4.0) * aB4, __builtin_fma(c2, aB4, cy), S);
-    S = __builtin_fma((ad * (ad - 5.0)) * aB5, __builtin_fma(c2, aB5, cy), S);
+    const double S = __builtin_fma(c2, second, cy * first);
     return FarParts{P, Q, S};
 }
 
@@ -296,9 +305,9 @@ SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__r
         c0 = 0.5 * t + 0.5 * ti;
     }
     const double near = (dx * dx + dy * dy > 1e-6) ? f_given_cosh(dx, c0, cy) : f_taylor(dx, dy);
-    double far = far_images(dx, cy, t, ti);
-    far = dx < 0 ? -far : far;
-    return near + far * (2.0 * kPiSq);
+    // the far images are sg(dx) F(|dx|): the sign rides on the constant
+    // (negating one factor of a product negates the rounded product exactly)
+    return near + far_images(dx, cy, t, ti) * __builtin_copysign(2.0 * kPiSq, dx);
 }
 
 // The production field for a pair with |dx| >= kTrigMinDx (far pairs of the
@@ -309,12 +318,15 @@ SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__r
 // here, so the near image needs no care against cancellation).
 SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
 {
-    const double c0 = 0.5 * t + 0.5 * ti;
+    // 0.5 t and 0.5 ti are exact, so the fma rounds their sum once, as the
+    // sum of the two products does
+    const double c0 = __builtin_fma(0.5, t, 0.5 * ti);
     const double d = c0 - cy;
-    const double A = dx * (c0 * cy - 1.0), B = d * d;
+    // every term is odd in dx: evaluated at |dx|, the sign applied with the
+    // constant factor (exact: each step's rounding is symmetric in sign)
+    const double A = sdlibm::fabsd(dx) * (c0 * cy - 1.0), B = d * d;
     const FarParts f = far_image_parts(dx, cy, t, ti);
-    const double P = dx < 0 ? -f.P : f.P, S = dx < 0 ? -f.S : f.S;
-    return (__builtin_fma(A, f.Q, P * B) / (B * f.Q) + S) * (2.0 * kPiSq);
+    return (__builtin_fma(A, f.Q, f.P * B) / (B * f.Q) + f.S) * __builtin_copysign(2.0 * kPiSq, dx);
 }
 
 // Per-dislocation factors of the far pairs: the wrapped x w = x - rint(x)
@@ -339,13 +351,20 @@ SDLIBM_FN double circ(double z) { return sdlibm::fabsd(z - __builtin_rint(z)); }
 SDLIBM_FN double far_pair(double wi, double wj, double epi, double emi, double epj, double emj, double cy)
 {
     const double s = wi - wj;
-    const double k = __builtin_rint(s);
-    const double dx = s - k;
-    const double gk = k == 0.0 ? 1.0 : (k > 0.0 ? sdlibm_exp_m2pik[1] : sdlibm_exp_2pik[1]);
-    const double gmk = k == 0.0 ? 1.0 : (k > 0.0 ? sdlibm_exp_2pik[1] : sdlibm_exp_m2pik[1]);
-    const double pp = (epi * emj) * gk, pm = (emi * epj) * gmk;
-    const bool pos = dx >= 0.0;
-    return xy_far(dx, cy, pos ? pp : pm, pos ? pm : pp);
+    const double dx = s - __builtin_rint(s);
+    // u = e^{2 pi s}, v = e^{-2 pi s}: the larger is e^{2 pi |s|}.  |dx| is
+    // |s| when rint(s) = 0 (|s| <= 1/2, ties to even) and 1 - |s| otherwise,
+    // so t = e^{2 pi |dx|} and ti = e^{-2 pi |dx|} are the larger and the
+    // smaller of u and v, the smaller times e^{2 pi} and the larger times
+    // e^{-2 pi} when s wrapped.  These are the products e^{2 pi wi}
+    // e^{-2 pi wj} e^{-2 pi k} of either sign, ordered by the sign of dx,
+    // without selecting on k or on that sign.
+    const double u = epi * emj, v = emi * epj;
+    const double hi = fmax(u, v), lo = fmin(u, v);
+    const bool wrapped = sdlibm::fabsd(s) > 0.5;
+    const double t = wrapped ? lo * sdlibm_exp_2pik[1] : hi;
+    const double ti = wrapped ? hi * sdlibm_exp_m2pik[1] : lo;
+    return xy_far(dx, cy, t, ti);
 }
 
 // Production form of xy() (see header comment).

@@ -18,9 +18,9 @@
 // recorded with an atomic min instead of a rotated accumulator.
 //
 // With Perm, tile positions are sorted positions: perm[p] is the dislocation
-// at position p (the single-system kernel orders them by x, so that most
-// tiles hold only pairs with |dx| above the small-argument range of cosh and
-// its wavefronts skip that code).  Every pair term is still evaluated lower
+// at position p (the single-system kernel orders them by cells of a 2D grid
+// over the unit cell, so that most tiles hold only pairs far apart in x or in
+// y, whose wavefronts take the short far form throughout).  Every pair term is still evaluated lower
 // original index first -- the difference is negated when the source has the
 // lower index, and the term then enters the row with a minus sign and the
 // column with a plus sign.  The partial sums stay at sorted positions
@@ -55,11 +55,24 @@ struct TileLds {
 struct FarData {
     const double *ca = nullptr, *sa = nullptr;   // cos, sin of 2 pi y
     const double *w = nullptr, *ep = nullptr, *em = nullptr;   // x - rint(x), e^{+-2 pi w}
+    const double *yw = nullptr;                  // y - rint(y)
 };
 
+// A pair takes the far form when its circular x distance is at least
+// kTrigMinDx or cos(2 pi dy) (by angle addition) is at most kFarMaxCy:
+// either way cosh(2 pi dx) - cos(2 pi dy) >= 0.2, where few-ulp errors in
+// the exponentials and the cosine move the term by at most ~1e-14 of
+// |tau| + 1.  Tiles classify from their members' extents: all pairs far when
+// the x gap is at least kTrigMinDx or the y gap at least kFarMinDy
+// (cos(2 pi 0.11) = 0.77 < kFarMaxCy with margin), none when the x spread is
+// below kTrigMinDx and the y spread below kNearMaxDy (cos(2 pi 0.095) = 0.83).
+constexpr double kFarMaxCy = 0.8;
+constexpr double kFarMinDy = 0.11;
+constexpr double kNearMaxDy = 0.095;
+
 // How a tile's pairs are evaluated: every pair in the reference form (all
-// circular x distances provably below kTrigMinDx, or no far-pair factors),
-// every pair in the far form (all provably at or above it), or per pair.
+// provably near by both rules above, or no far-pair factors), every pair in
+// the far form (all provably far), or per pair.
 enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2 };
 
 __device__ __forceinline__ double rotate_down1(double v, int lane)
@@ -87,8 +100,10 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
     bi = t - c * (c + 1) / 2;
 }
 
-// One pair term for target (xi, yi, bI) of original index io and source
-// slot m, lower original index first; returns the row contribution.  The
+// One pair's field for target (xi, yi) of original index io and source slot
+// m, lower original index first, with the sign of the row contribution and
+// without the Burgers vectors (the sweep folds b_j into the row sum and b_i
+// into the column sum, and multiplies each sum by its own b once).  The
 // fast field is odd bit for bit (tau(-d) = -tau(d): cos is even, the near
 // image's expression and the far images' sg(dx) F(|dx|) flip sign exactly),
 // and the wrap satisfies normalize(-d) = -normalize(d) except at |d| = 1/2
@@ -96,9 +111,9 @@ __device__ __forceinline__ void tile_coords(int t, int &bi, int &bj)
 // lower-first term with its sign; dx = +-0 differs only by the far images'
 // rounding residue (below 1e-17).
 template <int Mode, bool Perm>
-__device__ __forceinline__ double pair_term(double xi, double yi, double bI, int io, int jo, double sxm, double sym,
-                                            double sbm, bool lower_by_lane, double &r2,
-                                            const double *__restrict__ sct, const uint64_t *__restrict__ e2t)
+__device__ __forceinline__ double pair_term(double xi, double yi, int io, int jo, double sxm, double sym,
+                                            bool lower_by_lane, double &r2, const double *__restrict__ sct,
+                                            const uint64_t *__restrict__ e2t)
 {
     const bool lower = (Perm && Mode != SDDDST_FIELD_FAST) ? io < jo : (Perm ? true : lower_by_lane);
     const double d = xi - sxm, e = yi - sym;
@@ -112,7 +127,7 @@ __device__ __forceinline__ double pair_term(double xi, double yi, double bI, int
         dx = sdfield::normalize(lower ? d : -d);
         dy = sdfield::normalize(lower ? e : -e);
     }
-    const double t = bI * sbm * field_tau<Mode>(dx, dy, sct, e2t);
+    const double t = field_tau<Mode>(dx, dy, sct, e2t);
     r2 = dx * dx + dy * dy;
     return lower ? t : -t;
 }
@@ -142,14 +157,11 @@ __device__ __forceinline__ void far_stage(TileLds &L, int l, const FarSelf &v)
     L.em[l] = v.em;
 }
 
-// The tile kind from the wrapped x of the target lanes (tw) and of the source
-// slots (sw), NaN for lanes outside the system; the same on every lane.  The
-// differences s = tw_i - sw_j of the tile lie in [lo, hi] (rounding is
-// monotone), and circ() peaks at half-integers and vanishes at integers, so
-// its extremes over [lo, hi] bound every pair's circ(s).
-__device__ __forceinline__ int tile_kind(double tw, double sw)
+// Bounds of circ(t_i - s_j) over a tile from the wrapped coordinates of the
+// target lanes (t) and the source slots (s), NaN outside the system.
+__device__ __forceinline__ void circ_bounds(double t, double s, double &mind, double &maxd)
 {
-    double tmin = tw, tmax = tw, smin = sw, smax = sw;
+    double tmin = t, tmax = t, smin = s, smax = s;
     for (int off = 32; off > 0; off >>= 1) {
         tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
         tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
@@ -160,32 +172,48 @@ __device__ __forceinline__ int tile_kind(double tw, double sw)
     const double clo = sdfield::circ(lo), chi = sdfield::circ(hi);
     const bool has_int = (lo <= 0.0 && hi >= 0.0) || (lo <= 1.0 && hi >= 1.0) || (lo <= -1.0 && hi >= -1.0);
     const bool has_half = (lo <= 0.5 && hi >= 0.5) || (lo <= -0.5 && hi >= -0.5);
-    const double mind = has_int ? 0.0 : fmin(clo, chi);
-    const double maxd = has_half ? 0.5 : fmax(clo, chi);
-    const int kind = mind >= sdfield::kTrigMinDx ? kFarTile : (maxd < sdfield::kTrigMinDx ? kNearTile : kMixedTile);
+    mind = has_int ? 0.0 : fmin(clo, chi);
+    maxd = has_half ? 0.5 : fmax(clo, chi);
+}
+
+// The tile kind from the wrapped x and y of its members; the same on every
+// lane.  The differences of the tile lie in [lo, hi] (rounding is
+// monotone), and circ() peaks at half-integers and vanishes at integers, so
+// its extremes over [lo, hi] bound every pair's circular distance.
+__device__ __forceinline__ int tile_kind(double tw, double sw, double ty, double sy)
+{
+    double xmin, xmax, ymin, ymax;
+    circ_bounds(tw, sw, xmin, xmax);
+    circ_bounds(ty, sy, ymin, ymax);
+    const int kind = (xmin >= sdfield::kTrigMinDx || ymin >= kFarMinDy)
+                         ? kFarTile
+                         : ((xmax < sdfield::kTrigMinDx && ymax < kNearMaxDy) ? kNearTile : kMixedTile);
     return __builtin_amdgcn_readfirstlane(kind);
 }
 
 // One pair of a far or mixed tile: the far form when circ(w_i - w_j) >=
-// kTrigMinDx (always in a far tile), else the reference form.  r2 is set
-// for the reference form only (far pairs are never tolerance candidates:
-// kTrigMinDx^2 exceeds every r2_keep the caller passes on this path).
+// kTrigMinDx or cos(2 pi dy) <= kFarMaxCy (always in a far tile), else the
+// reference form.  The choice is symmetric in the two dislocations bit for
+// bit (circ of a negated difference, cos_diff's commuted products).  r2 is
+// set for the reference form only (far pairs are never tolerance
+// candidates: their r^2 >= 0.01 exceeds every r2_keep the caller passes on
+// this path).
 template <int Mode, bool Perm, int Kind>
-__device__ __forceinline__ double tile_term(double xi, double yi, double bI, int io, int jm, const TileLds &L, int m,
+__device__ __forceinline__ double tile_term(double xi, double yi, int io, int jm, const TileLds &L, int m,
                                             const FarSelf &fi, bool lower_by_lane, double &r2,
                                             const double *__restrict__ sct, const uint64_t *__restrict__ e2t)
 {
     if constexpr (Kind == kNearTile) {
-        return pair_term<Mode, Perm>(xi, yi, bI, io, jm, L.x[m], L.y[m], L.b[m], lower_by_lane, r2, sct, e2t);
+        return pair_term<Mode, Perm>(xi, yi, io, jm, L.x[m], L.y[m], lower_by_lane, r2, sct, e2t);
     } else {
         const double wj = L.w[m];
-        const bool far = Kind == kFarTile || sdfield::circ(fi.w - wj) >= sdfield::kTrigMinDx;
+        const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
+        const bool far = Kind == kFarTile || sdfield::circ(fi.w - wj) >= sdfield::kTrigMinDx || cy <= kFarMaxCy;
         if (far) {
             r2 = 1.0;
-            const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
-            return bI * L.b[m] * sdfield::far_pair(fi.w, wj, fi.ep, fi.em, L.ep[m], L.em[m], cy);
+            return sdfield::far_pair(fi.w, wj, fi.ep, fi.em, L.ep[m], L.em[m], cy);
         }
-        return pair_term<Mode, Perm>(xi, yi, bI, io, jm, L.x[m], L.y[m], L.b[m], lower_by_lane, r2, sct, e2t);
+        return pair_term<Mode, Perm>(xi, yi, io, jm, L.x[m], L.y[m], lower_by_lane, r2, sct, e2t);
     }
 }
 
@@ -193,11 +221,12 @@ __device__ __forceinline__ double tile_term(double xi, double yi, double bI, int
 // and bi < bj, so every pair of the tile counts.  sct / e2t (fast field
 // only) may point at LDS copies of the tables written before the call: the
 // tile's barrier orders them before the first read.  With far-pair factors
-// (fast field) a pair whose circular x distance is at least kTrigMinDx takes
-// the far form (cos by angle addition, the exponentials from per-dislocation
-// factors); the choice is made per pair, so a pair's value does not depend
-// on the tile that holds it, and the tile's x intervals decide once whether
-// all, none or some of its pairs are far.
+// (fast field) a pair whose circular x distance is at least kTrigMinDx, or
+// whose cos(2 pi dy) is at most kFarMaxCy, takes the far form (cos by angle
+// addition, the exponentials from per-dislocation factors); the choice is
+// made per pair, so a pair's value does not depend on the tile that holds
+// it, and the tile's x and y intervals decide once whether all, none or some
+// of its pairs are far.
 template <int Mode, bool Perm = false, bool Full = false>
 __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,
@@ -225,7 +254,8 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
     if constexpr (Mode == SDDDST_FIELD_FAST) {
         if (fd.w) {
             const double nan = __builtin_nan("");
-            kind = tile_kind(p < n ? fd.w[io] : nan, q < n ? fd.w[jo] : nan);
+            kind = tile_kind(p < n ? fd.w[io] : nan, q < n ? fd.w[jo] : nan, p < n ? fd.yw[io] : nan,
+                             q < n ? fd.yw[jo] : nan);
             if (kind != kNearTile) {
                 far_stage(L, l, q < n ? far_load(fd, jo) : FarSelf());
                 if (p < n) fi = far_load(fd, io);
@@ -246,10 +276,12 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
             const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
             double r2;
             // without Perm the target has the lower index in every tile it meets
-            const double t = tile_term<Mode, Perm, K>(xi, yi, bI, io, jm, L, m, fi, true, r2, sct, e2t);
+            const double u = tile_term<Mode, Perm, K>(xi, yi, io, jm, L, m, fi, true, r2, sct, e2t);
             if (valid) {
-                row += t;
-                col -= t;
+                // term b_i b_j u: b_j here, b_i on the row sum below; the
+                // column sum of source j gets -b_i u here and b_j below
+                row = __builtin_fma(sb[m], u, row);
+                col = __builtin_fma(-bI, u, col);
                 if (K != kFarTile && r2 < r2_keep) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
                     atomicMin(&minr2[io], bits);
@@ -258,11 +290,12 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
             }
             col = rotate_down1(col, l);
         }
+        // after the 64 sweeps lane l holds the column sum of source slot l
         if (diag) {
-            if (p < n) partial[(size_t)bi * n + p] = row + col;
+            if (p < n) partial[(size_t)bi * n + p] = bI * (row + col);
         } else {
-            if (p < n) partial[(size_t)bj * n + p] = row;
-            if (q < n) partial[(size_t)bi * n + q] = col;
+            if (p < n) partial[(size_t)bj * n + p] = bI * row;
+            if (q < n) partial[(size_t)bi * n + q] = sb[l] * col;
         }
     };
     if constexpr (Mode == SDDDST_FIELD_FAST) {
@@ -311,14 +344,15 @@ __device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const doubl
     sy[l] = yi;
     sb[l] = bI;
     sid[l] = io;
-    // a block against itself: never all far (its x interval overlaps itself);
-    // mixed when its members lie kTrigMinDx or more apart
+    // a block against itself: never all far (its intervals overlap
+    // themselves); mixed when its members spread far enough apart
     int kind = kNearTile;
     FarSelf fi;
     if constexpr (Mode == SDDDST_FIELD_FAST) {
         if (fd.w) {
             const double wv = p < n ? fd.w[io] : __builtin_nan("");
-            kind = tile_kind(wv, wv);
+            const double yv = p < n ? fd.yw[io] : __builtin_nan("");
+            kind = tile_kind(wv, wv, yv, yv);
             if (kind != kNearTile) {
                 if (p < n) fi = far_load(fd, io);
                 far_stage(L, l, fi);
@@ -335,10 +369,10 @@ __device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const doubl
             const bool valid = (p < n) && (j < n) && (s < kTile / 2 || l < kTile / 2);
             const int jm = (Perm && Mode != SDDDST_FIELD_FAST) ? sid[m] : j;
             double r2;
-            const double t = tile_term<Mode, Perm, K>(xi, yi, bI, io, jm, L, m, fi, m > l, r2, sct, e2t);
+            const double u = tile_term<Mode, Perm, K>(xi, yi, io, jm, L, m, fi, m > l, r2, sct, e2t);
             if (valid) {
-                row += t;
-                col -= t;
+                row = __builtin_fma(sb[m], u, row);
+                col = __builtin_fma(-bI, u, col);
                 if (r2 < r2_keep) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
                     atomicMin(&minr2[io], bits);
@@ -349,7 +383,7 @@ __device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const doubl
         }
         // after sweeps 1..32 the column sum of source m sits in lane (m - 33) mod 64
         const double cs = __shfl(col, (l + 31) & 63, 64);
-        if (p < n) partial[(size_t)bk * n + p] = row + cs;
+        if (p < n) partial[(size_t)bk * n + p] = bI * (row + cs);
     };
     if constexpr (Mode == SDDDST_FIELD_FAST) {
         if (kind == kMixedTile) sweep(std::integral_constant<int, kMixedTile>{});

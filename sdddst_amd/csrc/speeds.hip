@@ -97,7 +97,8 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
 // Far-pair factors of the fast field: cos and sin of 2 pi y per dislocation
 // (y wrapped first; y changes only by upload), and the wrapped x with
 // e^{+-2 pi x} (per speeds call).
-__global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict__ yc, double *__restrict__ ys)
+__global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict__ yc, double *__restrict__ ys,
+                        double *__restrict__ yw)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
@@ -105,20 +106,25 @@ __global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict_
     sdfield::ytrig(y[i], cv, sv);
     yc[i] = cv;
     ys[i] = sv;
+    yw[i] = y[i] - __builtin_rint(y[i]);
 }
 
-// Sort keys of the pair-tile order: the bin of the wrapped x among nb equal
-// bins, and the index (a stable radix sort keeps equal bins in index order,
-// so the order is a function of the positions alone).
-__global__ void k_xbin(int n, int nbins, const double *__restrict__ x, int *__restrict__ key, int *__restrict__ idx)
+// Sort keys of the pair-tile order: the cell of the wrapped (x, y) in a
+// G x G grid of the unit cell, rows of cells by y, cells in a row by x, with
+// G^2 <= nb so that a tile of 64 holds about one cell; and the index (a
+// stable radix sort keeps equal cells in index order, so the order is a
+// function of the positions alone).  Tiles compact in both x and y make most
+// tile pairs all-far by either coordinate.
+__global__ void k_cellbin(int n, int g, const double *__restrict__ x, const double *__restrict__ y,
+                          int *__restrict__ key, int *__restrict__ idx)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    const double xi = x[i];
-    const double w = xi - floor(xi + 0.5);
-    int k = (int)((w + 0.5) * (double)nbins);
-    k = k < 0 ? 0 : (k >= nbins ? nbins - 1 : k);
-    key[i] = k;
+    const double wx = x[i] - floor(x[i] + 0.5), wy = y[i] - floor(y[i] + 0.5);
+    int cx = (int)((wx + 0.5) * (double)g), cy = (int)((wy + 0.5) * (double)g);
+    cx = cx < 0 ? 0 : (cx >= g ? g - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= g ? g - 1 : cy);
+    key[i] = cy * g + cx;
     idx[i] = i;
 }
 
@@ -245,17 +251,26 @@ static int key_bits(int nbins)
     return bits;
 }
 
+// side of the cell grid of the pair-tile order: G^2 <= nb cells
+static int cell_side(int nb)
+{
+    int g = 1;
+    while ((g + 1) * (g + 1) <= nb) ++g;
+    return g;
+}
+
 int pair_order_alloc(Context &c)
 {
     const int n = c.n;
     for (int **p : {&c.perm, &c.xkey, &c.xkey_sorted, &c.xidx})
         SD_HIP(hipMalloc((void **)p, sizeof(int) * (size_t)(n > 0 ? n : 1)));
     size_t bytes = 0;
+    const int g = cell_side(c.nb);
     SD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,
-                                              key_bits(c.nb), c.stream));
+                                              key_bits(g * g), c.stream));
     c.sort_tmp_bytes = bytes > 0 ? bytes : 1;
     SD_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes));
-    for (double **p : {&c.ytc, &c.yts, &c.xw, &c.xep, &c.xem})
+    for (double **p : {&c.ytc, &c.yts, &c.yw, &c.xw, &c.xep, &c.xem})
         SD_HIP(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
     return SDDDST_OK;
 }
@@ -263,22 +278,23 @@ int pair_order_alloc(Context &c)
 int launch_ytrig(Context &c)
 {
     if (c.n <= 0) return SDDDST_OK;
-    hipLaunchKernelGGL(k_ytrig, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.y, c.ytc, c.yts);
+    hipLaunchKernelGGL(k_ytrig, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.y, c.ytc, c.yts, c.yw);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
 constexpr int kOrderReuse = 16;
 
-// perm = the dislocations ordered by x bin (index order within a bin).
+// perm = the dislocations ordered by cell (index order within a cell).
 static int pair_order(Context &c, const double *x)
 {
     const int n = c.n;
-    hipLaunchKernelGGL(k_xbin, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.nb, x, c.xkey, c.xidx);
+    const int g = cell_side(c.nb);
+    hipLaunchKernelGGL(k_cellbin, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, g, x, c.y, c.xkey, c.xidx);
     SD_HIP(hipGetLastError());
     size_t bytes = c.sort_tmp_bytes;
     SD_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,
-                                              key_bits(c.nb), c.stream));
+                                              key_bits(g * g), c.stream));
     return SDDDST_OK;
 }
 
@@ -310,8 +326,9 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     }
     ++c.order_age;
     // far-pair factors (fast field): every pair at circular x distance >=
-    // kTrigMinDx takes the far form, which skips the tolerance candidates, so
-    // they are used only when no such pair can be one
+    // kTrigMinDx, or with cos(2 pi dy) <= 0.8 (|dy| > 0.102), takes the far
+    // form, which skips the tolerance candidates, so they are used only when
+    // no such pair can be one (r^2 >= 0.01 for all of them)
     pairtile::FarData fd;
     const double r2_keep = pairtile::tolerance_r2(minPrecSqr);
     const bool far = c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kTrigMinDx * sdfield::kTrigMinDx;
@@ -324,6 +341,7 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         fd.w = c.xw;
         fd.ep = c.xep;
         fd.em = c.xem;
+        fd.yw = c.yw;
     }
     if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
@@ -359,12 +377,12 @@ namespace {
 
 constexpr int kNoAllocMaxN = 8192;
 constexpr int kNoAllocMaxBlocks = kNoAllocMaxN / kTile;
-// layout of scratch: partial sums [nb][n], then five per-dislocation FP64
-// arrays (cos 2 pi y, sin 2 pi y, wrapped x, e^{+-2 pi w}), the tolerance
-// candidates (u64) and the order (int)
+// layout of scratch: partial sums [nb][n], then six per-dislocation FP64
+// arrays (cos 2 pi y, sin 2 pi y, wrapped y, wrapped x, e^{+-2 pi w}), the
+// tolerance candidates (u64) and the order (int)
 constexpr size_t kNoAllocPartialBytes = sizeof(double) * (size_t)kNoAllocMaxBlocks * kNoAllocMaxN;
 constexpr size_t kNoAllocVecBytes = sizeof(double) * (size_t)kNoAllocMaxN;
-constexpr size_t kNoAllocBytes = kNoAllocPartialBytes + 6 * kNoAllocVecBytes + sizeof(int) * (size_t)kNoAllocMaxN;
+constexpr size_t kNoAllocBytes = kNoAllocPartialBytes + 7 * kNoAllocVecBytes + sizeof(int) * (size_t)kNoAllocMaxN;
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -410,10 +428,10 @@ int speeds_noalloc(const double *x, const double *y, const double *b, int n, dou
     if (rc) return rc;
     double *partial = reinterpret_cast<double *>(base);
     double *vec = reinterpret_cast<double *>(base + kNoAllocPartialBytes);
-    double *ytc = vec, *yts = vec + kNoAllocMaxN, *xw = vec + 2 * kNoAllocMaxN, *xep = vec + 3 * kNoAllocMaxN,
-           *xem = vec + 4 * kNoAllocMaxN;
-    unsigned long long *minr2 = reinterpret_cast<unsigned long long *>(vec + 5 * kNoAllocMaxN);
-    int *perm = reinterpret_cast<int *>(base + kNoAllocPartialBytes + 6 * kNoAllocVecBytes);
+    double *ytc = vec, *yts = vec + kNoAllocMaxN, *yw = vec + 2 * kNoAllocMaxN, *xw = vec + 3 * kNoAllocMaxN,
+           *xep = vec + 4 * kNoAllocMaxN, *xem = vec + 5 * kNoAllocMaxN;
+    unsigned long long *minr2 = reinterpret_cast<unsigned long long *>(vec + 6 * kNoAllocMaxN);
+    int *perm = reinterpret_cast<int *>(base + kNoAllocPartialBytes + 7 * kNoAllocVecBytes);
     const int nb = (n + kTile - 1) / kTile;
     const long nfull = (long)nb * (nb - 1) / 2;
     const long units = nfull + (nb + 1) / 2;
@@ -422,7 +440,7 @@ int speeds_noalloc(const double *x, const double *y, const double *b, int n, dou
     // pair at a far circular x distance may take the far form
     const bool far = field_mode == SDDDST_FIELD_FAST;
     hipLaunchKernelGGL(k_iota, dim3(blocks), dim3(256), 0, stream, n, perm);
-    if (far) hipLaunchKernelGGL(k_ytrig, dim3(blocks), dim3(256), 0, stream, n, y, ytc, yts);
+    if (far) hipLaunchKernelGGL(k_ytrig, dim3(blocks), dim3(256), 0, stream, n, y, ytc, yts, yw);
     hipLaunchKernelGGL(k_speeds_prep, dim3(blocks), dim3(256), 0, stream, n, minr2, x, far ? xw : nullptr, xep, xem);
     SD_HIP(hipGetLastError());
     if (far) {
@@ -432,6 +450,7 @@ int speeds_noalloc(const double *x, const double *y, const double *b, int n, dou
         fd.w = xw;
         fd.ep = xep;
         fd.em = xem;
+        fd.yw = yw;
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_FAST>, dim3((unsigned)units), dim3(kTile), 0, stream, n, nb,
                            (int)nfull, x, y, b, perm, partial, minr2, 0.0, fd);
     } else {

@@ -145,6 +145,37 @@ def test_far_pair_term():
     assert np.array_equal(out[far].view(np.int64), (-swp[far]).view(np.int64))
 
 
+def test_far_pair_term_close_in_x_far_in_y():
+    # pairs with |dx| < 0.1 (down to 1e-12) take the far form too when
+    # cos(2 pi dy) <= 0.8: cosh(2 pi dx) - cos(2 pi dy) >= 0.2 bounds the
+    # cancellation as for the x-far pairs (3.2e5-sample max observed 1.6e-14,
+    # 99.99th percentile 1.2e-14), and the form stays odd bit for bit
+    mc = O.mathcheck()
+    rng = np.random.default_rng(13)
+    n = 400_000
+    xi, yi = rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n)
+    ddx = np.concatenate([rng.uniform(-0.1, 0.1, n // 2),
+                          rng.choice([-1.0, 1.0], n // 2) * 10.0 ** rng.uniform(-12, -1, n // 2)])
+    ddy = rng.choice([-1.0, 1.0], n) * rng.uniform(0.1, 0.5, n)
+    xj, yj = xi - ddx, yi - ddy
+    xi[: n // 4] += rng.integers(-3, 4, n // 4)
+    yj[: n // 4] += rng.integers(-3, 4, n // 4)
+    out, swp = np.empty(n), np.empty(n)
+    mc.mc_batch_pair_far(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(out))
+    mc.mc_batch_pair_far(n, O.dp(xj), O.dp(yj), O.dp(xi), O.dp(yi), O.dp(swp))
+    dx, dy = np.empty(n), np.empty(n)
+    ex, ey = xi - xj, yi - yj
+    mc.mc_batch1(6, n, O.dp(ex), O.dp(dx))
+    mc.mc_batch1(6, n, O.dp(ey), O.dp(dy))
+    ref, _ = O.xy_batch(dx, dy)
+    sel = np.cos(2.0 * np.pi * dy) <= 0.8
+    assert sel.mean() > 0.9 and np.all(np.abs(dx[sel]) < 0.1 + 1e-12)
+    err = np.abs(out - ref) / (np.abs(ref) + 1.0)
+    assert err[sel].max() < 3.5e-14
+    assert err[sel].mean() < 1e-15
+    assert np.array_equal(out[sel].view(np.int64), (-swp[sel]).view(np.int64))
+
+
 def test_integer_powers_correctly_rounded():
     import mpmath
     mpmath.mp.prec = 200

@@ -75,7 +75,7 @@ struct Context {
     unsigned long long *minr2 = nullptr; // per-call minimum r^2 bits
     double *partial = nullptr;          // [nb][n] pair partial sums
     int nb = 0;                         // number of 64-wide blocks
-    // pair-tile order: perm[p] = dislocation at tile position p (by x bin)
+    // pair-tile order: perm[p] = dislocation at tile position p (curve order)
     int *perm = nullptr, *xkey = nullptr, *xkey_sorted = nullptr, *xidx = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;

@@ -206,14 +206,17 @@ SDLIBM_FN void ytrig(double y, double &c, double &s)
 
 // cos(2 pi (y_i - y_j)) from the two dislocations' ytrig values (angle
 // addition; symmetric in i and j bit for bit).  Within a few ulps of the
-// reference's cos(2 pi dy): used only for pairs at circular x distance
-// |dx| >= kTrigMinDx, where cosh(2 pi dx) - cos(2 pi dy) >= 0.2 keeps the
-// field's sensitivity to it (and to few-ulp exponentials) low.  The far form
-// stays within 2.2e-14 of the reference's term relative to |tau| + 1 (3e6
-// samples; 99.99th percentile 9e-15, mean 3e-16) -- below the reference's own
+// reference's cos(2 pi dy): used only for far pairs, those with
+// cosh(2 pi dx) - cos(2 pi dy) >= kFarMinGap, where the field's sensitivity
+// to it (and to few-ulp exponentials) is low.  The far form stays within
+// 2.2e-14 of the reference's term relative to |tau| + 1 (3e6 samples;
+// 99.99th percentile 1.3e-14, mean 3e-16) -- below the reference's own
 // truncation of the image sum at +-4 images with a half-weight fifth.
 SDLIBM_FN double cos_diff(double ci, double si, double cj, double sj) { return __builtin_fma(ci, cj, si * sj); }
-constexpr double kTrigMinDx = 0.1;
+constexpr double kFarMinGap = 0.2;
+// Every far pair has r^2 = dx^2 + dy^2 > 0.0081: cosh(2 pi dx) - cos(2 pi dy)
+// = 2 sinh^2(pi dx) + 2 sin^2(pi dy) <= 2 sinh^2(pi r) < 0.17 for r <= 0.09.
+constexpr double kFarMinR2 = 0.0081;
 
 // The nine far images of xy_fast (see below) as sg(dx) F(|dx|) without the
 // sign and the factor 2 pi^2, from t = e^{2 pi |dx|} and ti = 1 / t: the
@@ -310,12 +313,13 @@ SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__r
     return near + far_images(dx, cy, t, ti) * __builtin_copysign(2.0 * kPiSq, dx);
 }
 
-// The production field for a pair with |dx| >= kTrigMinDx (far pairs of the
-// pair kernel): cos(2 pi dy) and t = e^{2 pi |dx|}, ti = e^{-2 pi |dx|}
-// supplied by the caller (from per-dislocation factors, within a few ulps),
-// cosh(2 pi dx) = (t + ti) / 2.  The near image dx (c cy - 1) / (c - cy)^2
-// and the four inner far images P / Q share one division (cosh - cos >= 0.2
-// here, so the near image needs no care against cancellation).
+// The production field of a far pair (cosh(2 pi dx) - cos(2 pi dy) >=
+// kFarMinGap) in the pair kernel: cos(2 pi dy) and t = e^{2 pi |dx|},
+// ti = e^{-2 pi |dx|} supplied by the caller (from per-dislocation factors,
+// within a few ulps), cosh(2 pi dx) = (t + ti) / 2.  The near image
+// dx (c cy - 1) / (c - cy)^2 and the four inner far images P / Q share one
+// division (c - cy >= 0.2 here, so the near image needs no care against
+// cancellation).
 SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
 {
     // 0.5 t and 0.5 ti are exact, so the fma rounds their sum once, as the
@@ -342,29 +346,75 @@ SDLIBM_FN void xexp(double x, double &w, double &ep, double &em)
 // Circular distance |z - rint(z)| (exact) of a difference of wrapped x.
 SDLIBM_FN double circ(double z) { return sdlibm::fabsd(z - __builtin_rint(z)); }
 
-// The production field of a far pair (circ(wi - wj) >= kTrigMinDx) from the
-// two dislocations' wrapped x, exp factors and cos(2 pi dy): dx = s - rint(s)
-// with s = wi - wj, and e^{2 pi dx} = e^{2 pi wi} e^{-2 pi wj} e^{-2 pi k}
-// (k = rint(s) in {-1, 0, 1}), each exponential within a few ulps.  Swapping
-// the two dislocations negates s, k and dx exactly and exchanges the two
-// products, so the term is odd bit for bit.
-SDLIBM_FN double far_pair(double wi, double wj, double epi, double emi, double epj, double emj, double cy)
+// Bounds of circ(t - s) over t in [tmin, tmax] and s in [smin, smax], the
+// intervals widened by 1e-12 for the rounding of how the caller formed them:
+// every difference lies in [lo, hi]; circ vanishes at integers, peaks at
+// half-integers and is monotone between them, so its extremes over [lo, hi]
+// are 0 / 1/2 when the interval holds such a point and otherwise at its ends.
+SDLIBM_FN void circ_range(double tmin, double tmax, double smin, double smax, double &mind, double &maxd)
+{
+    const double lo = tmin - smax - 1e-12, hi = tmax - smin + 1e-12;
+    const double clo = circ(lo), chi = circ(hi);
+    const bool has_int = __builtin_floor(hi) >= __builtin_ceil(lo);
+    const bool has_half = __builtin_floor(hi - 0.5) >= __builtin_ceil(lo - 0.5);
+    mind = has_int ? 0.0 : fmin(clo, chi);
+    maxd = has_half ? 0.5 : fmax(clo, chi);
+}
+
+// How the pairs of a group whose circular x and y distances lie in
+// [xmin, xmax] and [ymin, ymax] stand against kFarMinGap: 1 all far, 0 all
+// near, 2 either.  With a = pi gx, c = pi gy the gap is 2 sinh^2 a +
+// 2 sin^2 c, and a <= sinh a, c (1 - c^2/6) <= sin c <= c (c <= pi/2),
+// sinh a <= a (1 + a^2/5) (a <= 1).  The bounds clear kFarMinGap with a
+// relative margin of 1e-9, far above their rounding and the far form's.
+SDLIBM_FN int gap_kind(double xmin, double xmax, double ymin, double ymax)
+{
+    const double a0 = kPi * xmin, c0 = kPi * ymin;
+    const double a1 = kPi * xmax, c1 = kPi * ymax;
+    const double s0 = c0 * (1.0 - c0 * c0 * (1.0 / 6.0));
+    const double lower = 2.0 * (a0 * a0 + s0 * s0);
+    const double h1 = a1 * (1.0 + a1 * a1 * 0.2);
+    const double upper = 2.0 * (h1 * h1 + c1 * c1);
+    if (lower >= kFarMinGap * (1.0 + 1e-9)) return 1;
+    return (a1 <= 1.0 && upper <= kFarMinGap * (1.0 - 1e-9)) ? 0 : 2;
+}
+
+// dx and t = e^{2 pi |dx|}, ti = e^{-2 pi |dx|} of a pair from the two
+// dislocations' wrapped x and exp factors: dx = s - rint(s) with s = wi - wj,
+// and e^{2 pi dx} = e^{2 pi wi} e^{-2 pi wj} e^{-2 pi k} (k = rint(s) in
+// {-1, 0, 1}), each exponential within a few ulps.  u = e^{2 pi s} and
+// v = e^{-2 pi s}: the larger is e^{2 pi |s|}.  |dx| is |s| when rint(s) = 0
+// (|s| <= 1/2, ties to even) and 1 - |s| otherwise, so t and ti are the
+// larger and the smaller of u and v, the smaller times e^{2 pi} and the
+// larger times e^{-2 pi} when s wrapped -- without selecting on k or on the
+// sign of dx.  Swapping the two dislocations negates s, k and dx exactly and
+// exchanges u and v, so t and ti are the same and dx flips sign.
+struct FarPrelude {
+    double dx, t, ti;
+};
+SDLIBM_FN FarPrelude far_prelude(double wi, double wj, double epi, double emi, double epj, double emj)
 {
     const double s = wi - wj;
-    const double dx = s - __builtin_rint(s);
-    // u = e^{2 pi s}, v = e^{-2 pi s}: the larger is e^{2 pi |s|}.  |dx| is
-    // |s| when rint(s) = 0 (|s| <= 1/2, ties to even) and 1 - |s| otherwise,
-    // so t = e^{2 pi |dx|} and ti = e^{-2 pi |dx|} are the larger and the
-    // smaller of u and v, the smaller times e^{2 pi} and the larger times
-    // e^{-2 pi} when s wrapped.  These are the products e^{2 pi wi}
-    // e^{-2 pi wj} e^{-2 pi k} of either sign, ordered by the sign of dx,
-    // without selecting on k or on that sign.
     const double u = epi * emj, v = emi * epj;
     const double hi = fmax(u, v), lo = fmin(u, v);
     const bool wrapped = sdlibm::fabsd(s) > 0.5;
-    const double t = wrapped ? lo * sdlibm_exp_2pik[1] : hi;
-    const double ti = wrapped ? hi * sdlibm_exp_m2pik[1] : lo;
-    return xy_far(dx, cy, t, ti);
+    FarPrelude f;
+    f.dx = s - __builtin_rint(s);
+    f.t = wrapped ? lo * sdlibm_exp_2pik[1] : hi;
+    f.ti = wrapped ? hi * sdlibm_exp_m2pik[1] : lo;
+    return f;
+}
+
+// cosh(2 pi dx) - cos(2 pi dy) as the far form computes it (symmetric in the
+// two dislocations bit for bit): the pair is far when it is >= kFarMinGap.
+SDLIBM_FN double far_gap(const FarPrelude &f, double cy) { return __builtin_fma(0.5, f.t, 0.5 * f.ti) - cy; }
+
+// The production field of a far pair from the two dislocations' wrapped x,
+// exp factors and cos(2 pi dy); odd bit for bit under swapping them.
+SDLIBM_FN double far_pair(double wi, double wj, double epi, double emi, double epj, double emj, double cy)
+{
+    const FarPrelude f = far_prelude(wi, wj, epi, emi, epj, emj);
+    return xy_far(f.dx, cy, f.t, f.ti);
 }
 
 // Production form of xy() (see header comment).

@@ -18,9 +18,9 @@
 // recorded with an atomic min instead of a rotated accumulator.
 //
 // With Perm, tile positions are sorted positions: perm[p] is the dislocation
-// at position p (the single-system kernel orders them by cells of a 2D grid
-// over the unit cell, so that most tiles hold only pairs far apart in x or in
-// y, whose wavefronts take the short far form throughout).  Every pair term is still evaluated lower
+// at position p (the single-system kernel orders them along a Hilbert curve
+// over the unit cell, so that tiles are compact in x and y and most hold
+// only far pairs, whose wavefronts take the short far form throughout).  Every pair term is still evaluated lower
 // original index first -- the difference is negated when the source has the
 // lower index, and the term then enters the row with a minus sign and the
 // column with a plus sign.  The partial sums stay at sorted positions
@@ -58,21 +58,16 @@ struct FarData {
     const double *yw = nullptr;                  // y - rint(y)
 };
 
-// A pair takes the far form when its circular x distance is at least
-// kTrigMinDx or cos(2 pi dy) (by angle addition) is at most kFarMaxCy:
-// either way cosh(2 pi dx) - cos(2 pi dy) >= 0.2, where few-ulp errors in
-// the exponentials and the cosine move the term by at most ~1e-14 of
-// |tau| + 1.  Tiles classify from their members' extents: all pairs far when
-// the x gap is at least kTrigMinDx or the y gap at least kFarMinDy
-// (cos(2 pi 0.11) = 0.77 < kFarMaxCy with margin), none when the x spread is
-// below kTrigMinDx and the y spread below kNearMaxDy (cos(2 pi 0.095) = 0.83).
-constexpr double kFarMaxCy = 0.8;
-constexpr double kFarMinDy = 0.11;
-constexpr double kNearMaxDy = 0.095;
-
+// A pair takes the far form when cosh(2 pi dx) - cos(2 pi dy), as the far
+// form computes it, is at least kFarMinGap: few-ulp errors in the
+// exponentials and the cosine then move the term by at most ~2e-14 of
+// |tau| + 1.  Tiles classify from their members' extents (below): all pairs
+// far when a lower bound of the gap over the tile clears kFarMinGap, none
+// when an upper bound stays below it.
 // How a tile's pairs are evaluated: every pair in the reference form (all
 // provably near by both rules above, or no far-pair factors), every pair in
 // the far form (all provably far), or per pair.
+// (the values of sdfield::gap_kind)
 enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2 };
 
 __device__ __forceinline__ double rotate_down1(double v, int lane)
@@ -157,47 +152,44 @@ __device__ __forceinline__ void far_stage(TileLds &L, int l, const FarSelf &v)
     L.em[l] = v.em;
 }
 
-// Bounds of circ(t_i - s_j) over a tile from the wrapped coordinates of the
-// target lanes (t) and the source slots (s), NaN outside the system.
-__device__ __forceinline__ void circ_bounds(double t, double s, double &mind, double &maxd)
+// min and max over the 64 lanes of a wrapped coordinate v (NaN outside the
+// system) after moving each lane's v by a whole period to within 1/2 of lane
+// 0's (a lane inside the system in every tile), so that a tile straddling
+// the cell's edge keeps a short interval.  The shifted values are exact up
+// to one rounding of v -+ 1 (below 2^-53).
+__device__ __forceinline__ void unwrapped_extent(double v, double &vmin, double &vmax)
 {
-    double tmin = t, tmax = t, smin = s, smax = s;
+    const double r = __shfl(v, 0, 64);
+    const double u = v - __builtin_rint(v - r);
+    vmin = u;
+    vmax = u;
     for (int off = 32; off > 0; off >>= 1) {
-        tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
-        tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
-        smin = fmin(smin, __shfl_xor(smin, off, 64));
-        smax = fmax(smax, __shfl_xor(smax, off, 64));
+        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
     }
-    const double lo = tmin - smax, hi = tmax - smin;
-    const double clo = sdfield::circ(lo), chi = sdfield::circ(hi);
-    const bool has_int = (lo <= 0.0 && hi >= 0.0) || (lo <= 1.0 && hi >= 1.0) || (lo <= -1.0 && hi >= -1.0);
-    const bool has_half = (lo <= 0.5 && hi >= 0.5) || (lo <= -0.5 && hi >= -0.5);
-    mind = has_int ? 0.0 : fmin(clo, chi);
-    maxd = has_half ? 0.5 : fmax(clo, chi);
 }
 
-// The tile kind from the wrapped x and y of its members; the same on every
-// lane.  The differences of the tile lie in [lo, hi] (rounding is
-// monotone), and circ() peaks at half-integers and vanishes at integers, so
-// its extremes over [lo, hi] bound every pair's circular distance.
+// The tile kind from the wrapped x and y of the target lanes (tw, ty) and
+// the source slots (sw, sy); the same on every lane.
 __device__ __forceinline__ int tile_kind(double tw, double sw, double ty, double sy)
 {
-    double xmin, xmax, ymin, ymax;
-    circ_bounds(tw, sw, xmin, xmax);
-    circ_bounds(ty, sy, ymin, ymax);
-    const int kind = (xmin >= sdfield::kTrigMinDx || ymin >= kFarMinDy)
-                         ? kFarTile
-                         : ((xmax < sdfield::kTrigMinDx && ymax < kNearMaxDy) ? kNearTile : kMixedTile);
-    return __builtin_amdgcn_readfirstlane(kind);
+    double tmin, tmax, smin, smax, xmin, xmax, ymin, ymax;
+    unwrapped_extent(tw, tmin, tmax);
+    unwrapped_extent(sw, smin, smax);
+    sdfield::circ_range(tmin, tmax, smin, smax, xmin, xmax);
+    unwrapped_extent(ty, tmin, tmax);
+    unwrapped_extent(sy, smin, smax);
+    sdfield::circ_range(tmin, tmax, smin, smax, ymin, ymax);
+    return __builtin_amdgcn_readfirstlane(sdfield::gap_kind(xmin, xmax, ymin, ymax));
 }
 
-// One pair of a far or mixed tile: the far form when circ(w_i - w_j) >=
-// kTrigMinDx or cos(2 pi dy) <= kFarMaxCy (always in a far tile), else the
-// reference form.  The choice is symmetric in the two dislocations bit for
-// bit (circ of a negated difference, cos_diff's commuted products).  r2 is
-// set for the reference form only (far pairs are never tolerance
-// candidates: their r^2 >= 0.01 exceeds every r2_keep the caller passes on
-// this path).
+// One pair of a far or mixed tile: the far form when its gap
+// cosh(2 pi dx) - cos(2 pi dy) (far_gap) is at least kFarMinGap (always in a
+// far tile), else the reference form.  The choice is symmetric in the two
+// dislocations bit for bit (the prelude's t and ti and cos_diff's commuted
+// products are).  r2 is set for the reference form only (far pairs are never
+// tolerance candidates: their r^2 > kFarMinR2 exceeds every r2_keep the
+// caller passes on this path).
 template <int Mode, bool Perm, int Kind>
 __device__ __forceinline__ double tile_term(double xi, double yi, int io, int jm, const TileLds &L, int m,
                                             const FarSelf &fi, bool lower_by_lane, double &r2,
@@ -206,12 +198,12 @@ __device__ __forceinline__ double tile_term(double xi, double yi, int io, int jm
     if constexpr (Kind == kNearTile) {
         return pair_term<Mode, Perm>(xi, yi, io, jm, L.x[m], L.y[m], lower_by_lane, r2, sct, e2t);
     } else {
-        const double wj = L.w[m];
         const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
-        const bool far = Kind == kFarTile || sdfield::circ(fi.w - wj) >= sdfield::kTrigMinDx || cy <= kFarMaxCy;
+        const sdfield::FarPrelude f = sdfield::far_prelude(fi.w, L.w[m], fi.ep, fi.em, L.ep[m], L.em[m]);
+        const bool far = Kind == kFarTile || sdfield::far_gap(f, cy) >= sdfield::kFarMinGap;
         if (far) {
             r2 = 1.0;
-            return sdfield::far_pair(fi.w, wj, fi.ep, fi.em, L.ep[m], L.em[m], cy);
+            return sdfield::xy_far(f.dx, cy, f.t, f.ti);
         }
         return pair_term<Mode, Perm>(xi, yi, io, jm, L.x[m], L.y[m], lower_by_lane, r2, sct, e2t);
     }
@@ -221,12 +213,11 @@ __device__ __forceinline__ double tile_term(double xi, double yi, int io, int jm
 // and bi < bj, so every pair of the tile counts.  sct / e2t (fast field
 // only) may point at LDS copies of the tables written before the call: the
 // tile's barrier orders them before the first read.  With far-pair factors
-// (fast field) a pair whose circular x distance is at least kTrigMinDx, or
-// whose cos(2 pi dy) is at most kFarMaxCy, takes the far form (cos by angle
-// addition, the exponentials from per-dislocation factors); the choice is
-// made per pair, so a pair's value does not depend on the tile that holds
-// it, and the tile's x and y intervals decide once whether all, none or some
-// of its pairs are far.
+// (fast field) a pair whose gap cosh(2 pi dx) - cos(2 pi dy) is at least
+// kFarMinGap takes the far form (cos by angle addition, the exponentials from
+// per-dislocation factors); the choice is made per pair, so a pair's value
+// does not depend on the tile that holds it, and the tile's x and y extents
+// decide once whether all, none or some of its pairs are far.
 template <int Mode, bool Perm = false, bool Full = false>
 __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const double *__restrict__ x,
                                      const double *__restrict__ y, const double *__restrict__ b,

@@ -3,10 +3,10 @@
 // (PrecisionHandler::updateTolerance, precision_handler.cpp:72-87).
 //
 // Layout: x, y, b are SoA FP64 arrays of N.  The dislocations are ordered
-// by x bin (perm, a stable radix sort of the bin of the wrapped x, refreshed
-// every 16 calls and after an upload):
-// tile positions are positions in that order, so most tiles hold only pairs
-// far apart in x.  The N(N-1)/2 unordered pairs are cut into 64x64 tiles
+// along a Hilbert curve over the unit cell (perm, a stable radix sort of the
+// curve index of the wrapped (x, y), refreshed every 16 calls and after an
+// upload): tile positions are positions in that order, so most tiles hold
+// only pairs far apart.  The N(N-1)/2 unordered pairs are cut into 64x64 tiles
 // (bi <= bj) of the upper triangle; one wavefront owns one tile.  Lane l
 // holds target p = 64*bi + l in registers and walks the 64 sources of block
 // bj in a rotated order (lane l meets source (l + s) mod 64 at step s), so
@@ -52,7 +52,7 @@ using pairtile::kTile;
 // Work units of equal cost: the nb (nb - 1) / 2 off-diagonal tiles, then
 // the diagonal tiles two to a unit (each in half the sweeps), so that the
 // units divide evenly over the resident waves.
-// Tile positions follow perm (dislocations ordered by x bin).
+// Tile positions follow perm (dislocations ordered along the curve).
 template <int Mode>
 __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, const double *__restrict__ x,
                                                    const double *__restrict__ y, const double *__restrict__ b,
@@ -109,22 +109,40 @@ __global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict_
     yw[i] = y[i] - __builtin_rint(y[i]);
 }
 
-// Sort keys of the pair-tile order: the cell of the wrapped (x, y) in a
-// G x G grid of the unit cell, rows of cells by y, cells in a row by x, with
-// G^2 <= nb so that a tile of 64 holds about one cell; and the index (a
-// stable radix sort keeps equal cells in index order, so the order is a
-// function of the positions alone).  Tiles compact in both x and y make most
-// tile pairs all-far by either coordinate.
-__global__ void k_cellbin(int n, int g, const double *__restrict__ x, const double *__restrict__ y,
-                          int *__restrict__ key, int *__restrict__ idx)
+// Sort keys of the pair-tile order: the index along a Hilbert curve of the
+// cell that holds the wrapped (x, y) in a 2^kOrderBits x 2^kOrderBits grid
+// of the unit cell (the curve's published iterative construction: per bit
+// level the quadrant's rank, then the quadrant's rotation or reflection),
+// and the index (a stable radix sort keeps equal cells in index order, so
+// the order is a function of the positions alone).  Consecutive runs of 64
+// along the curve are compact in both x and y, so most tile pairs are
+// all-far.
+constexpr int kOrderBits = 10;
+__global__ void k_curvekey(int n, const double *__restrict__ x, const double *__restrict__ y,
+                           int *__restrict__ key, int *__restrict__ idx)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
+    constexpr int g = 1 << kOrderBits;
     const double wx = x[i] - floor(x[i] + 0.5), wy = y[i] - floor(y[i] + 0.5);
     int cx = (int)((wx + 0.5) * (double)g), cy = (int)((wy + 0.5) * (double)g);
     cx = cx < 0 ? 0 : (cx >= g ? g - 1 : cx);
     cy = cy < 0 ? 0 : (cy >= g ? g - 1 : cy);
-    key[i] = cy * g + cx;
+    int d = 0;
+    for (int s = g >> 1; s > 0; s >>= 1) {
+        const int rx = (cx & s) ? 1 : 0, ry = (cy & s) ? 1 : 0;
+        d += s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                cx = g - 1 - cx;
+                cy = g - 1 - cy;
+            }
+            const int t = cx;
+            cx = cy;
+            cy = t;
+        }
+    }
+    key[i] = d;
     idx[i] = i;
 }
 
@@ -244,30 +262,14 @@ __global__ void k_speeds_prep(int n, unsigned long long *__restrict__ minr2, con
 
 }  // namespace
 
-static int key_bits(int nbins)
-{
-    int bits = 1;
-    while ((1 << bits) < nbins) ++bits;
-    return bits;
-}
-
-// side of the cell grid of the pair-tile order: G^2 <= nb cells
-static int cell_side(int nb)
-{
-    int g = 1;
-    while ((g + 1) * (g + 1) <= nb) ++g;
-    return g;
-}
-
 int pair_order_alloc(Context &c)
 {
     const int n = c.n;
     for (int **p : {&c.perm, &c.xkey, &c.xkey_sorted, &c.xidx})
         SD_HIP(hipMalloc((void **)p, sizeof(int) * (size_t)(n > 0 ? n : 1)));
     size_t bytes = 0;
-    const int g = cell_side(c.nb);
     SD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,
-                                              key_bits(g * g), c.stream));
+                                              2 * kOrderBits, c.stream));
     c.sort_tmp_bytes = bytes > 0 ? bytes : 1;
     SD_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes));
     for (double **p : {&c.ytc, &c.yts, &c.yw, &c.xw, &c.xep, &c.xem})
@@ -285,16 +287,16 @@ int launch_ytrig(Context &c)
 
 constexpr int kOrderReuse = 16;
 
-// perm = the dislocations ordered by cell (index order within a cell).
+// perm = the dislocations ordered along the curve (index order within a
+// cell).
 static int pair_order(Context &c, const double *x)
 {
     const int n = c.n;
-    const int g = cell_side(c.nb);
-    hipLaunchKernelGGL(k_cellbin, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, g, x, c.y, c.xkey, c.xidx);
+    hipLaunchKernelGGL(k_curvekey, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, x, c.y, c.xkey, c.xidx);
     SD_HIP(hipGetLastError());
     size_t bytes = c.sort_tmp_bytes;
     SD_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,
-                                              key_bits(g * g), c.stream));
+                                              2 * kOrderBits, c.stream));
     return SDDDST_OK;
 }
 
@@ -325,13 +327,13 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         c.order_age = 0;
     }
     ++c.order_age;
-    // far-pair factors (fast field): every pair at circular x distance >=
-    // kTrigMinDx, or with cos(2 pi dy) <= 0.8 (|dy| > 0.102), takes the far
-    // form, which skips the tolerance candidates, so they are used only when
-    // no such pair can be one (r^2 >= 0.01 for all of them)
+    // far-pair factors (fast field): every pair whose gap cosh(2 pi dx) -
+    // cos(2 pi dy) is at least kFarMinGap takes the far form, which skips the
+    // tolerance candidates, so they are used only when no such pair can be
+    // one (r^2 > kFarMinR2 for all of them)
     pairtile::FarData fd;
     const double r2_keep = pairtile::tolerance_r2(minPrecSqr);
-    const bool far = c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kTrigMinDx * sdfield::kTrigMinDx;
+    const bool far = c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kFarMinR2;
     hipLaunchKernelGGL(k_speeds_prep, dim3(blocks), dim3(threads), 0, c.stream, n, c.minr2, c.x[slot],
                        far ? c.xw : nullptr, c.xep, c.xem);
     SD_HIP(hipGetLastError());
@@ -371,7 +373,7 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
 // nothing.  Host code addresses the array through the pointer that
 // hipGetSymbolAddress returns for the current device (taken on the first
 // call on each device), never through its name.  Tiles follow index order
-// (no x-bin sort, whose temporary storage would be one more buffer): the
+// (no curve sort, whose temporary storage would be one more buffer): the
 // same pair terms, another fixed summation order.
 namespace {
 

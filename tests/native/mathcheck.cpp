@@ -40,8 +40,8 @@ void mc_batch1(int fn, long n, const double *x, double *out)
     }
 }
 
-// The pair kernel's far-pair term (circ(wi - wj) >= kTrigMinDx): cos by
-// angle addition and the exponentials from per-dislocation factors.
+// The pair kernel's far-pair term (its pairs with far_gap >= kFarMinGap):
+// cos by angle addition and the exponentials from per-dislocation factors.
 void mc_batch_pair_far(long n, const double *xi, const double *yi, const double *xj, const double *yj,
                        double *out)
 {
@@ -53,6 +53,50 @@ void mc_batch_pair_far(long n, const double *xi, const double *yi, const double 
         sdfield::xexp(xj[k], wj, epj, emj);
         out[k] = sdfield::far_pair(wi, wj, epi, emi, epj, emj, sdfield::cos_diff(ci, si, cj, sj));
     }
+}
+
+// The far form's gap cosh(2 pi dx) - cos(2 pi dy) of each pair, from which
+// the pair kernel decides far (>= kFarMinGap) or near.
+void mc_batch_pair_gap(long n, const double *xi, const double *yi, const double *xj, const double *yj,
+                       double *out)
+{
+    for (long k = 0; k < n; ++k) {
+        double ci, si, cj, sj, wi, epi, emi, wj, epj, emj;
+        sdfield::ytrig(yi[k], ci, si);
+        sdfield::ytrig(yj[k], cj, sj);
+        sdfield::xexp(xi[k], wi, epi, emi);
+        sdfield::xexp(xj[k], wj, epj, emj);
+        const sdfield::FarPrelude f = sdfield::far_prelude(wi, wj, epi, emi, epj, emj);
+        out[k] = sdfield::far_gap(f, sdfield::cos_diff(ci, si, cj, sj));
+    }
+}
+
+// The pair kernel's tile classification for 64 targets and 64 sources
+// (x, y as stored; NaN for slots outside the system, never the first): the
+// wrapped coordinates shifted by whole periods to within 1/2 of the first
+// slot's, their extremes, the circular-distance bounds and gap_kind -- the
+// wave reductions of the kernel written as host loops.
+static void mc_extent(const double *v, double &vmin, double &vmax)
+{
+    const double r = v[0] - __builtin_rint(v[0]);
+    vmin = vmax = __builtin_nan("");
+    for (int l = 0; l < 64; ++l) {
+        const double w = v[l] - __builtin_rint(v[l]);
+        const double u = w - __builtin_rint(w - r);
+        vmin = fmin(vmin, u);
+        vmax = fmax(vmax, u);
+    }
+}
+int mc_tile_kind(const double *tx, const double *ty, const double *sx, const double *sy)
+{
+    double tmin, tmax, smin, smax, xmin, xmax, ymin, ymax;
+    mc_extent(tx, tmin, tmax);
+    mc_extent(sx, smin, smax);
+    sdfield::circ_range(tmin, tmax, smin, smax, xmin, xmax);
+    mc_extent(ty, tmin, tmax);
+    mc_extent(sy, smin, smax);
+    sdfield::circ_range(tmin, tmax, smin, smax, ymin, ymax);
+    return sdfield::gap_kind(xmin, xmax, ymin, ymax);
 }
 
 // fn = 0 xy, 1 xy_fast, 2 xy_diff_x, 3 xy_diff_x_fast, 4 xy_diff_x2

@@ -144,6 +144,9 @@ def mathcheck():
     lib.mc_batch1.argtypes = [ctypes.c_int, L, DP, DP]
     lib.mc_batch2.argtypes = [ctypes.c_int, L, DP, DP, DP]
     lib.mc_batch_pair_far.argtypes = [L, DP, DP, DP, DP, DP]
+    lib.mc_batch_pair_gap.argtypes = [L, DP, DP, DP, DP, DP]
+    lib.mc_tile_kind.argtypes = [DP, DP, DP, DP]
+    lib.mc_tile_kind.restype = ctypes.c_int
     lib.mc_normalize.restype = D
     lib.mc_normalize.argtypes = [D]
     lib.mc_cos_2pi.restype = D

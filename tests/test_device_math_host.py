@@ -145,35 +145,91 @@ def test_far_pair_term():
     assert np.array_equal(out[far].view(np.int64), (-swp[far]).view(np.int64))
 
 
-def test_far_pair_term_close_in_x_far_in_y():
-    # pairs with |dx| < 0.1 (down to 1e-12) take the far form too when
-    # cos(2 pi dy) <= 0.8: cosh(2 pi dx) - cos(2 pi dy) >= 0.2 bounds the
-    # cancellation as for the x-far pairs (3.2e5-sample max observed 1.6e-14,
-    # 99.99th percentile 1.2e-14), and the form stays odd bit for bit
+def test_far_pair_term_by_gap():
+    # the pair kernel takes the far form for every pair whose gap
+    # cosh(2 pi dx) - cos(2 pi dy), as the far form computes it, is >= 0.2,
+    # whatever |dx| (down to 1e-12): the gap bounds the cancellation as for
+    # the x-far pairs (2.4e5-sample max observed 1.6e-14).  The gap is
+    # symmetric in the two dislocations bit for bit, the term odd, and every
+    # far pair has r^2 > 0.0081 (the far form is enabled only when no
+    # tolerance candidate can be that far)
     mc = O.mathcheck()
     rng = np.random.default_rng(13)
     n = 400_000
     xi, yi = rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n)
-    ddx = np.concatenate([rng.uniform(-0.1, 0.1, n // 2),
-                          rng.choice([-1.0, 1.0], n // 2) * 10.0 ** rng.uniform(-12, -1, n // 2)])
-    ddy = rng.choice([-1.0, 1.0], n) * rng.uniform(0.1, 0.5, n)
+    r = rng.uniform(0.06, 0.3, n)
+    th = rng.uniform(0.0, 2.0 * np.pi, n)
+    ddx, ddy = r * np.cos(th), r * np.sin(th)
+    ddx[: n // 8] = rng.choice([-1.0, 1.0], n // 8) * 10.0 ** rng.uniform(-12, -2, n // 8)
     xj, yj = xi - ddx, yi - ddy
     xi[: n // 4] += rng.integers(-3, 4, n // 4)
-    yj[: n // 4] += rng.integers(-3, 4, n // 4)
-    out, swp = np.empty(n), np.empty(n)
+    yj[n // 4: n // 2] += rng.integers(-3, 4, n // 4)
+    out, swp, gap, gsw = np.empty(n), np.empty(n), np.empty(n), np.empty(n)
     mc.mc_batch_pair_far(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(out))
     mc.mc_batch_pair_far(n, O.dp(xj), O.dp(yj), O.dp(xi), O.dp(yi), O.dp(swp))
+    mc.mc_batch_pair_gap(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(gap))
+    mc.mc_batch_pair_gap(n, O.dp(xj), O.dp(yj), O.dp(xi), O.dp(yi), O.dp(gsw))
+    assert np.array_equal(gap.view(np.int64), gsw.view(np.int64))
     dx, dy = np.empty(n), np.empty(n)
     ex, ey = xi - xj, yi - yj
     mc.mc_batch1(6, n, O.dp(ex), O.dp(dx))
     mc.mc_batch1(6, n, O.dp(ey), O.dp(dy))
+    exact_gap = 2.0 * np.sinh(np.pi * dx) ** 2 + 2.0 * np.sin(np.pi * dy) ** 2
+    assert np.max(np.abs(gap - exact_gap)) < 1e-13
+    sel = gap >= 0.2
+    assert 0.5 < sel.mean() < 0.95 and (sel & (np.abs(dx) < 0.1)).sum() > 50_000
+    assert np.all(dx[sel] ** 2 + dy[sel] ** 2 > 0.0081)
     ref, _ = O.xy_batch(dx, dy)
-    sel = np.cos(2.0 * np.pi * dy) <= 0.8
-    assert sel.mean() > 0.9 and np.all(np.abs(dx[sel]) < 0.1 + 1e-12)
     err = np.abs(out - ref) / (np.abs(ref) + 1.0)
     assert err[sel].max() < 3.5e-14
     assert err[sel].mean() < 1e-15
     assert np.array_equal(out[sel].view(np.int64), (-swp[sel]).view(np.int64))
+
+
+def _tile(rng, cx, cy, spread, count):
+    x = cx + spread * rng.uniform(-1.0, 1.0, 64) + rng.integers(-2, 3, 64)
+    y = cy + spread * rng.uniform(-1.0, 1.0, 64) + rng.integers(-2, 3, 64)
+    x[count:] = np.nan
+    y[count:] = np.nan
+    return x, y
+
+
+def test_tile_kind_bounds_every_pair():
+    # a tile is all-far only if every pair's gap 2 sinh^2(pi gx) +
+    # 2 sin^2(pi gy) is >= 0.2, all-near only if every one is below, over
+    # tiles of random centre, spread and fill, straddling the cell's edges;
+    # and the bounds are tight enough that a tile whose extreme distances
+    # give a gap beyond 0.25 (below 0.15) is classified all-far (all-near)
+    mc = O.mathcheck()
+    rng = np.random.default_rng(21)
+    kinds = [0, 0, 0]
+    for _ in range(6000):
+        spread = 10.0 ** rng.uniform(-3.5, -0.7)
+        ca, cb = rng.uniform(-0.5, 0.5, 2)
+        dist = rng.uniform(0.0, 0.35)
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        tx, ty = _tile(rng, ca, cb, spread, int(rng.integers(1, 65)))
+        sx, sy = _tile(rng, ca + dist * np.cos(ang), cb + dist * np.sin(ang), spread, int(rng.integers(1, 65)))
+        k = mc.mc_tile_kind(O.dp(tx), O.dp(ty), O.dp(sx), O.dp(sy))
+        kinds[k] += 1
+        ok_t, ok_s = ~np.isnan(tx), ~np.isnan(sx)
+        gx = np.abs(tx[ok_t][:, None] - sx[ok_s][None, :])
+        gy = np.abs(ty[ok_t][:, None] - sy[ok_s][None, :])
+        gx, gy = np.abs(gx - np.rint(gx)), np.abs(gy - np.rint(gy))
+        g = 2.0 * np.sinh(np.pi * gx) ** 2 + 2.0 * np.sin(np.pi * gy) ** 2
+        if k == 1:
+            assert g.min() >= 0.2
+        elif k == 0:
+            assert g.max() < 0.2
+        # the gap at the extremes of gx and gy over the tile (the tile's
+        # bounding boxes, which the bounds see)
+        lo = 2.0 * np.sinh(np.pi * gx.min()) ** 2 + 2.0 * np.sin(np.pi * gy.min()) ** 2
+        hi = 2.0 * np.sinh(np.pi * gx.max()) ** 2 + 2.0 * np.sin(np.pi * gy.max()) ** 2
+        if lo >= 0.25:
+            assert k == 1
+        if hi <= 0.15:
+            assert k == 0
+    assert min(kinds) > 300
 
 
 def test_integer_powers_correctly_rounded():

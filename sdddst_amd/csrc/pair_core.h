@@ -20,10 +20,10 @@
 // With Perm, tile positions are sorted positions: perm[p] is the dislocation
 // at position p (the single-system kernel orders them along a Hilbert curve
 // over the unit cell, so that tiles are compact in x and y and most hold
-// only far pairs, whose wavefronts take the short far form throughout).  Every pair term is still evaluated lower
-// original index first -- the difference is negated when the source has the
-// lower index, and the term then enters the row with a minus sign and the
-// column with a plus sign.  The partial sums stay at sorted positions
+// only far pairs, whose wavefronts take the short far form throughout).
+// Every pair term is still evaluated lower original index first -- the
+// difference is negated when the source has the lower index, and the term
+// then enters the row with a minus sign and the column with a plus sign.  The partial sums stay at sorted positions
 // (coalesced stores); the finishing pass maps them back through perm.
 #pragma once
 
@@ -64,10 +64,10 @@ struct FarData {
 // |tau| + 1.  Tiles classify from their members' extents (below): all pairs
 // far when a lower bound of the gap over the tile clears kFarMinGap, none
 // when an upper bound stays below it.
-// How a tile's pairs are evaluated: every pair in the reference form (all
-// provably near by both rules above, or no far-pair factors), every pair in
-// the far form (all provably far), or per pair.
-// (the values of sdfield::gap_kind)
+//
+// How a tile's pairs are evaluated (the values of sdfield::gap_kind): every
+// pair in the reference form (all provably near, or no far-pair factors),
+// every pair in the far form (all provably far), or per pair.
 enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2 };
 
 __device__ __forceinline__ double rotate_down1(double v, int lane)

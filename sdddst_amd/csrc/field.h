@@ -330,7 +330,8 @@ SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
     // constant factor (exact: each step's rounding is symmetric in sign)
     const double A = sdlibm::fabsd(dx) * (c0 * cy - 1.0), B = d * d;
     const FarParts f = far_image_parts(dx, cy, t, ti);
-    return (__builtin_fma(A, f.Q, f.P * B) / (B * f.Q) + f.S) * __builtin_copysign(2.0 * kPiSq, dx);
+    return (sdlibm::div_newton(__builtin_fma(A, f.Q, f.P * B), B * f.Q) + f.S) *
+           __builtin_copysign(2.0 * kPiSq, dx);
 }
 
 // Per-dislocation factors of the far pairs: the wrapped x w = x - rint(x)
@@ -383,12 +384,12 @@ SDLIBM_FN int gap_kind(double xmin, double xmax, double ymin, double ymax)
 // dislocations' wrapped x and exp factors: dx = s - rint(s) with s = wi - wj,
 // and e^{2 pi dx} = e^{2 pi wi} e^{-2 pi wj} e^{-2 pi k} (k = rint(s) in
 // {-1, 0, 1}), each exponential within a few ulps.  u = e^{2 pi s} and
-// v = e^{-2 pi s}: the larger is e^{2 pi |s|}.  |dx| is |s| when rint(s) = 0
-// (|s| <= 1/2, ties to even) and 1 - |s| otherwise, so t and ti are the
-// larger and the smaller of u and v, the smaller times e^{2 pi} and the
-// larger times e^{-2 pi} when s wrapped -- without selecting on k or on the
-// sign of dx.  Swapping the two dislocations negates s, k and dx exactly and
-// exchanges u and v, so t and ti are the same and dx flips sign.
+// v = e^{-2 pi s}: hi = max(u, v) = e^{2 pi |s|}, lo = min(u, v).  |dx| is
+// min(|s|, 1 - |s|) for |s| < 1, so t = min(hi, lo e^{2 pi}) and
+// ti = max(lo, hi e^{-2 pi}) -- without selecting on k, on the sign of dx
+// or on whether s wrapped (where |s| is near 1/2 the two candidates agree to
+// a few ulps).  Swapping the two dislocations negates s, k and dx exactly
+// and exchanges u and v, so t and ti are the same and dx flips sign.
 struct FarPrelude {
     double dx, t, ti;
 };
@@ -397,11 +398,10 @@ SDLIBM_FN FarPrelude far_prelude(double wi, double wj, double epi, double emi, d
     const double s = wi - wj;
     const double u = epi * emj, v = emi * epj;
     const double hi = fmax(u, v), lo = fmin(u, v);
-    const bool wrapped = sdlibm::fabsd(s) > 0.5;
     FarPrelude f;
     f.dx = s - __builtin_rint(s);
-    f.t = wrapped ? lo * sdlibm_exp_2pik[1] : hi;
-    f.ti = wrapped ? hi * sdlibm_exp_m2pik[1] : lo;
+    f.t = fmin(hi, lo * sdlibm_exp_2pik[1]);
+    f.ti = fmax(lo, hi * sdlibm_exp_m2pik[1]);
     return f;
 }
 

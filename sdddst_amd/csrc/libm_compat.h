@@ -48,6 +48,22 @@ SDLIBM_FN uint32_t hiword(double d) { return (uint32_t)(asu64(d) >> 32); }
 SDLIBM_FN double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 SDLIBM_FN double fabsd(double x) { return __builtin_fabs(x); }
 SDLIBM_FN double copysignd(double x, double s) { return __builtin_copysign(x, s); }
+// a / b for b > 0 normal and a, a / b finite: the hardware reciprocal
+// refined by two Newton steps, then one correction of the quotient (within
+// an ulp of the rounded quotient).  For the far-pair form, whose error budget
+// is far above an ulp; the host build seeds with the division itself.
+SDLIBM_FN double div_newton(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+#else
+    double r = 1.0 / b;
+#endif
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+}
 // 2^k for -1022 <= k <= 1023, exact.
 SDLIBM_FN double pow2i(int k) { return asf64((uint64_t)(1023 + k) << 52); }
 

@@ -205,6 +205,24 @@ def pair_roofline(pairs, avg_ms, traffic, traffic_rec):
     return out
 
 
+def issue_probe(roof):
+    """The FP64 VALU rate this device sustains on independent FMA and
+    multiply streams (sdddst_fp64_probe), measured after the timed run, and
+    the pair kernel's own VALU instruction rate against it: the published
+    78.6 TFLOP/s assumes the 2.4 GHz peak clock, which FP64-heavy kernels do
+    not hold."""
+    from sdddst_amd._capi import fp64_probe
+    tf, fma_rate, mul_rate = fp64_probe()
+    rec = {"fma_stream_tflops": tf, "fma_stream_valu_per_s": fma_rate, "mul_stream_valu_per_s": mul_rate,
+           "source": "sdddst_fp64_probe: 8 independent chains per lane, 16 waves per CU, best of 3 launches"}
+    insts = roof.get("valu_insts_per_pair")
+    if insts and roof.get("avg_launch_ms"):
+        k = insts * roof["pairs_per_launch"] / 64.0 / (roof["avg_launch_ms"] * 1e-3)
+        rec.update({"pair_kernel_valu_per_s": k, "pair_kernel_frac_of_fma_stream": k / fma_rate,
+                    "executed_fp64_frac_of_fma_stream": roof["achieved"] / tf if roof.get("achieved") else None})
+    return rec
+
+
 def assembly_run(n=16384, m=2.0, reps=10):
     """One full Jacobian assembly (calculateJacobian, simulation.cpp:236-384:
     count, scan, fill, values, column sums, d, scaling) of the headline
@@ -620,6 +638,7 @@ def main():
         "solver_fallbacks_total": st.solver_fallbacks,
         "solver_precond_fallbacks_total": st.solver_precond_fallbacks,
     }
+    line["roofline"]["issue_probe"] = issue_probe(line["roofline"])
     if args.assembly_reps > 0:
         log("Jacobian assembly")
         line["assembly"] = assembly_run(n, 2.0, args.assembly_reps)

@@ -133,6 +133,15 @@ int sdddst_speeds(sdddst_ctx *ctx, int cfg_slot, int out_buf, double sigma, int 
 int sdddst_speeds_noalloc(const double *x, const double *y, const double *b, int n, double sigma, int field_mode,
                           double *v, void *stream);
 
+/* No counterpart in the reference: a measurement helper.  The FP64 VALU
+ * rate this device sustains on independent FMA and multiply streams (8
+ * chains of `iters` operations per lane, 16 waves per CU, the best of three
+ * timed launches after a warm-up): wave-level instructions per second and
+ * the FMA stream's TFLOP/s.  Beside the published FP64 peak, which assumes
+ * the peak engine clock, this is the ceiling the pair kernel's VALU rate is
+ * compared with.  Allocates a small output buffer and a stream of its own. */
+int sdddst_fp64_probe(int iters, double *fma_tflops, double *fma_valu_per_s, double *mul_valu_per_s);
+
 /* PrecisionHandler state: tolerance download / reset (precision_handler.cpp:62-70). */
 int sdddst_get_tolerance(sdddst_ctx *ctx, double *tol_out, int n);
 int sdddst_reset_tolerance(sdddst_ctx *ctx);

@@ -80,6 +80,7 @@ _SIGNATURES = {
     "sdddst_solver_precond_fallbacks": (ctypes.c_int, [_vp, _ip]),
     "sdddst_solver_history": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _ip, _ip]),
     "sdddst_speeds_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
+    "sdddst_fp64_probe": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
     "sdddst_solver_set_direct_limit": (ctypes.c_int, [_vp, ctypes.c_int]),
     "sdddst_spai_param_count": (ctypes.c_int, []),
     "sdddst_spai_set_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
@@ -446,6 +447,17 @@ def kernel_stats(ctx_handle, kernel: str):
         raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
     return cnt.value, ms.value
 
+
+
+def fp64_probe(iters: int = 20000):
+    """sdddst_fp64_probe: (FMA-stream TFLOP/s, FMA and multiply wave-level
+    VALU instructions per second) sustained by this device."""
+    lib = load_library()
+    t, f, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib.sdddst_fp64_probe(int(iters), ctypes.byref(t), ctypes.byref(f), ctypes.byref(m))
+    if rc != OK:
+        raise SdddstError(lib.sdddst_last_error().decode(errors="replace"))
+    return t.value, f.value, m.value
 
 
 def solver_history(ctx_handle, max_records: int = 1024):

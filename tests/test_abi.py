@@ -53,3 +53,10 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_fp64_probe_rejects_bad_iteration_count():
+    # checked before any device call
+    from sdddst_amd._capi import SdddstError, fp64_probe
+    with pytest.raises(SdddstError, match="iters"):
+        fp64_probe(0)

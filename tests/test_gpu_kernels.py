@@ -325,3 +325,15 @@ def test_jacobian_with_point_defects():
     rel = np.abs(Ax - rAx) / np.maximum(np.abs(rAx), 1e-300)
     assert rel.max() < 1e-11
     assert np.max(np.abs(dv - rdv)) < 1e-12
+
+
+def test_fp64_probe_rates_are_physical():
+    # the sustained FP64 stream rates lie below the issue limit of one
+    # wave instruction per SIMD every 4 clocks at 2.5 GHz, and the FMA
+    # stream's flops below the published 78.6 TFLOP/s (+5 %)
+    from sdddst_amd._capi import fp64_probe
+    tf, fma_rate, mul_rate = fp64_probe(4000)
+    print(f"fp64 probe: {tf:.1f} TFLOP/s, {fma_rate:.3e} / {mul_rate:.3e} VALU/s", flush=True)
+    limit = 256 * 4 * 2.5e9 / 4
+    assert 0.1 * limit < fma_rate <= limit and 0.1 * limit < mul_rate <= limit
+    assert 5.0 < tf <= 78.6 * 1.05

@@ -28,7 +28,14 @@
  *     differences of the residual at an infinite cutoff (with and without
  *     point defects), the speed weights sum to one, a dipole's linear mode
  *     follows the theta-scheme with theta = (1+d)/2, local error order 3,
- *     second-order convergence to a DOP853 solution, N = 1 closed forms.
+ *     second-order convergence to a DOP853 solution, N = 1 closed forms;
+ *   - stress protocols: the golden table made from the reference's own
+ *     protocol translation units;
+ *   - point-defect terms: mpmath derivatives of the speed term;
+ *   - the run loop (tests/test_oracle_runloop.py): each limit, the avalanche
+ *     counter, the cutoff switch, post-relax and sub-configurations end the
+ *     run where the loop's semantics place them on the log of an
+ *     unconstrained run, whose logged prefix they reproduce.
  * The dense LU solver below stands in for UMFPACK (absent from this image);
  * its solves are pinned only by residual, "parity unpinned" against UMFPACK.
  */

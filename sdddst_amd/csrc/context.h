@@ -92,6 +92,7 @@ struct Context {
     int64_t nnz = 0, cap = 0;
     double *jtau = nullptr, *jmult = nullptr;   // tau_x and taper per strictly-lower entry
     int *jmirror = nullptr;                     // position of the mirror entry
+    bool aux_valid = false;                     // jtau / jmult / jmirror hold the current pattern's
     unsigned char *jcore = nullptr;             // 1 where the pair lies within the cutoff (untapered)
     double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask

@@ -138,7 +138,9 @@ __device__ __forceinline__ int row_position(int words, const unsigned long long 
 // of both positions.  Mode FAST takes the production derivative
 // (xy_diff_x_fast), the other modes the reference's expression.  With raw,
 // also the unsigned value for step size h at the entry and at its mirror.
-template <int Mode>
+// Without Aux, tau_x, the taper and the mirror positions are not kept (a
+// later call that needs them runs the pass again with Aux).
+template <int Mode, bool Aux = true>
 __device__ __forceinline__ void values_column(int j, const double *__restrict__ x, const double *__restrict__ y,
                                               const CutParams &cp, const int *__restrict__ Ap,
                                               const int *__restrict__ Ai, const int *__restrict__ diag,
@@ -170,9 +172,11 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
         double t;
         if constexpr (Mode == SDDDST_FIELD_FAST) t = sdfield::xy_diff_x_fast(dx, dy);
         else t = sdfield::xy_diff_x(dx, dy);
-        tau[k] = t;
-        mult[k] = m;
-        mirror[k] = mk;
+        if constexpr (Aux) {
+            tau[k] = t;
+            mult[k] = m;
+            mirror[k] = mk;
+        }
         if (raw) {
             // the unsigned value h * b_i * b_j * tau * taper in the operation
             // order of simulation.cpp:337-343; the upper copy (row j of column

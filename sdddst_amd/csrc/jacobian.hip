@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsign
 // tau_x and the taper of every strictly-lower entry (row i > column j), the
 // position of its mirror (row j in column i), and the unsigned value for step
 // size h at both positions.
-template <int Mode>
+template <int Mode, bool Aux>
 __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                     CutParams cp, const int *__restrict__ Ap,
                                                     const int *__restrict__ Ai, const int *__restrict__ diag,
@@ -256,8 +256,8 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    jac::values_column<Mode>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart, b, h, raw,
-                             flags);
+    jac::values_column<Mode, Aux>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart, b, h,
+                                   raw, flags);
 }
 
 // simulation.cpp:269-324: the point-defect sum tmp of column j (defects in
@@ -624,16 +624,20 @@ static int build_pattern(Context &c, int slot, double h)
                        c.jwstart);
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
+    // tau_x, the taper and the mirror positions are kept only when a later
+    // call needs them (ensure_aux): the step never does, and the three arrays
+    // were a fifth of the values pass's stores
     if (c.p.field_mode == SDDDST_FIELD_FAST)
-        hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_FAST>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp, c.Ap,
-                           c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart, c.b, h,
-                           c.Araw, c.d_flags);
+        hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_FAST, false>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           c.b, h, c.Araw, c.d_flags);
     else
-        hipLaunchKernelGGL(k_jac_values<SDDDST_FIELD_EXACT>, dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y, cp,
-                           c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart, c.b, h,
-                           c.Araw, c.d_flags);
+        hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_EXACT, false>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           c.b, h, c.Araw, c.d_flags);
     SD_HIP(hipGetLastError());
     c.araw_h = h;
+    c.aux_valid = false;
     if (c.pc > 0) {
         hipLaunchKernelGGL(k_jac_pd, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.pc, x, c.y, c.b, c.px,
                            c.py, c.p.A, c.p.KASQR, cp, c.jpd);
@@ -641,6 +645,31 @@ static int build_pattern(Context &c, int slot, double h)
     }
     c.pattern_valid = true;
     c.pattern_slot = slot;
+    return SDDDST_OK;
+}
+
+// tau_x, the taper and the mirror positions of the current pattern (for a
+// reuse at another step size and for the zero-value compaction): the full
+// values pass again over the positions in slot (those the pattern was built
+// from), which also rewrites the unsigned values for araw_h.
+static int ensure_aux(Context &c, int slot)
+{
+    if (c.aux_valid) return SDDDST_OK;
+    const int n = c.n;
+    const int wblocks = (n + 3) / 4;
+    const int words = (n + 63) / 64;
+    const CutParams cp = cut_params(c);
+    const double *x = c.x[slot];
+    if (c.p.field_mode == SDDDST_FIELD_FAST)
+        hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_FAST, true>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           c.b, c.araw_h, c.Araw, c.d_flags);
+    else
+        hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_EXACT, true>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           c.b, c.araw_h, c.Araw, c.d_flags);
+    SD_HIP(hipGetLastError());
+    c.aux_valid = true;
     return SDDDST_OK;
 }
 
@@ -672,6 +701,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
         if (c.araw_h != h) {
             // the pattern is reused with another step size: the unsigned
             // values again, from tau_x and the taper (with the zero flag)
+            if ((rc = ensure_aux(c, slot))) return rc;
             hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
             hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
                                c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Araw, c.d_flags);
@@ -689,6 +719,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
         c.raw_h = h;
         if (flag) {
             // an exactly zero value: its upper copy goes (simulation.cpp:263-267)
+            if ((rc = ensure_aux(c, slot))) return rc;
             hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
                                c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Ax, c.d_flags);
             SD_HIP(hipGetLastError());

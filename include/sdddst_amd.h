@@ -243,6 +243,18 @@ int sdddst_error_control(sdddst_ctx *ctx, double h, double hmax, int *accept, do
 int sdddst_speed_sums(sdddst_ctx *ctx, int buf, double *sum_abs, double *sum_sq, double *sum_bv);
 int sdddst_strain_increment(sdddst_ctx *ctx, int old_slot, int new_slot, double *out);
 
+/* The end of stage III (simulation.cpp:505-530) in one device round trip:
+ * vsq_init / vsq_init2 = sum v^2 of SDDDST_V_INIT / SDDDST_V_INIT2 (the
+ * energy terms), the error estimate and controller exactly as
+ * sdddst_error_control, and, with strain != 0, the strain increments of the
+ * dislocations slot -> first small step -> second small step (as two
+ * sdddst_strain_increment calls; the caller adds them only for an accepted
+ * step, as the reference does).  The same kernels and values as the
+ * separate calls, one host wait instead of five. */
+int sdddst_stage_end(sdddst_ctx *ctx, double h, double hmax, int strain, double *vsq_init, double *vsq_init2,
+                     int *accept, double *h_next, double *max_ratio, int *selected, double *strain_first,
+                     double *strain_second);
+
 int sdddst_synchronize(sdddst_ctx *ctx);
 
 /* AnalyticField::xy / xy_diff_x / xy_diff_x2 (AnalyticField.cpp:145-202,

@@ -98,6 +98,8 @@ _SIGNATURES = {
     ),
     "sdddst_speed_sums": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
     "sdddst_strain_increment": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "sdddst_stage_end": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                        ctypes.POINTER(ctypes.c_int), _dp, _dp, ctypes.POINTER(ctypes.c_int), _dp, _dp]),
     "sdddst_synchronize": (ctypes.c_int, [_vp]),
     "sdddst_field_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]),
     "sdddst_libm_eval": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp]),

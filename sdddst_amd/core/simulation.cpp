@@ -504,15 +504,15 @@ void Simulation::stepStageIII()
     check(sdddst_copy_slot(ctx, SS, FS), "copy");
     integrateSlots(0.5 * sD->stepSize, SS, FS, true, true, EndOfFirstSmallStep, EndOfSecondSmallStep);
 
-    check(sdddst_speed_sums(ctx, SDDDST_V_INIT, nullptr, &vsquare1, nullptr), "sums");
-    check(sdddst_speed_sums(ctx, SDDDST_V_INIT2, nullptr, &vsquare2, nullptr), "sums");
-    energyAccum = (vsquare1 + vsquare2) * 0.5 * sD->stepSize * 0.5;
-
-    // calculateXError + the controller; the device also resets the tolerance.
+    // the energy sums, calculateXError + the controller (the device also
+    // resets the tolerance) and the strain increments, in one round trip
     int accept = 0, sel = 0;
-    double hnext = 0, maxRatio = 0;
+    double hnext = 0, maxRatio = 0, strainA = 0, strainB = 0;
     const double hmax = sD->isMaxStepSizeLimit ? sD->maxStepSizeLimit : -1.0;
-    check(sdddst_error_control(ctx, sD->stepSize, hmax, &accept, &hnext, &maxRatio, &sel), "error control");
+    check(sdddst_stage_end(ctx, sD->stepSize, hmax, sD->calculateStrainDuringSimulation ? 1 : 0, &vsquare1,
+                           &vsquare2, &accept, &hnext, &maxRatio, &sel, &strainA, &strainB),
+          "stage end");
+    energyAccum = (vsquare1 + vsquare2) * 0.5 * sD->stepSize * 0.5;
     pH->setFromDevice(maxRatio, (unsigned)sel);
 
     if (pH->getMaxErrorRatioSqr() < 1.0) {
@@ -520,11 +520,8 @@ void Simulation::stepStageIII()
         initSpeedCalculationIsNeeded = true;
         if (sD->inFinal) sD->remainingFinalSteps--;
         if (sD->calculateStrainDuringSimulation) {
-            double a = 0, b = 0;
-            check(sdddst_strain_increment(ctx, DISL, FS, &a), "strain");
-            check(sdddst_strain_increment(ctx, FS, SS, &b), "strain");
-            sD->totalAccumulatedStrainIncrease += a;
-            sD->totalAccumulatedStrainIncrease += b;
+            sD->totalAccumulatedStrainIncrease += strainA;
+            sD->totalAccumulatedStrainIncrease += strainB;
         }
         check(sdddst_swap_slots(ctx, DISL, SS), "swap");
         sD->simTime += sD->stepSize;

@@ -123,7 +123,7 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
         (rc = dalloc(&c.tol, n)) || (rc = dalloc(&c.minr2, n)) || (rc = dalloc(&c.partial, (size_t)c.nb * n)) ||
         (rc = dalloc(&c.dvec, n)) || (rc = dalloc(&c.g, n)) || (rc = dalloc(&c.delta, n)) ||
         (rc = dalloc(&c.Ap, (size_t)n + 1)) || (rc = dalloc(&c.diag, n)) || (rc = dalloc(&c.colcount, n)) ||
-        (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, 1)) || (rc = dalloc(&c.jpd, n)) ||
+        (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, kNumDevScalars)) || (rc = dalloc(&c.jpd, n)) ||
         (rc = dalloc(&c.sraw, n)) || (rc = dalloc(&c.partner, n)) || (rc = dalloc(&c.strength, n)) ||
         (rc = dalloc(&c.colinfo, n)) ||
         (rc = pair_order_alloc(c)))
@@ -595,6 +595,35 @@ int sdddst_strain_increment(sdddst_ctx *ctx, int old_slot, int new_slot, double 
     if (rc) return rc;
     if ((rc = host_wait(c, c.ds, (int)(sizeof(DevScalars) / 8), c.hs))) return rc;
     *out = c.hs->strain;
+    return SDDDST_OK;
+}
+
+int sdddst_stage_end(sdddst_ctx *ctx, double h, double hmax, int strain, double *vsq_init, double *vsq_init2,
+                     int *accept, double *h_next, double *max_ratio, int *selected, double *strain_first,
+                     double *strain_second)
+{
+    CTX_CHECK(ctx);
+    Context &c = ctx->c;
+    int rc;
+    if ((rc = launch_speed_sums(c, SDDDST_V_INIT, c.ds + kStageSumA)) ||
+        (rc = launch_speed_sums(c, SDDDST_V_INIT2, c.ds + kStageSumB)) || (rc = launch_error_control(c, h, hmax)))
+        return rc;
+    if (strain && ((rc = launch_strain(c, SDDDST_SLOT_DISLOCATIONS, SDDDST_SLOT_FIRST_SMALL, c.ds + kStageStrainA)) ||
+                   (rc = launch_strain(c, SDDDST_SLOT_FIRST_SMALL, SDDDST_SLOT_SECOND_SMALL, c.ds + kStageStrainB))))
+        return rc;
+    if ((rc = launch_stage_pack(c))) return rc;
+    double out[7];
+    if ((rc = host_wait(c, c.ds + kStageOut, 7, out))) return rc;
+    int ints[2];
+    std::memcpy(ints, &out[4], sizeof ints);
+    if (vsq_init) *vsq_init = out[0];
+    if (vsq_init2) *vsq_init2 = out[1];
+    if (max_ratio) *max_ratio = out[2];
+    if (h_next) *h_next = out[3];
+    if (accept) *accept = ints[0];
+    if (selected) *selected = ints[1];
+    if (strain_first) *strain_first = strain ? out[5] : 0.0;
+    if (strain_second) *strain_second = strain ? out[6] : 0.0;
     return SDDDST_OK;
 }
 

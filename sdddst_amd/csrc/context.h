@@ -127,7 +127,7 @@ struct Context {
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     // controller scalars
-    DevScalars *ds = nullptr;
+    DevScalars *ds = nullptr;           // kNumDevScalars records; [0] for the single-result calls
     DevScalars *hs = nullptr;           // pinned host mirror
     HostSignal *sig = nullptr;          // host view of the completion signal (mapped, coherent)
     HostSignal *sig_dev = nullptr;      // its device address
@@ -209,8 +209,12 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern);
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
 int launch_newton_update(Context &c, int slot);
 int launch_error_control(Context &c, double h, double hmax);
-int launch_speed_sums(Context &c, int buf);
-int launch_strain(Context &c, int old_slot, int new_slot);
+// the sums / the increment into dst (nullptr: c.ds)
+int launch_speed_sums(Context &c, int buf, DevScalars *dst = nullptr);
+int launch_strain(Context &c, int old_slot, int new_slot, DevScalars *dst = nullptr);
+// the end of stage III in one record (c.ds[kStageOut]): see sdddst_stage_end
+constexpr int kStageSumA = 1, kStageSumB = 2, kStageStrainA = 3, kStageStrainB = 4, kStageOut = 5, kNumDevScalars = 6;
+int launch_stage_pack(Context &c);
 int launch_reset_tolerance(Context &c);
 int field_eval(Context &c, int which, int64_t n, const double *dx, const double *dy, double *out);
 int libm_eval(Context &c, int fn, int64_t n, const double *x, double *out);

@@ -40,6 +40,8 @@
 // Algorithmic HBM traffic of one assembly: 12*nnz + 40*N + 4 bytes.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <vector>
 
 #include "context.h"
@@ -232,12 +234,19 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cn
     }
 }
 
+// The kernels after the entry count run only while it fits the storage the
+// host allocated (total <= cap); the host reads the count with the
+// assembly's one wait and, on overflow, grows the storage and assembles
+// again.
+__device__ __forceinline__ bool over_capacity(const long long *total, long long cap) { return *total > cap; }
+
 __global__ __launch_bounds__(256) void k_jac_fill(int n, int words, const unsigned long long *__restrict__ mask,
                                                   const int *__restrict__ Ap, int *__restrict__ Ai,
-                                                  int *__restrict__ diag, int *__restrict__ wstart)
+                                                  int *__restrict__ diag, int *__restrict__ wstart,
+                                                  const long long *__restrict__ total, long long cap)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (j >= n) return;
+    if (j >= n || over_capacity(total, cap)) return;
     jac::fill_column(j, words, mask + (size_t)j * words, Ap[j], Ai, diag + j, wstart + (size_t)j * words);
 }
 
@@ -252,10 +261,11 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
                                                     int *__restrict__ mirror, unsigned char *__restrict__ core,
                                                     int words, const unsigned long long *__restrict__ mask,
                                                     const int *__restrict__ wstart, const double *__restrict__ b,
-                                                    double h, double *__restrict__ raw, int *__restrict__ flags)
+                                                    double h, double *__restrict__ raw, int *__restrict__ flags,
+                                                    const long long *__restrict__ total, long long cap)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (j >= n) return;
+    if (j >= n || over_capacity(total, cap)) return;
     jac::values_column<Mode, Aux>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart, b, h,
                                    raw, flags);
 }
@@ -327,12 +337,13 @@ constexpr int kSumPer = kSumChunk / 64;
 __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *__restrict__ Ap,
                                                     const int *__restrict__ diag, const double *__restrict__ pdtmp,
                                                     double *__restrict__ Araw, double *__restrict__ sraw,
-                                                    double *__restrict__ dvec, double2 *__restrict__ colinfo)
+                                                    double *__restrict__ dvec, double2 *__restrict__ colinfo,
+                                                    const long long *__restrict__ total, long long cap)
 {
     __shared__ double buf[4][kSumChunk];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int j = (int)(blockIdx.x * 4 + wave);
-    if (j >= n) return;
+    if (j >= n || over_capacity(total, cap)) return;
     const int beg = Ap[j], end = Ap[j + 1], d = diag[j];
     const double pdh = -(pdtmp ? pdtmp[j] : 0.0) * h;
     double *lb = buf[wave];
@@ -614,14 +625,16 @@ static int build_pattern(Context &c, int slot, double h)
     long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
     int rc = launch_scan(c, n, c.colcount, c.Ap, d_total);
     if (rc) return rc;
-    long long total = 0;
-    if ((rc = host_wait(c, d_total, 1, &total))) return rc;
-    SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
-    rc = ensure_capacity(c, total);
-    if (rc) return rc;
-    c.nnz = total;
+    if (c.cap == 0) {
+        // the first assembly sizes the storage; later ones run against the
+        // capacity and the count is read with the assembly's one wait
+        long long total = 0;
+        if ((rc = host_wait(c, d_total, 1, &total))) return rc;
+        SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
+        if ((rc = ensure_capacity(c, total))) return rc;
+    }
     hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag,
-                       c.jwstart);
+                       c.jwstart, d_total, (long long)c.cap);
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
     // tau_x, the taper and the mirror positions are kept only when a later
@@ -630,11 +643,11 @@ static int build_pattern(Context &c, int slot, double h)
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_FAST, false>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
                            cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
-                           c.b, h, c.Araw, c.d_flags);
+                           c.b, h, c.Araw, c.d_flags, d_total, (long long)c.cap);
     else
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_EXACT, false>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
                            cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
-                           c.b, h, c.Araw, c.d_flags);
+                           c.b, h, c.Araw, c.d_flags, d_total, (long long)c.cap);
     SD_HIP(hipGetLastError());
     c.araw_h = h;
     c.aux_valid = false;
@@ -647,6 +660,8 @@ static int build_pattern(Context &c, int slot, double h)
     c.pattern_slot = slot;
     return SDDDST_OK;
 }
+
+static long long *jac_total(Context &c) { return reinterpret_cast<long long *>(c.d_flags + 2); }
 
 // tau_x, the taper and the mirror positions of the current pattern (for a
 // reuse at another step size and for the zero-value compaction): the full
@@ -663,11 +678,11 @@ static int ensure_aux(Context &c, int slot)
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_FAST, true>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
                            cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
-                           c.b, c.araw_h, c.Araw, c.d_flags);
+                           c.b, c.araw_h, c.Araw, c.d_flags, jac_total(c), (long long)c.cap);
     else
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_EXACT, true>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
                            cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
-                           c.b, c.araw_h, c.Araw, c.d_flags);
+                           c.b, c.araw_h, c.Araw, c.d_flags, jac_total(c), (long long)c.cap);
     SD_HIP(hipGetLastError());
     c.aux_valid = true;
     return SDDDST_OK;
@@ -709,10 +724,25 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             c.araw_h = h;
         }
         hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, h, c.Ap, c.diag,
-                           c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec, c.colinfo);
+                           c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec, c.colinfo, jac_total(c),
+                           (long long)c.cap);
         SD_HIP(hipGetLastError());
-        int flags2[2] = {0, 0};   // d_flags[0..1] as one word; [0] is the zero-value flag
-        if ((rc = host_wait(c, c.d_flags, 1, flags2))) return rc;
+        // d_flags[0..1] as one word ([0]: the zero-value flag), then the
+        // entry count of the last build
+        long long words2[2] = {0, 0};
+        if ((rc = host_wait(c, c.d_flags, 2, words2))) return rc;
+        const long long total = words2[1];
+        SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
+        if (total > c.cap) {
+            // the pattern outgrew the storage: grow it and assemble again
+            // (every kernel after the count skipped its work)
+            if ((rc = ensure_capacity(c, total))) return rc;
+            c.pattern_valid = false;
+            return launch_jacobian(c, slot, h, false);
+        }
+        c.nnz = total;
+        int flags2[2];
+        std::memcpy(flags2, &words2[0], sizeof flags2);
         const int flag = flags2[0];
         c.jac_symmetric = flag == 0;
         c.raw_valid = flag == 0;

@@ -206,6 +206,25 @@ __global__ void k_signal(const uint64_t *__restrict__ src, int words, HostSignal
     __hip_atomic_store(&sig->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// The stage-III record of sdddst_stage_end as seven words: the squared-speed
+// sums of V_INIT and V_INIT2, the controller's max ratio and next step, its
+// accept flag and selected index (two ints in one word), the two strain
+// increments.
+__global__ void k_stage_pack(DevScalars *ds)
+{
+    if (threadIdx.x != 0) return;
+    double *out = reinterpret_cast<double *>(ds + kStageOut);
+    out[0] = ds[kStageSumA].sum_sq;
+    out[1] = ds[kStageSumB].sum_sq;
+    out[2] = ds[0].max_ratio;
+    out[3] = ds[0].h_next;
+    int *ints = reinterpret_cast<int *>(out + 4);
+    ints[0] = ds[0].accept;
+    ints[1] = ds[0].selected;
+    out[5] = ds[kStageStrainA].strain;
+    out[6] = ds[kStageStrainB].strain;
+}
+
 __global__ void k_fill(double *p, double v, int n)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -240,23 +259,32 @@ int launch_error_control(Context &c, double h, double hmax)
     return SDDDST_OK;
 }
 
-int launch_speed_sums(Context &c, int buf)
+int launch_speed_sums(Context &c, int buf, DevScalars *dst)
 {
+    DevScalars *ds = dst ? dst : c.ds;
     if (c.p.field_mode == SDDDST_FIELD_FAST)
-        hipLaunchKernelGGL(k_speed_sums_tree, dim3(1), dim3(1024), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
+        hipLaunchKernelGGL(k_speed_sums_tree, dim3(1), dim3(1024), 0, c.stream, c.n, c.v[buf], c.b, ds);
     else
-        hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(256), 0, c.stream, c.n, c.v[buf], c.b, c.ds);
+        hipLaunchKernelGGL(k_speed_sums, dim3(1), dim3(256), 0, c.stream, c.n, c.v[buf], c.b, ds);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
 
-int launch_strain(Context &c, int old_slot, int new_slot)
+int launch_strain(Context &c, int old_slot, int new_slot, DevScalars *dst)
 {
+    DevScalars *ds = dst ? dst : c.ds;
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL(k_strain_tree, dim3(1), dim3(1024), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b,
-                           c.ds);
+                           ds);
     else
-        hipLaunchKernelGGL(k_strain, dim3(1), dim3(256), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, c.ds);
+        hipLaunchKernelGGL(k_strain, dim3(1), dim3(256), 0, c.stream, c.n, c.x[old_slot], c.x[new_slot], c.b, ds);
+    SD_HIP(hipGetLastError());
+    return SDDDST_OK;
+}
+
+int launch_stage_pack(Context &c)
+{
+    hipLaunchKernelGGL(k_stage_pack, dim3(1), dim3(64), 0, c.stream, c.ds);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }

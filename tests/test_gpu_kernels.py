@@ -246,6 +246,27 @@ def _assemble(c, h, rebuild):
     return Ap, Ai, Ax, dv, dg
 
 
+def test_assembly_outgrowing_its_storage_assembles_again():
+    # assemblies after the first run against the storage the first one sized
+    # and read the entry count afterwards; a configuration with far more
+    # neighbours overflows it: the library grows the storage and assembles
+    # again, with the same matrix as a fresh context
+    n, m, h = 2048, 1.0, 3e-5
+    x, y, b = configs.dislocations(n, 77)
+    xs, ys = 0.25 * x, 0.25 * y          # 16x the density: about 16x the entries
+    with S.Context(n, cutoff_multiplier=m, field_mode=S.FIELD_FAST) as c:
+        c.upload_state(0, x, y, b)
+        small = _assemble(c, h, 1)
+        c.upload_state(0, xs, ys, b)
+        grown = _assemble(c, h, 1)
+    with S.Context(n, cutoff_multiplier=m, field_mode=S.FIELD_FAST) as c:
+        c.upload_state(0, xs, ys, b)
+        fresh = _assemble(c, h, 1)
+    assert len(grown[1]) > 4 * len(small[1])
+    for u, v in zip(grown, fresh):
+        assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+
+
 def _pair_solve(c, h):
     c.solver_select(S.SOLVER_FGMRES, 1e-10, 300, 40)
     c.set_preconditioner(S.PRECOND_PAIR)

@@ -12,10 +12,11 @@
 // minimum-image offset (the only one that can suffer cancellation) is
 // evaluated exactly as the reference does; the nine far images take their
 // cosh from one shared exp(2*pi*dx) and fixed exp(+-2*pi*k) factors.  The
-// four images within 2.5 of the cell are summed as one fraction (numerator
-// and product of the denominators (cosh - cos)^2; one division); the five
-// beyond it take the two-term expansion in 1/cosh, exact to 1e-19.  Far-image terms then differ from the reference by a few
-// ulps of terms that are at least e^-pi smaller than the near term.
+// two images within 1.5 of the cell are summed as one fraction (numerator
+// and product of the denominators (cosh - cos)^2; one division); the seven
+// beyond it take their Fourier series in y to the order that reaches 1e-17.
+// Far-image terms then differ from the reference by a few ulps of terms
+// that are at least e^-pi smaller than the near term.
 #pragma once
 
 #include "libm_compat.h"
@@ -220,7 +221,7 @@ constexpr double kFarMinR2 = 0.0081;
 
 // The nine far images of xy_fast (see below) as sg(dx) F(|dx|) without the
 // sign and the factor 2 pi^2, from t = e^{2 pi |dx|} and ti = 1 / t: the
-// four within 2.5 cells as the fraction P / Q, the five beyond as the sum S.
+// two within 1.5 cells as the fraction P / Q, the seven beyond as the sum S.
 struct FarParts {
     double P, Q, S;
 };
@@ -235,46 +236,57 @@ SDLIBM_FN FarParts far_image_parts(double dx, double cy, double t, double ti)
     // (t e^{2 pi k} + e^{-2 pi k} / t) / 2 and B_k = cosh(2 pi (k - ad)) =
     // (e^{2 pi k} / t + t e^{-2 pi k}) / 2.
     const double ad = sdlibm::fabsd(dx);
-    // |u| = ad + k and k - ad for k = 1, 2: the four fractions summed as one,
-    // P / Q with Q the product of their denominators, one division
-    double num[4], den[4];
-    for (int k = 1; k <= 2; ++k) {
-        const double hp = 0.5 * sdlibm_exp_2pik[k], hm = 0.5 * sdlibm_exp_m2pik[k];
-        const double A = __builtin_fma(t, hp, ti * hm);
-        const double B = __builtin_fma(ti, hp, t * hm);
-        const double dA = A - cy, dB = B - cy;
-        num[2 * k - 2] = (ad + (double)k) * __builtin_fma(A, cy, -1.0);
-        num[2 * k - 1] = (ad - (double)k) * __builtin_fma(B, cy, -1.0);
-        den[2 * k - 2] = dA * dA;
-        den[2 * k - 1] = dB * dB;
-    }
-    double P = num[0], Q = den[0];
-    for (int m = 1; m < 4; ++m) {
-        P = __builtin_fma(num[m], Q, P * den[m]);
-        Q *= den[m];
-    }
-    // |u| >= 2.5 (k = 3 both sides, the weighted 4th pair, the 5th): with
-    // a = 1 / cosh(2 pi u) <= 3e-7, (c cy - 1) / (c - cy)^2 =
-    // a cy + a^2 (2 cy^2 - 1) + O(a^3), and a = 2 e^{-2 pi |u|} to within
-    // e^{-4 pi |u|} <= 3e-14 relative.  With a = ti e_k (images ad + k) or
-    // t e_k (images k - ad), e_k = 2 e^{-2 pi k}, the weighted first-order
-    // sum is ti p1(ad) + t p2(ad) for two quadratics in ad:
-    //   p1 = e3 (ad + 3) + e4 (1 - ad)(ad + 4)
-    //   p2 = e3 (ad - 3) + e4 (ad - 4) + e5 ad (ad - 5)
-    // and of the second-order terms only k = 3 reaches 1e-16 (a^2 <= 1e-13);
-    // those of k = 4, 5 (a^2 <= 3e-19) and the third-order ones are omitted,
-    // below 1e-17 of the field after the factor 2 pi^2.
-    constexpr double e3 = 2.0 * sdlibm_exp_m2pik[3], e4 = 2.0 * sdlibm_exp_m2pik[4];
-    constexpr double e5 = 2.0 * sdlibm_exp_m2pik[kImages + 1];
-    constexpr double p10 = 3.0 * e3 + 4.0 * e4, p11 = e3 - 3.0 * e4, p12 = -e4;
-    constexpr double p20 = -3.0 * e3 - 4.0 * e4, p21 = e3 + e4 - 5.0 * e5, p22 = e5;
-    constexpr double e3sq = e3 * e3;
+    // |u| = ad + 1 and 1 - ad: the two fractions summed as one, P / Q with Q
+    // the product of their denominators
+    const double hp = 0.5 * sdlibm_exp_2pik[1], hm = 0.5 * sdlibm_exp_m2pik[1];
+    const double A = __builtin_fma(t, hp, ti * hm);
+    const double B = __builtin_fma(ti, hp, t * hm);
+    const double dA = A - cy, dB = B - cy;
+    const double nA = (ad + 1.0) * __builtin_fma(A, cy, -1.0);
+    const double nB = (ad - 1.0) * __builtin_fma(B, cy, -1.0);
+    const double QA = dA * dA, QB = dB * dB;
+    const double P = __builtin_fma(nB, QA, nA * QB);
+    const double Q = QA * QB;
+    // |u| >= 1.5 (k = 2, 3 both sides, the weighted 4th pair, the 5th): the
+    // Fourier series in y of one image (the y-derivative of the Poisson
+    // kernel), (c cy - 1) / (c - cy)^2 = 2 sum_n n q^n T_n(cy) with
+    // q = e^{-2 pi |u|} and T_n the Chebyshev polynomials (T_n(cy) =
+    // cos(2 pi n dy)).  q = E_k ti for the images ad + k and E_k t for
+    // k - ad (E_k = e^{-2 pi k}), so the weighted sum over the images is
+    //   2 sum_n n T_n(cy) (ti^n Pp_n(ad) + t^n Pm_n(ad)),
+    // Pp_n = sum_k w+_k(ad) E_k^n, Pm_n = sum_k w-_k(ad) E_k^n, with the
+    // weights w of the image list above.  q <= e^{-3 pi} (k = 2),
+    // e^{-5 pi} (k = 3), e^{-7 pi} (k = 4, 5): n <= 4 for k = 2, n <= 2 for
+    // k = 3 and n = 1 beyond; the omitted terms are below 3e-17 of the
+    // field after the factor 2 pi^2.
+    constexpr double E2 = sdlibm_exp_m2pik[2], E3 = sdlibm_exp_m2pik[3], E4 = sdlibm_exp_m2pik[4];
+    constexpr double E5 = sdlibm_exp_m2pik[kImages + 1];
+    // n = 1: Pp = (ad + 2) E2 + (ad + 3) E3 + (1 - ad)(ad + 4) E4,
+    //        Pm = (ad - 2) E2 + (ad - 3) E3 + (ad - 4) E4 + ad (ad - 5) E5 (times 2)
+    constexpr double p10 = 2.0 * (2.0 * E2 + 3.0 * E3 + 4.0 * E4), p11 = 2.0 * (E2 + E3 - 3.0 * E4), p12 = -2.0 * E4;
+    constexpr double p20 = -p10, p21 = 2.0 * (E2 + E3 + E4 - 5.0 * E5), p22 = 2.0 * E5;
+    // n = 2: (ad +- 2) E2^2 + (ad +- 3) E3^2 (times 4); n = 3, 4: (ad +- 2) E2^n (times 2n)
+    constexpr double s2 = 4.0 * (E2 * E2 + E3 * E3), c2k = 4.0 * (2.0 * E2 * E2 + 3.0 * E3 * E3);
+    constexpr double C3 = 6.0 * E2 * E2 * E2, C4 = 8.0 * E2 * E2 * E2 * E2;
+    const double ti2 = ti * ti, t2 = t * t;
+    const double ti3 = ti2 * ti, t3 = t2 * t;
+    const double ti4 = ti2 * ti2, t4 = t2 * t2;
     const double p1 = __builtin_fma(__builtin_fma(p12, ad, p11), ad, p10);
     const double p2 = __builtin_fma(__builtin_fma(p22, ad, p21), ad, p20);
     const double first = __builtin_fma(ti, p1, t * p2);
-    const double second = e3sq * __builtin_fma(ti * ti, ad + 3.0, (t * t) * (ad - 3.0));
-    const double c2 = __builtin_fma(2.0 * cy, cy, -1.0);
-    const double S = __builtin_fma(c2, second, cy * first);
+    const double second = __builtin_fma(ti2, __builtin_fma(s2, ad, c2k), t2 * __builtin_fma(s2, ad, -c2k));
+    const double ap2 = ad + 2.0, am2 = ad - 2.0;
+    const double third = __builtin_fma(ti3, ap2, t3 * am2);
+    const double fourth = __builtin_fma(ti4, ap2, t4 * am2);
+    const double cy2 = 2.0 * cy;
+    const double T2 = __builtin_fma(cy2, cy, -1.0);
+    const double T3 = __builtin_fma(cy2, T2, -cy);
+    const double T4 = __builtin_fma(cy2, T3, -T2);
+    // smallest terms first
+    double S = (C4 * T4) * fourth;
+    S = __builtin_fma(C3 * T3, third, S);
+    S = __builtin_fma(T2, second, S);
+    S = __builtin_fma(cy, first, S);
     return FarParts{P, Q, S};
 }
 
@@ -317,7 +329,7 @@ SDLIBM_FN double xy_fast_cy(double dx, double dy, double cy, const uint64_t *__r
 // kFarMinGap) in the pair kernel: cos(2 pi dy) and t = e^{2 pi |dx|},
 // ti = e^{-2 pi |dx|} supplied by the caller (from per-dislocation factors,
 // within a few ulps), cosh(2 pi dx) = (t + ti) / 2.  The near image
-// dx (c cy - 1) / (c - cy)^2 and the four inner far images P / Q share one
+// dx (c cy - 1) / (c - cy)^2 and the two inner far images P / Q share one
 // division (c - cy >= 0.2 here, so the near image needs no care against
 // cancellation).
 SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
@@ -415,6 +427,71 @@ SDLIBM_FN double far_pair(double wi, double wj, double epi, double emi, double e
 {
     const FarPrelude f = far_prelude(wi, wj, epi, emi, epj, emj);
     return xy_far(f.dx, cy, f.t, f.ti);
+}
+
+// The minimum-image term of the reference's sum (AnalyticField.cpp:24-39)
+// exactly as xy_fast_cy forms it, for a pair whose cosh(2 pi dx) -
+// cos(2 pi dy) is below kFarMinGap: cos(2 pi dy) as glibc forms it (such a
+// pair has cos(2 pi dy) > 0.8 - 1e-15, so |2 pi dy| < 0.6436 lies in glibc's
+// first range, cos_r1) and cosh(2 pi dx) likewise (expm1 below 0.5 ln 2,
+// exp above), then f or its Taylor branch.  Each cosh branch has one
+// division (m^2 / 2w, or 1 / t for 0.5 t + 0.5 / t); it is taken once after
+// the branches, so lanes that diverge over them share it.  Odd in dx bit
+// for bit.
+SDLIBM_FN double near_image(double dx, double dy, const double *__restrict__ sctab,
+                            const uint64_t *__restrict__ e2tab)
+{
+    const double cy = sdlibm::cos_r1(kPi * 2.0 * dy, sctab);
+    const double ax = sdlibm::fabsd(dx * 2.0 * kPi);
+    const bool small = sdlibm::hiword(ax) < 0x3fd62e43u;
+    double num, den, t = 0.0;
+    if (small) {
+        const double m = sdlibm::expm1(ax);
+        const double w = 1.0 + m;
+        num = m * m;
+        den = w + w;
+    } else {
+        t = sdlibm::exp(ax, e2tab);
+        num = 1.0;
+        den = t;
+    }
+    const double q = num / den;
+    const double c0 = small ? 1.0 + q : 0.5 * t + 0.5 * q;
+    return (dx * dx + dy * dy > 1e-6) ? f_given_cosh(dx, c0, cy) : f_taylor(dx, dy);
+}
+
+// The production field of a pair of a mixed tile (some pairs far, some
+// near) from the two dislocations' far-pair factors (prelude f, cy by angle
+// addition) and their minimum-image differences (dxw, dyw; read only for a
+// near pair).  A far pair (gap cosh(2 pi dx) - cos(2 pi dy) >= kFarMinGap,
+// as far_gap forms it) gets xy_far's value bit for bit.  A near pair gets
+// the minimum image exactly as the reference forms it (near_image, the one
+// image whose cosh - cos can cancel) plus the nine far images from the same
+// factors as a far pair's (P / Q + S of far_image_parts; every far image has
+// |u| >= 1/2, so cosh(2 pi u) - cos >= 10.5 and few-ulp factors move it by
+// a few ulps of itself).  Both kinds share far_image_parts and one
+// Newton-refined division, so a wavefront with near and far lanes runs the
+// far images once and only the near image diverges.  far is set for a far
+// pair; r2 = dxw^2 + dyw^2 for a near pair (the tolerance candidates), 1
+// otherwise.  Odd under swapping the dislocations bit for bit.
+SDLIBM_FN double mixed_pair(const FarPrelude &f, double cy, double dxw, double dyw, bool &far, double &r2,
+                            const double *__restrict__ sctab, const uint64_t *__restrict__ e2tab)
+{
+    const double c0 = __builtin_fma(0.5, f.t, 0.5 * f.ti);
+    const double d = c0 - cy;
+    const double B = d * d;
+    const FarParts p = far_image_parts(f.dx, cy, f.t, f.ti);
+    double num = __builtin_fma(sdlibm::fabsd(f.dx) * (c0 * cy - 1.0), p.Q, p.P * B), den = B * p.Q;
+    double near = 0.0;
+    far = d >= kFarMinGap;
+    r2 = 1.0;
+    if (!far) {
+        near = near_image(dxw, dyw, sctab, e2tab);
+        r2 = dxw * dxw + dyw * dyw;
+        num = p.P;
+        den = p.Q;
+    }
+    return (sdlibm::div_newton(num, den) + p.S) * __builtin_copysign(2.0 * kPiSq, f.dx) + near;
 }
 
 // Production form of xy() (see header comment).

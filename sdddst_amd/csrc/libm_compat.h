@@ -350,6 +350,25 @@ SDLIBM_FN double cos(double x, const double *__restrict__ sctab = sdlibm_sincos_
     return hk < 0x3e400000u ? 1.0 : v;                 // |x| < 2^-27
 }
 
+// cos(x) for |x| < 0.855469: glibc's first range alone, do_cos(x, 0) (1
+// below 2^-27), the same value as cos() there with none of the other
+// ranges' work.  Larger arguments are clamped into the range, so the table
+// lookup stays in bounds (their value is then meaningless).
+SDLIBM_FN double cos_r1(double x, const double *__restrict__ sctab = sdlibm_sincos_tab)
+{
+    using namespace detail;
+    const double ax = fmin(fabsd(x), 0.8555);   // table entry 110 at most
+    const double u = big + ax;
+    const double y = ax - (u - big);
+    const int k = tab_index(u);
+    const double xx = y * y;
+    const double s = fmad(y * xx, fmad(xx, sn5, sn3), y);
+    const double c = xx * fmad(xx, fmad(xx, cs6, cs4), cs2);
+    const double A = sctab[k + 2], Alo = sctab[k + 3], B = sctab[k], Blo = sctab[k + 1];
+    const double res = A + fmad(B, -s, fmad(-A, c, fmad(-s, Blo, Alo)));
+    return hiword(ax) < 0x3e400000u ? 1.0 : res;
+}
+
 // sin(x) for |x| <= 4.
 SDLIBM_FN double sin(double x)
 {

@@ -65,10 +65,19 @@ struct FarData {
 // far when a lower bound of the gap over the tile clears kFarMinGap, none
 // when an upper bound stays below it.
 //
-// How a tile's pairs are evaluated (the values of sdfield::gap_kind): every
-// pair in the reference form (all provably near, or no far-pair factors),
-// every pair in the far form (all provably far), or per pair.
-enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2 };
+// How a tile's pairs are evaluated: every pair in the reference form (no
+// far-pair factors: the exact and ordered fields, the ensemble), every pair
+// in the far form (all provably far by sdfield::gap_kind), or per pair by
+// sdfield::mixed_pair (every other tile of the fast field): the far form for
+// a far pair, for a near one the reference's minimum image plus the far
+// images from the same factors.
+//
+// A far tile whose x-differences s = w_i - w_j all lie strictly inside
+// (0, 1/2) (kFarPos) or all inside (-1/2, 0) (kFarNeg) -- no wrap, one
+// sign: most far tiles of the curve order -- has dx = s, and the prelude's
+// e^{2 pi |dx|}, e^{-2 pi |dx|} are the products e^{+-2 pi w_i} e^{-+2 pi w_j}
+// it would select, formed directly: the same values bit for bit.
+enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2, kFarPos = 3, kFarNeg = 4 };
 
 __device__ __forceinline__ double rotate_down1(double v, int lane)
 {
@@ -183,11 +192,31 @@ __device__ __forceinline__ int tile_kind(double tw, double sw, double ty, double
     return __builtin_amdgcn_readfirstlane(sdfield::gap_kind(xmin, xmax, ymin, ymax));
 }
 
+// The common sign of the x-differences s = w_i - w_j over a tile when every
+// one lies inside (0, 1/2) (+1) or inside (-1/2, 0) (-1) with a margin of
+// 1e-12 for the rounding of s, else 0; from the extremes of the wrapped x
+// of the target lanes (tw) and the source slots (sw), NaN outside the
+// system (lane 0 is inside in every tile).  The same on every lane.
+__device__ __forceinline__ int tile_sign(double tw, double sw)
+{
+    double tmin = tw, tmax = tw, smin = sw, smax = sw;
+    for (int off = 32; off > 0; off >>= 1) {
+        tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
+        tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
+        smin = fmin(smin, __shfl_xor(smin, off, 64));
+        smax = fmax(smax, __shfl_xor(smax, off, 64));
+    }
+    const double lo = tmin - smax, hi = tmax - smin;
+    const int sg = (lo > 1e-12 && hi < 0.5 - 1e-12) ? 1 : ((hi < -1e-12 && lo > -0.5 + 1e-12) ? -1 : 0);
+    return __builtin_amdgcn_readfirstlane(sg);
+}
+
 // One pair of a far or mixed tile: the far form when its gap
 // cosh(2 pi dx) - cos(2 pi dy) (far_gap) is at least kFarMinGap (always in a
-// far tile), else the reference form.  The choice is symmetric in the two
+// far tile), else the reference's minimum image plus the far images from the
+// factors (sdfield::mixed_pair).  The choice is symmetric in the two
 // dislocations bit for bit (the prelude's t and ti and cos_diff's commuted
-// products are).  r2 is set for the reference form only (far pairs are never
+// products are).  r2 is set for near pairs only (far pairs are never
 // tolerance candidates: their r^2 > kFarMinR2 exceeds every r2_keep the
 // caller passes on this path).
 template <int Mode, bool Perm, int Kind>
@@ -197,15 +226,28 @@ __device__ __forceinline__ double tile_term(double xi, double yi, int io, int jm
 {
     if constexpr (Kind == kNearTile) {
         return pair_term<Mode, Perm>(xi, yi, io, jm, L.x[m], L.y[m], lower_by_lane, r2, sct, e2t);
+    } else if constexpr (Kind == kFarPos || Kind == kFarNeg) {
+        // t = e^{2 pi |s|}: e^{2 pi w_i} e^{-2 pi w_j} for s > 0, the mirror
+        // product for s < 0 (the far prelude's choices, same operands)
+        const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
+        const double u = fi.ep * L.em[m], v = fi.em * L.ep[m];
+        r2 = 1.0;
+        return Kind == kFarPos ? sdfield::xy_far(fi.w - L.w[m], cy, u, v)
+                               : sdfield::xy_far(fi.w - L.w[m], cy, v, u);
     } else {
         const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
         const sdfield::FarPrelude f = sdfield::far_prelude(fi.w, L.w[m], fi.ep, fi.em, L.ep[m], L.em[m]);
-        const bool far = Kind == kFarTile || sdfield::far_gap(f, cy) >= sdfield::kFarMinGap;
-        if (far) {
+        if constexpr (Kind == kFarTile) {
             r2 = 1.0;
             return sdfield::xy_far(f.dx, cy, f.t, f.ti);
+        } else {
+            // the fast field only (the far-pair factors exist for it alone):
+            // odd bit for bit, so target minus source serves every pair
+            static_assert(Mode == SDDDST_FIELD_FAST, "mixed tiles are a fast-field form");
+            bool far;
+            return sdfield::mixed_pair(f, cy, sdfield::wrap_fast(xi - L.x[m]), sdfield::wrap_fast(yi - L.y[m]), far,
+                                       r2, sct, e2t);
         }
-        return pair_term<Mode, Perm>(xi, yi, io, jm, L.x[m], L.y[m], lower_by_lane, r2, sct, e2t);
     }
 }
 
@@ -247,9 +289,15 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
             const double nan = __builtin_nan("");
             kind = tile_kind(p < n ? fd.w[io] : nan, q < n ? fd.w[jo] : nan, p < n ? fd.yw[io] : nan,
                              q < n ? fd.yw[jo] : nan);
-            if (kind != kNearTile) {
-                far_stage(L, l, q < n ? far_load(fd, jo) : FarSelf());
-                if (p < n) fi = far_load(fd, io);
+            // every pair of a tile that is not all far takes the mixed form
+            // (so a near pair's value does not depend on its tile's kind)
+            if (kind == kNearTile) kind = kMixedTile;
+            const FarSelf src = q < n ? far_load(fd, jo) : FarSelf();
+            far_stage(L, l, src);
+            if (p < n) fi = far_load(fd, io);
+            if (kind == kFarTile) {
+                const int sgn = tile_sign(p < n ? fi.w : nan, q < n ? src.w : nan);
+                kind = sgn > 0 ? kFarPos : (sgn < 0 ? kFarNeg : kFarTile);
             }
         }
     }
@@ -273,7 +321,7 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
                 // column sum of source j gets -b_i u here and b_j below
                 row = __builtin_fma(sb[m], u, row);
                 col = __builtin_fma(-bI, u, col);
-                if (K != kFarTile && r2 < r2_keep) {
+                if ((K == kNearTile || K == kMixedTile) && r2 < r2_keep) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(r2);
                     atomicMin(&minr2[io], bits);
                     atomicMin(&minr2[Perm ? sid[m] : j], bits);
@@ -290,7 +338,9 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
         }
     };
     if constexpr (Mode == SDDDST_FIELD_FAST) {
-        if (kind == kFarTile) sweep(std::integral_constant<int, kFarTile>{});
+        if (kind == kFarPos) sweep(std::integral_constant<int, kFarPos>{});
+        else if (kind == kFarNeg) sweep(std::integral_constant<int, kFarNeg>{});
+        else if (kind == kFarTile) sweep(std::integral_constant<int, kFarTile>{});
         else if (kind == kMixedTile) sweep(std::integral_constant<int, kMixedTile>{});
         else sweep(std::integral_constant<int, kNearTile>{});
     } else {
@@ -335,19 +385,16 @@ __device__ __forceinline__ void diag_tile(TileLds &L, int bk, int n, const doubl
     sy[l] = yi;
     sb[l] = bI;
     sid[l] = io;
-    // a block against itself: never all far (its intervals overlap
-    // themselves); mixed when its members spread far enough apart
+    // a block against itself is never all far (its intervals overlap
+    // themselves): with far-pair factors every pair takes the mixed form, as
+    // in the off-diagonal tiles that are not all far
     int kind = kNearTile;
     FarSelf fi;
     if constexpr (Mode == SDDDST_FIELD_FAST) {
         if (fd.w) {
-            const double wv = p < n ? fd.w[io] : __builtin_nan("");
-            const double yv = p < n ? fd.yw[io] : __builtin_nan("");
-            kind = tile_kind(wv, wv, yv, yv);
-            if (kind != kNearTile) {
-                if (p < n) fi = far_load(fd, io);
-                far_stage(L, l, fi);
-            }
+            kind = kMixedTile;
+            if (p < n) fi = far_load(fd, io);
+            far_stage(L, l, fi);
         }
     }
     __syncthreads();

@@ -35,6 +35,7 @@ void mc_batch1(int fn, long n, const double *x, double *out)
         case 6: r = sdfield::normalize(v); break;
         case 9: r = sdlibm::pow_e(v); break;
         case 10: r = sdlibm::pow_m2(v); break;
+        case 11: r = sdlibm::cos_r1(v); break;
         }
         out[i] = r;
     }
@@ -52,6 +53,27 @@ void mc_batch_pair_far(long n, const double *xi, const double *yi, const double 
         sdfield::xexp(xi[k], wi, epi, emi);
         sdfield::xexp(xj[k], wj, epj, emj);
         out[k] = sdfield::far_pair(wi, wj, epi, emi, epj, emj, sdfield::cos_diff(ci, si, cj, sj));
+    }
+}
+
+// The pair kernel's form for the pairs of its mixed tiles (far pairs: the
+// far form; near pairs: the reference's minimum image plus the far images
+// from the per-dislocation factors), with the far decision in far_out.
+void mc_batch_pair_mixed(long n, const double *xi, const double *yi, const double *xj, const double *yj,
+                         double *out, int *far_out)
+{
+    for (long k = 0; k < n; ++k) {
+        double ci, si, cj, sj, wi, epi, emi, wj, epj, emj;
+        sdfield::ytrig(yi[k], ci, si);
+        sdfield::ytrig(yj[k], cj, sj);
+        sdfield::xexp(xi[k], wi, epi, emi);
+        sdfield::xexp(xj[k], wj, epj, emj);
+        const sdfield::FarPrelude f = sdfield::far_prelude(wi, wj, epi, emi, epj, emj);
+        bool far;
+        double r2;
+        out[k] = sdfield::mixed_pair(f, sdfield::cos_diff(ci, si, cj, sj), sdfield::wrap_fast(xi[k] - xj[k]),
+                                     sdfield::wrap_fast(yi[k] - yj[k]), far, r2, sdlibm_sincos_tab, sdlibm_exp2_tab);
+        far_out[k] = far ? 1 : 0;
     }
 }
 

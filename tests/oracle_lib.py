@@ -145,6 +145,7 @@ def mathcheck():
     lib.mc_batch2.argtypes = [ctypes.c_int, L, DP, DP, DP]
     lib.mc_batch_pair_far.argtypes = [L, DP, DP, DP, DP, DP]
     lib.mc_batch_pair_gap.argtypes = [L, DP, DP, DP, DP, DP]
+    lib.mc_batch_pair_mixed.argtypes = [L, DP, DP, DP, DP, DP, IP]
     lib.mc_tile_kind.argtypes = [DP, DP, DP, DP]
     lib.mc_tile_kind.restype = ctypes.c_int
     lib.mc_normalize.restype = D

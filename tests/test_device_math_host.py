@@ -36,6 +36,24 @@ def test_host_build_matches_glibc(name, lo, hi):
     assert mism <= (2 if name == "sin" else 0)
 
 
+def test_cos_first_range_matches_glibc():
+    # cos_r1, glibc's first cos range alone (the near pairs' |2 pi dy| <
+    # 0.6436), bit for bit with glibc's cos below 0.855469, at random
+    # arguments, near the range's end and below 2^-27
+    rng = np.random.default_rng(21)
+    t = 0.855469
+    x = np.concatenate([rng.uniform(-t, t, 400_000), rng.uniform(-0.05, 0.05, 100_000),
+                        np.nextafter(t, 0) - np.arange(4000) * np.spacing(t),
+                        2.0 ** -27 + np.arange(-2000, 2000) * np.spacing(2.0 ** -27), [0.0, -0.0]])
+    x = np.concatenate([x, -x])
+    x = x[np.abs(x) < float.fromhex("0x1.b6p-1")]    # hiword below 0x3feb6000
+    got = _mc1(11, x)
+    want = O.libm_batch(0, x)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+    # outside the range the lookup stays in the table (clamped), finite
+    assert np.all(np.isfinite(_mc1(11, np.linspace(-4.0, 4.0, 10_001))))
+
+
 def test_thresholds_of_the_branches():
     # dense sampling around the branch points of cosh / sinh / expm1 / cos
     pts = []
@@ -184,6 +202,58 @@ def test_far_pair_term_by_gap():
     assert err[sel].max() < 3.5e-14
     assert err[sel].mean() < 1e-15
     assert np.array_equal(out[sel].view(np.int64), (-swp[sel]).view(np.int64))
+
+
+def test_mixed_pair_term():
+    # the pairs of the pair kernel's mixed tiles: a far pair (gap >= 0.2)
+    # gets the far form's value bit for bit; a near pair the reference's
+    # minimum image (same operations as the reference, so bit-identical when
+    # the far images vanish) plus the nine far images from the
+    # per-dislocation factors, within 4e-15 of the reference's term relative
+    # to |tau| + 1, down to the Taylor branch (r^2 <= 1e-6) and the exact
+    # zero; odd under swapping the two dislocations bit for bit
+    mc = O.mathcheck()
+    rng = np.random.default_rng(14)
+    n = 400_000
+    xi, yi = rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n)
+    r = 10.0 ** rng.uniform(-5, np.log10(0.3), n)
+    th = rng.uniform(0.0, 2.0 * np.pi, n)
+    ddx, ddy = r * np.cos(th), r * np.sin(th)
+    ddx[: n // 16] = 0.0
+    ddy[n // 16: n // 8] = 0.0
+    xj, yj = xi - ddx, yi - ddy
+    xi[: n // 4] += rng.integers(-3, 4, n // 4)
+    yj[n // 4: n // 2] += rng.integers(-3, 4, n // 4)
+    out, swp, far_ref = np.empty(n), np.empty(n), np.empty(n)
+    isfar, isfar_sw = np.empty(n, np.int32), np.empty(n, np.int32)
+    mc.mc_batch_pair_mixed(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(out), O.ip(isfar))
+    mc.mc_batch_pair_mixed(n, O.dp(xj), O.dp(yj), O.dp(xi), O.dp(yi), O.dp(swp), O.ip(isfar_sw))
+    mc.mc_batch_pair_far(n, O.dp(xi), O.dp(yi), O.dp(xj), O.dp(yj), O.dp(far_ref))
+    assert np.array_equal(isfar, isfar_sw)
+    far = isfar.astype(bool)
+    assert 0.05 < far.mean() < 0.8
+    assert np.array_equal(out[far].view(np.int64), far_ref[far].view(np.int64))
+    dx, dy = np.empty(n), np.empty(n)
+    ex, ey = xi - xj, yi - yj
+    mc.mc_batch1(6, n, O.dp(ex), O.dp(dx))
+    mc.mc_batch1(6, n, O.dp(ey), O.dp(dy))
+    # at dx = 0 exactly (here: x shifted by whole periods) the factors' dx
+    # is a rounding residue of either sign, as in the far form
+    nz = (out != 0.0) & (dx != 0.0)
+    assert np.array_equal(out[nz].view(np.int64), (-swp[nz]).view(np.int64))
+    assert np.all(swp[out == 0.0] == 0.0)
+    assert np.max(np.abs(out[(dx == 0.0) & ~far])) < 1e-15
+    ref, _ = O.xy_batch(dx, dy)
+    near = ~far
+    assert (near & (dx * dx + dy * dy <= 1e-6)).sum() > 10_000
+    err = np.abs(out - ref) / (np.abs(ref) + 1.0)
+    assert err[near].max() < 4e-15
+    assert err[near].mean() < 2e-16
+    # the exact near image dominates: on the close pairs the term is the
+    # reference's to a few ulps of itself
+    close = near & (np.abs(ref) > 1e3)
+    assert close.sum() > 50_000
+    assert np.max(np.abs(out[close] - ref[close]) / np.abs(ref[close])) < 1e-15
 
 
 def _tile(rng, cx, cy, spread, count):

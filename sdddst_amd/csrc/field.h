@@ -242,8 +242,10 @@ SDLIBM_FN FarParts far_image_parts(double dx, double cy, double t, double ti)
     const double A = __builtin_fma(t, hp, ti * hm);
     const double B = __builtin_fma(ti, hp, t * hm);
     const double dA = A - cy, dB = B - cy;
-    const double nA = (ad + 1.0) * __builtin_fma(A, cy, -1.0);
-    const double nB = (ad - 1.0) * __builtin_fma(B, cy, -1.0);
+    // (ad + 1) X and (ad - 1) Y as single roundings
+    const double X = __builtin_fma(A, cy, -1.0), Y = __builtin_fma(B, cy, -1.0);
+    const double nA = __builtin_fma(ad, X, X);
+    const double nB = __builtin_fma(ad, Y, -Y);
     const double QA = dA * dA, QB = dB * dB;
     const double P = __builtin_fma(nB, QA, nA * QB);
     const double Q = QA * QB;
@@ -278,15 +280,14 @@ SDLIBM_FN FarParts far_image_parts(double dx, double cy, double t, double ti)
     const double ap2 = ad + 2.0, am2 = ad - 2.0;
     const double third = __builtin_fma(ti3, ap2, t3 * am2);
     const double fourth = __builtin_fma(ti4, ap2, t4 * am2);
+    // sum_{n=1..4} a_n T_n(cy) by Clenshaw's recurrence (a_1 = first,
+    // a_2 = second, a_3 = C3 third, a_4 = C4 fourth)
     const double cy2 = 2.0 * cy;
-    const double T2 = __builtin_fma(cy2, cy, -1.0);
-    const double T3 = __builtin_fma(cy2, T2, -cy);
-    const double T4 = __builtin_fma(cy2, T3, -T2);
-    // smallest terms first
-    double S = (C4 * T4) * fourth;
-    S = __builtin_fma(C3 * T3, third, S);
-    S = __builtin_fma(T2, second, S);
-    S = __builtin_fma(cy, first, S);
+    const double b4 = C4 * fourth;
+    const double b3 = __builtin_fma(cy2, b4, C3 * third);
+    const double b2 = __builtin_fma(cy2, b3, second - b4);
+    const double b1 = __builtin_fma(cy2, b2, first - b3);
+    const double S = __builtin_fma(cy, b1, -b2);
     return FarParts{P, Q, S};
 }
 
@@ -340,7 +341,7 @@ SDLIBM_FN double xy_far(double dx, double cy, double t, double ti)
     const double d = c0 - cy;
     // every term is odd in dx: evaluated at |dx|, the sign applied with the
     // constant factor (exact: each step's rounding is symmetric in sign)
-    const double A = sdlibm::fabsd(dx) * (c0 * cy - 1.0), B = d * d;
+    const double A = sdlibm::fabsd(dx) * __builtin_fma(c0, cy, -1.0), B = d * d;
     const FarParts f = far_image_parts(dx, cy, t, ti);
     return (sdlibm::div_newton(__builtin_fma(A, f.Q, f.P * B), B * f.Q) + f.S) *
            __builtin_copysign(2.0 * kPiSq, dx);
@@ -481,7 +482,7 @@ SDLIBM_FN double mixed_pair(const FarPrelude &f, double cy, double dxw, double d
     const double d = c0 - cy;
     const double B = d * d;
     const FarParts p = far_image_parts(f.dx, cy, f.t, f.ti);
-    double num = __builtin_fma(sdlibm::fabsd(f.dx) * (c0 * cy - 1.0), p.Q, p.P * B), den = B * p.Q;
+    double num = __builtin_fma(sdlibm::fabsd(f.dx) * __builtin_fma(c0, cy, -1.0), p.Q, p.P * B), den = B * p.Q;
     double near = 0.0;
     far = d >= kFarMinGap;
     r2 = 1.0;

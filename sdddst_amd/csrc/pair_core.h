@@ -76,8 +76,11 @@ struct FarData {
 // (0, 1/2) (kFarPos) or all inside (-1/2, 0) (kFarNeg) -- no wrap, one
 // sign: most far tiles of the curve order -- has dx = s, and the prelude's
 // e^{2 pi |dx|}, e^{-2 pi |dx|} are the products e^{+-2 pi w_i} e^{-+2 pi w_j}
-// it would select, formed directly: the same values bit for bit.
-enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2, kFarPos = 3, kFarNeg = 4 };
+// it would select, formed directly: the same values bit for bit.  Inside
+// (1/2, 1) (kFarWrapPos: dx = s - 1) or (-1, -1/2) (kFarWrapNeg: dx = s + 1)
+// the prelude's choice is the mirror product times e^{+-2 pi}, likewise.
+enum TileKind { kNearTile = 0, kFarTile = 1, kMixedTile = 2, kFarPos = 3, kFarNeg = 4, kFarWrapPos = 5,
+                kFarWrapNeg = 6 };
 
 __device__ __forceinline__ double rotate_down1(double v, int lane)
 {
@@ -192,12 +195,13 @@ __device__ __forceinline__ int tile_kind(double tw, double sw, double ty, double
     return __builtin_amdgcn_readfirstlane(sdfield::gap_kind(xmin, xmax, ymin, ymax));
 }
 
-// The common sign of the x-differences s = w_i - w_j over a tile when every
-// one lies inside (0, 1/2) (+1) or inside (-1/2, 0) (-1) with a margin of
-// 1e-12 for the rounding of s, else 0; from the extremes of the wrapped x
-// of the target lanes (tw) and the source slots (sw), NaN outside the
-// system (lane 0 is inside in every tile).  The same on every lane.
-__device__ __forceinline__ int tile_sign(double tw, double sw)
+// The interval that holds every x-difference s = w_i - w_j of a tile, with
+// a margin of 1e-12 for the rounding of s: (0, 1/2) -> kFarPos,
+// (-1/2, 0) -> kFarNeg, (1/2, 1) -> kFarWrapPos, (-1, -1/2) -> kFarWrapNeg,
+// none of them -> kFarTile; from the extremes of the wrapped x of the target
+// lanes (tw) and the source slots (sw), NaN outside the system (lane 0 is
+// inside in every tile).  The same on every lane.
+__device__ __forceinline__ int far_tile_kind(double tw, double sw)
 {
     double tmin = tw, tmax = tw, smin = sw, smax = sw;
     for (int off = 32; off > 0; off >>= 1) {
@@ -206,9 +210,13 @@ __device__ __forceinline__ int tile_sign(double tw, double sw)
         smin = fmin(smin, __shfl_xor(smin, off, 64));
         smax = fmax(smax, __shfl_xor(smax, off, 64));
     }
-    const double lo = tmin - smax, hi = tmax - smin;
-    const int sg = (lo > 1e-12 && hi < 0.5 - 1e-12) ? 1 : ((hi < -1e-12 && lo > -0.5 + 1e-12) ? -1 : 0);
-    return __builtin_amdgcn_readfirstlane(sg);
+    const double lo = tmin - smax, hi = tmax - smin, e = 1e-12;
+    int k = kFarTile;
+    if (lo > e && hi < 0.5 - e) k = kFarPos;
+    else if (hi < -e && lo > -0.5 + e) k = kFarNeg;
+    else if (lo > 0.5 + e && hi < 1.0 - e) k = kFarWrapPos;
+    else if (hi < -0.5 - e && lo > -1.0 + e) k = kFarWrapNeg;
+    return __builtin_amdgcn_readfirstlane(k);
 }
 
 // One pair of a far or mixed tile: the far form when its gap
@@ -234,6 +242,16 @@ __device__ __forceinline__ double tile_term(double xi, double yi, int io, int jm
         r2 = 1.0;
         return Kind == kFarPos ? sdfield::xy_far(fi.w - L.w[m], cy, u, v)
                                : sdfield::xy_far(fi.w - L.w[m], cy, v, u);
+    } else if constexpr (Kind == kFarWrapPos || Kind == kFarWrapNeg) {
+        // dx = s -+ 1: t = e^{2 pi (1 -+ s)} is the mirror product times
+        // e^{2 pi}, ti the product times e^{-2 pi} (as the prelude forms them)
+        const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
+        const double u = fi.ep * L.em[m], v = fi.em * L.ep[m];
+        const double s = fi.w - L.w[m];
+        r2 = 1.0;
+        return Kind == kFarWrapPos
+                   ? sdfield::xy_far(s - 1.0, cy, v * sdlibm_exp_2pik[1], u * sdlibm_exp_m2pik[1])
+                   : sdfield::xy_far(s + 1.0, cy, u * sdlibm_exp_2pik[1], v * sdlibm_exp_m2pik[1]);
     } else {
         const double cy = sdfield::cos_diff(fi.ca, fi.sa, L.ca[m], L.sa[m]);
         const sdfield::FarPrelude f = sdfield::far_prelude(fi.w, L.w[m], fi.ep, fi.em, L.ep[m], L.em[m]);
@@ -296,8 +314,7 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
             far_stage(L, l, src);
             if (p < n) fi = far_load(fd, io);
             if (kind == kFarTile) {
-                const int sgn = tile_sign(p < n ? fi.w : nan, q < n ? src.w : nan);
-                kind = sgn > 0 ? kFarPos : (sgn < 0 ? kFarNeg : kFarTile);
+                kind = far_tile_kind(p < n ? fi.w : nan, q < n ? src.w : nan);
             }
         }
     }
@@ -340,6 +357,8 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
     if constexpr (Mode == SDDDST_FIELD_FAST) {
         if (kind == kFarPos) sweep(std::integral_constant<int, kFarPos>{});
         else if (kind == kFarNeg) sweep(std::integral_constant<int, kFarNeg>{});
+        else if (kind == kFarWrapPos) sweep(std::integral_constant<int, kFarWrapPos>{});
+        else if (kind == kFarWrapNeg) sweep(std::integral_constant<int, kFarWrapNeg>{});
         else if (kind == kFarTile) sweep(std::integral_constant<int, kFarTile>{});
         else if (kind == kMixedTile) sweep(std::integral_constant<int, kMixedTile>{});
         else sweep(std::integral_constant<int, kNearTile>{});

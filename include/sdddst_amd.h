@@ -196,6 +196,14 @@ int sdddst_solver_history(sdddst_ctx *ctx, double *out, int max_records, int *co
  * solve returns SDDDST_ERR_NOT_CONVERGED (sdddstCore::Simulation then
  * rejects the attempt and divides the step size by 8).  Default 32768. */
 int sdddst_solver_set_direct_limit(sdddst_ctx *ctx, int max_n);
+/* With SDDDST_PRECOND_LEARNED: the most columns whose scaled diagonal
+ * differs from 1 by more than 0.25 a Newton matrix may have for the
+ * network to precondition it; above that (a step size large for the
+ * configuration, where the pair blocks alone converge faster) the factor
+ * keeps the pair blocks only.  -1: the network always; -2 (default):
+ * max(16, N / 256).  The solver history records the preconditioner that
+ * served each solve.  No counterpart in the reference. */
+int sdddst_set_learned_gate(sdddst_ctx *ctx, int max_strong_rows);
 
 /* ---- Learned preconditioner (SDDDST_PRECOND_LEARNED) ------------------
  * No counterpart in the reference, which solves A delta = g with UMFPACK

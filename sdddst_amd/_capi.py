@@ -82,6 +82,7 @@ _SIGNATURES = {
     "sdddst_speeds_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "sdddst_fp64_probe": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
     "sdddst_solver_set_direct_limit": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "sdddst_set_learned_gate": (ctypes.c_int, [_vp, ctypes.c_int]),
     "sdddst_spai_param_count": (ctypes.c_int, []),
     "sdddst_spai_set_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
     "sdddst_spai_get_weights": (ctypes.c_int, [_vp, _fp, ctypes.c_int]),
@@ -298,6 +299,11 @@ class Context:
 
     def set_preconditioner(self, precond: int):
         self._check(self.lib.sdddst_set_preconditioner(self.h, precond))
+
+    def set_learned_gate(self, max_strong_rows: int):
+        """Most strongly shifted diagonals (|A_jj - 1| > 0.25) a matrix may
+        have for the learned preconditioner (-1: always; -2: max(16, N/256))."""
+        self._check(self.lib.sdddst_set_learned_gate(self.h, max_strong_rows))
 
     def solver_factor(self):
         self._check(self.lib.sdddst_solver_factor(self.h))

@@ -498,6 +498,14 @@ int sdddst_solver_set_direct_limit(sdddst_ctx *ctx, int max_n)
     return SDDDST_OK;
 }
 
+int sdddst_set_learned_gate(sdddst_ctx *ctx, int max_strong_rows)
+{
+    CTX_CHECK(ctx);
+    SD_CHECK(max_strong_rows >= -2, SDDDST_ERR_ARG, "learned gate %d below -2", max_strong_rows);
+    ctx->c.learned_gate = max_strong_rows;
+    return SDDDST_OK;
+}
+
 int sdddst_spai_param_count(void) { return spai_param_count(); }
 
 int sdddst_spai_set_weights(sdddst_ctx *ctx, const float *w, int count)

@@ -58,6 +58,11 @@ static_assert(sizeof(DevScalars) % 8 == 0 && sizeof(DevScalars) <= 14 * 8, "DevS
 
 constexpr int kTimePair = 1, kTimeSpeeds = 2, kTimeJacobian = 4;
 
+// A column whose scaled diagonal differs from 1 by more than this is
+// "strongly shifted": the Newton matrix is then far from the identity there
+// (a close pair or a pinned dislocation at a large step size).
+constexpr double kStrongDiag = 0.25;
+
 struct Context {
     int n = 0, pc = 0;
     sdddst_params p{};
@@ -123,7 +128,11 @@ struct Context {
     int *partner = nullptr;
     double *strength = nullptr;
     bool partner_valid = false;
-    int *d_flags = nullptr;             // device flags: [0] zero-valued upper entries found
+    int *d_flags = nullptr;             // device flags: [0] zero-valued upper entries found,
+                                        // [2..3] entry count, [4] strongly shifted diagonals
+    // columns with |A_jj - 1| > kStrongDiag in the last full assembly, and its h
+    int diag_strong = 0;
+    double diag_strong_h = 0.0;
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     // controller scalars
@@ -137,6 +146,12 @@ struct Context {
     double solver_tol = 1e-10;
     int solver_max_iter = 200, solver_restart = 30;
     int precond = SDDDST_PRECOND_PAIR;
+    // the preconditioner the current factorisation serves: precond, except
+    // that the learned one gives way to the pair blocks on a matrix with more
+    // strongly shifted diagonals than learned_gate (-1: never; -2: the
+    // default max(16, N / 256))
+    int precond_active = SDDDST_PRECOND_PAIR;
+    int learned_gate = -2;
     int precond_override = -1;          // >= 0: preconditioner of the current retry
     int precond_fallbacks = 0;          // learned solves retried with the pair blocks
     struct Spai *spai = nullptr;       // learned preconditioner state

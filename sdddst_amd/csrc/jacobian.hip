@@ -338,7 +338,8 @@ __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *
                                                     const int *__restrict__ diag, const double *__restrict__ pdtmp,
                                                     double *__restrict__ Araw, double *__restrict__ sraw,
                                                     double *__restrict__ dvec, double2 *__restrict__ colinfo,
-                                                    const long long *__restrict__ total, long long cap)
+                                                    const long long *__restrict__ total, long long cap,
+                                                    int *__restrict__ strong)
 {
     __shared__ double buf[4][kSumChunk];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -391,6 +392,10 @@ __global__ __launch_bounds__(256) void k_jac_rawsum(int n, double h, const int *
     dvec[j] = dv;
     const double Dj = (1.0 + dv) * 0.5;
     colinfo[j] = make_double2(Dj, __builtin_fabs(subSum * Dj + 1.0));
+    // a diagonal far from 1 marks a matrix far from the identity (a step
+    // size large for the configuration), counted for the preconditioner
+    // choice
+    if (__builtin_fabs(subSum * Dj) > kStrongDiag) atomicAdd(strong, 1);
 }
 
 // d_j for the column sums s * factor (factor a power of two: the unscaled
@@ -495,6 +500,15 @@ __global__ void k_fill_int(int *p, int v, int n)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) p[i] = v;
+}
+
+// the zero-value flag and the strong-diagonal count of the assembly flags
+__global__ void k_flags_reset(int *f)
+{
+    if (threadIdx.x == 0) {
+        f[0] = 0;
+        f[4] = 0;
+    }
 }
 
 }  // namespace
@@ -636,7 +650,7 @@ static int build_pattern(Context &c, int slot, double h)
     hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag,
                        c.jwstart, d_total, (long long)c.cap);
     SD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
+    hipLaunchKernelGGL(k_flags_reset, dim3(1), dim3(64), 0, c.stream, c.d_flags);
     // tau_x, the taper and the mirror positions are kept only when a later
     // call needs them (ensure_aux): the step never does, and the three arrays
     // were a fifth of the values pass's stores
@@ -717,7 +731,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             // the pattern is reused with another step size: the unsigned
             // values again, from tau_x and the taper (with the zero flag)
             if ((rc = ensure_aux(c, slot))) return rc;
-            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, c.d_flags, 0, 1);
+            hipLaunchKernelGGL(k_flags_reset, dim3(1), dim3(64), 0, c.stream, c.d_flags);
             hipLaunchKernelGGL(k_jac_raw, dim3(wblocks), dim3(256), 0, c.stream, n, c.b, h, c.Ap, c.Ai, c.diag,
                                c.jtau, c.jmult, c.jmirror, c.pc > 0 ? c.jpd : nullptr, c.Araw, c.d_flags);
             SD_HIP(hipGetLastError());
@@ -725,13 +739,20 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
         }
         hipLaunchKernelGGL(k_jac_rawsum, dim3(wblocks), dim3(256), 0, c.stream, n, h, c.Ap, c.diag,
                            c.pc > 0 ? c.jpd : nullptr, c.Araw, c.sraw, c.dvec, c.colinfo, jac_total(c),
-                           (long long)c.cap);
+                           (long long)c.cap, c.d_flags + 4);
         SD_HIP(hipGetLastError());
         // d_flags[0..1] as one word ([0]: the zero-value flag), then the
-        // entry count of the last build
-        long long words2[2] = {0, 0};
-        if ((rc = host_wait(c, c.d_flags, 2, words2))) return rc;
+        // entry count of the last build, then [4]: the strongly shifted
+        // diagonals
+        long long words2[3] = {0, 0, 0};
+        if ((rc = host_wait(c, c.d_flags, 3, words2))) return rc;
         const long long total = words2[1];
+        {
+            int w4[2];
+            std::memcpy(w4, &words2[2], sizeof w4);
+            c.diag_strong = w4[0];
+            c.diag_strong_h = h;
+        }
         SD_CHECK(total < (1LL << 31) - 1, SDDDST_ERR_ARG, "nnz %lld exceeds the int32 index range", total);
         if (total > c.cap) {
             // the pattern outgrew the storage: grow it and assemble again

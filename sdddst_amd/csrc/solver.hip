@@ -40,9 +40,9 @@ int solver_solve(Context &c)
         int rc = c.gm->solve(c);
         c.total_iters += c.last_iters;
         c.total_solves += 1;
-        c.log_solve(c.precond, c.last_iters, c.last_rel_res, c.last_gnorm, rc == SDDDST_OK ? 0 : 1);
-        const int pair = c.precond == SDDDST_PRECOND_JACOBI ? SDDDST_PRECOND_JACOBI : SDDDST_PRECOND_PAIR;
-        if (rc == SDDDST_ERR_SOLVER && c.precond == SDDDST_PRECOND_LEARNED) {
+        c.log_solve(c.precond_active, c.last_iters, c.last_rel_res, c.last_gnorm, rc == SDDDST_OK ? 0 : 1);
+        const int pair = c.precond_active == SDDDST_PRECOND_JACOBI ? SDDDST_PRECOND_JACOBI : SDDDST_PRECOND_PAIR;
+        if (rc == SDDDST_ERR_SOLVER && c.precond_active == SDDDST_PRECOND_LEARNED) {
             // the pair blocks are part of the learned preconditioner's factor
             c.precond_fallbacks++;
             c.precond_override = pair;
@@ -51,7 +51,7 @@ int solver_solve(Context &c)
             c.total_iters += c.last_iters;
             c.log_solve(pair, c.last_iters, c.last_rel_res, c.last_gnorm, rc == SDDDST_OK ? 0 : 1);
         }
-        if (rc == SDDDST_ERR_SOLVER && c.precond != SDDDST_PRECOND_STRONG) {
+        if (rc == SDDDST_ERR_SOLVER && c.precond_active != SDDDST_PRECOND_STRONG) {
             // the pair blocks stagnate on indefinite systems (a step size
             // far too large for the configuration): once more with the
             // strong-row block, factored once for this matrix, before the

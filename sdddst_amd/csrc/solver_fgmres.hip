@@ -533,11 +533,32 @@ int Fgmres::pair_factor(Context &c)
     return SDDDST_OK;
 }
 
+// Whether the learned preconditioner serves the current matrix.  The
+// network was trained on the Newton systems of running simulations, close to
+// the identity; at a step size large for the configuration (the first
+// attempts from h0 = 1e-6 at N = 16384) many diagonals move far from 1 and
+// the pair blocks alone take fewer iterations (measured at N = 16384, m = 2:
+// learned 40 without converging / 30 / 20 / 10 iterations against 37 / 24 /
+// 18 / 9 for the pair blocks at h = 1e-6 / 5e-7 / 2.5e-7 / 1e-7, with 1034 /
+// 490 / 248 / 90 strongly shifted diagonals; 6 against 8 at 5e-8 with 39, 3
+// against 5 at 1e-9 with 2).  The count is the last full assembly's; a reuse
+// of its values at another step size (stage II, h / 2) scales it by the
+// step-size ratio, as every shift scales with h.
+static bool learned_serves(const Context &c)
+{
+    if (c.learned_gate == -1) return true;
+    const int gate = c.learned_gate >= 0 ? c.learned_gate : std::max(16, c.n / 256);
+    double count = (double)c.diag_strong;
+    if (c.diag_strong_h > 0.0 && c.jac_h != c.diag_strong_h) count *= c.jac_h / c.diag_strong_h;
+    return count <= (double)gate;
+}
+
 int Fgmres::factor(Context &c)
 {
     int rc = init(c);
     if (rc) return rc;
     strong_valid = false;
+    c.precond_active = c.precond;
     if (c.precond == SDDDST_PRECOND_STRONG) return strong_factor(c);
     if (c.precond == SDDDST_PRECOND_JACOBI) {
         hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
@@ -545,13 +566,19 @@ int Fgmres::factor(Context &c)
         return SDDDST_OK;
     }
     if ((rc = pair_factor(c))) return rc;
-    if (c.precond == SDDDST_PRECOND_LEARNED) return spai_factor(c, *this);
+    if (c.precond == SDDDST_PRECOND_LEARNED) {
+        if (!learned_serves(c)) {
+            c.precond_active = SDDDST_PRECOND_PAIR;
+            return SDDDST_OK;
+        }
+        return spai_factor(c, *this);
+    }
     return SDDDST_OK;
 }
 
 int Fgmres::apply(Context &c, const double *v, double *z)
 {
-    const int precond = c.precond_override >= 0 ? c.precond_override : c.precond;
+    const int precond = c.precond_override >= 0 ? c.precond_override : c.precond_active;
     if (precond == SDDDST_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_jacobi_apply, dim3(nb256(n)), dim3(256), 0, c.stream, n, dinv, v, z);
     else if (precond == SDDDST_PRECOND_LEARNED)
@@ -607,7 +634,7 @@ int Fgmres::solve(Context &c)
     const double tol = c.solver_tol;
     // a learned-preconditioned solve gets one restart cycle; the caller
     // retries with the pair blocks when it does not converge in it
-    const int precond = c.precond_override >= 0 ? c.precond_override : c.precond;
+    const int precond = c.precond_override >= 0 ? c.precond_override : c.precond_active;
     const int max_it = precond == SDDDST_PRECOND_LEARNED ? std::min(c.solver_max_iter, m) : c.solver_max_iter;
     int rc;
     // delta = 0, r = g: ||g||^2 into the device slot gs, v_0 = g / ||g||

@@ -2,6 +2,7 @@
 (oracle/spai_gnn.py): core graph and pair-block inverse bit-exact, network
 outputs, loss and weight gradient within FP32 tolerances, training lowers
 the loss, and FGMRES with the learned preconditioner solves to 1e-10."""
+import ctypes
 import os
 import sys
 
@@ -82,6 +83,7 @@ def test_learned_fgmres_matches_direct_solve():
             c.spai_set_weights(th)
             c.solver_select(solver, 1e-10, 300, 40)
             c.set_preconditioner(precond)
+            c.set_learned_gate(-1)     # the network itself, whatever the matrix
             g = c.residual(h, 0, 0, S.V_INIT, S.V_INIT2)
             c.solver_factor()
             d = c.solver_solve(download=True)
@@ -110,6 +112,7 @@ def _solve_iters(n, m, x, y, b, h, precond, weights):
         c.spai_set_weights(weights)
         c.solver_select(S.SOLVER_FGMRES, 1e-10, 300, 40)
         c.set_preconditioner(precond)
+        c.set_learned_gate(-1)         # the network itself, whatever the matrix
         c.residual(h, 0, 0, S.V_INIT, S.V_INIT2)
         c.solver_factor()
         c.solver_solve()
@@ -146,3 +149,58 @@ def test_trained_network_gains_over_pair_blocks(m, seed):
     print(f"m={m}: FGMRES iterations pair {pair} learned {learned}")
     assert sum(learned) <= 0.85 * sum(pair)
     assert all(l <= p + 1 for l, p in zip(learned, pair))
+
+
+def test_learned_gate_follows_the_shifted_diagonals():
+    """With the learned preconditioner selected, a Newton matrix with more
+    than max(16, N/256) columns whose scaled diagonal is more than 0.25 from
+    1 (a step size large for the configuration) is preconditioned by the pair
+    blocks alone, one with fewer by the network; the count is taken on the
+    downloaded matrix here and the choice read from the solver history (1:
+    pair blocks, 2: learned).  A stage-II style reuse at h / 2 scales the
+    count by 1/2; gate -1 keeps the network at any step size.  Every solve
+    still reaches 1e-10."""
+    from sdddst_amd.spai_train import DEFAULT_WEIGHTS, load_weights
+    w = load_weights(DEFAULT_WEIGHTS)
+    n, m = 4096, 2.0
+    x, y, b = configs.dislocations(n, 9)
+    gate = max(16, n // 256)
+    seen = set()
+    with S.Context(n, cutoff_multiplier=m) as c:
+        c.upload_state(0, x, y, b)
+        c.speeds(0, S.V_INIT, 0.02)
+        c.speeds(0, S.V_INIT2, 0.02)
+        c.spai_set_weights(w)
+        c.solver_select(S.SOLVER_FGMRES, 1e-10, 300, 40)
+        c.set_preconditioner(S.PRECOND_LEARNED)
+        c.set_direct_limit(0)
+        for h in (1e-6, 1e-7, 1e-8, 1e-9):
+            Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h)
+            count = int(np.count_nonzero(np.abs(Ax[dg] - 1.0) > 0.25))
+            for gate_set, want in ((-2, 2 if count <= gate else 1), (-1, 2)):
+                c.set_learned_gate(gate_set)
+                c.residual(h, 0, 0, S.V_INIT, S.V_INIT2)
+                c.solver_factor()
+                c.solver_history(1024)
+                c.solver_solve()
+                hist, _ = c.solver_history(1024)
+                assert int(hist[0, 0]) == want, (h, count, gate_set, hist)
+                assert hist[-1, 4] == 0 and hist[-1, 2] <= 1e-10
+                if gate_set == -2:
+                    seen.add(want)
+        # the reuse at h / 2 of the last full build (rebuild = 2, stage II):
+        # the count of that build, halved
+        c.set_learned_gate(-2)
+        for h_full in (1e-6, 1e-8):
+            Ap, Ai, Ax, dv, dg = c.jacobian_csc(0, h_full)
+            full = int(np.count_nonzero(np.abs(Ax[dg] - 1.0) > 0.25))
+            nnz = ctypes.c_int64()
+            c._check(c.lib.sdddst_jacobian_csc(c.h, 0, 0.5 * h_full, 2, None, None, None, None, None,
+                                               ctypes.byref(nnz)))
+            c.residual(0.5 * h_full, 0, 0, S.V_INIT, S.V_INIT2)
+            c.solver_factor()
+            c.solver_history(1024)
+            c.solver_solve()
+            hist, _ = c.solver_history(1024)
+            assert int(hist[0, 0]) == (2 if 0.5 * full <= gate else 1), (h_full, full, hist)
+    assert seen == {1, 2}, "the step sizes should cover both choices"

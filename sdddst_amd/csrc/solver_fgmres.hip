@@ -53,9 +53,10 @@ struct KrylovDev {
     int k, kmax, stop;
 };
 
-// What the host reads after a wait.
+// What the host reads after a wait (res: the cycle's residual estimate
+// relative to ||g||).
 struct KrylovHost {
-    double gnorm;
+    double gnorm, res;
     int k, stop;
 };
 
@@ -178,6 +179,7 @@ __device__ void arnoldi_update(KrylovDev *__restrict__ st, KrylovHost *__restric
     if (fabs(sk1) / st->gnorm <= halftol || hn == 0.0) stop = hn == 0.0 ? kBreakdown : kConverged;
     else if (k + 1 >= st->kmax) stop = kBudget;
     st->stop = stop;
+    kh->res = fabs(sk1) / st->gnorm;
     kh->k = k + 1;
     kh->stop = stop;
     __threadfence_system();
@@ -357,6 +359,7 @@ __global__ void k_cycle_start(int n, const double *__restrict__ r, const double 
     st->k = 0;
     st->kmax = kmax;
     st->stop = st->gnorm == 0.0 ? kZeroRhs : kRunning;
+    kh->res = st->gnorm > 0.0 ? beta / st->gnorm : 0.0;
     kh->k = 0;
     kh->stop = st->stop;
     __threadfence_system();
@@ -625,8 +628,22 @@ static int norm2(Context &c, Fgmres &f, const double *v, double *out)
 // after the cycle stops, and the closing kernels while it still runs, return
 // at once, so a solve that converges within a batch costs one wait.  Only an
 // Arnoldi step whose second pass removed more than a quarter of ||w'|| comes
-// back to the host, for an explicit norm.
-constexpr int kBatch = 2;
+// back to the host, for an explicit norm.  After the first batch (as long
+// as the context's recent solves) each batch is the number of iterations
+// the cycle's mean reduction per iteration so far needs to bring the
+// residual estimate to the target (at most kBatchMax), so a long solve
+// (tens of iterations at a large step size) waits a few times instead of
+// once every kBatch iterations.
+constexpr int kBatch = 2, kBatchMax = 16;
+
+static int next_batch(double res, double res_start, int k, double target)
+{
+    if (!(res > 0.0) || !(res_start > 0.0) || k <= 0 || res <= target) return 1;
+    const double rate = std::log(res / res_start) / k;   // log of the mean reduction per iteration
+    if (!(rate < -1e-3)) return kBatch;
+    const double need = std::ceil(std::log(target / res) / rate);
+    return (int)std::max(1.0, std::min((double)kBatchMax, need));
+}
 
 int Fgmres::solve(Context &c)
 {
@@ -659,9 +676,10 @@ int Fgmres::solve(Context &c)
         int k = 0;   // iterations of this cycle the host has seen completed
         bool ended = false;
         // the first batch as long as this context's recent solves (rounded),
-        // later ones kBatch: iterations past convergence return at once but
-        // still cost their launches
+        // later ones from the observed reduction (next_batch): iterations
+        // past convergence return at once but still cost their launches
         int batch = std::max(1, std::min(m, (int)(iter_mean + 0.5)));
+        const double res_start = cycle == 0 ? 1.0 : rel;
         while (!ended) {
             const int B = std::min(batch, kmax - k);
             batch = kBatch;
@@ -686,6 +704,7 @@ int Fgmres::solve(Context &c)
             const volatile KrylovHost *vkh = kh;
             KrylovHost st;
             st.gnorm = vkh->gnorm;
+            st.res = vkh->res;
             st.k = vkh->k;
             st.stop = vkh->stop;
             if (gnorm < 0) gnorm = st.gnorm;
@@ -717,6 +736,7 @@ int Fgmres::solve(Context &c)
                 break;
             }
             case kRunning:
+                batch = next_batch(st.res, res_start, k, 0.5 * tol);
                 break;
             default:   // converged estimate, breakdown or budget: the closing kernels ran
                 beta = sqrt(H[0]);

@@ -88,6 +88,7 @@ struct Context {
     double *ytc = nullptr, *yts = nullptr;   // cos, sin of 2 pi y per dislocation (set with y)
     double *yw = nullptr;                    // y - rint(y) per dislocation (set with y)
     double *xw = nullptr, *xep = nullptr, *xem = nullptr;   // x - rint(x), e^{+-2 pi (x - rint(x))}
+    double *bext = nullptr;                  // per 64-block extents of xw and yw (pairtile::kExtWords each)
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
     // Jacobian (CSC)

@@ -51,12 +51,20 @@ struct TileLds {
 };
 
 // Per-dislocation far-pair factors of the fast field (null members: every
-// pair takes the reference-form path).
+// pair takes the reference-form path), and optionally the extents of each
+// 64-block of tile positions (block_extents), from which a tile classifies
+// without wave reductions.
 struct FarData {
     const double *ca = nullptr, *sa = nullptr;   // cos, sin of 2 pi y
     const double *w = nullptr, *ep = nullptr, *em = nullptr;   // x - rint(x), e^{+-2 pi w}
     const double *yw = nullptr;                  // y - rint(y)
+    const double *ext = nullptr;                 // [nb][kExtWords]
 };
+
+// Extents of one block of tile positions: the wrapped x and y of its
+// members moved by whole periods to within 1/2 of its first member's
+// (min, max of each), and the raw wrapped x (min, max).
+constexpr int kExtWords = 6;
 
 // A pair takes the far form when cosh(2 pi dx) - cos(2 pi dy), as the far
 // form computes it, is at least kFarMinGap: few-ulp errors in the
@@ -181,6 +189,22 @@ __device__ __forceinline__ void unwrapped_extent(double v, double &vmin, double 
     }
 }
 
+// The extents of a block from its members' wrapped x and y (NaN outside the
+// system; the first member is inside): called by one whole 64-lane
+// wavefront, the result on every lane, in the order of kExtWords.
+__device__ __forceinline__ void block_extents(double w, double yw, double out[kExtWords])
+{
+    unwrapped_extent(w, out[0], out[1]);
+    unwrapped_extent(yw, out[2], out[3]);
+    double lo = w, hi = w;
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    out[4] = lo;
+    out[5] = hi;
+}
+
 // The tile kind from the wrapped x and y of the target lanes (tw, ty) and
 // the source slots (sw, sy); the same on every lane.
 __device__ __forceinline__ int tile_kind(double tw, double sw, double ty, double sy)
@@ -195,12 +219,27 @@ __device__ __forceinline__ int tile_kind(double tw, double sw, double ty, double
     return __builtin_amdgcn_readfirstlane(sdfield::gap_kind(xmin, xmax, ymin, ymax));
 }
 
-// The interval that holds every x-difference s = w_i - w_j of a tile, with
-// a margin of 1e-12 for the rounding of s: (0, 1/2) -> kFarPos,
+
+// The interval that holds every x-difference s = w_i - w_j of a far tile,
+// with a margin of 1e-12 for the rounding of s: (0, 1/2) -> kFarPos,
 // (-1/2, 0) -> kFarNeg, (1/2, 1) -> kFarWrapPos, (-1, -1/2) -> kFarWrapNeg,
-// none of them -> kFarTile; from the extremes of the wrapped x of the target
+// none of them -> kFarTile; from the extremes of the wrapped x of the
+// targets (tmin, tmax) and of the sources (smin, smax).  The same on every
+// lane.
+__device__ __forceinline__ int far_kind_from(double tmin, double tmax, double smin, double smax)
+{
+    const double lo = tmin - smax, hi = tmax - smin, e = 1e-12;
+    int k = kFarTile;
+    if (lo > e && hi < 0.5 - e) k = kFarPos;
+    else if (hi < -e && lo > -0.5 + e) k = kFarNeg;
+    else if (lo > 0.5 + e && hi < 1.0 - e) k = kFarWrapPos;
+    else if (hi < -0.5 - e && lo > -1.0 + e) k = kFarWrapNeg;
+    return __builtin_amdgcn_readfirstlane(k);
+}
+
+// far_kind_from with the extremes taken over the wrapped x of the target
 // lanes (tw) and the source slots (sw), NaN outside the system (lane 0 is
-// inside in every tile).  The same on every lane.
+// inside in every tile).
 __device__ __forceinline__ int far_tile_kind(double tw, double sw)
 {
     double tmin = tw, tmax = tw, smin = sw, smax = sw;
@@ -210,13 +249,20 @@ __device__ __forceinline__ int far_tile_kind(double tw, double sw)
         smin = fmin(smin, __shfl_xor(smin, off, 64));
         smax = fmax(smax, __shfl_xor(smax, off, 64));
     }
-    const double lo = tmin - smax, hi = tmax - smin, e = 1e-12;
-    int k = kFarTile;
-    if (lo > e && hi < 0.5 - e) k = kFarPos;
-    else if (hi < -e && lo > -0.5 + e) k = kFarNeg;
-    else if (lo > 0.5 + e && hi < 1.0 - e) k = kFarWrapPos;
-    else if (hi < -0.5 - e && lo > -1.0 + e) k = kFarWrapNeg;
-    return __builtin_amdgcn_readfirstlane(k);
+    return far_kind_from(tmin, tmax, smin, smax);
+}
+
+
+// tile_kind, and far_tile_kind for an all-far tile, from the precomputed
+// extents of the target block (et) and the source block (es): the same
+// intervals the reductions form, so the same kind.
+__device__ __forceinline__ int tile_kind_ext(const double *__restrict__ et, const double *__restrict__ es)
+{
+    double xmin, xmax, ymin, ymax;
+    sdfield::circ_range(et[0], et[1], es[0], es[1], xmin, xmax);
+    sdfield::circ_range(et[2], et[3], es[2], es[3], ymin, ymax);
+    const int kind = __builtin_amdgcn_readfirstlane(sdfield::gap_kind(xmin, xmax, ymin, ymax));
+    return kind == kFarTile ? far_kind_from(et[4], et[5], es[4], es[5]) : kind;
 }
 
 // One pair of a far or mixed tile: the far form when its gap
@@ -305,17 +351,19 @@ __device__ __forceinline__ void tile(TileLds &L, int bi, int bj, int n, const do
     if constexpr (Mode == SDDDST_FIELD_FAST) {
         if (fd.w) {
             const double nan = __builtin_nan("");
-            kind = tile_kind(p < n ? fd.w[io] : nan, q < n ? fd.w[jo] : nan, p < n ? fd.yw[io] : nan,
-                             q < n ? fd.yw[jo] : nan);
-            // every pair of a tile that is not all far takes the mixed form
-            // (so a near pair's value does not depend on its tile's kind)
-            if (kind == kNearTile) kind = kMixedTile;
             const FarSelf src = q < n ? far_load(fd, jo) : FarSelf();
             far_stage(L, l, src);
             if (p < n) fi = far_load(fd, io);
-            if (kind == kFarTile) {
-                kind = far_tile_kind(p < n ? fi.w : nan, q < n ? src.w : nan);
+            if (fd.ext) {
+                kind = tile_kind_ext(fd.ext + (size_t)bi * kExtWords, fd.ext + (size_t)bj * kExtWords);
+            } else {
+                kind = tile_kind(p < n ? fi.w : nan, q < n ? src.w : nan, p < n ? fd.yw[io] : nan,
+                                 q < n ? fd.yw[jo] : nan);
+                if (kind == kFarTile) kind = far_tile_kind(p < n ? fi.w : nan, q < n ? src.w : nan);
             }
+            // every pair of a tile that is not all far takes the mixed form
+            // (so a near pair's value does not depend on its tile's kind)
+            if (kind == kNearTile) kind = kMixedTile;
         }
     }
     __syncthreads();

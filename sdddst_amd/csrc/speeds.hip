@@ -260,6 +260,38 @@ __global__ void k_speeds_prep(int n, unsigned long long *__restrict__ minr2, con
     }
 }
 
+// The same for the fast field's far-pair path, in tile order: one
+// wavefront per block of tile positions resets and forms its members'
+// values (wrapped x, e^{+-2 pi w}) and the block's extents for the tile
+// classification (pairtile::block_extents).
+__global__ __launch_bounds__(64) void k_speeds_prep_blocks(int n, const int *__restrict__ perm,
+                                                           unsigned long long *__restrict__ minr2,
+                                                           const double *__restrict__ x,
+                                                           const double *__restrict__ yw, double *__restrict__ w,
+                                                           double *__restrict__ ep, double *__restrict__ em,
+                                                           double *__restrict__ ext)
+{
+    const int p = (int)(blockIdx.x * 64 + threadIdx.x);
+    double wv = __builtin_nan(""), yv = __builtin_nan("");
+    if (p < n) {
+        const int i = perm[p];
+        minr2[i] = kInfBits;
+        double pv, mv;
+        sdfield::xexp(x[i], wv, pv, mv);
+        w[i] = wv;
+        ep[i] = pv;
+        em[i] = mv;
+        yv = yw[i];
+    }
+    double e[pairtile::kExtWords];
+    pairtile::block_extents(wv, yv, e);
+    if (threadIdx.x == 0) {
+        double *o = ext + (size_t)blockIdx.x * pairtile::kExtWords;
+#pragma unroll
+        for (int k = 0; k < pairtile::kExtWords; ++k) o[k] = e[k];
+    }
+}
+
 }  // namespace
 
 int pair_order_alloc(Context &c)
@@ -274,6 +306,7 @@ int pair_order_alloc(Context &c)
     SD_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes));
     for (double **p : {&c.ytc, &c.yts, &c.yw, &c.xw, &c.xep, &c.xem})
         SD_HIP(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
+    SD_HIP(hipMalloc((void **)&c.bext, sizeof(double) * pairtile::kExtWords * (size_t)(c.nb > 0 ? c.nb : 1)));
     return SDDDST_OK;
 }
 
@@ -334,8 +367,12 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     pairtile::FarData fd;
     const double r2_keep = pairtile::tolerance_r2(minPrecSqr);
     const bool far = c.p.field_mode == SDDDST_FIELD_FAST && r2_keep < sdfield::kFarMinR2;
-    hipLaunchKernelGGL(k_speeds_prep, dim3(blocks), dim3(threads), 0, c.stream, n, c.minr2, c.x[slot],
-                       far ? c.xw : nullptr, c.xep, c.xem);
+    if (far)
+        hipLaunchKernelGGL(k_speeds_prep_blocks, dim3(nb), dim3(kTile), 0, c.stream, n, c.perm, c.minr2, c.x[slot],
+                           c.yw, c.xw, c.xep, c.xem, c.bext);
+    else
+        hipLaunchKernelGGL(k_speeds_prep, dim3(blocks), dim3(threads), 0, c.stream, n, c.minr2, c.x[slot], nullptr,
+                           c.xep, c.xem);
     SD_HIP(hipGetLastError());
     if (far) {
         fd.ca = c.ytc;
@@ -344,6 +381,7 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         fd.ep = c.xep;
         fd.em = c.xem;
         fd.yw = c.yw;
+        fd.ext = c.bext;
     }
     if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)

@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__r
                 a += 1.0;
                 r += 1.0;
             }
-            Ax[k] = a;
+            __builtin_nontemporal_store(a, &Ax[k]);
             AxRow[k] = r;
             if (i != j) {
                 const double st = __builtin_fabs(r * a) / (ajj * ci[u].y);

@@ -61,7 +61,7 @@ static void free_all(Context &c)
                     c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
                     c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo,
-                    c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext};
+                    c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext, c.Ai16};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);

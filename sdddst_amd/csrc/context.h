@@ -93,6 +93,10 @@ struct Context {
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
     // Jacobian (CSC)
     int *Ap = nullptr, *Ai = nullptr, *diag = nullptr, *colcount = nullptr;
+    // the row indices again as 16 bits (N <= 65536), for the Krylov products:
+    // 10 instead of 12 bytes per entry; valid for the pattern of the last fill
+    unsigned short *Ai16 = nullptr;
+    bool ai16_valid = false;
     double *Ax = nullptr;               // final values
     double *AxRow = nullptr;            // CSR-of-A values (same index arrays): for SpMV
     int64_t nnz = 0, cap = 0;

@@ -436,7 +436,8 @@ __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__r
                                                            const double2 *__restrict__ colinfo,
                                                            const double *__restrict__ Araw, double factor,
                                                            double *__restrict__ Ax, double *__restrict__ AxRow,
-                                                           int *__restrict__ partner, double *__restrict__ strength)
+                                                           int *__restrict__ partner, double *__restrict__ strength,
+                                                           unsigned short *__restrict__ Ai16)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -473,6 +474,7 @@ __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__r
             }
             __builtin_nontemporal_store(a, &Ax[k]);
             AxRow[k] = r;
+            if (Ai16) Ai16[k] = (unsigned short)i;
             if (i != j) {
                 const double st = __builtin_fabs(r * a) / (ajj * ci[u].y);
                 if (st > best || (st == best && i < bj)) {
@@ -553,6 +555,7 @@ static int compact_zero_upper(Context &c)
         nAp[j + 1] = (int)nAi.size();
     }
     c.nnz = (int64_t)nAi.size();
+    c.ai16_valid = false;   // the kept entries moved
     SD_HIP(hipMemcpyAsync(c.Ap, nAp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c.stream));
     SD_HIP(hipMemcpyAsync(c.Ai, nAi.data(), sizeof(int) * c.nnz, hipMemcpyHostToDevice, c.stream));
     SD_HIP(hipMemcpyAsync(c.Ax, nAx.data(), sizeof(double) * c.nnz, hipMemcpyHostToDevice, c.stream));
@@ -566,17 +569,22 @@ static int ensure_capacity(Context &c, int64_t need)
     if (need <= c.cap) return SDDDST_OK;
     int64_t ncap = need + need / 8 + 1024;
     void **bufs[] = {(void **)&c.Ai,   (void **)&c.Ax,      (void **)&c.AxRow, (void **)&c.jtau,
-                     (void **)&c.jmult, (void **)&c.jmirror, (void **)&c.jcore, (void **)&c.Araw};
+                     (void **)&c.jmult, (void **)&c.jmirror, (void **)&c.jcore, (void **)&c.Araw,
+                     (void **)&c.Ai16};
     for (void **p : bufs) {
         if (*p) SD_HIP(hipFree(*p));
         *p = nullptr;
     }
     c.cap = 0;
     c.raw_valid = false;
+    c.ai16_valid = false;
     c.araw_h = __builtin_nan("");
     const size_t sizes[] = {sizeof(int),    sizeof(double), sizeof(double),        sizeof(double),
-                            sizeof(double), sizeof(int),    sizeof(unsigned char), sizeof(double)};
-    for (int k = 0; k < 8; ++k) {
+                            sizeof(double), sizeof(int),    sizeof(unsigned char), sizeof(double),
+                            sizeof(unsigned short)};
+    // 16-bit row indices only where every index fits
+    const int nbufs = c.n <= 65536 ? 9 : 8;
+    for (int k = 0; k < nbufs; ++k) {
         if (hipMalloc(bufs[k], sizes[k] * ncap) != hipSuccess) {
             *bufs[k] = nullptr;
             set_error("device allocation of %lld Jacobian entries failed", (long long)ncap);
@@ -650,6 +658,7 @@ static int build_pattern(Context &c, int slot, double h)
     hipLaunchKernelGGL(k_jac_fill, dim3(wblocks), dim3(256), 0, c.stream, n, words, c.jmask, c.Ap, c.Ai, c.diag,
                        c.jwstart, d_total, (long long)c.cap);
     SD_HIP(hipGetLastError());
+    c.ai16_valid = false;   // the scaling pass copies the new row indices
     hipLaunchKernelGGL(k_flags_reset, dim3(1), dim3(64), 0, c.stream, c.d_flags);
     // tau_x, the taper and the mirror positions are kept only when a later
     // call needs them (ensure_aux): the step never does, and the three arrays
@@ -788,9 +797,12 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             return SDDDST_OK;
         }
     }
+    // the 16-bit row indices, once per pattern
+    unsigned short *i16 = (c.Ai16 && !c.ai16_valid) ? c.Ai16 : nullptr;
     hipLaunchKernelGGL(k_jac_scale_partner, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag,
-                       c.colinfo, c.Araw, factor, c.Ax, c.AxRow, c.partner, c.strength);
+                       c.colinfo, c.Araw, factor, c.Ax, c.AxRow, c.partner, c.strength, i16);
     SD_HIP(hipGetLastError());
+    if (c.Ai16) c.ai16_valid = true;
     c.partner_valid = true;
     if ((c.timing & kTimeJacobian) && (rc = timer_end(c, c.t_jac))) return rc;
     c.jac_valid = true;

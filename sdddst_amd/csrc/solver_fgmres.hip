@@ -69,8 +69,8 @@ namespace {
 // trips instead of one per 64 entries; the order of the additions is that
 // of the plain loop.
 constexpr int kRowUnroll = 8;
-template <typename XRead>
-__device__ __forceinline__ double row_dot(int b, int e, int lane, const int *__restrict__ Ai,
+template <typename Index, typename XRead>
+__device__ __forceinline__ double row_dot(int b, int e, int lane, const Index *__restrict__ Ai,
                                           const double *__restrict__ Av, XRead xr)
 {
     double s = 0.0;
@@ -81,7 +81,7 @@ __device__ __forceinline__ double row_dot(int b, int e, int lane, const int *__r
         for (int u = 0; u < kRowUnroll; ++u) {
             const int k = k0 + 64 * u;
             a[u] = k < e ? Av[k] : 0.0;
-            c[u] = k < e ? Ai[k] : 0;
+            c[u] = k < e ? (int)Ai[k] : 0;
         }
 #pragma unroll
         for (int u = 0; u < kRowUnroll; ++u)
@@ -91,8 +91,10 @@ __device__ __forceinline__ double row_dot(int b, int e, int lane, const int *__r
     return s;
 }
 
-// y = A x, one wavefront per row.
-__global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+// y = A x, one wavefront per row.  Index: int, or unsigned short (the
+// 16-bit copy of the row indices, N <= 65536).
+template <typename Index>
+__global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap, const Index *__restrict__ Ai,
                                               const double *__restrict__ Av, const double *__restrict__ x,
                                               double *__restrict__ y, const int *__restrict__ gate, int gmode,
                                               const double *__restrict__ gsub)
@@ -111,7 +113,8 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap,
 // instead of the cache hierarchy.  Each row's sum is formed exactly as in
 // k_spmv.
 constexpr int kSpmvLdsMax = 20480;   // 160 KiB of FP64
-__global__ __launch_bounds__(1024) void k_spmv_lds(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+template <typename Index>
+__global__ __launch_bounds__(1024) void k_spmv_lds(int n, const int *__restrict__ Ap, const Index *__restrict__ Ai,
                                                    const double *__restrict__ Av, const double *__restrict__ x,
                                                    double *__restrict__ y, const int *__restrict__ gate, int gmode,
                                                    const double *__restrict__ gsub)
@@ -423,8 +426,10 @@ static int spmv_grid()
         int dev = 0, cu = 0;
         SD_HIP(hipGetDevice(&dev));
         SD_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
-        SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+        SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds<int>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(sizeof(double) * kSpmvLdsMax)));
+        SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds<unsigned short>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * kSpmvLdsMax)));
         cus = cu < 1 ? 1 : (cu > kRedMax ? kRedMax : cu);
     }
     return cus;
@@ -435,14 +440,22 @@ static int launch_spmv(Context &c, const double *x, double *y, const int *gate =
                        const double *gsub = nullptr)
 {
     const int n = c.n;
+    const bool i16 = c.ai16_valid;
     if (n <= kSpmvLdsMax) {
         const int cus = spmv_grid();
         if (cus < 0) return cus;
-        hipLaunchKernelGGL(k_spmv_lds, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n, c.Ap, c.Ai, c.AxRow, x,
-                           y, gate, gmode, gsub);
+        if (i16)
+            hipLaunchKernelGGL(k_spmv_lds<unsigned short>, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n,
+                               c.Ap, c.Ai16, c.AxRow, x, y, gate, gmode, gsub);
+        else
+            hipLaunchKernelGGL(k_spmv_lds<int>, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n, c.Ap, c.Ai,
+                               c.AxRow, x, y, gate, gmode, gsub);
+    } else if (i16) {
+        hipLaunchKernelGGL(k_spmv<unsigned short>, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai16,
+                           c.AxRow, x, y, gate, gmode, gsub);
     } else {
-        hipLaunchKernelGGL(k_spmv, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.AxRow, x, y, gate,
-                           gmode, gsub);
+        hipLaunchKernelGGL(k_spmv<int>, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.AxRow, x, y,
+                           gate, gmode, gsub);
     }
     SD_HIP(hipGetLastError());
     return SDDDST_OK;

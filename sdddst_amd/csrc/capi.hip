@@ -61,7 +61,7 @@ static void free_all(Context &c)
                     c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
                     c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo,
-                    c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext, c.Ai16};
+                    c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext, c.Ai16, c.gnorm2, c.gpart, c.gticket};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);
@@ -125,9 +125,11 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
         (rc = dalloc(&c.Ap, (size_t)n + 1)) || (rc = dalloc(&c.diag, n)) || (rc = dalloc(&c.colcount, n)) ||
         (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, kNumDevScalars)) || (rc = dalloc(&c.jpd, n)) ||
         (rc = dalloc(&c.sraw, n)) || (rc = dalloc(&c.partner, n)) || (rc = dalloc(&c.strength, n)) ||
-        (rc = dalloc(&c.colinfo, n)) ||
+        (rc = dalloc(&c.colinfo, n)) || (rc = dalloc(&c.gnorm2, 1)) || (rc = dalloc(&c.gpart, (size_t)c.nb * 4)) ||
+        (rc = dalloc(&c.gticket, 1)) ||
         (rc = pair_order_alloc(c)))
         return rc;
+    SD_HIP(hipMemsetAsync(c.gticket, 0, sizeof(unsigned), c.stream));
     SD_HIP(hipHostMalloc((void **)&c.hs, sizeof(DevScalars), hipHostMallocDefault));
     SD_HIP(hipHostMalloc((void **)&c.sig, sizeof(HostSignal), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(c.sig, 0, sizeof(HostSignal));

@@ -91,6 +91,11 @@ struct Context {
     double *bext = nullptr;                  // per 64-block extents of xw and yw (pairtile::kExtWords each)
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
+    // ||g||^2 formed by the residual kernel (block partials, last-block
+    // ticket), valid until a Krylov solve takes it
+    double *gnorm2 = nullptr, *gpart = nullptr;
+    unsigned *gticket = nullptr;
+    bool gnorm2_valid = false;
     // Jacobian (CSC)
     int *Ap = nullptr, *Ai = nullptr, *diag = nullptr, *colcount = nullptr;
     // the row indices again as 16 bits (N <= 65536), for the Krylov products:

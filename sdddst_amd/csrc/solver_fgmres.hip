@@ -111,22 +111,57 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ Ap,
 // one persistent 1024-lane workgroup per CU copies x once, then its
 // wavefronts take rows with stride 16 * grid; the gathers of x hit LDS
 // instead of the cache hierarchy.  Each row's sum is formed exactly as in
-// k_spmv.
-constexpr int kSpmvLdsMax = 20480;   // 160 KiB of FP64
+// k_spmv.  With nrm (the closing product of a restart cycle, y = g - A
+// delta) the kernel also forms ||y||^2: each wavefront adds the squares of
+// its rows in row order, the workgroup its 16 wavefronts in order into
+// partial[block], and the last workgroup to take the ticket adds the
+// partials in block order into *nrm (and nrm_host, mapped host memory) --
+// deterministic, and one launch fewer per cycle than a separate reduction.
+constexpr int kSpmvLdsMax = 20448;   // 160 KiB of FP64 with 32 words of reduction scratch
 template <typename Index>
 __global__ __launch_bounds__(1024) void k_spmv_lds(int n, const int *__restrict__ Ap, const Index *__restrict__ Ai,
                                                    const double *__restrict__ Av, const double *__restrict__ x,
                                                    double *__restrict__ y, const int *__restrict__ gate, int gmode,
-                                                   const double *__restrict__ gsub)
+                                                   const double *__restrict__ gsub, double *__restrict__ partial,
+                                                   unsigned *__restrict__ ticket, double *__restrict__ nrm,
+                                                   double *__restrict__ nrm_host)
 {
-    if (gated(gate, gmode)) return;
+    if (gated(gate, gmode)) return;   // every workgroup: the ticket stays untouched
     extern __shared__ double xs[];
+    double *wsum = xs + n;            // 16 wavefront sums, then the last-workgroup flag
     for (int i = (int)threadIdx.x; i < n; i += 1024) xs[i] = x[i];
     __syncthreads();
-    const int lane = (int)(threadIdx.x & 63);
-    for (int r = (int)(blockIdx.x * 16 + (threadIdx.x >> 6)); r < n; r += (int)gridDim.x * 16) {
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    double ss = 0.0;
+    for (int r = (int)(blockIdx.x * 16 + wave); r < n; r += (int)gridDim.x * 16) {
         const double s = row_dot(Ap[r], Ap[r + 1], lane, Ai, Av, [&](int i) { return xs[i]; });
-        if (lane == 0) y[r] = gsub ? gsub[r] - s : s;
+        if (lane == 0) {
+            const double v = gsub ? gsub[r] - s : s;
+            y[r] = v;
+            ss += v * v;
+        }
+    }
+    if (!nrm) return;
+    if (lane == 0) wsum[wave] = ss;
+    __syncthreads();
+    int *last = reinterpret_cast<int *>(wsum + 16);
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w) t += wsum[w];
+        partial[blockIdx.x] = t;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        *last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (!*last || wave != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double t = 0.0;
+    for (int b = lane; b < (int)gridDim.x; b += 64) t += __builtin_nontemporal_load(&partial[b]);
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) {
+        *nrm = t;
+        if (nrm_host) *nrm_host = t;
+        *ticket = 0u;
     }
 }
 
@@ -427,29 +462,40 @@ static int spmv_grid()
         SD_HIP(hipGetDevice(&dev));
         SD_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
         SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds<int>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(sizeof(double) * kSpmvLdsMax)));
+                                   (int)(sizeof(double) * (kSpmvLdsMax + 32))));
         SD_HIP(hipFuncSetAttribute((const void *)k_spmv_lds<unsigned short>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * kSpmvLdsMax)));
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double) * (kSpmvLdsMax + 32))));
         cus = cu < 1 ? 1 : (cu > kRedMax ? kRedMax : cu);
     }
     return cus;
 }
 
-// y = A x on the CSR view (AxRow): LDS-staged x when it fits.
+// y = A x on the CSR view (AxRow): LDS-staged x when it fits.  With f and
+// nrm: also ||y||^2 into nrm (and nrm_host) when the LDS kernel runs
+// (*fused set), else the caller reduces.
 static int launch_spmv(Context &c, const double *x, double *y, const int *gate = nullptr, int gmode = kGateIter,
-                       const double *gsub = nullptr)
+                       const double *gsub = nullptr, Fgmres *f = nullptr, double *nrm = nullptr,
+                       double *nrm_host = nullptr, bool *fused = nullptr)
 {
     const int n = c.n;
     const bool i16 = c.ai16_valid;
+    if (fused) *fused = false;
     if (n <= kSpmvLdsMax) {
         const int cus = spmv_grid();
         if (cus < 0) return cus;
+        const bool red = f && nrm;
+        if (fused) *fused = red;
+        const size_t lds = sizeof(double) * (n + 32);
+        double *part = red ? f->red : nullptr;
+        unsigned *tk = red ? f->ticket : nullptr;
+        double *nm = red ? nrm : nullptr, *nh = red ? nrm_host : nullptr;
         if (i16)
-            hipLaunchKernelGGL(k_spmv_lds<unsigned short>, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n,
-                               c.Ap, c.Ai16, c.AxRow, x, y, gate, gmode, gsub);
+            hipLaunchKernelGGL(k_spmv_lds<unsigned short>, dim3(cus), dim3(1024), lds, c.stream, n, c.Ap, c.Ai16,
+                               c.AxRow, x, y, gate, gmode, gsub, part, tk, nm, nh);
         else
-            hipLaunchKernelGGL(k_spmv_lds<int>, dim3(cus), dim3(1024), sizeof(double) * n, c.stream, n, c.Ap, c.Ai,
-                               c.AxRow, x, y, gate, gmode, gsub);
+            hipLaunchKernelGGL(k_spmv_lds<int>, dim3(cus), dim3(1024), lds, c.stream, n, c.Ap, c.Ai, c.AxRow, x, y,
+                               gate, gmode, gsub, part, tk, nm, nh);
     } else if (i16) {
         hipLaunchKernelGGL(k_spmv<unsigned short>, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai16,
                            c.AxRow, x, y, gate, gmode, gsub);
@@ -671,8 +717,14 @@ int Fgmres::solve(Context &c)
     SD_HIP(hipMemsetAsync(c.delta, 0, sizeof(double) * N, c.stream));
     double *gs = hcol + 3 * (m + 2);
     double *rn2 = hcol + 3 * (m + 2) + 1;   // ||g - A delta||^2 of a cycle's end (device; H[0] on the host)
-    launch_proj(c, *this, 0, V, const_cast<double *>(c.g), nullptr, gs, nullptr, 0, 1);
-    SD_HIP(hipGetLastError());
+    if (c.gnorm2_valid) {
+        // formed by the residual kernel that wrote g
+        gs = c.gnorm2;
+        c.gnorm2_valid = false;
+    } else {
+        launch_proj(c, *this, 0, V, const_cast<double *>(c.g), nullptr, gs, nullptr, 0, 1);
+        SD_HIP(hipGetLastError());
+    }
     c.last_iters = 0;
     int total_it = 0;
     double gnorm = -1.0, rel = 1.0, beta = 0.0;
@@ -710,8 +762,9 @@ int Fgmres::solve(Context &c)
             }
             // the cycle's end, when it came: delta += Z y and the true residual
             hipLaunchKernelGGL(k_combine_dev, dim3(nb256(N)), dim3(256), 0, c.stream, N, Z, kst, c.delta);
-            if ((rc = launch_spmv(c, c.delta, w, stop, kGateClose, c.g))) return rc;
-            launch_proj(c, *this, 0, V, w, nullptr, rn2, nullptr, 0, 1, Hdev, 0, stop, kGateClose);
+            bool fused = false;
+            if ((rc = launch_spmv(c, c.delta, w, stop, kGateClose, c.g, this, rn2, Hdev, &fused))) return rc;
+            if (!fused) launch_proj(c, *this, 0, V, w, nullptr, rn2, nullptr, 0, 1, Hdev, 0, stop, kGateClose);
             SD_HIP(hipGetLastError());
             if ((rc = host_wait(c))) return rc;
             const volatile KrylovHost *vkh = kh;

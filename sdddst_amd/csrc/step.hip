@@ -23,13 +23,46 @@ namespace sdamd {
 
 namespace {
 
-__global__ void k_residual(int n, double h, const double *__restrict__ xn, const double *__restrict__ xo,
-                           const double *__restrict__ csp, const double *__restrict__ isp,
-                           const double *__restrict__ d, double *__restrict__ g)
+// g and, for the Krylov solve that follows, ||g||^2: each 256-lane
+// workgroup adds its lanes' squares by a wavefront butterfly and its four
+// wavefronts in order into part[block]; the last workgroup to take the
+// ticket adds the partials in block order (lanes strided by 64, then a
+// butterfly) -- the order of the solver's own reduction over 256-lane
+// blocks, so ||g||^2 is the value that reduction would give.
+__global__ __launch_bounds__(256) void k_residual(int n, double h, const double *__restrict__ xn,
+                                                  const double *__restrict__ xo, const double *__restrict__ csp,
+                                                  const double *__restrict__ isp, const double *__restrict__ d,
+                                                  double *__restrict__ g, double *__restrict__ part,
+                                                  unsigned *__restrict__ ticket, double *__restrict__ norm2)
 {
+    __shared__ double wsum[4];
+    __shared__ bool last;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    g[i] = stepf::residual(xn[i], xo[i], csp[i], isp[i], d[i], h);
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    double sq = 0.0;
+    if (i < n) {
+        const double v = stepf::residual(xn[i], xo[i], csp[i], isp[i], d[i], h);
+        g[i] = v;
+        sq = 0.0 + v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+    if (lane == 0) wsum[wave] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || wave != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double t = 0.0;
+    for (int b = lane; b < (int)gridDim.x; b += 64) t += __builtin_nontemporal_load(&part[b]);
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) {
+        *norm2 = t;
+        *ticket = 0u;
+    }
 }
 
 __global__ void k_newton_update(int n, double *__restrict__ x, const double *__restrict__ delta)
@@ -238,8 +271,9 @@ static inline int nblocks(int n, int t) { return (n + t - 1) / t; }
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp)
 {
     hipLaunchKernelGGL(k_residual, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, h, c.x[new_slot],
-                       c.x[old_slot], c.v[csp], c.v[isp], c.dvec, c.g);
+                       c.x[old_slot], c.v[csp], c.v[isp], c.dvec, c.g, c.gpart, c.gticket, c.gnorm2);
     SD_HIP(hipGetLastError());
+    c.gnorm2_valid = true;
     return SDDDST_OK;
 }
 

@@ -58,7 +58,7 @@ static void free_all(Context &c)
     for (int s = 0; s < SDDDST_NUM_VBUFS; ++s)
         if (c.v[s]) (void)hipFree(c.v[s]);
     void *ptrs[] = {c.y, c.b, c.px, c.py, c.tol, c.minr2, c.partial, c.dvec, c.g, c.delta, c.Ap, c.Ai, c.diag,
-                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jcore,
+                    c.colcount, c.Ax, c.AxRow, c.d_flags, c.ds, c.jtau, c.jmult, c.jmirror, c.jmask, c.jpd, c.jx, c.jcmask,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
                     c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo,
                     c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext, c.Ai16, c.gnorm2, c.gpart, c.gticket};
@@ -126,7 +126,7 @@ static int create(int n, int pc, const sdddst_params *params, Context &c)
         (rc = dalloc(&c.d_flags, 8)) || (rc = dalloc(&c.ds, kNumDevScalars)) || (rc = dalloc(&c.jpd, n)) ||
         (rc = dalloc(&c.sraw, n)) || (rc = dalloc(&c.partner, n)) || (rc = dalloc(&c.strength, n)) ||
         (rc = dalloc(&c.colinfo, n)) || (rc = dalloc(&c.gnorm2, 1)) || (rc = dalloc(&c.gpart, (size_t)c.nb * 4)) ||
-        (rc = dalloc(&c.gticket, 1)) ||
+        (rc = dalloc(&c.gticket, 1)) || (rc = dalloc(&c.jx, n)) ||
         (rc = pair_order_alloc(c)))
         return rc;
     SD_HIP(hipMemsetAsync(c.gticket, 0, sizeof(unsigned), c.stream));

@@ -108,7 +108,15 @@ struct Context {
     double *jtau = nullptr, *jmult = nullptr;   // tau_x and taper per strictly-lower entry
     int *jmirror = nullptr;                     // position of the mirror entry
     bool aux_valid = false;                     // jtau / jmult / jmirror hold the current pattern's
-    unsigned char *jcore = nullptr;             // 1 where the pair lies within the cutoff (untapered)
+    // x at the last pattern build (written by its values pass) and the
+    // squared cutoff radius then: an entry's pair lies within the cutoff
+    // (untapered) where its r^2 from these positions is not above jrc2
+    double *jx = nullptr;
+    double jrc2 = 0;
+    // the same relation as a [n][ceil(n/64)] bitmask, written by the count
+    // pass of a build while the learned preconditioner is selected
+    unsigned long long *jcmask = nullptr;
+    bool jcmask_valid = false;
     double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask
     int *jwstart = nullptr;                     // [n][ceil(n/64)] position of each mask word's first row

@@ -300,15 +300,14 @@ __global__ __launch_bounds__(256) void k_e_jac_values(int nt, const int *__restr
                                                       const double *__restrict__ x, const double *__restrict__ y,
                                                       const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                       const int *__restrict__ diag, double *__restrict__ tau,
-                                                      double *__restrict__ mult, int *__restrict__ mirror,
-                                                      unsigned char *__restrict__ core)
+                                                      double *__restrict__ mult, int *__restrict__ mirror)
 {
     ColView cv;
     const int g = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (!column_of(g, nt, sys, D, st, true, cv)) return;
     const SysDesc &d = D[cv.s];
     jac::values_column<Mode>(cv.j, x + d.off, y + d.off, d.cp, Ap + d.off + cv.s, Ai + d.nzoff, diag + d.off,
-                       tau + d.nzoff, mult + d.nzoff, mirror + d.nzoff, core + d.nzoff);
+                       tau + d.nzoff, mult + d.nzoff, mirror + d.nzoff, nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_e_jac_raw(int nt, const int *__restrict__ sys, const SysDesc *__restrict__ D,
@@ -799,7 +798,6 @@ struct Ensemble {
     unsigned long long *mask = nullptr;
     int *Ai = nullptr, *mirror = nullptr;
     double *Ax = nullptr, *AxRow = nullptr, *tau = nullptr, *mult = nullptr;
-    unsigned char *core = nullptr;
     int *partner = nullptr, *bcol = nullptr;
     double *strength = nullptr, *binv = nullptr;
     double *V = nullptr, *Z = nullptr, *dense = nullptr;
@@ -866,8 +864,7 @@ int ens_layout(Ensemble &e)
     const bool grow_nz = nz != e.nzTotal;
     if (grow_nz) {
         if ((rc = ealloc(&e.Ai, nz)) || (rc = ealloc(&e.mirror, nz)) || (rc = ealloc(&e.Ax, nz)) ||
-            (rc = ealloc(&e.AxRow, nz)) || (rc = ealloc(&e.tau, nz)) || (rc = ealloc(&e.mult, nz)) ||
-            (rc = ealloc(&e.core, nz)))
+            (rc = ealloc(&e.AxRow, nz)) || (rc = ealloc(&e.tau, nz)) || (rc = ealloc(&e.mult, nz)))
             return rc;
         e.nzTotal = nz;
     }
@@ -970,10 +967,10 @@ int launch_jacobian(Ensemble &e, int slot, double hscale, bool build)
                            e.diag);
         if (e.p.field_mode == SDDDST_FIELD_FAST)
             hipLaunchKernelGGL(k_e_jac_values<SDDDST_FIELD_FAST>, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D,
-                               e.st, e.x[slot], e.y, e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror, e.core);
+                               e.st, e.x[slot], e.y, e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror);
         else
             hipLaunchKernelGGL(k_e_jac_values<SDDDST_FIELD_EXACT>, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D,
-                               e.st, e.x[slot], e.y, e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror, e.core);
+                               e.st, e.x[slot], e.y, e.Ap, e.Ai, e.diag, e.tau, e.mult, e.mirror);
     }
     hipLaunchKernelGGL(k_e_jac_raw, dim3(wb), dim3(256), 0, e.stream, e.NT, e.sys, e.D, e.st, hscale, e.b, e.Ap, e.Ai,
                        e.diag, e.tau, e.mult, e.mirror, e.Ax, e.flags);
@@ -1133,7 +1130,7 @@ void ensemble_free(Ensemble *e)
     drop_graph(*e);
     void *ptrs[] = {e->D,    e->st,   e->sys,     e->y,      e->b,     e->tol,  e->g,       e->delta,
                     e->dvec, e->cnt,  e->Ap,      e->diag,   e->flags, e->mask, e->Ai,      e->mirror,
-                    e->Ax,   e->AxRow, e->tau,    e->mult,   e->core,  e->partner, e->bcol, e->strength,
+                    e->Ax,   e->AxRow, e->tau,    e->mult,   e->partner, e->bcol, e->strength,
                     e->binv, e->V,    e->Z,       e->dense,  e->blk,  e->tl,   e->partial, e->minr2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);

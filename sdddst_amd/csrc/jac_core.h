@@ -58,17 +58,20 @@ __device__ __forceinline__ int find_row(const int *__restrict__ Ai, int lo, int 
 }
 
 // Membership bitmask of column j (mj[words]) and its entry count; the
-// predicate runs once per ordered pair with a cheap r^2 pre-filter.
+// predicate runs once per ordered pair with a cheap r^2 pre-filter.  With
+// cmj, also the bitmask of the off-diagonal rows whose pair lies within the
+// cutoff (r^2 <= rc^2: the untapered entries).
 __device__ __forceinline__ void count_column(int j, int n, int words, const double *__restrict__ x,
                                              const double *__restrict__ y, const CutParams &cp,
-                                             int *__restrict__ cnt_j, unsigned long long *__restrict__ mj)
+                                             int *__restrict__ cnt_j, unsigned long long *__restrict__ mj,
+                                             unsigned long long *__restrict__ cmj = nullptr)
 {
     const int lane = (int)(threadIdx.x & 63);
     const double xj = x[j], yj = y[j];
     int c = 0;
     for (int w = 0; w < words; ++w) {
         const int i = w * 64 + lane;
-        bool in = false;
+        bool in = false, core = false;
         if (i < n) {
             if (i == j) {
                 in = true;
@@ -79,11 +82,16 @@ __device__ __forceinline__ void count_column(int j, int n, int words, const doub
                 const double dy = sdfield::normalize(i > j ? yi - yj : yj - yi);
                 double r2;
                 in = pair_pred(dx, dy, cp, r2);
+                core = !(r2 > cp.rc2);
             }
         }
         const unsigned long long m = __ballot(in);
         if (lane == 0) mj[w] = m;
         c += __popcll(m);
+        if (cmj) {
+            const unsigned long long cm = __ballot(core);
+            if (lane == 0) cmj[w] = cm;
+        }
     }
     if (lane == 0) *cnt_j = c;
 }
@@ -133,9 +141,10 @@ __device__ __forceinline__ int row_position(int words, const unsigned long long 
     return wstart[w] + __popcll(mask[w] & below);
 }
 
-// tau_x and the taper of every strictly-lower entry of column j (row i > j),
-// the position of its mirror (row j in column i) and the within-cutoff flag
-// of both positions.  Mode FAST takes the production derivative
+// tau_x and the taper of every strictly-lower entry of column j (row i > j)
+// and the position of its mirror (row j in column i); with xsnap, x_j is kept
+// there (the positions the pattern was built from, from which a later pass
+// tells the entries within the cutoff).  Mode FAST takes the production derivative
 // (xy_diff_x_fast), the other modes the reference's expression.  With raw,
 // also the unsigned value for step size h at the entry and at its mirror.
 // Without Aux, tau_x, the taper and the mirror positions are not kept (a
@@ -145,7 +154,7 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
                                               const CutParams &cp, const int *__restrict__ Ap,
                                               const int *__restrict__ Ai, const int *__restrict__ diag,
                                               double *__restrict__ tau, double *__restrict__ mult,
-                                              int *__restrict__ mirror, unsigned char *__restrict__ core,
+                                              int *__restrict__ mirror, double *__restrict__ xsnap,
                                               int words = 0, const unsigned long long *__restrict__ mask = nullptr,
                                               const int *__restrict__ wstart = nullptr,
                                               const double *__restrict__ b = nullptr, double h = 0.0,
@@ -156,6 +165,7 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
     const int end = Ap[j + 1];
     const double xj = x[j], yj = y[j];
     const double bj = raw ? b[j] : 0.0;
+    if (xsnap && lane == 0) xsnap[j] = xj;
     for (int k = d + 1 + lane; k < end; k += 64) {
         const int i = Ai[k];
         // mirror: row j in column i (two loads with the word starts, else a
@@ -187,9 +197,6 @@ __device__ __forceinline__ void values_column(int j, const double *__restrict__ 
             raw[mk] = v;
             if (v == 0.0) atomicOr(flags, 1);
         }
-        const unsigned char inside = r2 > cp.rc2 ? 0 : 1;
-        core[k] = inside;
-        core[mk] = inside;
     }
 }
 

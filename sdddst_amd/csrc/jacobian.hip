@@ -56,11 +56,13 @@ using jac::CutParams;
 
 __global__ __launch_bounds__(256) void k_jac_count(int n, int words, const double *__restrict__ x,
                                                    const double *__restrict__ y, CutParams cp,
-                                                   int *__restrict__ cnt, unsigned long long *__restrict__ mask)
+                                                   int *__restrict__ cnt, unsigned long long *__restrict__ mask,
+                                                   unsigned long long *__restrict__ cmask)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
-    jac::count_column(j, n, words, x, y, cp, cnt + j, mask + (size_t)j * words);
+    jac::count_column(j, n, words, x, y, cp, cnt + j, mask + (size_t)j * words,
+                      cmask ? cmask + (size_t)j * words : nullptr);
 }
 
 // Cell binning for the count: the unit cell is cut into G x G periodic cells
@@ -76,19 +78,36 @@ __device__ __forceinline__ int cell_of(double x, double y, int G)
     return cy * G + cx;
 }
 
-__global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y, int G,
-                             int *__restrict__ cell, int *__restrict__ ccount)
+// Cell of each dislocation and the cells' counts: each workgroup counts its
+// kCellPer * 256 dislocations in LDS and adds one total per occupied cell.
+constexpr int kCellPer = 4;
+constexpr int kCellMaxG = 64;   // G * G counts (and bases) fit in LDS
+
+__global__ __launch_bounds__(256) void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                    int G, int *__restrict__ cell, int *__restrict__ ccount)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    const int c = cell_of(x[i], y[i], G);
-    cell[i] = c;
-    atomicAdd(&ccount[c], 1);
+    extern __shared__ int hist[];
+    const int cells = G * G, t = (int)threadIdx.x;
+    for (int k = t; k < cells; k += 256) hist[k] = 0;
+    __syncthreads();
+    const int i0 = (int)blockIdx.x * 256 * kCellPer + t;
+#pragma unroll
+    for (int e = 0; e < kCellPer; ++e) {
+        const int i = i0 + e * 256;
+        if (i < n) {
+            const int c = cell_of(x[i], y[i], G);
+            cell[i] = c;
+            atomicAdd(&hist[c], 1);
+        }
+    }
+    __syncthreads();
+    for (int k = t; k < cells; k += 256)
+        if (hist[k]) atomicAdd(&ccount[k], hist[k]);
 }
 
 // cstart = exclusive scan of ccount (G*G cells, one workgroup); the fill
-// cursor starts at cstart.
-__global__ __launch_bounds__(1024) void k_cell_scan(int cells, const int *__restrict__ ccount,
+// cursor starts at cstart, and ccount is left zero for the next count.
+__global__ __launch_bounds__(1024) void k_cell_scan(int cells, int *__restrict__ ccount,
                                                     int *__restrict__ cstart, int *__restrict__ cursor)
 {
     __shared__ int part[1024];
@@ -110,27 +129,53 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int cells, const int *__rest
         cstart[k] = run;
         cursor[k] = run;
         run += ccount[k];
+        ccount[k] = 0;
     }
     if (t == 1023) cstart[cells] = part[1023];
 }
 
 // Dislocations by cell, with copies of their positions in the same order
-// (the order within a cell is arbitrary: the count only sets bits).
-__global__ void k_cell_scatter(int n, const int *__restrict__ cell, const double *__restrict__ x,
-                               const double *__restrict__ y, int *__restrict__ cursor, int *__restrict__ sorted,
-                               double *__restrict__ sx, double *__restrict__ sy)
+// (the order within a cell is arbitrary: the count only sets bits).  Each
+// workgroup ranks its dislocations per cell in LDS and reserves one range
+// per occupied cell.
+__global__ __launch_bounds__(256) void k_cell_scatter(int n, int G, const int *__restrict__ cell,
+                                                      const double *__restrict__ x, const double *__restrict__ y,
+                                                      int *__restrict__ cursor, int *__restrict__ sorted,
+                                                      double *__restrict__ sx, double *__restrict__ sy)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    const int p = atomicAdd(&cursor[cell[i]], 1);
-    sorted[p] = i;
-    sx[p] = x[i];
-    sy[p] = y[i];
+    extern __shared__ int lds[];
+    const int cells = G * G, t = (int)threadIdx.x;
+    int *hist = lds, *base = lds + cells;
+    for (int k = t; k < cells; k += 256) hist[k] = 0;
+    __syncthreads();
+    const int i0 = (int)blockIdx.x * 256 * kCellPer + t;
+    int c[kCellPer], r[kCellPer];
+#pragma unroll
+    for (int e = 0; e < kCellPer; ++e) {
+        const int i = i0 + e * 256;
+        c[e] = i < n ? cell[i] : -1;
+        r[e] = c[e] >= 0 ? atomicAdd(&hist[c[e]], 1) : 0;
+    }
+    __syncthreads();
+    for (int k = t; k < cells; k += 256)
+        if (hist[k]) base[k] = atomicAdd(&cursor[k], hist[k]);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < kCellPer; ++e) {
+        const int i = i0 + e * 256;
+        if (c[e] >= 0) {
+            const int p = base[c[e]] + r[e];
+            sorted[p] = i;
+            sx[p] = x[i];
+            sy[p] = y[i];
+        }
+    }
 }
 
 // Count of column j from the 3 x 3 neighbouring cells: the same predicate as
 // count_column, bits gathered in an LDS copy of the column's bitmask (the
-// order of the candidates does not matter), then written out whole.
+// order of the candidates does not matter), then written out whole; with
+// cmask, the within-cutoff bitmask of count_column the same way.
 constexpr int kCellMaxWords = 1024;
 __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const double *__restrict__ x,
                                                          const double *__restrict__ y, CutParams cp, int G,
@@ -139,13 +184,17 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
                                                          const int *__restrict__ sorted,
                                                          const double *__restrict__ sx,
                                                          const double *__restrict__ sy, int *__restrict__ cnt,
-                                                         unsigned long long *__restrict__ mask)
+                                                         unsigned long long *__restrict__ mask,
+                                                         unsigned long long *__restrict__ cmask)
 {
     extern __shared__ unsigned long long lm[];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int j = (int)(blockIdx.x * 4 + wave);
     unsigned long long *m = lm + (size_t)wave * words;
+    unsigned long long *cm = lm + (size_t)(4 + wave) * words;   // used with cmask only
     for (int w = lane; w < words; w += 64) m[w] = 0ull;
+    if (cmask)
+        for (int w = lane; w < words; w += 64) cm[w] = 0ull;
     __syncthreads();
     if (j < n) {
         const double xj = x[j], yj = y[j];
@@ -162,7 +211,7 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
                 for (int k = k0 + lane; k < k1; k += 64) {
                     const int i = sorted[k];
                     const double xi = sx[k], yi = sy[k];
-                    bool in;
+                    bool in, core = false;
                     if (i == j) {
                         in = true;
                     } else {
@@ -170,8 +219,10 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
                         const double dy = sdfield::normalize(i > j ? yi - yj : yj - yi);
                         double r2;
                         in = jac::pair_pred(dx, dy, cp, r2);
+                        core = !(r2 > cp.rc2);
                     }
                     if (in) atomicOr(&m[i >> 6], 1ull << (i & 63));
+                    if (core && cmask) atomicOr(&cm[i >> 6], 1ull << (i & 63));
                 }
             }
         }
@@ -185,48 +236,83 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
         mj[w] = v;
         c += __popcll(v);
     }
+    if (cmask) {
+        unsigned long long *cj = cmask + (size_t)j * words;
+        for (int w = lane; w < words; w += 64) cj[w] = cm[w];
+    }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
     if (lane == 0) cnt[j] = c;
 }
 
 // Exclusive scan of n counts into Ap[0..n] (and the total) by one workgroup
-// of 1024 lanes: tiles of 8192 counts go through LDS with coalesced loads
-// and stores; each lane scans 8 consecutive counts, the 1024 lane sums are
-// scanned (Hillis-Steele) and a 64-bit carry joins the tiles.
+// of 1024 lanes over tiles of 16384 counts: each lane sums 16 consecutive
+// counts (four 16-byte loads), each wave scans its 64 lane sums with
+// shuffles, the first wave scans the 16 wave sums, and a 64-bit carry joins
+// the tiles (three barriers per tile).
 __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ cnt, int *__restrict__ Ap,
                                                long long *__restrict__ total)
 {
-    constexpr int kPer = 8, kTileN = 1024 * kPer;
-    __shared__ int buf[kTileN];
-    __shared__ long long part[1024];
-    const int t = (int)threadIdx.x;
+    constexpr int kPer = 16, kTileN = 1024 * kPer;
+    __shared__ long long wsum[16];
+    const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+    const bool vec = ((reinterpret_cast<uintptr_t>(cnt) | reinterpret_cast<uintptr_t>(Ap)) & 15) == 0;
     long long carry = 0;
     for (int base = 0; base < n; base += kTileN) {
-        const int m = min(kTileN, n - base);
-        for (int q = t; q < kTileN; q += 1024) buf[q] = q < m ? cnt[base + q] : 0;
-        __syncthreads();
+        const int q0 = base + t * kPer;
+        const bool full = vec && q0 + kPer <= n;
+        int v[kPer];
+        if (full) {
+#pragma unroll
+            for (int u = 0; u < kPer / 4; ++u) {
+                const int4 a = reinterpret_cast<const int4 *>(cnt + q0)[u];
+                v[4 * u] = a.x;
+                v[4 * u + 1] = a.y;
+                v[4 * u + 2] = a.z;
+                v[4 * u + 3] = a.w;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) v[u] = q0 + u < n ? cnt[q0 + u] : 0;
+        }
         long long acc = 0;
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) acc += buf[t * kPer + u];
-        part[t] = acc;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const long long v = t >= off ? part[t - off] : 0;
-            __syncthreads();
-            part[t] += v;
-            __syncthreads();
+        for (int u = 0; u < kPer; ++u) acc += v[u];
+        long long incl = acc;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
         }
-        long long run = carry + part[t] - acc;   // exclusive prefix of this lane's counts
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        if (w == 0) {
+            long long sw = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const long long o = __shfl_up(sw, off, 64);
+                if (lane >= off) sw += o;
+            }
+            if (lane < 16) wsum[lane] = sw;
+        }
+        __syncthreads();
+        long long run = carry + (w > 0 ? wsum[w - 1] : 0) + incl - acc;   // exclusive prefix of this lane's counts
+        int o[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-            const int v = buf[t * kPer + u];
-            buf[t * kPer + u] = (int)run;
-            run += v;
+            o[u] = (int)run;
+            run += v[u];
         }
-        __syncthreads();
-        for (int q = t; q < m; q += 1024) Ap[base + q] = buf[q];
-        carry += part[1023];
-        __syncthreads();
+        if (full) {
+#pragma unroll
+            for (int u = 0; u < kPer / 4; ++u)
+                reinterpret_cast<int4 *>(Ap + q0)[u] = make_int4(o[4 * u], o[4 * u + 1], o[4 * u + 2], o[4 * u + 3]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u)
+                if (q0 + u < n) Ap[q0 + u] = o[u];
+        }
+        carry += wsum[15];
+        __syncthreads();   // wsum is rewritten by the next tile
     }
     if (t == 0) {
         *total = carry;
@@ -258,7 +344,7 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
                                                     CutParams cp, const int *__restrict__ Ap,
                                                     const int *__restrict__ Ai, const int *__restrict__ diag,
                                                     double *__restrict__ tau, double *__restrict__ mult,
-                                                    int *__restrict__ mirror, unsigned char *__restrict__ core,
+                                                    int *__restrict__ mirror, double *__restrict__ xsnap,
                                                     int words, const unsigned long long *__restrict__ mask,
                                                     const int *__restrict__ wstart, const double *__restrict__ b,
                                                     double h, double *__restrict__ raw, int *__restrict__ flags,
@@ -266,7 +352,7 @@ __global__ __launch_bounds__(256) void k_jac_values(int n, const double *__restr
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n || over_capacity(total, cap)) return;
-    jac::values_column<Mode, Aux>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, core, words, mask, wstart, b, h,
+    jac::values_column<Mode, Aux>(j, x, y, cp, Ap, Ai, diag, tau, mult, mirror, xsnap, words, mask, wstart, b, h,
                                    raw, flags);
 }
 
@@ -498,12 +584,6 @@ __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__r
     }
 }
 
-__global__ void k_fill_int(int *p, int v, int n)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i < n) p[i] = v;
-}
-
 // the zero-value flag and the strong-diagonal count of the assembly flags
 __global__ void k_flags_reset(int *f)
 {
@@ -568,9 +648,8 @@ static int ensure_capacity(Context &c, int64_t need)
 {
     if (need <= c.cap) return SDDDST_OK;
     int64_t ncap = need + need / 8 + 1024;
-    void **bufs[] = {(void **)&c.Ai,   (void **)&c.Ax,      (void **)&c.AxRow, (void **)&c.jtau,
-                     (void **)&c.jmult, (void **)&c.jmirror, (void **)&c.jcore, (void **)&c.Araw,
-                     (void **)&c.Ai16};
+    void **bufs[] = {(void **)&c.Ai,      (void **)&c.Ax,   (void **)&c.AxRow, (void **)&c.jtau, (void **)&c.jmult,
+                     (void **)&c.jmirror, (void **)&c.Araw, (void **)&c.Ai16};
     for (void **p : bufs) {
         if (*p) SD_HIP(hipFree(*p));
         *p = nullptr;
@@ -579,11 +658,10 @@ static int ensure_capacity(Context &c, int64_t need)
     c.raw_valid = false;
     c.ai16_valid = false;
     c.araw_h = __builtin_nan("");
-    const size_t sizes[] = {sizeof(int),    sizeof(double), sizeof(double),        sizeof(double),
-                            sizeof(double), sizeof(int),    sizeof(unsigned char), sizeof(double),
-                            sizeof(unsigned short)};
+    const size_t sizes[] = {sizeof(int), sizeof(double), sizeof(double), sizeof(double),
+                            sizeof(double), sizeof(int), sizeof(double), sizeof(unsigned short)};
     // 16-bit row indices only where every index fits
-    const int nbufs = c.n <= 65536 ? 9 : 8;
+    const int nbufs = c.n <= 65536 ? 8 : 7;
     for (int k = 0; k < nbufs; ++k) {
         if (hipMalloc(bufs[k], sizes[k] * ncap) != hipSuccess) {
             *bufs[k] = nullptr;
@@ -614,9 +692,18 @@ static int build_pattern(Context &c, int slot, double h)
         set_error("device allocation of the %d x %d mask word starts failed", n, words);
         return SDDDST_ERR_NOMEM;
     }
+    // the within-cutoff bitmask, for the learned preconditioner's core graph
+    const bool want_core = c.precond == SDDDST_PRECOND_LEARNED;
+    if (want_core && !c.jcmask && hipMalloc(&c.jcmask, sizeof(unsigned long long) * (size_t)n * words) != hipSuccess) {
+        c.jcmask = nullptr;
+        set_error("device allocation of the %d x %d within-cutoff bitmask failed", n, words);
+        return SDDDST_ERR_NOMEM;
+    }
+    unsigned long long *cmask = want_core ? c.jcmask : nullptr;
     // cells at least the pre-filter radius wide; 3 x 3 neighbourhoods need G >= 3
     const double rpre = sqrt(cp.prefilter);
-    const int G = rpre > 0 && rpre < 1.0 / 3.0 ? (int)floor(1.0 / (rpre * (1.0 + 1e-6))) : 0;
+    // (fewer, wider cells are as valid: at most kCellMaxG per side)
+    const int G = rpre > 0 && rpre < 1.0 / 3.0 ? min(kCellMaxG, (int)floor(1.0 / (rpre * (1.0 + 1e-6)))) : 0;
     if (G >= 4 && words <= kCellMaxWords) {
         const int cells = G * G;
         if (cells > c.cell_cap) {
@@ -625,24 +712,28 @@ static int build_pattern(Context &c, int slot, double h)
                 *q = nullptr;
                 SD_HIP(hipMalloc(q, sizeof(int) * (cells + 1)));
             }
+            // the counts start at zero; each scan leaves them zero again
+            SD_HIP(hipMemsetAsync(c.ccount, 0, sizeof(int) * (cells + 1), c.stream));
             c.cell_cap = cells;
         }
         if (!c.cell) SD_HIP(hipMalloc(&c.cell, sizeof(int) * n));
         if (!c.csorted) SD_HIP(hipMalloc(&c.csorted, sizeof(int) * n));
         if (!c.csx) SD_HIP(hipMalloc(&c.csx, sizeof(double) * n));
         if (!c.csy) SD_HIP(hipMalloc(&c.csy, sizeof(double) * n));
-        hipLaunchKernelGGL(k_fill_int, dim3((cells + 255) / 256), dim3(256), 0, c.stream, c.ccount, 0, cells);
-        hipLaunchKernelGGL(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, x, c.y, G, c.cell, c.ccount);
+        const int cblocks = (n + 256 * kCellPer - 1) / (256 * kCellPer);
+        hipLaunchKernelGGL(k_cell_count, dim3(cblocks), dim3(256), sizeof(int) * cells, c.stream, n, x, c.y, G,
+                           c.cell, c.ccount);
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c.stream, cells, c.ccount, c.cstart, c.ccursor);
-        hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.cell, x, c.y,
-                           c.ccursor, c.csorted, c.csx, c.csy);
-        hipLaunchKernelGGL(k_jac_count_cells, dim3(wblocks), dim3(256), sizeof(unsigned long long) * 4 * words,
-                           c.stream, n, words, x, c.y, cp, G, c.cell, c.cstart, c.csorted, c.csx, c.csy, c.colcount,
-                           c.jmask);
+        hipLaunchKernelGGL(k_cell_scatter, dim3(cblocks), dim3(256), 2 * sizeof(int) * cells, c.stream, n, G, c.cell,
+                           x, c.y, c.ccursor, c.csorted, c.csx, c.csy);
+        hipLaunchKernelGGL(k_jac_count_cells, dim3(wblocks), dim3(256),
+                           sizeof(unsigned long long) * (cmask ? 8 : 4) * words, c.stream, n, words, x, c.y, cp, G,
+                           c.cell, c.cstart, c.csorted, c.csx, c.csy, c.colcount, c.jmask, cmask);
     } else {
         hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, words, x, c.y, cp, c.colcount,
-                           c.jmask);
+                           c.jmask, cmask);
     }
+    c.jcmask_valid = want_core;
     SD_HIP(hipGetLastError());
     long long *d_total = reinterpret_cast<long long *>(c.d_flags + 2);
     int rc = launch_scan(c, n, c.colcount, c.Ap, d_total);
@@ -665,11 +756,11 @@ static int build_pattern(Context &c, int slot, double h)
     // were a fifth of the values pass's stores
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_FAST, false>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
-                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jx, words, c.jmask, c.jwstart,
                            c.b, h, c.Araw, c.d_flags, d_total, (long long)c.cap);
     else
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_EXACT, false>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
-                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jx, words, c.jmask, c.jwstart,
                            c.b, h, c.Araw, c.d_flags, d_total, (long long)c.cap);
     SD_HIP(hipGetLastError());
     c.araw_h = h;
@@ -681,6 +772,7 @@ static int build_pattern(Context &c, int slot, double h)
     }
     c.pattern_valid = true;
     c.pattern_slot = slot;
+    c.jrc2 = cp.rc2;
     return SDDDST_OK;
 }
 
@@ -700,11 +792,11 @@ static int ensure_aux(Context &c, int slot)
     const double *x = c.x[slot];
     if (c.p.field_mode == SDDDST_FIELD_FAST)
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_FAST, true>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
-                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jx, words, c.jmask, c.jwstart,
                            c.b, c.araw_h, c.Araw, c.d_flags, jac_total(c), (long long)c.cap);
     else
         hipLaunchKernelGGL((k_jac_values<SDDDST_FIELD_EXACT, true>), dim3(wblocks), dim3(256), 0, c.stream, n, x, c.y,
-                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jcore, words, c.jmask, c.jwstart,
+                           cp, c.Ap, c.Ai, c.diag, c.jtau, c.jmult, c.jmirror, c.jx, words, c.jmask, c.jwstart,
                            c.b, c.araw_h, c.Araw, c.d_flags, jac_total(c), (long long)c.cap);
     SD_HIP(hipGetLastError());
     c.aux_valid = true;

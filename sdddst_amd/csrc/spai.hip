@@ -5,8 +5,8 @@
 // M = B^-1 + C.  B^-1 is the pair-block inverse of solver_fgmres.hip (bd =
 // its diagonal, bo / bcol the partner entry).  C lives on the core graph of
 // A: the off-diagonal entries whose pair lies within the cutoff radius (the
-// untapered part of the Jacobian, flagged in jcore when the pattern is
-// built), minus the pair partner, plus the diagonal:
+// untapered part of the Jacobian, told from the positions the pattern was
+// built from), minus the pair partner, plus the diagonal:
 //     C_ij = -gamma_ij * bd_i * A_ij * bd_j / (1 + |x_ij| + |w_ij|)
 //     C_ii =  gamma_i  * bd_i * sum_j x_ij w_ij / (1 + |x_ij w_ij|)
 // with x_ij = bd_i A_ij and w_ij = bd_j A_ji: the first-order (Neumann)
@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "context.h"
+#include "field.h"
 #include "solvers.h"
 
 namespace sdamd {
@@ -110,16 +111,32 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
 
 // -- core graph -------------------------------------------------------------
 
-__device__ __forceinline__ bool core_entry(int i, int k, const int *__restrict__ Ai,
-                                           const unsigned char *__restrict__ jcore, int partner)
+// The entry (i, j) is a core edge when its pair's r^2 at the pattern's build,
+// formed as the assembly forms it (higher index minus lower), is not above
+// the squared cutoff: a bit of the count pass's within-cutoff bitmask, or,
+// for a pattern built without it, from the positions of the build.
+struct CoreTest {
+    const unsigned long long *mask;   // [n][words] or nullptr
+    int words;
+    const double *x, *y;
+    double rc2;
+};
+
+__device__ __forceinline__ bool core_entry(int i, int k, const int *__restrict__ Ai, const CoreTest &ct,
+                                           int partner)
 {
     const int j = Ai[k];
-    return j != i && j != partner && jcore[k] != 0;
+    if (j == i || j == partner) return false;
+    if (ct.mask) return (ct.mask[(size_t)i * ct.words + (j >> 6)] >> (j & 63)) & 1ull;
+    const int lo = j < i ? j : i, hi = j < i ? i : j;
+    const double dx = sdfield::normalize(ct.x[hi] - ct.x[lo]);
+    const double dy = sdfield::normalize(ct.y[hi] - ct.y[lo]);
+    return !(dx * dx + dy * dy > ct.rc2);
 }
 
 __global__ __launch_bounds__(256) void k_core_count(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
-                                                    const unsigned char *__restrict__ jcore,
-                                                    const int *__restrict__ bcol, int *__restrict__ cnt)
+                                                    CoreTest ct, const int *__restrict__ bcol,
+                                                    int *__restrict__ cnt)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int i = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -128,15 +145,15 @@ __global__ __launch_bounds__(256) void k_core_count(int n, const int *__restrict
     int c = 0;
     for (int base = Ap[i]; base < end; base += 64) {
         const int k = base + lane;
-        const bool in = k < end && core_entry(i, k, Ai, jcore, p);
+        const bool in = k < end && core_entry(i, k, Ai, ct, p);
         c += __popcll(__ballot(in));
     }
     if (lane == 0) cnt[i] = c;
 }
 
 __global__ __launch_bounds__(256) void k_core_fill(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
-                                                   const unsigned char *__restrict__ jcore,
-                                                   const int *__restrict__ bcol, const int *__restrict__ Cp,
+                                                   CoreTest ct, const int *__restrict__ bcol,
+                                                   const int *__restrict__ Cp,
                                                    int *__restrict__ Ci, int *__restrict__ Cr, int *__restrict__ Ck)
 {
     const int lane = (int)(threadIdx.x & 63);
@@ -147,7 +164,7 @@ __global__ __launch_bounds__(256) void k_core_fill(int n, const int *__restrict_
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     for (int base = Ap[i]; base < end; base += 64) {
         const int k = base + lane;
-        const bool in = k < end && core_entry(i, k, Ai, jcore, p);
+        const bool in = k < end && core_entry(i, k, Ai, ct, p);
         const unsigned long long m = __ballot(in);
         if (in) {
             const int e = pos + __popcll(m & below);
@@ -760,14 +777,15 @@ static int spai_forward(Context &c, const Fgmres &f)
     const int n = c.n;
     int rc = spai_alloc_nodes(c, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_core_count, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.jcore, f.bcol, s.cnt);
+    const CoreTest ct{c.jcmask_valid ? c.jcmask : nullptr, (n + 63) / 64, c.jx, c.y, c.jrc2};
+    hipLaunchKernelGGL(k_core_count, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, ct, f.bcol, s.cnt);
     SD_HIP(hipGetLastError());
     if ((rc = launch_scan(c, n, s.cnt, s.Cp, s.d_total))) return rc;
     long long ne = 0;
     if ((rc = host_wait(c, s.d_total, 1, &ne))) return rc;
     if ((rc = spai_alloc_edges(s, ne))) return rc;
     s.ne = ne;
-    hipLaunchKernelGGL(k_core_fill, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.jcore, f.bcol, s.Cp,
+    hipLaunchKernelGGL(k_core_fill, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, ct, f.bcol, s.Cp,
                        s.Ci, s.Cr, s.Ck);
     if (ne > 0)
         hipLaunchKernelGGL(k_core_mirror, dim3(nblk(ne)), dim3(256), 0, c.stream, (int64_t)ne, s.Cp, s.Ci, s.Cr,

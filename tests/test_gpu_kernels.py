@@ -178,8 +178,9 @@ def test_speeds_sum_rule_and_determinism_n16384():
 
 # -- K2 Jacobian -----------------------------------------------------------
 
+# 20000 columns: the count scan's second, partial tile of 16384
 @pytest.mark.parametrize("n,m,seed", [(256, 1.0, 1), (300, 1.0, 4), (1024, 2.0, 7), (64, 1.0, 3), (4096, 1.0, 9),
-                                      (4096, 2.0, 10), (4096, 4.0, 11)])
+                                      (4096, 2.0, 10), (4096, 4.0, 11), (20000, 1.0, 12)])
 def test_jacobian_pattern_bit_exact(n, m, seed):
     x, y, b = configs.dislocations(n, seed)
     h = 1e-4

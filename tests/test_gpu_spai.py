@@ -18,18 +18,24 @@ from sdddst_amd import configs  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-def _device_system(n, m, seed, h):
+def _device_system(n, m, seed, h, learned_first=False):
+    # learned_first: the learned preconditioner is selected before the
+    # assembly, whose count pass then writes the within-cutoff bitmask the
+    # core graph reads (else the core graph tests the build's positions)
     x, y, b = configs.dislocations(n, seed)
     c = S.Context(n, cutoff_multiplier=m)
     c.upload_state(0, x, y, b)
+    if learned_first:
+        c.set_preconditioner(S.PRECOND_LEARNED)
     Ap, Ai, Ax, dv, diag = c.jacobian_csc(0, h)
     rc, _ = c.cutoff()
     return c, G.system_from_jacobian(x, y, Ap, Ai, Ax, diag, rc)
 
 
-@pytest.mark.parametrize("n,m,seed,h", [(1024, 2.0, 31, 2e-5), (700, 4.0, 32, 5e-6)])
-def test_device_network_and_gradient_match_oracle(n, m, seed, h):
-    c, s = _device_system(n, m, seed, h)
+@pytest.mark.parametrize("n,m,seed,h,learned_first", [(1024, 2.0, 31, 2e-5, False), (700, 4.0, 32, 5e-6, False),
+                                                       (1024, 2.0, 34, 2e-5, True), (700, 4.0, 35, 5e-6, True)])
+def test_device_network_and_gradient_match_oracle(n, m, seed, h, learned_first):
+    c, s = _device_system(n, m, seed, h, learned_first)
     with c:
         rng = np.random.default_rng(seed)
         th = (G.init_params(7) + 0.05 * rng.standard_normal(G.NPARAM)).astype(np.float32)

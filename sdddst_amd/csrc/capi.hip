@@ -386,6 +386,8 @@ int sdddst_jacobian_csc(sdddst_ctx *ctx, int cfg_slot, double h, int rebuild, in
     if (nnz) *nnz = c.nnz;
     if (Ap) {
         SD_CHECK(Ai && Ax, SDDDST_ERR_ARG, "Ai/Ax are NULL");
+        int rc = ensure_ax(c);
+        if (rc) return rc;
         SD_HIP(hipMemcpyAsync(Ap, c.Ap, sizeof(int) * (c.n + 1), hipMemcpyDeviceToHost, c.stream));
         SD_HIP(hipMemcpyAsync(Ai, c.Ai, sizeof(int) * c.nnz, hipMemcpyDeviceToHost, c.stream));
         SD_HIP(hipMemcpyAsync(Ax, c.Ax, sizeof(double) * c.nnz, hipMemcpyDeviceToHost, c.stream));

@@ -102,7 +102,11 @@ struct Context {
     // 10 instead of 12 bytes per entry; valid for the pattern of the last fill
     unsigned short *Ai16 = nullptr;
     bool ai16_valid = false;
-    double *Ax = nullptr;               // final values
+    double *Ax = nullptr;               // final values (formed on request: ensure_ax)
+    // Ax holds the current matrix; else it is (Araw * ax_factor) times the
+    // row scales of colinfo, +1 on the diagonal
+    bool ax_valid = false;
+    double ax_factor = 1.0;
     double *AxRow = nullptr;            // CSR-of-A values (same index arrays): for SpMV
     int64_t nnz = 0, cap = 0;
     double *jtau = nullptr, *jmult = nullptr;   // tau_x and taper per strictly-lower entry
@@ -239,6 +243,9 @@ int ev_decompose(Context &c, int slot, double *eigenvalues, double *eigenvectors
 // then hipStreamSynchronize, which also reports device errors).
 int host_wait(Context &c, const void *src = nullptr, int words = 0, void *dst = nullptr);
 int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern);
+// The column-major values Ax of the current matrix (the assembly writes the
+// row-major AxRow only).
+int ensure_ax(Context &c);
 int launch_residual(Context &c, double h, int new_slot, int old_slot, int csp, int isp);
 int launch_newton_update(Context &c, int slot);
 int launch_error_control(Context &c, double h, double hmax);

@@ -31,9 +31,11 @@
 //          stored coalesced, and their ordered sum
 //          (simulation.cpp:349-374) giving the raw diagonal and d_j;
 //   scale  one wavefront per column: A = row-scaled values (+1 on the
-//          diagonal, simulation.cpp:376-383), the same matrix row-major
-//          (AxRow: row j of A on the index arrays of column j) for SpMV, and
-//          the pair partner of row j.  Stage II of a step assembles at the
+//          diagonal, simulation.cpp:376-383) stored row-major (AxRow: row j
+//          of A on the index arrays of column j), which the solvers read, and
+//          the pair partner of row j; the column-major Ax the reference
+//          returns is formed from the same factors on request (ensure_ax:
+//          downloads, the dense solver).  Stage II of a step assembles at the
 //          same positions as stage I with h/2: its unscaled values and sums
 //          are stage I's times 0.5 exactly, so only d_j and scale rerun.
 //
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__r
                 a += 1.0;
                 r += 1.0;
             }
-            __builtin_nontemporal_store(a, &Ax[k]);
+            if (Ax) __builtin_nontemporal_store(a, &Ax[k]);
             AxRow[k] = r;
             if (Ai16) Ai16[k] = (unsigned short)i;
             if (i != j) {
@@ -581,6 +583,27 @@ __global__ __launch_bounds__(256) void k_jac_scale_partner(int n, const int *__r
     if (lane == 0) {
         partner[j] = bj;
         strength[j] = best;
+    }
+}
+
+// The CSC values A = (factor * unsigned) * row scale (+1 on the diagonal),
+// formed exactly as the scaling pass forms them, for the callers that read
+// the column-major values (downloads, the dense solver, training).
+__global__ __launch_bounds__(256) void k_jac_ax(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                const int *__restrict__ diag, const double2 *__restrict__ colinfo,
+                                                const double *__restrict__ Araw, double factor,
+                                                double *__restrict__ Ax)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (j >= n) return;
+    const int dpos = diag[j];
+    const int end = Ap[j + 1];
+    for (int k = Ap[j] + lane; k < end; k += 64) {
+        const double r0 = Araw[k] * factor;
+        double a = r0 * colinfo[Ai[k]].x;
+        if (k == dpos) a += 1.0;
+        Ax[k] = a;
     }
 }
 
@@ -883,6 +906,7 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
             hipLaunchKernelGGL(k_jac_scale, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.dvec,
                                c.Ax, c.AxRow);
             SD_HIP(hipGetLastError());
+            c.ax_valid = true;
             if ((c.timing & kTimeJacobian) && (rc = timer_end(c, c.t_jac))) return rc;
             c.jac_valid = true;
             c.jac_h = h;
@@ -891,14 +915,29 @@ int launch_jacobian(Context &c, int slot, double h, bool reuse_pattern)
     }
     // the 16-bit row indices, once per pattern
     unsigned short *i16 = (c.Ai16 && !c.ai16_valid) ? c.Ai16 : nullptr;
+    // the row-major values only: the column-major ones follow on request
+    // (ensure_ax); the step's solvers read the row-major ones
     hipLaunchKernelGGL(k_jac_scale_partner, dim3(wblocks), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag,
-                       c.colinfo, c.Araw, factor, c.Ax, c.AxRow, c.partner, c.strength, i16);
+                       c.colinfo, c.Araw, factor, (double *)nullptr, c.AxRow, c.partner, c.strength, i16);
     SD_HIP(hipGetLastError());
+    c.ax_valid = false;
+    c.ax_factor = factor;
     if (c.Ai16) c.ai16_valid = true;
     c.partner_valid = true;
     if ((c.timing & kTimeJacobian) && (rc = timer_end(c, c.t_jac))) return rc;
     c.jac_valid = true;
     c.jac_h = h;
+    return SDDDST_OK;
+}
+
+int ensure_ax(Context &c)
+{
+    if (c.ax_valid) return SDDDST_OK;
+    SD_CHECK(c.jac_valid, SDDDST_ERR_STATE, "no Jacobian assembled");
+    hipLaunchKernelGGL(k_jac_ax, dim3((c.n + 3) / 4), dim3(256), 0, c.stream, c.n, c.Ap, c.Ai, c.diag, c.colinfo,
+                       c.Araw, c.ax_factor, c.Ax);
+    SD_HIP(hipGetLastError());
+    c.ax_valid = true;
     return SDDDST_OK;
 }
 

@@ -46,6 +46,21 @@ __device__ __forceinline__ void partner_row(int i, const int *__restrict__ Ap, c
 
 // Mutual partners with coupling above the threshold form a 2x2 block; the
 // row of the block inverse is stored as (binv_ii, binv_ij, j), else 1/A_ii.
+// Without Ax (symmetric pattern), A_ji is read as row j's entry at column i
+// (AxRow in column j): the same value, since both are the pair's unsigned
+// value times row j's scale.
+__device__ __forceinline__ int find_in_column(const int *__restrict__ Ap, const int *__restrict__ Ai, int col,
+                                              int row)
+{
+    int lo = Ap[col], hi = Ap[col + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (Ai[mid] < row) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void block_row(int i, const int *__restrict__ Ap, const int *__restrict__ Ai,
                                           const int *__restrict__ diag, const double *__restrict__ Ax,
                                           const double *__restrict__ AxRow, const int *__restrict__ partner,
@@ -55,13 +70,9 @@ __device__ __forceinline__ void block_row(int i, const int *__restrict__ Ap, con
     const double aii = AxRow[diag[i]];
     const int j = partner[i];
     if (j >= 0 && partner[j] == i && strength[i] > threshold) {
-        int lo = Ap[i], hi = Ap[i + 1];
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (Ai[mid] < j) lo = mid + 1;
-            else hi = mid;
-        }
-        const double aij = AxRow[lo], aji = Ax[lo], ajj = AxRow[diag[j]];
+        const int lo = find_in_column(Ap, Ai, i, j);
+        const double aij = AxRow[lo], ajj = AxRow[diag[j]];
+        const double aji = Ax ? Ax[lo] : AxRow[find_in_column(Ap, Ai, j, i)];
         const double det = aii * ajj - aij * aji;
         binv[2 * i] = ajj / det;
         binv[2 * i + 1] = -aij / det;

@@ -74,6 +74,7 @@ int DenseLu::factor(Context &c)
     if (rc) return rc;
     const int n = c.n;
     const size_t total = (size_t)n * n;
+    if ((rc = ensure_ax(c))) return rc;
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, c.stream, A, total);
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_scatter_dense, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.Ax, A);

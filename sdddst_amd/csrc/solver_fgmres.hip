@@ -338,11 +338,12 @@ __global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const dou
     if (i < n) z[i] = dinv[i] * v[i];
 }
 
-__global__ void k_jacobi_setup(int n, const int *__restrict__ diag, const double *__restrict__ Ax,
+// (the diagonal is the same in the row- and column-major values)
+__global__ void k_jacobi_setup(int n, const int *__restrict__ diag, const double *__restrict__ AxRow,
                                double *__restrict__ dinv)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i < n) dinv[i] = 1.0 / Ax[diag[i]];
+    if (i < n) dinv[i] = 1.0 / AxRow[diag[i]];
 }
 
 // Pair-block Jacobi (row bodies in precond_core.h), one wavefront per row.
@@ -585,12 +586,16 @@ int Fgmres::pair_factor(Context &c)
     // the fused scaling pass of the assembly leaves the partners of this
     // matrix in the context; otherwise (compacted pattern) find them here
     if (!c.partner_valid) {
+        if ((rc = ensure_ax(c))) return rc;
         hipLaunchKernelGGL(k_pair_partner, dim3((n + 3) / 4), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax,
                            c.AxRow, c.partner, c.strength);
         c.partner_valid = true;
     }
-    hipLaunchKernelGGL(k_pair_blocks, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag, c.Ax, c.AxRow,
-                       c.partner, c.strength, 1e-3, binv, bcol);
+    // without the column-major values the blocks read A_ji from the mirror
+    // entry, which a symmetric pattern has
+    if (!c.jac_symmetric && (rc = ensure_ax(c))) return rc;
+    hipLaunchKernelGGL(k_pair_blocks, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.diag,
+                       c.ax_valid ? c.Ax : nullptr, c.AxRow, c.partner, c.strength, 1e-3, binv, bcol);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }
@@ -623,7 +628,7 @@ int Fgmres::factor(Context &c)
     c.precond_active = c.precond;
     if (c.precond == SDDDST_PRECOND_STRONG) return strong_factor(c);
     if (c.precond == SDDDST_PRECOND_JACOBI) {
-        hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.Ax, dinv);
+        hipLaunchKernelGGL(k_jacobi_setup, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.diag, c.AxRow, dinv);
         SD_HIP(hipGetLastError());
         return SDDDST_OK;
     }

@@ -205,8 +205,8 @@ __device__ __forceinline__ double group_sum(double s)
 // 16-byte feature stores), the row sums close with a butterfly in the group.
 template <int G>
 __global__ __launch_bounds__(256) void k_features(int n, const int *__restrict__ Cp, const int *__restrict__ Ci,
-                                                  const int *__restrict__ Ck, const int *__restrict__ diag,
-                                                  const double *__restrict__ Ax, const double *__restrict__ AxRow,
+                                                  const int *__restrict__ Ck, const int *__restrict__ Cmir,
+                                                  const int *__restrict__ diag, const double *__restrict__ AxRow,
                                                   const double *__restrict__ binv, const int *__restrict__ bcol,
                                                   float *__restrict__ U, float *__restrict__ E,
                                                   double *__restrict__ se, double *__restrict__ sd)
@@ -219,8 +219,9 @@ __global__ __launch_bounds__(256) void k_features(int n, const int *__restrict__
         bdi = binv[2 * i];
         const int end = Cp[i + 1];
         for (int e = Cp[i] + g; e < end; e += G) {
+            // A_ji: row j's entry at column i, the mirror edge's
             const int k = Ck[e], j = Ci[e];
-            const double aij = AxRow[k], aji = Ax[k], bdj = binv[2 * j];
+            const double aij = AxRow[k], aji = AxRow[Ck[Cmir[e]]], bdj = binv[2 * j];
             const double xe = bdi * aij, ye = aij * bdj, we = bdj * aji;
             se[e] = -bdi * aij * bdj / (1.0 + fabs(xe) + fabs(we));
             t += xe * we / (1.0 + fabs(xe * we));
@@ -687,7 +688,7 @@ template <int G>
 static void launch_features(Context &c, const Spai &s, const Fgmres &f)
 {
     hipLaunchKernelGGL(k_features<G>, dim3(nblk((int64_t)c.n * G)), dim3(256), 0, c.stream, c.n, s.Cp, s.Ci, s.Ck,
-                       c.diag, c.Ax, c.AxRow, f.binv, f.bcol, s.U, s.E, s.se, s.sd);
+                       s.Cmir, c.diag, c.AxRow, f.binv, f.bcol, s.U, s.E, s.se, s.sd);
 }
 
 template <int G>
@@ -862,6 +863,7 @@ int spai_train_step(Context &c, Fgmres &f, int k, unsigned long long seed, doubl
     hipLaunchKernelGGL(k_loss_rows, dim3(nblk(n)), dim3(256), 0, c.stream, k, n, eps, s.W, s.R, s.lrow);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, c.stream, n, s.lrow, 1.0 / n, s.loss);
     // dY = A^T dR (CSC values as the CSR of A^T)
+    if ((rc = ensure_ax(c))) return rc;
     hipLaunchKernelGGL(k_spmv_batch, dim3(nblk(n, 4), k), dim3(256), 0, c.stream, n, c.Ap, c.Ai, c.Ax, s.R, s.Y);
     hipLaunchKernelGGL(k_dgamma, dim3(nblk(n)), dim3(256), 0, c.stream, n, k, s.Cp, s.Ci, s.Y, s.W, s.se, s.sd, s.dge,
                        s.dgd);

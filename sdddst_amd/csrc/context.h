@@ -120,6 +120,7 @@ struct Context {
     // the same relation as a [n][ceil(n/64)] bitmask, written by the count
     // pass of a build while the learned preconditioner is selected
     unsigned long long *jcmask = nullptr;
+    int *jccnt = nullptr;                       // set bits per column of jcmask
     bool jcmask_valid = false;
     double *jpd = nullptr;                      // point-defect sum per column (h-free)
     unsigned long long *jmask = nullptr;        // [n][ceil(n/64)] membership bitmask

@@ -60,15 +60,16 @@ __device__ __forceinline__ int find_row(const int *__restrict__ Ai, int lo, int 
 // Membership bitmask of column j (mj[words]) and its entry count; the
 // predicate runs once per ordered pair with a cheap r^2 pre-filter.  With
 // cmj, also the bitmask of the off-diagonal rows whose pair lies within the
-// cutoff (r^2 <= rc^2: the untapered entries).
+// cutoff (r^2 <= rc^2: the untapered entries) and its count into *ccnt_j.
 __device__ __forceinline__ void count_column(int j, int n, int words, const double *__restrict__ x,
                                              const double *__restrict__ y, const CutParams &cp,
                                              int *__restrict__ cnt_j, unsigned long long *__restrict__ mj,
-                                             unsigned long long *__restrict__ cmj = nullptr)
+                                             unsigned long long *__restrict__ cmj = nullptr,
+                                             int *__restrict__ ccnt_j = nullptr)
 {
     const int lane = (int)(threadIdx.x & 63);
     const double xj = x[j], yj = y[j];
-    int c = 0;
+    int c = 0, cc = 0;
     for (int w = 0; w < words; ++w) {
         const int i = w * 64 + lane;
         bool in = false, core = false;
@@ -91,9 +92,11 @@ __device__ __forceinline__ void count_column(int j, int n, int words, const doub
         if (cmj) {
             const unsigned long long cm = __ballot(core);
             if (lane == 0) cmj[w] = cm;
+            cc += __popcll(cm);
         }
     }
     if (lane == 0) *cnt_j = c;
+    if (cmj && lane == 0) *ccnt_j = cc;
 }
 
 // Rows of column j in ascending order from its bitmask, starting at base;

@@ -59,12 +59,12 @@ using jac::CutParams;
 __global__ __launch_bounds__(256) void k_jac_count(int n, int words, const double *__restrict__ x,
                                                    const double *__restrict__ y, CutParams cp,
                                                    int *__restrict__ cnt, unsigned long long *__restrict__ mask,
-                                                   unsigned long long *__restrict__ cmask)
+                                                   unsigned long long *__restrict__ cmask, int *__restrict__ ccnt)
 {
     const int j = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (j >= n) return;
     jac::count_column(j, n, words, x, y, cp, cnt + j, mask + (size_t)j * words,
-                      cmask ? cmask + (size_t)j * words : nullptr);
+                      cmask ? cmask + (size_t)j * words : nullptr, cmask ? ccnt + j : nullptr);
 }
 
 // Cell binning for the count: the unit cell is cut into G x G periodic cells
@@ -177,7 +177,8 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, int G, const int *_
 // Count of column j from the 3 x 3 neighbouring cells: the same predicate as
 // count_column, bits gathered in an LDS copy of the column's bitmask (the
 // order of the candidates does not matter), then written out whole; with
-// cmask, the within-cutoff bitmask of count_column the same way.
+// cmask, the within-cutoff bitmask of count_column the same way (and its
+// count into ccnt).
 constexpr int kCellMaxWords = 1024;
 __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const double *__restrict__ x,
                                                          const double *__restrict__ y, CutParams cp, int G,
@@ -187,7 +188,8 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
                                                          const double *__restrict__ sx,
                                                          const double *__restrict__ sy, int *__restrict__ cnt,
                                                          unsigned long long *__restrict__ mask,
-                                                         unsigned long long *__restrict__ cmask)
+                                                         unsigned long long *__restrict__ cmask,
+                                                         int *__restrict__ ccnt)
 {
     extern __shared__ unsigned long long lm[];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -240,7 +242,14 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
     }
     if (cmask) {
         unsigned long long *cj = cmask + (size_t)j * words;
-        for (int w = lane; w < words; w += 64) cj[w] = cm[w];
+        int cc = 0;
+        for (int w = lane; w < words; w += 64) {
+            const unsigned long long v = cm[w];
+            cj[w] = v;
+            cc += __popcll(v);
+        }
+        for (int off = 32; off > 0; off >>= 1) cc += __shfl_down(cc, off, 64);
+        if (lane == 0) ccnt[j] = cc;
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
     if (lane == 0) cnt[j] = c;
@@ -722,6 +731,7 @@ static int build_pattern(Context &c, int slot, double h)
         set_error("device allocation of the %d x %d within-cutoff bitmask failed", n, words);
         return SDDDST_ERR_NOMEM;
     }
+    if (want_core && !c.jccnt) SD_HIP(hipMalloc(&c.jccnt, sizeof(int) * n));
     unsigned long long *cmask = want_core ? c.jcmask : nullptr;
     // cells at least the pre-filter radius wide; 3 x 3 neighbourhoods need G >= 3
     const double rpre = sqrt(cp.prefilter);
@@ -751,10 +761,10 @@ static int build_pattern(Context &c, int slot, double h)
                            x, c.y, c.ccursor, c.csorted, c.csx, c.csy);
         hipLaunchKernelGGL(k_jac_count_cells, dim3(wblocks), dim3(256),
                            sizeof(unsigned long long) * (cmask ? 8 : 4) * words, c.stream, n, words, x, c.y, cp, G,
-                           c.cell, c.cstart, c.csorted, c.csx, c.csy, c.colcount, c.jmask, cmask);
+                           c.cell, c.cstart, c.csorted, c.csx, c.csy, c.colcount, c.jmask, cmask, c.jccnt);
     } else {
         hipLaunchKernelGGL(k_jac_count, dim3(wblocks), dim3(256), 0, c.stream, n, words, x, c.y, cp, c.colcount,
-                           c.jmask, cmask);
+                           c.jmask, cmask, c.jccnt);
     }
     c.jcmask_valid = want_core;
     SD_HIP(hipGetLastError());

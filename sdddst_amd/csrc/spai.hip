@@ -113,11 +113,10 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
 
 // The entry (i, j) is a core edge when its pair's r^2 at the pattern's build,
 // formed as the assembly forms it (higher index minus lower), is not above
-// the squared cutoff: a bit of the count pass's within-cutoff bitmask, or,
-// for a pattern built without it, from the positions of the build.
+// the squared cutoff.  For a pattern built with the within-cutoff bitmask the
+// graph comes from its bits (k_core_count_mask, k_core_fill_mask); else this
+// test runs on the positions of the build.
 struct CoreTest {
-    const unsigned long long *mask;   // [n][words] or nullptr
-    int words;
     const double *x, *y;
     double rc2;
 };
@@ -127,7 +126,6 @@ __device__ __forceinline__ bool core_entry(int i, int k, const int *__restrict__
 {
     const int j = Ai[k];
     if (j == i || j == partner) return false;
-    if (ct.mask) return (ct.mask[(size_t)i * ct.words + (j >> 6)] >> (j & 63)) & 1ull;
     const int lo = j < i ? j : i, hi = j < i ? i : j;
     const double dx = sdfield::normalize(ct.x[hi] - ct.x[lo]);
     const double dy = sdfield::normalize(ct.y[hi] - ct.y[lo]);
@@ -177,6 +175,61 @@ __global__ __launch_bounds__(256) void k_core_fill(int n, const int *__restrict_
 }
 
 // Cmir[e] for e = (i, j): the edge (j, i), by binary search of row j.
+// The same graph from the count pass's within-cutoff bitmask (row i's bits
+// are its core rows; the diagonal is never set): the count is the row's
+// total less its partner, and the rows come out of the mask words in
+// ascending order, each entry's position in A from the pattern's mask word
+// and word start.
+__global__ void k_core_count_mask(int n, int words, const unsigned long long *__restrict__ cmask,
+                                  const int *__restrict__ ccnt, const int *__restrict__ bcol, int *__restrict__ cnt)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int p = bcol[i];
+    int c = ccnt[i];
+    if (p >= 0 && ((cmask[(size_t)i * words + (p >> 6)] >> (p & 63)) & 1ull)) --c;
+    cnt[i] = c;
+}
+
+__global__ __launch_bounds__(256) void k_core_fill_mask(int n, int words, const unsigned long long *__restrict__ cmask,
+                                                        const unsigned long long *__restrict__ mask,
+                                                        const int *__restrict__ wstart, const int *__restrict__ bcol,
+                                                        const int *__restrict__ Cp, int *__restrict__ Ci,
+                                                        int *__restrict__ Cr, int *__restrict__ Ck)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int i = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= n) return;
+    const int p = bcol[i];
+    const size_t row = (size_t)i * words;
+    int pos = Cp[i];
+    for (int w0 = 0; w0 < words; w0 += 64) {
+        const int w = w0 + lane;
+        unsigned long long bits = w < words ? cmask[row + w] : 0ull;
+        if (p >= 0 && (p >> 6) == w) bits &= ~(1ull << (p & 63));
+        const int pc = __popcll(bits);
+        int incl = pc;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        int e = pos + incl - pc;
+        if (bits) {
+            const unsigned long long pm = mask[row + w];
+            const int ws = wstart[row + w];
+            while (bits) {
+                const int b = __builtin_ctzll(bits);
+                Ci[e] = w * 64 + b;
+                Cr[e] = i;
+                Ck[e] = ws + __popcll(pm & ((1ull << b) - 1ull));
+                ++e;
+                bits &= bits - 1ull;
+            }
+        }
+        pos += __shfl(incl, 63, 64);
+    }
+}
+
 __global__ void k_core_mirror(int64_t ne, const int *__restrict__ Cp, const int *__restrict__ Ci,
                               const int *__restrict__ Cr, int *__restrict__ Cmir)
 {
@@ -778,16 +831,26 @@ static int spai_forward(Context &c, const Fgmres &f)
     const int n = c.n;
     int rc = spai_alloc_nodes(c, s);
     if (rc) return rc;
-    const CoreTest ct{c.jcmask_valid ? c.jcmask : nullptr, (n + 63) / 64, c.jx, c.y, c.jrc2};
-    hipLaunchKernelGGL(k_core_count, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, ct, f.bcol, s.cnt);
+    const int words = (n + 63) / 64;
+    const bool from_mask = c.jcmask_valid;
+    const CoreTest ct{c.jx, c.y, c.jrc2};
+    if (from_mask)
+        hipLaunchKernelGGL(k_core_count_mask, dim3(nblk(n)), dim3(256), 0, c.stream, n, words, c.jcmask, c.jccnt,
+                           f.bcol, s.cnt);
+    else
+        hipLaunchKernelGGL(k_core_count, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, ct, f.bcol, s.cnt);
     SD_HIP(hipGetLastError());
     if ((rc = launch_scan(c, n, s.cnt, s.Cp, s.d_total))) return rc;
     long long ne = 0;
     if ((rc = host_wait(c, s.d_total, 1, &ne))) return rc;
     if ((rc = spai_alloc_edges(s, ne))) return rc;
     s.ne = ne;
-    hipLaunchKernelGGL(k_core_fill, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, ct, f.bcol, s.Cp,
-                       s.Ci, s.Cr, s.Ck);
+    if (from_mask)
+        hipLaunchKernelGGL(k_core_fill_mask, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, words, c.jcmask, c.jmask,
+                           c.jwstart, f.bcol, s.Cp, s.Ci, s.Cr, s.Ck);
+    else
+        hipLaunchKernelGGL(k_core_fill, dim3(nblk(n, 4)), dim3(256), 0, c.stream, n, c.Ap, c.Ai, ct, f.bcol, s.Cp,
+                           s.Ci, s.Cr, s.Ck);
     if (ne > 0)
         hipLaunchKernelGGL(k_core_mirror, dim3(nblk(ne)), dim3(256), 0, c.stream, (int64_t)ne, s.Cp, s.Ci, s.Cr,
                            s.Cmir);

@@ -363,7 +363,27 @@ __global__ __launch_bounds__(256) void k_gnn_mp(int n, int l, const float *__res
         const float cc = L[lay::LC + ch];
         const int e0 = Cp[i], e1 = Cp[i + 1];
         float acc = 0.f;
-        for (int e = e0; e < e1; ++e) {
+        // four edges' loads issued before their arithmetic (same order of
+        // additions)
+        int e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            int jj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) jj[u] = Ci[e + u];
+            float zv[4];
+            float4 ef[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                zv[u] = z[(size_t)jj[u] * 32 + ch];
+                ef[u] = *reinterpret_cast<const float4 *>(E + 4 * (size_t)(e + u));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float pe = zv[u] + cc + q0 * ef[u].x + q1 * ef[u].y + q2 * ef[u].z + q3 * ef[u].w;
+                acc += fmaxf(pe, 0.f);
+            }
+        }
+        for (; e < e1; ++e) {
             const float *ef = E + 4 * (size_t)e;
             const float pe = z[(size_t)Ci[e] * 32 + ch] + cc + q0 * ef[0] + q1 * ef[1] + q2 * ef[2] + q3 * ef[3];
             acc += fmaxf(pe, 0.f);

@@ -62,7 +62,7 @@ static void free_all(Context &c)
                     c.jcmask, c.jccnt,
                     c.cell, c.csorted, c.ccount, c.cstart, c.ccursor, c.perm, c.xkey, c.xkey_sorted, c.xidx,
                     c.sort_tmp, c.Araw, c.sraw, c.partner, c.strength, c.jwstart, c.csx, c.csy, c.colinfo,
-                    c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext, c.Ai16, c.gnorm2, c.gpart, c.gticket};
+                    c.ytc, c.yts, c.yw, c.xw, c.xep, c.xem, c.bext, c.farx_mom, c.farx_mom1, c.vfar, c.Ai16, c.gnorm2, c.gpart, c.gticket};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hs) (void)hipHostFree(c.hs);

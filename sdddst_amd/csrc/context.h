@@ -89,6 +89,9 @@ struct Context {
     double *yw = nullptr;                    // y - rint(y) per dislocation (set with y)
     double *xw = nullptr, *xep = nullptr, *xem = nullptr;   // x - rint(x), e^{+-2 pi (x - rint(x))}
     double *bext = nullptr;                  // per 64-block extents of xw and yw (pairtile::kExtWords each)
+    // series of the x-separated tiles: source moments per block, their
+    // n = 1 squares, and the tiles' share of each speed (sorted positions)
+    double *farx_mom = nullptr, *farx_mom1 = nullptr, *vfar = nullptr;
     // Newton
     double *dvec = nullptr, *g = nullptr, *delta = nullptr;
     // ||g||^2 formed by the residual kernel (block partials, last-block

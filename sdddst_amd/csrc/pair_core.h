@@ -59,6 +59,7 @@ struct FarData {
     const double *w = nullptr, *ep = nullptr, *em = nullptr;   // x - rint(x), e^{+-2 pi w}
     const double *yw = nullptr;                  // y - rint(y)
     const double *ext = nullptr;                 // [nb][kExtWords]
+    int expand = 0;                              // x-separated tiles summed elsewhere (far_expand.h)
 };
 
 // Extents of one block of tile positions: the wrapped x and y of its

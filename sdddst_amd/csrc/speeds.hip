@@ -38,6 +38,7 @@
 #include <mutex>
 
 #include "context.h"
+#include "far_expand.h"
 #include "field.h"
 #include "pair_core.h"
 #include "step_core.h"
@@ -61,6 +62,21 @@ __global__ __launch_bounds__(64) void k_pair_tiles(int n, int nb, int nfull, con
                                                    pairtile::FarData fd)
 {
     const int u = (int)blockIdx.x;
+    if constexpr (Mode == SDDDST_FIELD_FAST) {
+        // an x-separated tile is summed by its series (k_farx_gather): its
+        // partial sums are zero here
+        if (fd.expand && u < nfull) {
+            int bi, bj;
+            pairtile::strict_coords(u, bi, bj);
+            if (farx::tile_code(fd.ext + (size_t)bi * pairtile::kExtWords, fd.ext + (size_t)bj * pairtile::kExtWords)) {
+                const int l = (int)threadIdx.x;
+                const int p = bi * kTile + l, q = bj * kTile + l;
+                if (p < n) partial[(size_t)bj * n + p] = 0.0;
+                if (q < n) partial[(size_t)bi * n + q] = 0.0;
+                return;
+            }
+        }
+    }
     __shared__ pairtile::TileLds L;
     // The fast field's table lookups (cos and exp, per lane at data-dependent
     // indices) read an LDS copy: 32-bit LDS addressing instead of 64-bit
@@ -173,9 +189,10 @@ __global__ void k_speeds_finish(int n, int nb, int pc, const int *__restrict__ p
                                 const double *__restrict__ px, const double *__restrict__ py, double A, double K,
                                 double sigma, const unsigned long long *__restrict__ minr2,
                                 double *__restrict__ tol, double minPrecSqr, int update_tol,
-                                double *__restrict__ v)
+                                const double *__restrict__ vfar, double *__restrict__ v)
 {
-    // partial holds sorted positions; perm maps position q to dislocation i
+    // partial holds sorted positions; perm maps position q to dislocation i;
+    // vfar (nullable) the share of the x-separated tiles at the same positions
     const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= n) return;
     const int i = perm[q];
@@ -189,6 +206,7 @@ __global__ void k_speeds_finish(int n, int nb, int pc, const int *__restrict__ p
         for (int u = 0; u < 8; ++u) acc += p[u];
     }
     for (; s < nb; ++s) acc += partial[(size_t)s * n + q];
+    if (vfar) acc += vfar[q];
     double t = update_tol ? tol[i] : 0.0;
     if (update_tol) {
         const unsigned long long bits = minr2[i];
@@ -292,6 +310,172 @@ __global__ __launch_bounds__(64) void k_speeds_prep_blocks(int n, const int *__r
     }
 }
 
+// Source moments of the x-separated tiles (far_expand.h), one workgroup of
+// four wavefronts per block of tile positions: lane m holds member m, wave v
+// the terms n = v + 1, v + 5, ...; each moment is a butterfly sum over the
+// 64 lanes (fixed order).
+__global__ __launch_bounds__(256) void k_farx_moments(int n, const int *__restrict__ perm,
+                                                      const double *__restrict__ w, const double *__restrict__ ca,
+                                                      const double *__restrict__ sa, const double *__restrict__ b,
+                                                      const double *__restrict__ ext, double *__restrict__ mom,
+                                                      double *__restrict__ mom1)
+{
+    using namespace farx;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int J = (int)blockIdx.x;
+    const int p = J * 64 + lane;
+    const double X = centre(ext + (size_t)J * pairtile::kExtWords);
+    double xi = 0.0, bb = 0.0;
+    Cplx c{1.0, 0.0};
+    if (p < n) {
+        const int j = perm[p];
+        xi = w[j] - X;
+        bb = b[j];
+        c = Cplx{ca[j], -sa[j]};
+    }
+    const Cplx fp = cscale(c, sdlibm::exp(sdfield::kTwoPi * xi));    // e^{2 pi xi} e^{-2 pi i y}
+    const Cplx fm = cscale(c, sdlibm::exp(-sdfield::kTwoPi * xi));
+    const Cplx step_p = cpow(fp, 4), step_m = cpow(fm, 4);
+    Cplx pp = cpow(fp, wave + 1), pm = cpow(fm, wave + 1);
+    auto wsum = [&](double v) {
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        return v;
+    };
+    for (int t = wave + 1; t <= kTerms; t += 4) {
+        double v[8] = {bb * pp.re, bb * pp.im, bb * xi * pp.re, bb * xi * pp.im,
+                       bb * pm.re, bb * pm.im, bb * xi * pm.re, bb * xi * pm.im};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = wsum(v[k]);
+        if (lane == 0) {
+            double *o = mom + ((size_t)J * kTerms + (t - 1)) * kWords;
+            o[0] = sdlibm::exp(sdfield::kTwoPi * t * X);
+            o[1] = sdlibm::exp(-sdfield::kTwoPi * t * X);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) o[2 + k] = v[k];
+        }
+        if (t == 1) {
+            const double x2 = xi * xi;
+            double u[4] = {bb * x2 * pp.re, bb * x2 * pp.im, bb * x2 * pm.re, bb * x2 * pm.im};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) u[k] = wsum(u[k]);
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) mom1[(size_t)J * kWords1 + k] = u[k];
+        }
+        pp = cmul(pp, step_p);
+        pm = cmul(pm, step_m);
+    }
+}
+
+// Per target block I: the coefficients gathered over the source blocks of
+// its x-separated tiles (wave v takes the blocks J = v, v + 4, ...; lane
+// t - 1 the term t), summed over the four waves in order, then every member
+// of the block sums its series (lane quarter k the terms 15 k + 1 ..
+// 15 k + 15, added in order): vfar[p] = b_i 4 pi^2 sum_n n Re(...), the
+// tiles' share of the member's speed (the pair kernel skips them).
+__global__ __launch_bounds__(256) void k_farx_gather(int n, int nb, const int *__restrict__ perm,
+                                                     const double *__restrict__ w, const double *__restrict__ ca,
+                                                     const double *__restrict__ sa, const double *__restrict__ b,
+                                                     const double *__restrict__ ext, const double *__restrict__ mom,
+                                                     const double *__restrict__ mom1, double *__restrict__ vfar)
+{
+    using namespace farx;
+    static_assert(kTerms <= 64 && kTerms % 4 == 0, "one lane per term, four quarters");
+    __shared__ double acc[4][64][12];
+    __shared__ double part[4][64];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int I = (int)blockIdx.x;
+    const int t = lane + 1;
+    const bool live = t <= kTerms;
+    const double *eI = ext + (size_t)I * pairtile::kExtWords;
+    const double XI = centre(eI);
+    Cplx Lm[3] = {}, Lp[3] = {};   // psi- and psi+ families, powers 0..2 of xi_i
+    if (live) {
+        Poly alpha, beta;
+        term_polys(t, alpha, beta);
+        const double Gp = sdlibm::exp(sdfield::kTwoPi * t), Gm = sdlibm::exp(-sdfield::kTwoPi * t);
+        const double *mI = mom + ((size_t)I * kTerms + (t - 1)) * kWords;
+        const double EXI = mI[0], EXIinv = mI[1];
+        const bool withB = t <= kTermsB;
+        for (int J = wave; J < nb; J += 4) {
+            if (J == I) continue;
+            const int lo = I < J ? I : J, hi = I < J ? J : I;
+            const int code = tile_code(ext + (size_t)lo * pairtile::kExtWords, ext + (size_t)hi * pairtile::kExtWords);
+            if (!code) continue;
+            int sigma;
+            double o;
+            code_side(code, sigma, o);
+            if (I == hi) {
+                sigma = -sigma;
+                o = -o;
+            }
+            const double D = XI - centre(ext + (size_t)J * pairtile::kExtWords) + o;
+            const double *mJ = mom + ((size_t)J * kTerms + (t - 1)) * kWords;
+            const double EXJ = mJ[0], EXJinv = mJ[1];
+            const Cplx M0p{mJ[2], mJ[3]}, M1p{mJ[4], mJ[5]}, M0m{mJ[6], mJ[7]}, M1m{mJ[8], mJ[9]};
+            Cplx M2p{0.0, 0.0}, M2m{0.0, 0.0};
+            if (t == 1) {
+                const double *m1 = mom1 + (size_t)J * kWords1;
+                M2p = Cplx{m1[0], m1[1]};
+                M2m = Cplx{m1[2], m1[3]};
+            }
+            // e^{-+2 pi t o}
+            const double eo_m = o > 0 ? Gm : (o < 0 ? Gp : 1.0);
+            const double eo_p = o > 0 ? Gp : (o < 0 ? Gm : 1.0);
+            if (sigma > 0) {
+                // e^{-2 pi t D}: alpha terms, psi- with the + moments
+                gather(Lm, EXIinv * EXJ * eo_m, D, alpha, 1.0, -1.0, M0p, M1p, M2p);
+                // e^{-2 pi t (1 - D)}: beta terms, psi+ with the - moments
+                if (withB) gather(Lp, Gm * EXI * EXJinv * eo_p, D, beta, 1.0, -1.0, M0m, M1m, M2m);
+            } else {
+                // |dx| = D' - xi_i + xi_j, D' = -D, tau = -tau(|dx|)
+                gather(Lp, -(EXI * EXJinv * eo_p), -D, alpha, -1.0, 1.0, M0m, M1m, M2m);
+                if (withB) gather(Lm, -(Gm * EXIinv * EXJ * eo_m), -D, beta, -1.0, 1.0, M0p, M1p, M2p);
+            }
+        }
+    }
+    {
+        double *a = acc[wave][lane];
+        a[0] = Lm[0].re; a[1] = Lm[0].im; a[2] = Lm[1].re; a[3] = Lm[1].im; a[4] = Lm[2].re; a[5] = Lm[2].im;
+        a[6] = Lp[0].re; a[7] = Lp[0].im; a[8] = Lp[1].re; a[9] = Lp[1].im; a[10] = Lp[2].re; a[11] = Lp[2].im;
+    }
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < 64 * 12; k += 256) {
+        const int l = k / 12, f = k % 12;
+        acc[0][l][f] = ((acc[0][l][f] + acc[1][l][f]) + acc[2][l][f]) + acc[3][l][f];
+    }
+    __syncthreads();
+    // the block's members: lane m, quarter `wave` of the terms
+    const int p = I * 64 + lane;
+    double s = 0.0, bI = 0.0;
+    if (p < n) {
+        const int i = perm[p];
+        bI = b[i];
+        const double xi = w[i] - XI;
+        const Cplx cy{ca[i], sa[i]};
+        const Cplx gm = cscale(cy, sdlibm::exp(-sdfield::kTwoPi * xi));   // psi-
+        const Cplx gp = cscale(cy, sdlibm::exp(sdfield::kTwoPi * xi));    // psi+
+        constexpr int kQ = kTerms / 4;
+        const int t0 = wave * kQ + 1;
+        Cplx pm = cpow(gm, t0), pp = cpow(gp, t0);
+        for (int tt = t0; tt < t0 + kQ; ++tt) {
+            const double *a = acc[0][tt - 1];
+            const Cplx Lm0{a[0], a[1]}, Lm1{a[2], a[3]}, Lm2{a[4], a[5]};
+            const Cplx Lp0{a[6], a[7]}, Lp1{a[8], a[9]}, Lp2{a[10], a[11]};
+            const Cplx cm = cadd(Lm0, cscale(cadd(Lm1, cscale(Lm2, xi)), xi));
+            const Cplx cp = cadd(Lp0, cscale(cadd(Lp1, cscale(Lp2, xi)), xi));
+            const double re = (pm.re * cm.re - pm.im * cm.im) + (pp.re * cp.re - pp.im * cp.im);
+            s += (double)tt * re;
+            pm = cmul(pm, gm);
+            pp = cmul(pp, gp);
+        }
+    }
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && p < n)
+        vfar[p] = bI * (4.0 * sdfield::kPiSq) * (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+}
+
 }  // namespace
 
 int pair_order_alloc(Context &c)
@@ -307,6 +491,10 @@ int pair_order_alloc(Context &c)
     for (double **p : {&c.ytc, &c.yts, &c.yw, &c.xw, &c.xep, &c.xem})
         SD_HIP(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
     SD_HIP(hipMalloc((void **)&c.bext, sizeof(double) * pairtile::kExtWords * (size_t)(c.nb > 0 ? c.nb : 1)));
+    const size_t nbb = (size_t)(c.nb > 0 ? c.nb : 1);
+    SD_HIP(hipMalloc((void **)&c.farx_mom, sizeof(double) * farx::kTerms * farx::kWords * nbb));
+    SD_HIP(hipMalloc((void **)&c.farx_mom1, sizeof(double) * farx::kWords1 * nbb));
+    SD_HIP(hipMalloc((void **)&c.vfar, sizeof(double) * (size_t)(n > 0 ? n : 1)));
     return SDDDST_OK;
 }
 
@@ -382,8 +570,17 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         fd.em = c.xem;
         fd.yw = c.yw;
         fd.ext = c.bext;
+        fd.expand = nb >= 2 ? 1 : 0;
     }
+    // the pair timer spans the series of the x-separated tiles too
     if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
+    if (fd.expand) {
+        hipLaunchKernelGGL(k_farx_moments, dim3(nb), dim3(256), 0, c.stream, n, c.perm, c.xw, c.ytc, c.yts, c.b,
+                           c.bext, c.farx_mom, c.farx_mom1);
+        hipLaunchKernelGGL(k_farx_gather, dim3(nb), dim3(256), 0, c.stream, n, nb, c.perm, c.xw, c.ytc, c.yts, c.b,
+                           c.bext, c.farx_mom, c.farx_mom1, c.vfar);
+        SD_HIP(hipGetLastError());
+    }
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,
@@ -397,7 +594,7 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(threads), 0, c.stream, n, nb, c.pc, c.perm, c.partial,
                        c.x[slot],
                        c.y, c.b, c.px, c.py, c.p.A, c.p.KASQR, sigma, c.minr2, c.tol, minPrecSqr, update_tol,
-                       c.v[out_buf]);
+                       fd.expand ? c.vfar : nullptr, c.v[out_buf]);
     SD_HIP(hipGetLastError());
     if ((c.timing & kTimeSpeeds) && (rc = timer_end(c, c.t_speeds))) return rc;
     return SDDDST_OK;
@@ -499,7 +696,7 @@ int speeds_noalloc(const double *x, const double *y, const double *b, int n, dou
     }
     SD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_speeds_finish, dim3(blocks), dim3(256), 0, stream, n, nb, 0, perm, partial, x, y, b,
-                       nullptr, nullptr, 0.0, 0.0, sigma, minr2, nullptr, 0.0, 0, v);
+                       nullptr, nullptr, 0.0, 0.0, sigma, minr2, nullptr, 0.0, 0, nullptr, v);
     SD_HIP(hipGetLastError());
     return SDDDST_OK;
 }

@@ -37,8 +37,8 @@
 namespace sdamd {
 namespace farx {
 
-constexpr double kMinDx = 0.12;
-constexpr int kTerms = 60;    // n = 1..kTerms (one lane each)
+constexpr double kMinDx = 0.11;
+constexpr int kTerms = 64;    // n = 1..kTerms (one lane each)
 constexpr int kTermsB = 14;   // the beta terms, n = 1..kTermsB
 // per block and term n: e^{2 pi n X}, e^{-2 pi n X}, then the complex
 // moments M0+, M1+, M0-, M1- (xi^p e^{+-2 pi n xi} b e^{-2 pi i n y})
@@ -50,20 +50,27 @@ constexpr int kWords1 = 4;
 __device__ __forceinline__ double centre(const double *__restrict__ ext) { return 0.5 * (ext[4] + ext[5]); }
 
 // Whether the tile of target block t and source block s qualifies, from the
-// blocks' raw wrapped-x extents (ext[4], ext[5]; NaN-free for every block
-// with a member): 0 no; 1 s in [kMinDx, 1/2) (dx = s); 2 s in (-1/2,
+// blocks' raw wrapped-x extents (minimum and maximum; NaN-free for every
+// block with a member): 0 no; 1 s in [kMinDx, 1/2) (dx = s); 2 s in (-1/2,
 // -kMinDx] (dx = s); 3 s in (1/2, 1 - kMinDx] (dx = s - 1); 4 s in
 // [-1 + kMinDx, -1/2) (dx = s + 1).  The margins at the half and whole
 // periods are those of the pair kernel's far kinds.
-__device__ __forceinline__ int tile_code(const double *__restrict__ et, const double *__restrict__ es)
+__device__ __forceinline__ int tile_code_x(double tmin, double tmax, double smin, double smax)
 {
-    const double lo = et[4] - es[5], hi = et[5] - es[4], e = 1e-12;
+    const double lo = tmin - smax, hi = tmax - smin, e = 1e-12;
     int k = 0;
     if (lo >= kMinDx && hi < 0.5 - e) k = 1;
     else if (hi <= -kMinDx && lo > -0.5 + e) k = 2;
     else if (lo > 0.5 + e && hi <= 1.0 - kMinDx) k = 3;
     else if (hi < -0.5 - e && lo >= -1.0 + kMinDx) k = 4;
-    return __builtin_amdgcn_readfirstlane(k);
+    return k;
+}
+
+// The same from the blocks' extent records (words 4 and 5), for a tile the
+// whole wavefront holds.
+__device__ __forceinline__ int tile_code(const double *__restrict__ et, const double *__restrict__ es)
+{
+    return __builtin_amdgcn_readfirstlane(tile_code_x(et[4], et[5], es[4], es[5]));
 }
 
 // Side (+1: dx > 0) and wrap offset o (dx = w_i - w_j + o) of the pairs of
@@ -102,7 +109,7 @@ struct Poly {
 };
 __device__ __forceinline__ void term_polys(int n, Poly &alpha, Poly &beta)
 {
-    const double g1 = sdlibm::exp(-sdfield::kTwoPi * n);
+    const double g1 = exp(-sdfield::kTwoPi * n);
     const double g2 = g1 * g1, g3 = g2 * g1, g4 = n == 1 ? g2 * g2 : 0.0;
     alpha = Poly{-g4, 1.0 + g1 + g2 + g3 - 3.0 * g4, g1 + 2.0 * g2 + 3.0 * g3 + 4.0 * g4};
     beta = Poly{g4, 1.0 + g1 + g2 + g3 - 5.0 * g4, -(1.0 + 2.0 * g1 + 3.0 * g2 + 4.0 * g3)};
@@ -113,14 +120,14 @@ __device__ __forceinline__ void term_polys(int n, Poly &alpha, Poly &beta)
 //   poly(p + sj xi_j) = c0(p) + sj c1(p) xi_j + q2 xi_j^2,
 //   c0(P + si xi_i) = c0(P) + si c1(P) xi_i + q2 xi_i^2,
 //   c1(P + si xi_i) = c1(P) + 2 q2 si xi_i.
-__device__ __forceinline__ void gather(Cplx L[3], double e, double P, const Poly &q, double si, double sj, Cplx M0,
-                                       Cplx M1, Cplx M2)
+__device__ __forceinline__ void gather(Cplx &L0, Cplx &L1, Cplx &L2, double e, double P, const Poly &q, double si,
+                                       double sj, Cplx M0, Cplx M1, Cplx M2)
 {
     const double c0 = (q.q2 * P + q.q1) * P + q.q0;
     const double c1 = 2.0 * q.q2 * P + q.q1;
-    L[0] = cadd(L[0], cscale(cadd(cadd(cscale(M0, c0), cscale(M1, sj * c1)), cscale(M2, q.q2)), e));
-    L[1] = cadd(L[1], cscale(cadd(cscale(M0, si * c1), cscale(M1, 2.0 * q.q2 * si * sj)), e));
-    L[2] = cadd(L[2], cscale(M0, q.q2 * e));
+    L0 = cadd(L0, cscale(cadd(cadd(cscale(M0, c0), cscale(M1, sj * c1)), cscale(M2, q.q2)), e));
+    L1 = cadd(L1, cscale(cadd(cscale(M0, si * c1), cscale(M1, 2.0 * q.q2 * si * sj)), e));
+    L2 = cadd(L2, cscale(M0, q.q2 * e));
 }
 
 }  // namespace farx

@@ -311,105 +311,186 @@ __global__ __launch_bounds__(64) void k_speeds_prep_blocks(int n, const int *__r
 }
 
 // Source moments of the x-separated tiles (far_expand.h), one workgroup of
-// four wavefronts per block of tile positions: lane m holds member m, wave v
-// the terms n = v + 1, v + 5, ...; each moment is a butterfly sum over the
-// 64 lanes (fixed order).
-__global__ __launch_bounds__(256) void k_farx_moments(int n, const int *__restrict__ perm,
-                                                      const double *__restrict__ w, const double *__restrict__ ca,
-                                                      const double *__restrict__ sa, const double *__restrict__ b,
-                                                      const double *__restrict__ ext, double *__restrict__ mom,
-                                                      double *__restrict__ mom1)
+// eight wavefronts per block of tile positions: the members' factors are
+// staged in LDS, lane t - 1 of every wave takes term t over an eighth of the
+// members (their powers by one binary exponentiation, the eight members in
+// lockstep), and the eighths are added in order.
+constexpr int kMomWaves = 8;
+__global__ __launch_bounds__(64 * kMomWaves) void k_farx_moments(int n, const int *__restrict__ perm,
+                                                                 const double *__restrict__ w,
+                                                                 const double *__restrict__ ca,
+                                                                 const double *__restrict__ sa,
+                                                                 const double *__restrict__ b,
+                                                                 const double *__restrict__ ext,
+                                                                 double *__restrict__ mom, double *__restrict__ mom1)
 {
     using namespace farx;
+    static_assert(kTerms <= 64, "one lane per term");
+    constexpr int W = kMomWaves, kPer = 64 / W, kLock = 8;
+    static_assert(kPer % kLock == 0, "members in lockstep groups");
+    __shared__ double sxi[64], sb[64], spr[64], spi[64], sr[64];
+    __shared__ double part[W][64][kWords - 2];
+    __shared__ double part1[W][kWords1];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int J = (int)blockIdx.x;
-    const int p = J * 64 + lane;
     const double X = centre(ext + (size_t)J * pairtile::kExtWords);
-    double xi = 0.0, bb = 0.0;
-    Cplx c{1.0, 0.0};
-    if (p < n) {
-        const int j = perm[p];
-        xi = w[j] - X;
-        bb = b[j];
-        c = Cplx{ca[j], -sa[j]};
+    if (wave == 0) {
+        const int p = J * 64 + lane;
+        double xi = 0.0, bb = 0.0, c = 1.0, s = 0.0;
+        if (p < n) {
+            const int j = perm[p];
+            xi = w[j] - X;
+            bb = b[j];
+            c = ca[j];
+            s = sa[j];
+        }
+        // phi+ = e^{2 pi xi} e^{-2 pi i y}; phi- = phi+ e^{-4 pi xi}
+        const double ep = exp(sdfield::kTwoPi * xi), em = exp(-sdfield::kTwoPi * xi);
+        sxi[lane] = xi;
+        sb[lane] = bb;
+        spr[lane] = c * ep;
+        spi[lane] = -s * ep;
+        sr[lane] = em * em;
     }
-    const Cplx fp = cscale(c, sdlibm::exp(sdfield::kTwoPi * xi));    // e^{2 pi xi} e^{-2 pi i y}
-    const Cplx fm = cscale(c, sdlibm::exp(-sdfield::kTwoPi * xi));
-    const Cplx step_p = cpow(fp, 4), step_m = cpow(fm, 4);
-    Cplx pp = cpow(fp, wave + 1), pm = cpow(fm, wave + 1);
-    auto wsum = [&](double v) {
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        return v;
-    };
-    for (int t = wave + 1; t <= kTerms; t += 4) {
-        double v[8] = {bb * pp.re, bb * pp.im, bb * xi * pp.re, bb * xi * pp.im,
-                       bb * pm.re, bb * pm.im, bb * xi * pm.re, bb * xi * pm.im};
+    __syncthreads();
+    const int t = lane + 1;
+    double v[kWords - 2] = {};
+    double v1[kWords1] = {};
+    if (t <= kTerms) {
+        for (int m0 = wave * kPer; m0 < wave * kPer + kPer; m0 += kLock) {
+            // phi+^t and e^{-4 pi t xi} of kLock members by one exponentiation
+            Cplx a[kLock], pp[kLock];
+            double r[kLock], rt[kLock];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = wsum(v[k]);
-        if (lane == 0) {
-            double *o = mom + ((size_t)J * kTerms + (t - 1)) * kWords;
-            o[0] = sdlibm::exp(sdfield::kTwoPi * t * X);
-            o[1] = sdlibm::exp(-sdfield::kTwoPi * t * X);
+            for (int u = 0; u < kLock; ++u) {
+                a[u] = Cplx{spr[m0 + u], spi[m0 + u]};
+                r[u] = sr[m0 + u];
+                pp[u] = Cplx{1.0, 0.0};
+                rt[u] = 1.0;
+            }
+            for (int k = t; k > 0; k >>= 1) {
+                if (k & 1) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) o[2 + k] = v[k];
+                    for (int u = 0; u < kLock; ++u) {
+                        pp[u] = cmul(pp[u], a[u]);
+                        rt[u] *= r[u];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kLock; ++u) {
+                    a[u] = cmul(a[u], a[u]);
+                    r[u] *= r[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kLock; ++u) {
+                const Cplx pm = cscale(pp[u], rt[u]);
+                const double xi = sxi[m0 + u], bb = sb[m0 + u], bx = bb * xi;
+                v[0] += bb * pp[u].re;
+                v[1] += bb * pp[u].im;
+                v[2] += bx * pp[u].re;
+                v[3] += bx * pp[u].im;
+                v[4] += bb * pm.re;
+                v[5] += bb * pm.im;
+                v[6] += bx * pm.re;
+                v[7] += bx * pm.im;
+                if (t == 1) {
+                    const double bxx = bx * xi;
+                    v1[0] += bxx * pp[u].re;
+                    v1[1] += bxx * pp[u].im;
+                    v1[2] += bxx * pm.re;
+                    v1[3] += bxx * pm.im;
+                }
+            }
         }
-        if (t == 1) {
-            const double x2 = xi * xi;
-            double u[4] = {bb * x2 * pp.re, bb * x2 * pp.im, bb * x2 * pm.re, bb * x2 * pm.im};
+    }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) u[k] = wsum(u[k]);
-            if (lane == 0)
+    for (int k = 0; k < kWords - 2; ++k) part[wave][lane][k] = v[k];
+    if (lane == 0)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) mom1[(size_t)J * kWords1 + k] = u[k];
+        for (int k = 0; k < kWords1; ++k) part1[wave][k] = v1[k];
+    __syncthreads();
+    if (wave == 0 && t <= kTerms) {
+        double *o = mom + ((size_t)J * kTerms + (t - 1)) * kWords;
+        o[0] = exp(sdfield::kTwoPi * t * X);
+        o[1] = exp(-sdfield::kTwoPi * t * X);
+#pragma unroll
+        for (int k = 0; k < kWords - 2; ++k) {
+            double sum = part[0][lane][k];
+            for (int q = 1; q < W; ++q) sum += part[q][lane][k];
+            o[2 + k] = sum;
         }
-        pp = cmul(pp, step_p);
-        pm = cmul(pm, step_m);
+        if (t == 1)
+#pragma unroll
+            for (int k = 0; k < kWords1; ++k) {
+                double sum = part1[0][k];
+                for (int q = 1; q < W; ++q) sum += part1[q][k];
+                mom1[(size_t)J * kWords1 + k] = sum;
+            }
     }
 }
 
 // Per target block I: the coefficients gathered over the source blocks of
-// its x-separated tiles (wave v takes the blocks J = v, v + 4, ...; lane
-// t - 1 the term t), summed over the four waves in order, then every member
-// of the block sums its series (lane quarter k the terms 15 k + 1 ..
-// 15 k + 15, added in order): vfar[p] = b_i 4 pi^2 sum_n n Re(...), the
-// tiles' share of the member's speed (the pair kernel skips them).
-__global__ __launch_bounds__(256) void k_farx_gather(int n, int nb, const int *__restrict__ perm,
+// its x-separated tiles, then every member of the block sums its series:
+// vfar[p] = b_i 4 pi^2 sum_n n Re(...), the tiles' share of the member's
+// speed (the pair kernel skips them).  The blocks' x-extents are staged in
+// LDS and the qualifying source blocks listed in ascending order first;
+// wave v of eight takes list entries v, v + 8, ... with lane t - 1 on term
+// t, and the eight partial coefficient sets are added in order.  For the series, wave v takes terms 8 v + 1 .. 8 v + 8
+// of every member (lane m), the eight partial sums again added in order.
+constexpr int kGatherWaves = 8;
+constexpr size_t kGatherDynLds = 96 * 1024;   // with the static 53 KiB, within the CU's 160 KiB
+__global__ __launch_bounds__(512) void k_farx_gather(int n, int nb, const int *__restrict__ perm,
                                                      const double *__restrict__ w, const double *__restrict__ ca,
                                                      const double *__restrict__ sa, const double *__restrict__ b,
                                                      const double *__restrict__ ext, const double *__restrict__ mom,
                                                      const double *__restrict__ mom1, double *__restrict__ vfar)
 {
     using namespace farx;
-    static_assert(kTerms <= 64 && kTerms % 4 == 0, "one lane per term, four quarters");
-    __shared__ double acc[4][64][12];
-    __shared__ double part[4][64];
+    static_assert(kTerms == 64, "one lane per term, eight terms per wave in the series");
+    constexpr int W = kGatherWaves;
+    extern __shared__ double xe[];                         // [nb][2] x-extents, then the list
+    int *list = reinterpret_cast<int *>(xe + 2 * (size_t)nb);   // J * 8 + code, J ascending
+    __shared__ double acc[W][64][12];
+    __shared__ double part[W][64];
+    __shared__ int wcount[W];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int I = (int)blockIdx.x;
+    for (int k = (int)threadIdx.x; k < nb; k += 64 * W) {
+        xe[2 * k] = ext[(size_t)k * pairtile::kExtWords + 4];
+        xe[2 * k + 1] = ext[(size_t)k * pairtile::kExtWords + 5];
+    }
+    __syncthreads();
+    const double XI = 0.5 * (xe[2 * I] + xe[2 * I + 1]);
+    int len = 0;
+    for (int base = 0; base < nb; base += 64 * W) {
+        const int J = base + (int)threadIdx.x;
+        int code = 0;
+        if (J < nb && J != I) {
+            const int lo = I < J ? I : J, hi = I < J ? J : I;
+            code = tile_code_x(xe[2 * lo], xe[2 * lo + 1], xe[2 * hi], xe[2 * hi + 1]);
+        }
+        const unsigned long long m = __ballot(code != 0);
+        if (lane == 0) wcount[wave] = __popcll(m);
+        __syncthreads();
+        int off = len;
+        for (int v = 0; v < wave; ++v) off += wcount[v];
+        if (code) list[off + __popcll(m & ((1ull << lane) - 1ull))] = J * 8 + code;
+        for (int v = 0; v < W; ++v) len += wcount[v];
+        __syncthreads();
+    }
     const int t = lane + 1;
-    const bool live = t <= kTerms;
-    const double *eI = ext + (size_t)I * pairtile::kExtWords;
-    const double XI = centre(eI);
-    Cplx Lm[3] = {}, Lp[3] = {};   // psi- and psi+ families, powers 0..2 of xi_i
-    if (live) {
+    // psi- and psi+ families, powers 0..2 of xi_i
+    Cplx Lm0{0.0, 0.0}, Lm1{0.0, 0.0}, Lm2{0.0, 0.0}, Lp0{0.0, 0.0}, Lp1{0.0, 0.0}, Lp2{0.0, 0.0};
+    {
         Poly alpha, beta;
         term_polys(t, alpha, beta);
-        const double Gp = sdlibm::exp(sdfield::kTwoPi * t), Gm = sdlibm::exp(-sdfield::kTwoPi * t);
+        const double Gp = exp(sdfield::kTwoPi * t), Gm = exp(-sdfield::kTwoPi * t);
         const double *mI = mom + ((size_t)I * kTerms + (t - 1)) * kWords;
         const double EXI = mI[0], EXIinv = mI[1];
         const bool withB = t <= kTermsB;
-        for (int J = wave; J < nb; J += 4) {
-            if (J == I) continue;
-            const int lo = I < J ? I : J, hi = I < J ? J : I;
-            const int code = tile_code(ext + (size_t)lo * pairtile::kExtWords, ext + (size_t)hi * pairtile::kExtWords);
-            if (!code) continue;
-            int sigma;
-            double o;
-            code_side(code, sigma, o);
-            if (I == hi) {
-                sigma = -sigma;
-                o = -o;
-            }
-            const double D = XI - centre(ext + (size_t)J * pairtile::kExtWords) + o;
+        for (int e = wave; e < len; e += W) {
+            const int J = list[e] >> 3, code = list[e] & 7;
             const double *mJ = mom + ((size_t)J * kTerms + (t - 1)) * kWords;
             const double EXJ = mJ[0], EXJinv = mJ[1];
             const Cplx M0p{mJ[2], mJ[3]}, M1p{mJ[4], mJ[5]}, M0m{mJ[6], mJ[7]}, M1m{mJ[8], mJ[9]};
@@ -419,33 +500,43 @@ __global__ __launch_bounds__(256) void k_farx_gather(int n, int nb, const int *_
                 M2p = Cplx{m1[0], m1[1]};
                 M2m = Cplx{m1[2], m1[3]};
             }
+            int sigma;
+            double o;
+            code_side(code, sigma, o);
+            if (I > J) {
+                sigma = -sigma;
+                o = -o;
+            }
+            const double D = XI - 0.5 * (xe[2 * J] + xe[2 * J + 1]) + o;
             // e^{-+2 pi t o}
             const double eo_m = o > 0 ? Gm : (o < 0 ? Gp : 1.0);
             const double eo_p = o > 0 ? Gp : (o < 0 ? Gm : 1.0);
             if (sigma > 0) {
                 // e^{-2 pi t D}: alpha terms, psi- with the + moments
-                gather(Lm, EXIinv * EXJ * eo_m, D, alpha, 1.0, -1.0, M0p, M1p, M2p);
+                gather(Lm0, Lm1, Lm2, EXIinv * EXJ * eo_m, D, alpha, 1.0, -1.0, M0p, M1p, M2p);
                 // e^{-2 pi t (1 - D)}: beta terms, psi+ with the - moments
-                if (withB) gather(Lp, Gm * EXI * EXJinv * eo_p, D, beta, 1.0, -1.0, M0m, M1m, M2m);
+                if (withB) gather(Lp0, Lp1, Lp2, Gm * EXI * EXJinv * eo_p, D, beta, 1.0, -1.0, M0m, M1m, M2m);
             } else {
                 // |dx| = D' - xi_i + xi_j, D' = -D, tau = -tau(|dx|)
-                gather(Lp, -(EXI * EXJinv * eo_p), -D, alpha, -1.0, 1.0, M0m, M1m, M2m);
-                if (withB) gather(Lm, -(Gm * EXIinv * EXJ * eo_m), -D, beta, -1.0, 1.0, M0p, M1p, M2p);
+                gather(Lp0, Lp1, Lp2, -(EXI * EXJinv * eo_p), -D, alpha, -1.0, 1.0, M0m, M1m, M2m);
+                if (withB) gather(Lm0, Lm1, Lm2, -(Gm * EXIinv * EXJ * eo_m), -D, beta, -1.0, 1.0, M0p, M1p, M2p);
             }
         }
     }
     {
         double *a = acc[wave][lane];
-        a[0] = Lm[0].re; a[1] = Lm[0].im; a[2] = Lm[1].re; a[3] = Lm[1].im; a[4] = Lm[2].re; a[5] = Lm[2].im;
-        a[6] = Lp[0].re; a[7] = Lp[0].im; a[8] = Lp[1].re; a[9] = Lp[1].im; a[10] = Lp[2].re; a[11] = Lp[2].im;
+        a[0] = Lm0.re; a[1] = Lm0.im; a[2] = Lm1.re; a[3] = Lm1.im; a[4] = Lm2.re; a[5] = Lm2.im;
+        a[6] = Lp0.re; a[7] = Lp0.im; a[8] = Lp1.re; a[9] = Lp1.im; a[10] = Lp2.re; a[11] = Lp2.im;
     }
     __syncthreads();
-    for (int k = (int)threadIdx.x; k < 64 * 12; k += 256) {
+    for (int k = (int)threadIdx.x; k < 64 * 12; k += 64 * W) {
         const int l = k / 12, f = k % 12;
-        acc[0][l][f] = ((acc[0][l][f] + acc[1][l][f]) + acc[2][l][f]) + acc[3][l][f];
+        double sum = acc[0][l][f];
+        for (int v = 1; v < W; ++v) sum += acc[v][l][f];
+        acc[0][l][f] = sum;
     }
     __syncthreads();
-    // the block's members: lane m, quarter `wave` of the terms
+    // the block's members: lane m, terms 8 wave + 1 .. 8 wave + 8
     const int p = I * 64 + lane;
     double s = 0.0, bI = 0.0;
     if (p < n) {
@@ -453,9 +544,9 @@ __global__ __launch_bounds__(256) void k_farx_gather(int n, int nb, const int *_
         bI = b[i];
         const double xi = w[i] - XI;
         const Cplx cy{ca[i], sa[i]};
-        const Cplx gm = cscale(cy, sdlibm::exp(-sdfield::kTwoPi * xi));   // psi-
-        const Cplx gp = cscale(cy, sdlibm::exp(sdfield::kTwoPi * xi));    // psi+
-        constexpr int kQ = kTerms / 4;
+        const Cplx gm = cscale(cy, exp(-sdfield::kTwoPi * xi));   // psi-
+        const Cplx gp = cscale(cy, exp(sdfield::kTwoPi * xi));    // psi+
+        constexpr int kQ = kTerms / W;
         const int t0 = wave * kQ + 1;
         Cplx pm = cpow(gm, t0), pp = cpow(gp, t0);
         for (int tt = t0; tt < t0 + kQ; ++tt) {
@@ -472,8 +563,11 @@ __global__ __launch_bounds__(256) void k_farx_gather(int n, int nb, const int *_
     }
     part[wave][lane] = s;
     __syncthreads();
-    if (wave == 0 && p < n)
-        vfar[p] = bI * (4.0 * sdfield::kPiSq) * (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+    if (wave == 0 && p < n) {
+        double sum = part[0][lane];
+        for (int v = 1; v < W; ++v) sum += part[v][lane];
+        vfar[p] = bI * (4.0 * sdfield::kPiSq) * sum;
+    }
 }
 
 }  // namespace
@@ -570,17 +664,28 @@ int launch_speeds(Context &c, int slot, int out_buf, double sigma, int update_to
         fd.em = c.xem;
         fd.yw = c.yw;
         fd.ext = c.bext;
-        fd.expand = nb >= 2 ? 1 : 0;
+        // (the series' gather stages 20 bytes per block in LDS)
+        fd.expand = nb >= 2 && (2 * sizeof(double) + sizeof(int)) * (size_t)nb <= kGatherDynLds ? 1 : 0;
     }
-    // the pair timer spans the series of the x-separated tiles too
-    if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
     if (fd.expand) {
-        hipLaunchKernelGGL(k_farx_moments, dim3(nb), dim3(256), 0, c.stream, n, c.perm, c.xw, c.ytc, c.yts, c.b,
-                           c.bext, c.farx_mom, c.farx_mom1);
-        hipLaunchKernelGGL(k_farx_gather, dim3(nb), dim3(256), 0, c.stream, n, nb, c.perm, c.xw, c.ytc, c.yts, c.b,
-                           c.bext, c.farx_mom, c.farx_mom1, c.vfar);
+        // the gather's LDS: 53 KiB of its own plus 20 bytes per block (above
+        // the default limit for the larger systems)
+        static std::once_flag lds_once;
+        static hipError_t lds_rc = hipSuccess;
+        std::call_once(lds_once, [] {
+            lds_rc = hipFuncSetAttribute((const void *)k_farx_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)kGatherDynLds);
+        });
+        SD_HIP(lds_rc);
+        hipLaunchKernelGGL(k_farx_moments, dim3(nb), dim3(64 * kMomWaves), 0, c.stream, n, c.perm, c.xw, c.ytc,
+                           c.yts, c.b, c.bext, c.farx_mom, c.farx_mom1);
+        hipLaunchKernelGGL(k_farx_gather, dim3(nb), dim3(64 * kGatherWaves),
+                           (2 * sizeof(double) + sizeof(int)) * (size_t)nb,
+                           c.stream, n, nb, c.perm, c.xw, c.ytc, c.yts, c.b, c.bext, c.farx_mom, c.farx_mom1,
+                           c.vfar);
         SD_HIP(hipGetLastError());
     }
+    if ((c.timing & kTimePair) && (rc = timer_begin(c, c.t_pair))) return rc;
     if (c.p.field_mode == SDDDST_FIELD_EXACT)
         hipLaunchKernelGGL(k_pair_tiles<SDDDST_FIELD_EXACT>, dim3((unsigned)units), dim3(kTile), 0, c.stream, n, nb,
                            (int)nfull, c.x[slot], c.y, c.b, c.perm, c.partial, c.minr2,

@@ -164,6 +164,19 @@ def test_speeds_tiled_vs_oracle(mode, n, seed):
     assert np.linalg.norm(v - vr) / np.linalg.norm(vr) < 1e-13
 
 
+# Sizes where most tiles are x-separated and summed by the field's Fourier
+# series in y (the fast field's far-tile path); 16384 is the headline system.
+@pytest.mark.parametrize("n,seed", [(8192, 13), (16384, 2)])
+def test_speeds_fast_series_vs_oracle(n, seed):
+    x, y, b, v, tol = _speeds_dev(n, seed, S.FIELD_FAST, sigma=0.02)
+    vr, tr = O.speeds(x, y, b, sigma=0.02)
+    assert np.array_equal(tol.view(np.int64), tr.view(np.int64))
+    strict, ok = O.speed_parity(v, vr, O.speeds_abs(x, y, b), n)
+    assert ok.all()
+    assert strict.mean() >= 0.995
+    assert np.linalg.norm(v - vr) / np.linalg.norm(vr) < 1e-13
+
+
 def test_speeds_sum_rule_and_determinism_n16384():
     n = 16384
     x, y, b = configs.dislocations(n, 2)

@@ -132,8 +132,13 @@ __global__ void k_ytrig(int n, const double *__restrict__ y, double *__restrict_
 // and the index (a stable radix sort keeps equal cells in index order, so
 // the order is a function of the positions alone).  Consecutive runs of 64
 // along the curve are compact in both x and y, so most tile pairs are
-// all-far.
+// all-far.  The context's speeds use the strip order below (more tiles
+// x-separated); the allocation-free entry point keeps the curve.
 constexpr int kOrderBits = 10;
+// strips = kStripFactor sqrt(blocks) (64 at N = 16384; measured on the
+// headline workload: pair kernel 0.282 ms along the curve, 0.231 with 32
+// strips, 0.224 with 64 or 96, 0.227 with 128); 0: the curve
+constexpr double kStripFactor = 4.0;
 __global__ void k_curvekey(int n, const double *__restrict__ x, const double *__restrict__ y,
                            int *__restrict__ key, int *__restrict__ idx)
 {
@@ -159,6 +164,25 @@ __global__ void k_curvekey(int n, const double *__restrict__ x, const double *__
         }
     }
     key[i] = d;
+    idx[i] = i;
+}
+
+// Strip order: the wrapped x cut into `strips` equal strips, taken left to
+// right, each walked in y up and down alternately (y in 2^kOrderBits
+// levels); equal keys keep index order under the stable sort.  Runs of 64
+// are then narrow in x, so more tiles are x-separated (summed by the series)
+// than along the curve.
+__global__ void k_stripkey(int n, int strips, const double *__restrict__ x, const double *__restrict__ y,
+                           int *__restrict__ key, int *__restrict__ idx)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    constexpr int g = 1 << kOrderBits;
+    const double wx = x[i] - floor(x[i] + 0.5), wy = y[i] - floor(y[i] + 0.5);
+    int sx = (int)((wx + 0.5) * (double)strips), cy = (int)((wy + 0.5) * (double)g);
+    sx = sx < 0 ? 0 : (sx >= strips ? strips - 1 : sx);
+    cy = cy < 0 ? 0 : (cy >= g ? g - 1 : cy);
+    key[i] = sx * g + ((sx & 1) ? g - 1 - cy : cy);
     idx[i] = i;
 }
 
@@ -602,12 +626,17 @@ int launch_ytrig(Context &c)
 
 constexpr int kOrderReuse = 16;
 
-// perm = the dislocations ordered along the curve (index order within a
-// cell).
+// perm = the dislocations in strip order (index order within equal keys).
 static int pair_order(Context &c, const double *x)
 {
     const int n = c.n;
-    hipLaunchKernelGGL(k_curvekey, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, x, c.y, c.xkey, c.xidx);
+    if (kStripFactor > 0) {
+        const int strips = std::min(1 << kOrderBits, std::max(1, (int)lround(kStripFactor * sqrt((double)c.nb))));
+        hipLaunchKernelGGL(k_stripkey, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, strips, x, c.y, c.xkey,
+                           c.xidx);
+    } else {
+        hipLaunchKernelGGL(k_curvekey, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, x, c.y, c.xkey, c.xidx);
+    }
     SD_HIP(hipGetLastError());
     size_t bytes = c.sort_tmp_bytes;
     SD_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp, bytes, c.xkey, c.xkey_sorted, c.xidx, c.perm, n, 0,

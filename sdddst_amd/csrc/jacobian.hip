@@ -13,9 +13,9 @@
 // exactly zero; that case is detected and compacted afterwards.
 //
 // Stages (all on the context stream):
-//   count  one wavefront per column; with cells at least the pre-filter
-//          radius wide, the candidates are the 3 x 3 neighbouring cells'
-//          dislocations (cell-ordered copies of their positions), else all
+//   count  one wavefront per column; with cells at least half the
+//          pre-filter radius wide, the candidates are the 5 x 5 neighbouring
+//          cells' dislocations (cell-ordered copies of their positions), else all
 //          rows 64 at a time; a cheap r^2 pre-filter precedes the square
 //          root; the column's membership bitmask is stored, so the predicate
 //          runs once per ordered pair;
@@ -174,12 +174,17 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, int G, const int *_
     }
 }
 
-// Count of column j from the 3 x 3 neighbouring cells: the same predicate as
+// Count of column j from the neighbouring cells: the same predicate as
 // count_column, bits gathered in an LDS copy of the column's bitmask (the
 // order of the candidates does not matter), then written out whole; with
 // cmask, the within-cutoff bitmask of count_column the same way (and its
 // count into ccnt).
 constexpr int kCellMaxWords = 1024;
+// cells at least 1 / kCellReach of the pre-filter radius wide: a column's
+// candidates are the (2 kCellReach + 1)^2 cells around its own (25 cells of
+// width rpre / 2 cover 6.25 rpre^2, against 9 rpre^2 for 3 x 3 cells of
+// width rpre)
+constexpr int kCellReach = 2;
 __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const double *__restrict__ x,
                                                          const double *__restrict__ y, CutParams cp, int G,
                                                          const int *__restrict__ cell,
@@ -204,14 +209,15 @@ __global__ __launch_bounds__(256) void k_jac_count_cells(int n, int words, const
         const double xj = x[j], yj = y[j];
         const int c = cell[j];
         const int cx = c % G, cy = c / G;
-        for (int oy = -1; oy <= 1; ++oy) {
+        constexpr int R = kCellReach;
+        for (int oy = -R; oy <= R; ++oy) {
             const int yy = (cy + oy + G) % G;
-            // the three cells of this row: one contiguous range unless the
+            // the 2R + 1 cells of this row: one contiguous range unless the
             // row wraps around the cell edge
-            const bool inner = cx >= 1 && cx <= G - 2;
-            for (int ox = inner ? 0 : -1; ox <= (inner ? 0 : 1); ++ox) {
+            const bool inner = cx >= R && cx <= G - 1 - R;
+            for (int ox = inner ? 0 : -R; ox <= (inner ? 0 : R); ++ox) {
                 const int q = yy * G + (cx + ox + G) % G;
-                const int k0 = inner ? cstart[q - 1] : cstart[q], k1 = inner ? cstart[q + 2] : cstart[q + 1];
+                const int k0 = inner ? cstart[q - R] : cstart[q], k1 = inner ? cstart[q + R + 1] : cstart[q + 1];
                 for (int k = k0 + lane; k < k1; k += 64) {
                     const int i = sorted[k];
                     const double xi = sx[k], yi = sy[k];
@@ -733,11 +739,13 @@ static int build_pattern(Context &c, int slot, double h)
     }
     if (want_core && !c.jccnt) SD_HIP(hipMalloc(&c.jccnt, sizeof(int) * n));
     unsigned long long *cmask = want_core ? c.jcmask : nullptr;
-    // cells at least the pre-filter radius wide; 3 x 3 neighbourhoods need G >= 3
+    // cells at least rpre / kCellReach wide; the neighbourhoods need G >= 2 kCellReach + 1
     const double rpre = sqrt(cp.prefilter);
     // (fewer, wider cells are as valid: at most kCellMaxG per side)
-    const int G = rpre > 0 && rpre < 1.0 / 3.0 ? min(kCellMaxG, (int)floor(1.0 / (rpre * (1.0 + 1e-6)))) : 0;
-    if (G >= 4 && words <= kCellMaxWords) {
+    const int G = rpre > 0 && rpre < 1.0 / 3.0
+                      ? min(kCellMaxG, (int)floor((double)kCellReach / (rpre * (1.0 + 1e-6))))
+                      : 0;
+    if (G >= 2 * kCellReach + 2 && words <= kCellMaxWords) {
         const int cells = G * G;
         if (cells > c.cell_cap) {
             for (int **q : {&c.ccount, &c.cstart, &c.ccursor}) {

@@ -634,8 +634,8 @@ def main():
         # series of the x-separated tiles, the pair kernel on the others,
         # the preparation and the finishing pass
         "pairs_per_s": pairs / (sp_ms / max(sp_cnt, 1) * 1e-3) if sp_cnt else None,
-        "speeds_eval_ms": sp_ms / max(sp_cnt, 1),
-        "jacobian_ms": jac_ms / max(jac_cnt, 1),
+        "speeds_eval_ms": sp_ms / sp_cnt if sp_cnt else None,
+        "jacobian_ms": jac_ms / jac_cnt if jac_cnt else None,
         "final_step_size": st.step_size,
         "solver_iterations_per_solve": iters / solves if solves else None,
         "solver_fallbacks_total": st.solver_fallbacks,
